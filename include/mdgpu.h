@@ -1,0 +1,223 @@
+/* mdgpu.h - C ABI of the MI355X (gfx950) energy-and-force evaluator.
+ *
+ * This is the thin boundary between the C++ host (md::System, which keeps the
+ * reference's class API) and the hand-written HIP kernels.  Plain C types and
+ * pointers only; no HIP or torch types appear in a signature.
+ *
+ * The reference has no FFI of its own: its evaluation is the private member
+ * chain System::init() -> calculate_energies()/calculate_forces()
+ * (src/System/init.cpp:10-81, src/System/System.cpp:9-39).  Each entry point
+ * below names the reference routine(s) whose work it replaces.
+ *
+ * Conventions
+ *   - every function returns an int status (MDGPU_OK == 0); no exception or
+ *     abort crosses the boundary; mdgpu_last_error() gives the message;
+ *   - host buffers belong to the caller, device buffers to the context;
+ *   - one context = one device + one HIP stream; calls are synchronous on
+ *     return unless stated; a context is not thread-safe (use one per thread);
+ *   - atom indices are 0-based; coordinates and forces are 3N interleaved
+ *     x,y,z doubles (the reference's layout, AmberTopology.h:252, System.h:175);
+ *   - energies in kcal/mol, forces in kcal/mol/A, lengths in A.
+ */
+#ifndef MDGPU_H
+#define MDGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MDGPU_OK = 0,
+    MDGPU_ERR_ARG = 1,         /* invalid argument or inconsistent system   */
+    MDGPU_ERR_HIP = 2,         /* a HIP runtime call failed                 */
+    MDGPU_ERR_STATE = 3,       /* call out of order (e.g. eval before upload) */
+    MDGPU_ERR_OVERFLOW = 4,    /* fixed-point force accumulator overflowed  */
+    MDGPU_ERR_UNSUPPORTED = 5  /* a parameter combination is not supported  */
+};
+
+/* Semantics.  REF reproduces the reference's printed energies and final
+ * forces, quirks included (empty exclusion lists, plain-Coulomb forces,
+ * O(N^2) "EE" energy; src/System/System.cpp:9-39).  AMBER evaluates the
+ * physics the reference's code is written for: exclusions and scaled 1-4
+ * pairs honoured, smooth PME reciprocal space, forces = -grad E. */
+enum { MDGPU_SEM_REF = 0, MDGPU_SEM_AMBER = 1 };
+
+/* Precision forms.  FP64: all arithmetic and accumulation in double.
+ * MIXED: pair, PME and bonded arithmetic in FP32, forces accumulated in
+ * 64-bit fixed point (bit-reproducible), FP64 for the excluded/1-4 Ewald
+ * corrections, self energy, energy/virial reductions and coordinate wrapping. */
+enum { MDGPU_PREC_FP64 = 0, MDGPU_PREC_MIXED = 1 };
+
+typedef struct mdgpu_ctx mdgpu_ctx;
+
+/* One system.  Arrays are read during mdgpu_upload_system only. */
+typedef struct {
+    int32_t n_atoms;
+    int32_t n_types;              /* LJ atom types T                          */
+    const double* charge;         /* [n_atoms] prmtop CHARGE / 18.2206899283247 */
+    const int32_t* type;          /* [n_atoms] 0-based LJ type                */
+    const int32_t* nb_index;      /* [T*T] 1-based LJ table index, 0 = none   */
+    int32_t n_lj;                 /* length of the LJ tables                  */
+    const double* lj_a;           /* [n_lj] A (r^-12) coefficients            */
+    const double* lj_b;           /* [n_lj] B (r^-6)                          */
+    const double* lj14_a;         /* [n_lj] 1-4 A                             */
+    const double* lj14_b;         /* [n_lj] 1-4 B                             */
+
+    int32_t n_bonds;              /* harmonic bonds E = k (r - r0)^2          */
+    const int32_t* bond_atoms;    /* [2*n]                                    */
+    const int32_t* bond_type;     /* [n] 0-based into bond_k/bond_r0          */
+    int32_t n_bond_types;
+    const double* bond_k;
+    const double* bond_r0;
+
+    int32_t n_ubs;                /* Urey-Bradley E = k (r13 - r0)^2          */
+    const int32_t* ub_atoms;      /* [2*n]                                    */
+    const int32_t* ub_type;
+    int32_t n_ub_types;
+    const double* ub_k;
+    const double* ub_r0;
+
+    int32_t n_angles;             /* harmonic angles E = k (theta - theta0)^2 */
+    const int32_t* angle_atoms;   /* [3*n]                                    */
+    const int32_t* angle_type;
+    int32_t n_angle_types;
+    const double* angle_k;
+    const double* angle_theta0;
+
+    int32_t n_dihedrals;          /* cosine E = k (1 + cos(n phi - phase))    */
+    const int32_t* dihedral_atoms;/* [4*n]                                    */
+    const int32_t* dihedral_type;
+    int32_t n_dihedral_types;
+    const double* dihedral_k;
+    const double* dihedral_n;     /* periodicity (double, as in the prmtop)   */
+    const double* dihedral_phase;
+
+    int32_t n_impropers;          /* CHARMM harmonic E = k (psi - psi0)^2     */
+    const int32_t* improper_atoms;/* [4*n]                                    */
+    const int32_t* improper_type;
+    int32_t n_improper_types;
+    const double* improper_k;
+    const double* improper_psi0;
+
+    int32_t n_cmaps;              /* CMAP bicubic patches on (phi, psi)       */
+    const int32_t* cmap_atoms;    /* [5*n]                                    */
+    const int32_t* cmap_set;      /* [n] 1-based parameter set                */
+    int32_t cmap_n_sets;          /* number of sets in cmap_coeff / cmap_grid */
+    int32_t cmap_resolution;      /* grid points per angle (same for all sets)*/
+    const double* cmap_coeff;     /* [n_sets*res*res*16] patch coefficients   */
+    const double* cmap_grid;      /* [n_sets*res*res] grid energies (phi-major)*/
+
+    /* Nonbonded exclusions (AMBER semantics; REF passes none).  excl_* is the
+     * full symmetric per-atom CSR list (both directions, 1-4 partners
+     * included) used to mask pair tiles.  The explicit pair lists drive the
+     * FP64 correction kernel. */
+    const int32_t* excl_start;    /* [n_atoms+1] or NULL                      */
+    const int32_t* excl_list;     /* [excl_start[n_atoms]] partner indices    */
+    int32_t n_excl_pairs;         /* pure exclusions (1-2, 1-3), i < j        */
+    const int32_t* excl_pairs;    /* [2*n]                                    */
+    int32_t n_pairs14;            /* 1-4 pairs from proper dihedrals          */
+    const int32_t* pairs14;       /* [2*n]                                    */
+    const double* scee14;         /* [n] Coulomb 1-4 divisor                  */
+} mdgpu_system_desc;
+
+/* Evaluation parameters (the reference hard-codes these in System::init(),
+ * src/System/init.cpp:18-23,53; its setters live in System.h:54-62). */
+typedef struct {
+    int32_t semantics;            /* MDGPU_SEM_*                              */
+    int32_t precision;            /* MDGPU_PREC_*                             */
+    double lj_cutoff;             /* A                                        */
+    double ee_cutoff;             /* A                                        */
+    double lj_switch;             /* switch radius (A); <= 0: force-shifted LJ */
+    double ewald_alpha;           /* 1/A                                      */
+    int32_t pme_grid[3];          /* 0,0,0 -> no reciprocal space             */
+    int32_t pme_order;            /* B-spline order (4 supported)             */
+} mdgpu_params;
+
+/* Per-term energies (kcal/mol).  Names follow the reference's members
+ * (System.h:176-188) plus the AMBER terms it does not compute. */
+typedef struct {
+    double bond;                  /* bond_energies_                           */
+    double angle;                 /* angle_energy_                            */
+    double dihedral;              /* dihedral_energy_                         */
+    double urey_bradley;          /* urey_bradley_energy_                     */
+    double improper;              /* improper_energy_                         */
+    double cmap;                  /* CMAP_energy_                             */
+    double vdw;                   /* LJ_energy_pairlist_ (switched / shifted) */
+    double ee_all;                /* EE_energy_: plain Coulomb, all i<j < rc  */
+    double ee_pairlist;           /* EE_energy_pairlist_                      */
+    double ee_rf;                 /* EE_energy_pairlist_cutoff_               */
+    double ewald_direct;          /* EE_energy_pairlist_ewald_direct_         */
+    double ewald_self;            /* EE_energy_pairlist_ewald_self_           */
+    double ewald_recip;           /* AMBER: smooth PME reciprocal energy      */
+    double ewald_excl;            /* AMBER: -k qq erf(a r)/r, pure exclusions */
+    double vdw14;                 /* AMBER: 1-4 LJ                            */
+    double ee14;                  /* AMBER: 1-4 Coulomb (scaled) minus erf part */
+    double total;                 /* energy whose -gradient the forces are    */
+} mdgpu_energies;
+
+/* Kernel ids for mdgpu_get_timing(). */
+enum {
+    MDGPU_K_SORT = 0,       /* spatial sort + cluster build                 */
+    MDGPU_K_PAIRLIST = 1,   /* cluster-pair search                          */
+    MDGPU_K_NONBONDED = 2,  /* LJ + Coulomb/Ewald pair kernel               */
+    MDGPU_K_BONDED = 3,     /* fused bonded kernel                          */
+    MDGPU_K_CORRECT = 4,    /* FP64 excluded / 1-4 corrections              */
+    MDGPU_K_SPREAD = 5,     /* PME charge spreading                         */
+    MDGPU_K_FFT = 6,        /* PME r2c + c2c passes (forward + inverse)     */
+    MDGPU_K_CONV = 7,       /* PME influence function + energy + virial     */
+    MDGPU_K_GATHER = 8,     /* PME force gather                             */
+    MDGPU_K_REDUCE = 9,     /* energy reductions + force conversion         */
+    MDGPU_K_COUNT = 10
+};
+
+/* Create a context on HIP device `device`. */
+int mdgpu_create(int device, mdgpu_ctx** out);
+/* Destroy a context and free its device memory (NULL is accepted). */
+int mdgpu_destroy(mdgpu_ctx* ctx);
+/* Message for the last non-zero status returned on this context. */
+const char* mdgpu_last_error(const mdgpu_ctx* ctx);
+
+/* Upload the topology (replaces System::build_nonbonded_cache,
+ * NonBondedUtils.cpp:134-160, and the term-list reads of the energy/force
+ * passes).  May be called again to replace the system. */
+int mdgpu_upload_system(mdgpu_ctx* ctx, const mdgpu_system_desc* sys);
+
+/* Set the evaluation parameters (replaces the setters and hard-coded values
+ * of System::init, init.cpp:15-23,53-55, System.h:54-62). */
+int mdgpu_set_params(mdgpu_ctx* ctx, const mdgpu_params* p);
+
+/* One energy + force evaluation from host buffers.  Replaces the evaluation
+ * part of System::init(): build_lj_pairlist (NonBondedUtils.cpp:185-282),
+ * calculate_energies (System.cpp:9-23) and calculate_forces (System.cpp:25-39).
+ *   coords     [3N] host, raw (unwrapped) coordinates;
+ *   box        [3]  orthorhombic box lengths (all > 0);
+ *   out_e      energies (may be NULL);
+ *   out_forces [3N] host (may be NULL);
+ *   out_virial [9]  host, reciprocal-space virial (may be NULL). */
+int mdgpu_eval(mdgpu_ctx* ctx, const double* coords, const double box[3],
+               mdgpu_energies* out_e, double* out_forces, double* out_virial);
+
+/* Same evaluation with device-resident coordinates and forces (3N doubles in
+ * device memory), for callers that keep state on the GPU.  The kernels run
+ * on the context's stream; the call returns after they finish. */
+int mdgpu_eval_device(mdgpu_ctx* ctx, const double* d_coords, const double box[3],
+                      mdgpu_energies* out_e, double* d_forces, double* out_virial);
+
+/* Per-kernel timing with HIP events on the context stream (for roofline
+ * reporting).  mdgpu_set_timing(ctx, 1) resets and enables; each later
+ * evaluation adds the event-measured duration of every kernel group. */
+int mdgpu_set_timing(mdgpu_ctx* ctx, int enable);
+int mdgpu_get_timing(mdgpu_ctx* ctx, int kernel_id, double* total_ms, int64_t* launches);
+/* Work counts of the last evaluation: [0] atom pairs within cutoff (set when
+ * counting is enabled), [1] cluster-pair tiles, [2] clusters, [3] atoms. */
+int mdgpu_get_counts(mdgpu_ctx* ctx, int64_t counts[4]);
+/* Enable (1) or disable (0) counting of pairs within the cutoff. */
+int mdgpu_set_counting(mdgpu_ctx* ctx, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDGPU_H */

@@ -1,0 +1,41 @@
+/* mdhost.h - C entry points of the host topology library (libmdhost.so).
+ *
+ * They expose the C++ prmtop/rst7 reader (md::AmberTopology::AmberTopology,
+ * which keeps the reference's parsing semantics, the src/AmberTopology sources) to C and
+ * Python callers as an mdgpu_system_desc ready for mdgpu_upload_system().
+ * No counterpart exists in the reference, whose topology is reachable only
+ * from C++ (AmberTopology.h:33).
+ */
+#ifndef MDHOST_H
+#define MDHOST_H
+
+#include "mdgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mdhost_system mdhost_system;
+
+/* Parse prmtop (+ optional rst7; NULL or "" for none) in MDGPU_SEM_REF or
+ * MDGPU_SEM_AMBER semantics.  Returns MDGPU_OK or MDGPU_ERR_ARG. */
+int mdhost_load(const char* prmtop, const char* rst7, int semantics, mdhost_system** out);
+/* Fill *out with pointers into the handle (valid until mdhost_free). */
+int mdhost_desc(const mdhost_system* h, mdgpu_system_desc* out);
+/* 3N coordinates read from the rst7 (zeros without one). */
+const double* mdhost_coords(const mdhost_system* h);
+/* Atom masses [N]. */
+const double* mdhost_masses(const mdhost_system* h);
+/* Box lengths (0,0,0 when the rst7 has no box). */
+int mdhost_box(const mdhost_system* h, double box[3]);
+/* Per-dihedral flags: bit 0 = no 1-4 (negative 3rd index), bit 1 = improper. */
+const int32_t* mdhost_dihedral_flags(const mdhost_system* h);
+void mdhost_free(mdhost_system* h);
+/* Message of the last failure on this thread. */
+const char* mdhost_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,745 @@
+// mdgpu C ABI: context, upload, parameters, evaluation and the final
+// reduction.  One evaluation runs, on the context's stream:
+//   [AMBER] excluded-pair extent -> sort + clusters -> cluster-pair search ->
+//   nonbonded -> bonded -> [AMBER] FP64 corrections -> [AMBER] PME ->
+//   reduction (energy rows in fixed order, fixed-point -> FP64 forces)
+// and returns after one device -> host copy of the energies and flags.
+#include "ctx.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+using mdg::Topo;
+using mdg::Work;
+
+namespace {
+
+int set_error(mdgpu_ctx* c, int code, const std::string& msg) {
+    if (c) c->error = msg;
+    return code;
+}
+
+int hip_error(mdgpu_ctx* c, hipError_t e, const char* where) {
+    return set_error(c, MDGPU_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, where)                          \
+    do {                                              \
+        hipError_t _e = (expr);                       \
+        if (_e != hipSuccess) return hip_error(c, _e, where); \
+    } while (0)
+
+template <typename T>
+hipError_t upload(mdgpu_ctx* c, T** dst, const T* src, size_t n) {
+    *dst = nullptr;
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(dst), n * sizeof(T));
+    if (e != hipSuccess) return e;
+    c->topo_allocs.push_back(*dst);
+    return hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice);
+}
+
+void free_topo(mdgpu_ctx* c) {
+    for (void* p : c->topo_allocs) (void)hipFree(p);
+    c->topo_allocs.clear();
+    c->topo = Topo{};
+}
+
+template <typename T>
+hipError_t ensure(T** p, size_t* cap, size_t n) {
+    if (*p && *cap >= n) return hipSuccess;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(n, 1) * sizeof(T));
+    if (e == hipSuccess) *cap = n;
+    return e;
+}
+
+void free_work(mdgpu_ctx* c) {
+    Work& w = c->work;
+    void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_count, w.col_fill, w.col_start, w.col_cstart, w.col_tmp,
+                    w.slot_atom, w.slot_type, w.slot_pos_d, w.slot_pos_f, w.cluster_col, w.cl_origin, w.cl_half,
+                    w.list_count, w.list_j, w.list_mask, w.list_delta_d, w.list_delta_f, w.force_d, w.force_fx,
+                    w.partial, w.energies, w.flags, w.excl_rmax, w.grid_d, w.grid_fx, w.grid_f, w.grid_phi_d,
+                    w.cgrid, w.bsp_mod, w.bsp_mod_f, w.twiddle_d, w.twiddle_f};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    c->work = Work{};
+}
+
+bool is_2a3b(int n) {
+    if (n < 2) return false;
+    while (n % 2 == 0) n /= 2;
+    while (n % 3 == 0) n /= 3;
+    return n == 1;
+}
+
+// LJ pair table over (T+1)^2 types with the force-shift constants at rc.
+int build_lj_table(mdgpu_ctx* c) {
+    const std::vector<double>& A = c->host_lj_a;
+    const std::vector<double>& B = c->host_lj_b;
+    const int T = c->topo.n_types, tp = T + 1;
+    const double rc2 = c->params.lj_cutoff * c->params.lj_cutoff;
+    std::vector<double4> td(static_cast<size_t>(tp) * tp);
+    std::vector<float4> tf(td.size());
+    for (int i = 0; i < tp; ++i)
+        for (int j = 0; j < tp; ++j) {
+            const size_t k = static_cast<size_t>(i) * tp + j;
+            double a = 0, b = 0;
+            if (i < T && j < T) {
+                a = A[static_cast<size_t>(i) * T + j];
+                b = B[static_cast<size_t>(i) * T + j];
+            }
+            // U(rc) and G(rc) = -U'(rc)/rc (NonBondedUtils.cpp:162-183)
+            const double s2 = 1.0 / rc2, s6 = s2 * s2 * s2;
+            const double u = a * s6 * s6 - b * s6;
+            const double g = 12.0 * a * s6 * s6 * s2 - 6.0 * b * s6 * s2;
+            td[k] = make_double4(a, b, u, g);
+            tf[k] = make_float4(static_cast<float>(a), static_cast<float>(b), static_cast<float>(u),
+                                static_cast<float>(g));
+        }
+    hipError_t e = hipMemcpy(c->topo.lj_d, td.data(), td.size() * sizeof(double4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->topo.lj_f, tf.data(), tf.size() * sizeof(float4), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_error(c, e, "lj table upload");
+    return MDGPU_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ API --
+
+extern "C" {
+
+int mdgpu_create(int device, mdgpu_ctx** out) {
+    if (!out) return MDGPU_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return MDGPU_ERR_ARG;
+    auto* c = new (std::nothrow) mdgpu_ctx();
+    if (!c) return MDGPU_ERR_HIP;
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (auto& ev : c->ev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.flags), 4 * sizeof(unsigned int));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.energies), mdg::E_NSLOTS * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.excl_rmax), sizeof(float));
+    if (e != hipSuccess) {
+        mdgpu_destroy(c);
+        return MDGPU_ERR_HIP;
+    }
+    *out = c;
+    return MDGPU_OK;
+}
+
+int mdgpu_destroy(mdgpu_ctx* c) {
+    if (!c) return MDGPU_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_topo(c);
+    free_work(c);
+    for (auto& ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MDGPU_OK;
+}
+
+const char* mdgpu_last_error(const mdgpu_ctx* c) { return c ? c->error.c_str() : "null context"; }
+
+}  // extern "C"
+
+namespace {
+
+int validate(mdgpu_ctx* c, const mdgpu_system_desc* s) {
+    const int n = s->n_atoms;
+    if (n <= 0) return set_error(c, MDGPU_ERR_ARG, "n_atoms must be positive");
+    if (s->n_types <= 0 || !s->charge || !s->type || !s->nb_index)
+        return set_error(c, MDGPU_ERR_ARG, "charges, types and nb_index are required");
+    for (int i = 0; i < n; ++i)
+        if (s->type[i] < 0 || s->type[i] >= s->n_types) return set_error(c, MDGPU_ERR_ARG, "atom type out of range");
+    for (long k = 0; k < static_cast<long>(s->n_types) * s->n_types; ++k)
+        if (s->nb_index[k] < 0 || s->nb_index[k] > s->n_lj)
+            return set_error(c, MDGPU_ERR_ARG, "nb_index out of range");
+    auto atoms_ok = [&](const int32_t* a, long cnt) {
+        for (long k = 0; k < cnt; ++k)
+            if (a[k] < 0 || a[k] >= n) return false;
+        return true;
+    };
+    auto types_ok = [&](const int32_t* t, long cnt, int ntypes) {
+        for (long k = 0; k < cnt; ++k)
+            if (t[k] < 0 || t[k] >= ntypes) return false;
+        return true;
+    };
+    if (!atoms_ok(s->bond_atoms, 2L * s->n_bonds) || !types_ok(s->bond_type, s->n_bonds, s->n_bond_types) ||
+        !atoms_ok(s->ub_atoms, 2L * s->n_ubs) || !types_ok(s->ub_type, s->n_ubs, s->n_ub_types) ||
+        !atoms_ok(s->angle_atoms, 3L * s->n_angles) || !types_ok(s->angle_type, s->n_angles, s->n_angle_types) ||
+        !atoms_ok(s->dihedral_atoms, 4L * s->n_dihedrals) ||
+        !types_ok(s->dihedral_type, s->n_dihedrals, s->n_dihedral_types) ||
+        !atoms_ok(s->improper_atoms, 4L * s->n_impropers) ||
+        !types_ok(s->improper_type, s->n_impropers, s->n_improper_types) ||
+        !atoms_ok(s->cmap_atoms, 5L * s->n_cmaps) || !atoms_ok(s->excl_pairs, 2L * s->n_excl_pairs) ||
+        !atoms_ok(s->pairs14, 2L * s->n_pairs14))
+        return set_error(c, MDGPU_ERR_ARG, "term list index out of range");
+    if (s->n_cmaps > 0) {
+        if (s->cmap_resolution < 1 || s->cmap_n_sets < 1 || !s->cmap_coeff)
+            return set_error(c, MDGPU_ERR_ARG, "CMAP terms need coefficient tables");
+        for (int k = 0; k < s->n_cmaps; ++k)
+            if (s->cmap_set[k] < 1 || s->cmap_set[k] > s->cmap_n_sets)
+                return set_error(c, MDGPU_ERR_ARG, "CMAP parameter set out of range");
+    }
+    if (s->excl_start) {
+        if (s->excl_start[0] != 0) return set_error(c, MDGPU_ERR_ARG, "excl_start[0] must be 0");
+        for (int i = 0; i < n; ++i) {
+            if (s->excl_start[i + 1] < s->excl_start[i]) return set_error(c, MDGPU_ERR_ARG, "excl_start not sorted");
+            for (int k = s->excl_start[i]; k < s->excl_start[i + 1]; ++k) {
+                if (s->excl_list[k] < 0 || s->excl_list[k] >= n)
+                    return set_error(c, MDGPU_ERR_ARG, "exclusion partner out of range");
+                if (k > s->excl_start[i] && s->excl_list[k] <= s->excl_list[k - 1])
+                    return set_error(c, MDGPU_ERR_ARG, "exclusion lists must be sorted and unique");
+            }
+        }
+    }
+    return MDGPU_OK;
+}
+
+}  // namespace
+
+extern "C" int mdgpu_upload_system(mdgpu_ctx* c, const mdgpu_system_desc* s) {
+    if (!c || !s) return MDGPU_ERR_ARG;
+    int st = validate(c, s);
+    if (st != MDGPU_OK) return st;
+    HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
+    HIP_TRY(hipStreamSynchronize(c->stream), "sync");
+    free_topo(c);
+    c->have_system = false;
+    Topo& t = c->topo;
+    const int n = s->n_atoms, T = s->n_types;
+    t.n_atoms = n;
+    t.n_types = T;
+    HIP_TRY(upload(c, &t.charge, s->charge, n), "upload charge");
+    HIP_TRY(upload(c, &t.type, reinterpret_cast<const int*>(s->type), n), "upload type");
+    HIP_TRY(upload(c, &t.nb_index, reinterpret_cast<const int*>(s->nb_index), static_cast<size_t>(T) * T),
+            "upload nb_index");
+    // LJ tables per type pair; listed = the pair has an LJ entry
+    c->host_lj_a.assign(static_cast<size_t>(T) * T, 0.0);
+    c->host_lj_b.assign(static_cast<size_t>(T) * T, 0.0);
+    c->host_listed.assign(static_cast<size_t>(T + 1) * (T + 1), 0);
+    for (int i = 0; i < T; ++i)
+        for (int j = 0; j < T; ++j) {
+            const int nb = s->nb_index[i * T + j];
+            if (nb > 0) {
+                c->host_lj_a[static_cast<size_t>(i) * T + j] = s->lj_a[nb - 1];
+                c->host_lj_b[static_cast<size_t>(i) * T + j] = s->lj_b[nb - 1];
+                c->host_listed[static_cast<size_t>(i) * (T + 1) + j] = 1;
+            }
+        }
+    const size_t tp2 = static_cast<size_t>(T + 1) * (T + 1);
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&t.lj_d), tp2 * sizeof(double4)), "alloc lj");
+    c->topo_allocs.push_back(t.lj_d);
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&t.lj_f), tp2 * sizeof(float4)), "alloc lj");
+    c->topo_allocs.push_back(t.lj_f);
+    HIP_TRY(upload(c, &t.lj_listed, c->host_listed.data(), tp2), "upload listed");
+    {
+        std::vector<double2> l14(std::max(s->n_lj, 1));
+        for (int k = 0; k < s->n_lj; ++k) l14[k] = make_double2(s->lj14_a ? s->lj14_a[k] : 0.0, s->lj14_b ? s->lj14_b[k] : 0.0);
+        HIP_TRY(upload(c, &t.lj14, l14.data(), l14.size()), "upload lj14");
+    }
+    // bonded terms
+    auto pack2 = [](const int32_t* a, int n2) {
+        std::vector<int2> v(n2);
+        for (int k = 0; k < n2; ++k) v[k] = make_int2(a[2 * k], a[2 * k + 1]);
+        return v;
+    };
+    auto par2 = [](const double* a, const double* b, int nt) {
+        std::vector<double2> v(nt);
+        for (int k = 0; k < nt; ++k) v[k] = make_double2(a[k], b[k]);
+        return v;
+    };
+    t.n_bonds = s->n_bonds;
+    {
+        auto a = pack2(s->bond_atoms, s->n_bonds);
+        auto p = par2(s->bond_k, s->bond_r0, s->n_bond_types);
+        HIP_TRY(upload(c, &t.bond_atoms, a.data(), a.size()), "upload bonds");
+        HIP_TRY(upload(c, &t.bond_type, reinterpret_cast<const int*>(s->bond_type), s->n_bonds), "upload bonds");
+        HIP_TRY(upload(c, &t.bond_par, p.data(), p.size()), "upload bonds");
+    }
+    t.n_ubs = s->n_ubs;
+    {
+        auto a = pack2(s->ub_atoms, s->n_ubs);
+        auto p = par2(s->ub_k, s->ub_r0, s->n_ub_types);
+        HIP_TRY(upload(c, &t.ub_atoms, a.data(), a.size()), "upload ub");
+        HIP_TRY(upload(c, &t.ub_type, reinterpret_cast<const int*>(s->ub_type), s->n_ubs), "upload ub");
+        HIP_TRY(upload(c, &t.ub_par, p.data(), p.size()), "upload ub");
+    }
+    t.n_angles = s->n_angles;
+    {
+        std::vector<int4> a(s->n_angles);
+        for (int k = 0; k < s->n_angles; ++k)
+            a[k] = make_int4(s->angle_atoms[3 * k], s->angle_atoms[3 * k + 1], s->angle_atoms[3 * k + 2], 0);
+        auto p = par2(s->angle_k, s->angle_theta0, s->n_angle_types);
+        HIP_TRY(upload(c, &t.angle_atoms, a.data(), a.size()), "upload angles");
+        HIP_TRY(upload(c, &t.angle_type, reinterpret_cast<const int*>(s->angle_type), s->n_angles), "upload angles");
+        HIP_TRY(upload(c, &t.angle_par, p.data(), p.size()), "upload angles");
+    }
+    t.n_dihedrals = s->n_dihedrals;
+    {
+        std::vector<int4> a(s->n_dihedrals);
+        for (int k = 0; k < s->n_dihedrals; ++k)
+            a[k] = make_int4(s->dihedral_atoms[4 * k], s->dihedral_atoms[4 * k + 1], s->dihedral_atoms[4 * k + 2],
+                             s->dihedral_atoms[4 * k + 3]);
+        std::vector<double4> p(s->n_dihedral_types);
+        for (int k = 0; k < s->n_dihedral_types; ++k)
+            p[k] = make_double4(s->dihedral_k[k], s->dihedral_n[k], s->dihedral_phase[k], 0.0);
+        HIP_TRY(upload(c, &t.dih_atoms, a.data(), a.size()), "upload dihedrals");
+        HIP_TRY(upload(c, &t.dih_type, reinterpret_cast<const int*>(s->dihedral_type), s->n_dihedrals),
+                "upload dihedrals");
+        HIP_TRY(upload(c, &t.dih_par, p.data(), p.size()), "upload dihedrals");
+    }
+    t.n_impropers = s->n_impropers;
+    {
+        std::vector<int4> a(s->n_impropers);
+        for (int k = 0; k < s->n_impropers; ++k)
+            a[k] = make_int4(s->improper_atoms[4 * k], s->improper_atoms[4 * k + 1], s->improper_atoms[4 * k + 2],
+                             s->improper_atoms[4 * k + 3]);
+        auto p = par2(s->improper_k, s->improper_psi0, s->n_improper_types);
+        HIP_TRY(upload(c, &t.imp_atoms, a.data(), a.size()), "upload impropers");
+        HIP_TRY(upload(c, &t.imp_type, reinterpret_cast<const int*>(s->improper_type), s->n_impropers),
+                "upload impropers");
+        HIP_TRY(upload(c, &t.imp_par, p.data(), p.size()), "upload impropers");
+    }
+    t.n_cmaps = s->n_cmaps;
+    t.cmap_res = s->cmap_resolution;
+    t.cmap_n_sets = s->cmap_n_sets;
+    if (s->n_cmaps > 0) {
+        const size_t nc = static_cast<size_t>(s->cmap_n_sets) * s->cmap_resolution * s->cmap_resolution * 16;
+        std::vector<float> cf(nc);
+        for (size_t k = 0; k < nc; ++k) cf[k] = static_cast<float>(s->cmap_coeff[k]);
+        HIP_TRY(upload(c, &t.cmap_atoms, reinterpret_cast<const int*>(s->cmap_atoms), 5 * static_cast<size_t>(s->n_cmaps)),
+                "upload cmap");
+        HIP_TRY(upload(c, &t.cmap_set, reinterpret_cast<const int*>(s->cmap_set), s->n_cmaps), "upload cmap");
+        HIP_TRY(upload(c, &t.cmap_coeff, s->cmap_coeff, nc), "upload cmap");
+        HIP_TRY(upload(c, &t.cmap_coeff_f, cf.data(), nc), "upload cmap");
+    }
+    // exclusions
+    t.has_excl = s->excl_start != nullptr && s->excl_start[n] > 0;
+    if (t.has_excl) {
+        HIP_TRY(upload(c, &t.excl_start, reinterpret_cast<const int*>(s->excl_start), static_cast<size_t>(n) + 1),
+                "upload exclusions");
+        HIP_TRY(upload(c, &t.excl_list, reinterpret_cast<const int*>(s->excl_list), static_cast<size_t>(s->excl_start[n])),
+                "upload exclusions");
+    }
+    t.n_excl_pairs = s->n_excl_pairs;
+    {
+        auto a = pack2(s->excl_pairs, s->n_excl_pairs);
+        HIP_TRY(upload(c, &t.excl_pairs, a.data(), a.size()), "upload excluded pairs");
+    }
+    t.n_pairs14 = s->n_pairs14;
+    {
+        auto a = pack2(s->pairs14, s->n_pairs14);
+        HIP_TRY(upload(c, &t.pairs14, a.data(), a.size()), "upload 1-4 pairs");
+        HIP_TRY(upload(c, &t.scee14, s->scee14, s->n_pairs14), "upload 1-4 pairs");
+    }
+    double q2 = 0.0;
+    for (int i = 0; i < n; ++i) q2 += s->charge[i] * s->charge[i];
+    t.sum_q2 = q2;
+    c->have_system = true;
+    if (c->have_params) return build_lj_table(c);
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_set_params(mdgpu_ctx* c, const mdgpu_params* p) {
+    if (!c || !p) return MDGPU_ERR_ARG;
+    if (p->semantics != MDGPU_SEM_REF && p->semantics != MDGPU_SEM_AMBER)
+        return set_error(c, MDGPU_ERR_ARG, "unknown semantics");
+    if (p->precision != MDGPU_PREC_FP64 && p->precision != MDGPU_PREC_MIXED)
+        return set_error(c, MDGPU_ERR_ARG, "unknown precision");
+    if (!(p->lj_cutoff > 0) || !(p->ee_cutoff > 0)) return set_error(c, MDGPU_ERR_ARG, "cutoffs must be positive");
+    // the reference throws for a switch radius >= cutoff (NonBondedUtils.cpp:30-37)
+    if (p->lj_switch > 0 && !(p->lj_switch < p->lj_cutoff))
+        return set_error(c, MDGPU_ERR_ARG, "lj_switch must be < lj_cutoff");
+    if (!(p->ewald_alpha >= 0)) return set_error(c, MDGPU_ERR_ARG, "ewald_alpha must be >= 0");
+    const bool pme = p->pme_grid[0] != 0 || p->pme_grid[1] != 0 || p->pme_grid[2] != 0;
+    if (pme) {
+        if (p->semantics != MDGPU_SEM_AMBER)
+            return set_error(c, MDGPU_ERR_UNSUPPORTED, "PME is evaluated in AMBER semantics only");
+        if (p->pme_order != 4) return set_error(c, MDGPU_ERR_UNSUPPORTED, "only PME order 4 is implemented");
+        for (int d = 0; d < 3; ++d)
+            if (!is_2a3b(p->pme_grid[d]) || p->pme_grid[d] < 8 || p->pme_grid[d] > 512)
+                return set_error(c, MDGPU_ERR_ARG, "PME grid sizes must be 2^a 3^b in [8, 512]");
+        if (!(p->ewald_alpha > 0)) return set_error(c, MDGPU_ERR_ARG, "PME needs ewald_alpha > 0");
+    }
+    HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
+    c->params = *p;
+    c->have_params = true;
+    if (c->have_system) return build_lj_table(c);
+    return MDGPU_OK;
+}
+
+namespace {
+
+int ensure_pme(mdgpu_ctx* c, const double box[3]) {
+    Work& w = c->work;
+    const int* K = c->params.pme_grid;
+    const bool same = w.pme_k[0] == K[0] && w.pme_k[1] == K[1] && w.pme_k[2] == K[2];
+    if (K[0] == 0) {
+        w.pme_k[0] = w.pme_k[1] = w.pme_k[2] = 0;
+        return MDGPU_OK;
+    }
+    if (!same) {
+        const size_t G = static_cast<size_t>(K[0]) * K[1] * K[2];
+        const size_t Gc = static_cast<size_t>(K[0]) * K[1] * (K[2] / 2 + 1);
+        void* ptrs[] = {w.grid_d, w.grid_fx, w.grid_f, w.grid_phi_d, w.cgrid, w.bsp_mod, w.twiddle_d, w.twiddle_f};
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.grid_d), G * sizeof(double)), "alloc grid");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.grid_fx), G * sizeof(unsigned long long)), "alloc grid");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.grid_f), G * sizeof(float)), "alloc grid");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.grid_phi_d), G * sizeof(double)), "alloc grid");
+        HIP_TRY(hipMalloc(&w.cgrid, Gc * sizeof(double2)), "alloc grid");
+        HIP_TRY(hipMemset(w.grid_d, 0, G * sizeof(double)), "clear grid");
+        HIP_TRY(hipMemset(w.grid_fx, 0, G * sizeof(unsigned long long)), "clear grid");
+        const int nk = K[0] + K[1] + K[2];
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.bsp_mod), nk * sizeof(double)), "alloc moduli");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.twiddle_d), 2 * nk * sizeof(double)), "alloc twiddles");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.twiddle_f), 2 * nk * sizeof(float)), "alloc twiddles");
+        std::vector<double> mod(nk), twd(2 * nk);
+        std::vector<float> twf(2 * nk);
+        int off = 0;
+        for (int d = 0; d < 3; ++d) {
+            const int n = K[d];
+            // 1/|b(m)|^2 = |sum_{k=0}^{2} M4(k+1) e^{2 pi i m k/n}|^2 with M4(1..3) = 1/6, 4/6, 1/6
+            const double m4[3] = {1.0 / 6.0, 4.0 / 6.0, 1.0 / 6.0};
+            for (int m = 0; m < n; ++m) {
+                long double sr = 0, si = 0;
+                for (int k = 0; k < 3; ++k) {
+                    const long double ang = 2.0L * 3.141592653589793238462643383279502884L * m * k / n;
+                    sr += m4[k] * cosl(ang);
+                    si += m4[k] * sinl(ang);
+                }
+                mod[off + m] = static_cast<double>(1.0L / (sr * sr + si * si));
+                const long double ang = -2.0L * 3.141592653589793238462643383279502884L * m / n;
+                twd[2 * (off + m)] = static_cast<double>(cosl(ang));
+                twd[2 * (off + m) + 1] = static_cast<double>(sinl(ang));
+                twf[2 * (off + m)] = static_cast<float>(cosl(ang));
+                twf[2 * (off + m) + 1] = static_cast<float>(sinl(ang));
+            }
+            off += n;
+        }
+        HIP_TRY(hipMemcpy(w.bsp_mod, mod.data(), nk * sizeof(double), hipMemcpyHostToDevice), "upload moduli");
+        HIP_TRY(hipMemcpy(w.twiddle_d, twd.data(), 2 * nk * sizeof(double), hipMemcpyHostToDevice), "upload tw");
+        HIP_TRY(hipMemcpy(w.twiddle_f, twf.data(), 2 * nk * sizeof(float), hipMemcpyHostToDevice), "upload tw");
+        w.pme_k[0] = K[0];
+        w.pme_k[1] = K[1];
+        w.pme_k[2] = K[2];
+    }
+    (void)box;
+    return MDGPU_OK;
+}
+
+int prepare(mdgpu_ctx* c, const double box[3]) {
+    const int n = c->topo.n_atoms;
+    const mdgpu_params& p = c->params;
+    for (int d = 0; d < 3; ++d)
+        if (!(box[d] > 0)) return set_error(c, MDGPU_ERR_ARG, "box lengths must be positive");
+    const double rc = std::max(p.lj_cutoff, p.ee_cutoff);
+    if (std::min({box[0], box[1], box[2]}) < 2.0 * rc)
+        return set_error(c, MDGPU_ERR_UNSUPPORTED, "every box length must be at least twice the cutoff");
+    mdg::Grid& g = c->grid;
+    g.box[0] = box[0];
+    g.box[1] = box[1];
+    g.box[2] = box[2];
+    const double dens = n / (box[0] * box[1] * box[2]);
+    const double s = std::cbrt(mdg::kCluster / std::max(dens, 1e-6));
+    g.ncx = std::max(1, static_cast<int>(box[0] / s));
+    g.ncy = std::max(1, static_cast<int>(box[1] / s));
+    g.sx = box[0] / g.ncx;
+    g.sy = box[1] / g.ncy;
+    g.dcx = static_cast<int>(std::floor(rc / g.sx)) + 1;
+    g.dcy = static_cast<int>(std::floor(rc / g.sy)) + 1;
+    const int ncol = g.ncx * g.ncy;
+    c->n_cols = ncol;
+    c->n_clusters = n / mdg::kCluster + ncol + 1;
+    Work& w = c->work;
+    size_t cap;
+    cap = w.cap_atoms;
+    if (w.cap_atoms < static_cast<size_t>(n)) {
+        void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_tmp, w.force_d, w.force_fx};
+        for (void* q : ptrs)
+            if (q) (void)hipFree(q);
+        w.coords = w.wrapped = w.force_d = nullptr;
+        w.col_of = w.col_tmp = nullptr;
+        w.force_fx = nullptr;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.coords), 3 * n * sizeof(double)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.wrapped), 3 * n * sizeof(double)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_of), n * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_tmp), n * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_d), 3 * n * sizeof(double)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_fx), 3 * n * sizeof(unsigned long long)), "alloc");
+        w.cap_atoms = n;
+    }
+    (void)cap;
+    if (w.cap_cols < static_cast<size_t>(ncol + 1)) {
+        void* ptrs[] = {w.col_count, w.col_fill, w.col_start, w.col_cstart};
+        for (void* q : ptrs)
+            if (q) (void)hipFree(q);
+        const size_t m = ncol + 1;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_count), m * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_fill), m * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_start), m * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_cstart), m * sizeof(int)), "alloc");
+        w.cap_cols = m;
+    }
+    const size_t ncl = c->n_clusters, nslots = ncl * mdg::kCluster;
+    if (w.cap_slots < nslots) {
+        void* ptrs[] = {w.slot_atom, w.slot_type, w.slot_pos_d, w.slot_pos_f, w.cluster_col, w.cl_origin, w.cl_half,
+                        w.list_count};
+        for (void* q : ptrs)
+            if (q) (void)hipFree(q);
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_atom), nslots * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_type), nslots * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_pos_d), nslots * sizeof(double4)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_pos_f), nslots * sizeof(float4)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.cluster_col), ncl * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.cl_origin), ncl * sizeof(double4)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.cl_half), ncl * sizeof(float4)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.list_count), ncl * sizeof(int)), "alloc");
+        w.cap_slots = nslots;
+        w.cap_list = 0;  // list arrays are sized by clusters x capacity
+    }
+    if (w.list_cap == 0) {
+        // half-list j-clusters per i-cluster: the clusters inside the cutoff
+        // sphere grown by one cluster width, halved, with headroom
+        const double width = std::cbrt(mdg::kCluster / std::max(dens, 1e-6));
+        const double r = rc + 2.0 * width;
+        const double est = 0.5 * (4.0 / 3.0) * 3.14159265 * r * r * r * dens / mdg::kCluster;
+        w.list_cap = std::max(64, static_cast<int>(1.5 * est) + 32);
+    }
+    const size_t nl = ncl * static_cast<size_t>(w.list_cap);
+    if (w.cap_list < nl) {
+        void* ptrs[] = {w.list_j, w.list_mask, w.list_delta_d, w.list_delta_f};
+        for (void* q : ptrs)
+            if (q) (void)hipFree(q);
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.list_j), nl * sizeof(int)), "alloc list");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.list_mask), nl * sizeof(unsigned long long)), "alloc list");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.list_delta_d), nl * sizeof(double4)), "alloc list");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.list_delta_f), nl * sizeof(float4)), "alloc list");
+        w.cap_list = nl;
+    }
+    int st = ensure_pme(c, box);
+    if (st != MDGPU_OK) return st;
+    const size_t rows = static_cast<size_t>(mdg::nonbonded_rows(c) + mdg::bonded_rows(c) +
+                                            mdg::corrections_rows(c) + mdg::pme_rows(c) + 1);
+    if (w.cap_partial < rows) {
+        if (w.partial) (void)hipFree(w.partial);
+        w.partial = nullptr;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.partial), rows * mdg::E_NSLOTS * sizeof(double)), "alloc");
+        w.cap_partial = rows;
+    }
+    return MDGPU_OK;
+}
+
+// Final reduction: one thread per energy slot sums the rows in order; the
+// remaining threads convert the fixed-point forces (mixed form) or copy the
+// FP64 forces to the output.
+__global__ void k_reduce(const double* __restrict__ partial, int rows, double* __restrict__ energies, int mixed,
+                         const unsigned long long* __restrict__ fx, const double* __restrict__ fd, double inv_scale,
+                         int n3, double* __restrict__ out, const int* __restrict__ list_count,
+                         const int* __restrict__ ncl_dev) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x < mdg::E_NSLOTS) {
+        double s = 0.0;
+        if (threadIdx.x == mdg::E_TILES) {
+            for (int k = 0; k < *ncl_dev; ++k) s += list_count[k];
+        } else {
+            for (int r = 0; r < rows; ++r) s += partial[static_cast<size_t>(r) * mdg::E_NSLOTS + threadIdx.x];
+        }
+        energies[threadIdx.x] = s;
+    }
+    for (int i = t; i < n3; i += gridDim.x * blockDim.x)
+        out[i] = mixed ? static_cast<double>(static_cast<long long>(fx[i])) * inv_scale : fd[i];
+}
+
+int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_energies* out_e, double* d_out,
+              double* out_virial, bool coords_are_work) {
+    if (!c->have_system || !c->have_params) return set_error(c, MDGPU_ERR_STATE, "upload a system and set params first");
+    HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
+    const mdgpu_params& p = c->params;
+    const bool mixed = p.precision == MDGPU_PREC_MIXED;
+    const bool amber = p.semantics == MDGPU_SEM_AMBER;
+    const int n = c->topo.n_atoms;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        int st = prepare(c, box);
+        if (st != MDGPU_OK) return st;
+        Work& w = c->work;
+        if (!coords_are_work)
+            HIP_TRY(hipMemcpyAsync(w.coords, d_coords, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
+                    "copy coords");
+        const double* x = w.coords;
+        auto& T = c->timing;
+        auto mark = [&](int k, bool start) {
+            if (T.on) (void)hipEventRecord(c->ev[2 * k + (start ? 0 : 1)], c->stream);
+        };
+        HIP_TRY(hipMemsetAsync(w.flags, 0, 4 * sizeof(unsigned int), c->stream), "clear flags");
+        HIP_TRY(hipMemsetAsync(w.partial, 0, w.cap_partial * mdg::E_NSLOTS * sizeof(double), c->stream), "clear");
+        if (mixed) HIP_TRY(hipMemsetAsync(w.force_fx, 0, 3 * n * sizeof(unsigned long long), c->stream), "clear");
+        else HIP_TRY(hipMemsetAsync(w.force_d, 0, 3 * n * sizeof(double), c->stream), "clear");
+        mark(MDGPU_K_SORT, true);
+        if (amber && c->topo.has_excl) HIP_TRY(mdg::launch_excl_extent(c), "excl extent");
+        HIP_TRY(mdg::launch_sort(c, x, mixed), "sort");
+        mark(MDGPU_K_SORT, false);
+        mark(MDGPU_K_PAIRLIST, true);
+        HIP_TRY(mdg::launch_pairsearch(c, mixed, amber), "pair search");
+        mark(MDGPU_K_PAIRLIST, false);
+        int row = 0;
+        mark(MDGPU_K_NONBONDED, true);
+        HIP_TRY(mdg::launch_nonbonded(c, mixed, amber, row), "nonbonded");
+        mark(MDGPU_K_NONBONDED, false);
+        row += mdg::nonbonded_rows(c);
+        mark(MDGPU_K_BONDED, true);
+        HIP_TRY(mdg::launch_bonded(c, x, mixed, amber, row), "bonded");
+        mark(MDGPU_K_BONDED, false);
+        row += mdg::bonded_rows(c);
+        if (amber) {
+            mark(MDGPU_K_CORRECT, true);
+            HIP_TRY(mdg::launch_corrections(c, mixed, row), "corrections");
+            mark(MDGPU_K_CORRECT, false);
+            row += mdg::corrections_rows(c);
+            int used = 0;
+            HIP_TRY(mdg::launch_pme(c, mixed, row, &used), "pme");
+            row += used;
+        }
+        mark(MDGPU_K_REDUCE, true);
+        hipLaunchKernelGGL(k_reduce, dim3(std::max(1, std::min(1024, (3 * n + 255) / 256))), dim3(256), 0, c->stream,
+                           w.partial, row, w.energies, mixed ? 1 : 0, w.force_fx, w.force_d,
+                           1.0 / mdg::fixed_scale(p.semantics), 3 * n, d_out ? d_out : w.force_d, w.list_count,
+                           w.col_cstart + c->n_cols);
+        HIP_TRY(hipGetLastError(), "reduce");
+        mark(MDGPU_K_REDUCE, false);
+        double e[mdg::E_NSLOTS];
+        unsigned int flags[4];
+        HIP_TRY(hipMemcpyAsync(e, w.energies, sizeof e, hipMemcpyDeviceToHost, c->stream), "read energies");
+        HIP_TRY(hipMemcpyAsync(flags, w.flags, sizeof flags, hipMemcpyDeviceToHost, c->stream), "read flags");
+        HIP_TRY(hipStreamSynchronize(c->stream), "evaluation");
+        if (flags[2]) return set_error(c, MDGPU_ERR_UNSUPPORTED, "a sort column holds more than 2048 atoms");
+        if (flags[3]) return set_error(c, MDGPU_ERR_UNSUPPORTED, "too many tiles near excluded pairs in one cluster");
+        if (flags[1]) {
+            // list capacity exceeded: grow and evaluate again
+            w.list_cap *= 2;
+            continue;
+        }
+        if (flags[0]) return set_error(c, MDGPU_ERR_OVERFLOW, "fixed-point force accumulator overflow");
+        if (T.on) {
+            for (int k = 0; k <= MDGPU_K_COUNT; ++k) {
+                if (k == MDGPU_K_SPREAD || k == MDGPU_K_FFT || k == MDGPU_K_CONV || k == MDGPU_K_GATHER ||
+                    k == MDGPU_K_COUNT) {
+                    if (!(amber && w.pme_k[0] > 0)) continue;
+                }
+                if (k == MDGPU_K_CORRECT && !amber) continue;
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]) == hipSuccess) {
+                    const int slot = k == MDGPU_K_COUNT ? MDGPU_K_FFT : k;
+                    T.total_ms[slot] += ms;
+                    if (k != MDGPU_K_COUNT) T.launches[slot] += 1;
+                }
+            }
+        }
+        c->counts[0] = static_cast<long long>(llround(e[mdg::E_PAIRS]));
+        c->counts[1] = static_cast<long long>(llround(e[mdg::E_TILES]));
+        c->counts[2] = c->n_clusters;
+        c->counts[3] = n;
+        if (out_e) {
+            mdgpu_energies& o = *out_e;
+            o.bond = e[mdg::E_BOND];
+            o.angle = e[mdg::E_ANGLE];
+            o.dihedral = e[mdg::E_DIHEDRAL];
+            o.urey_bradley = e[mdg::E_UB];
+            o.improper = e[mdg::E_IMPROPER];
+            o.cmap = e[mdg::E_CMAP];
+            o.vdw = e[mdg::E_VDW];
+            o.ee_all = amber ? 0.0 : e[mdg::E_EE_ALL];
+            o.ee_pairlist = amber ? 0.0 : e[mdg::E_EE_LIST];
+            o.ee_rf = amber ? 0.0 : e[mdg::E_EE_RF];
+            o.ewald_direct = e[mdg::E_EW_DIRECT];
+            o.ewald_self = -mdg::kCoulomb * (p.ewald_alpha / std::sqrt(mdg::kPi)) * c->topo.sum_q2;
+            o.ewald_recip = amber ? e[mdg::E_EW_RECIP] : 0.0;
+            o.ewald_excl = amber ? e[mdg::E_EW_EXCL] : 0.0;
+            o.vdw14 = amber ? e[mdg::E_VDW14] : 0.0;
+            o.ee14 = amber ? e[mdg::E_EE14] : 0.0;
+            const double bonded = o.bond + o.angle + o.dihedral + o.urey_bradley + o.improper + o.cmap;
+            o.total = amber ? bonded + o.vdw + o.vdw14 + o.ewald_direct + o.ewald_recip + o.ewald_self + o.ewald_excl +
+                                  o.ee14
+                            : bonded + o.vdw + o.ee_pairlist;
+        }
+        if (out_virial) {
+            const double xx = e[mdg::E_VIR_XX], yy = e[mdg::E_VIR_YY], zz = e[mdg::E_VIR_ZZ];
+            const double xy = e[mdg::E_VIR_XY], xz = e[mdg::E_VIR_XZ], yz = e[mdg::E_VIR_YZ];
+            const double v[9] = {xx, xy, xz, xy, yy, yz, xz, yz, zz};
+            std::memcpy(out_virial, v, sizeof v);
+        }
+        return MDGPU_OK;
+    }
+    return set_error(c, MDGPU_ERR_HIP, "pair list did not fit after regrowing");
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdgpu_eval(mdgpu_ctx* c, const double* coords, const double box[3], mdgpu_energies* out_e, double* out_forces,
+               double* out_virial) {
+    if (!c || !coords || !box) return set_error(c, MDGPU_ERR_ARG, "null argument");
+    if (!c->have_system) return set_error(c, MDGPU_ERR_STATE, "upload a system first");
+    HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
+    const int n = c->topo.n_atoms;
+    int st = prepare(c, box);
+    if (st != MDGPU_OK) return st;
+    HIP_TRY(hipMemcpyAsync(c->work.coords, coords, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream),
+            "upload coords");
+    st = eval_impl(c, nullptr, box, out_e, nullptr, out_virial, true);
+    if (st != MDGPU_OK) return st;
+    if (out_forces) {
+        HIP_TRY(hipMemcpy(out_forces, c->work.force_d, 3 * n * sizeof(double), hipMemcpyDeviceToHost), "read forces");
+    }
+    return MDGPU_OK;
+}
+
+int mdgpu_eval_device(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_energies* out_e,
+                      double* d_forces, double* out_virial) {
+    if (!c || !d_coords || !box) return set_error(c, MDGPU_ERR_ARG, "null argument");
+    return eval_impl(c, d_coords, box, out_e, d_forces, out_virial, false);
+}
+
+int mdgpu_set_timing(mdgpu_ctx* c, int enable) {
+    if (!c) return MDGPU_ERR_ARG;
+    c->timing = mdg::Timing{};
+    c->timing.on = enable != 0;
+    return MDGPU_OK;
+}
+
+int mdgpu_get_timing(mdgpu_ctx* c, int k, double* total_ms, int64_t* launches) {
+    if (!c || k < 0 || k >= MDGPU_K_COUNT) return MDGPU_ERR_ARG;
+    if (total_ms) *total_ms = c->timing.total_ms[k];
+    if (launches) *launches = c->timing.launches[k];
+    return MDGPU_OK;
+}
+
+int mdgpu_get_counts(mdgpu_ctx* c, int64_t counts[4]) {
+    if (!c || !counts) return MDGPU_ERR_ARG;
+    for (int k = 0; k < 4; ++k) counts[k] = c->counts[k];
+    return MDGPU_OK;
+}
+
+int mdgpu_set_counting(mdgpu_ctx* c, int enable) {
+    if (!c) return MDGPU_ERR_ARG;
+    c->counting = enable != 0;
+    return MDGPU_OK;
+}
+
+}  // extern "C"

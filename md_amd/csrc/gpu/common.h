@@ -1,0 +1,110 @@
+// Shared device definitions for the gfx950 kernels.
+#ifndef MD_AMD_GPU_COMMON_H
+#define MD_AMD_GPU_COMMON_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mdg {
+
+constexpr int kWave = 64;          // CDNA wavefront
+constexpr int kCluster = 8;        // atoms per cluster; a tile is 8 x 8 = one wave
+// 18.2206899283247^2, the reference's Coulomb constant (CoulombicEE.cpp:12)
+constexpr double kCoulomb = 331.99354146411776;
+constexpr double kPi = 3.14159265358979323846;
+// A padding slot of a cluster sits here; every real atom is far beyond the
+// cutoff from it, and it has zero charge and LJ parameters.
+constexpr double kFarAway = 1.0e5;
+
+// Energy accumulator slots of the partial-sum buffers.
+enum EnergySlot {
+    E_BOND = 0, E_ANGLE, E_DIHEDRAL, E_UB, E_IMPROPER, E_CMAP,
+    E_VDW, E_EE_ALL, E_EE_LIST, E_EE_RF, E_EW_DIRECT,
+    E_EW_RECIP, E_EW_EXCL, E_VDW14, E_EE14,
+    E_VIR_XX, E_VIR_YY, E_VIR_ZZ, E_VIR_XY, E_VIR_XZ, E_VIR_YZ,
+    E_PAIRS,   // number of atom pairs inside the cutoff (work count)
+    E_TILES,   // number of cluster-pair tiles (work count)
+    E_NSLOTS
+};
+
+// ---------------------------------------------------------------------------
+// 64-bit fixed-point force accumulation.  A contribution f is added as
+// llrint(f * 2^s); integer addition is associative, so the sum is independent
+// of the order in which atomics land (bit-reproducible).  A contribution that
+// does not fit sets the overflow flag instead.
+// ---------------------------------------------------------------------------
+struct FixedAcc {
+    unsigned long long* f;   // [3N]
+    double scale;            // 2^s
+    unsigned int* overflow;
+};
+
+__device__ __forceinline__ long long to_fixed(double v, double scale, unsigned int* overflow) {
+    const double s = v * scale;
+    if (!(fabs(s) < 9.0e18)) {   // also catches NaN
+        atomicOr(overflow, 1u);
+        return 0;
+    }
+    return __double2ll_rn(s);
+}
+
+__device__ __forceinline__ void acc_add(const FixedAcc& a, int idx, double v) {
+    const long long q = to_fixed(v, a.scale, a.overflow);
+    if (q != 0) atomicAdd(a.f + idx, static_cast<unsigned long long>(q));
+}
+
+__device__ __forceinline__ void acc_add(double* f, int idx, double v) {
+    if (v != 0.0) atomicAdd(f + idx, v);
+}
+
+// ---------------------------------------------------------------------------
+// wave-level helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T shfl_xor(T v, int mask) {
+    return __shfl_xor(v, mask, kWave);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+    return v;
+}
+
+// Block-wide sum of several doubles into partial[slot] (one writer per
+// block); blockDim.x must be a multiple of 64 and <= 1024.
+template <int NV>
+__device__ __forceinline__ void block_sum_store(double (&v)[NV], double* partial_row, const int* slots) {
+    __shared__ double red[16][NV];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) red[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0.0;
+            for (int q = 0; q < nw; ++q) s += red[q][k];
+            partial_row[slots[k]] = s;
+        }
+    }
+}
+
+// XCD-aware block order: consecutive logical blocks land on one XCD (the
+// hardware deals workgroups round-robin over the 8 XCDs), so neighbouring
+// clusters share that XCD's L2.
+// The map is a bijection of [0, nblocks): XCD x receives full + (x < rem)
+// blocks, placed after the blocks of XCDs 0..x-1.
+__device__ __forceinline__ int xcd_block(int b, int nblocks) {
+    constexpr int kXcd = 8;
+    const int full = nblocks / kXcd, rem = nblocks % kXcd;
+    const int xcd = b % kXcd, k = b / kXcd;
+    return xcd * full + (xcd < rem ? xcd : rem) + k;
+}
+
+}  // namespace mdg
+
+#endif

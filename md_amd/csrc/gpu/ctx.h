@@ -1,0 +1,163 @@
+// Device-side state of one mdgpu context and the kernel launchers.
+#ifndef MD_AMD_GPU_CTX_H
+#define MD_AMD_GPU_CTX_H
+
+#include "common.h"
+#include "mdgpu.h"
+
+#include <string>
+#include <vector>
+
+namespace mdg {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+};
+
+// Static topology arrays on the device (uploaded once per system).
+struct Topo {
+    int n_atoms = 0, n_types = 0;          // n_types excludes the padding type
+    double* charge = nullptr;              // [N]
+    int* type = nullptr;                   // [N]
+    // pair table over (T+1)^2 types, the last type being the padding type:
+    // x = A, y = B, z = U(rc), w = G(rc) (the force-shift constants)
+    double4* lj_d = nullptr;
+    float4* lj_f = nullptr;
+    unsigned char* lj_listed = nullptr;    // (T+1)^2
+    int* nb_index = nullptr;               // [T*T] 1-based LJ table index (0 = none)
+    double2* lj14 = nullptr;               // [n_lj] (A14, B14)
+    // bonded terms: per-term atoms and per-type parameters
+    int n_bonds = 0, n_ubs = 0, n_angles = 0, n_dihedrals = 0, n_impropers = 0, n_cmaps = 0;
+    int2* bond_atoms = nullptr;  int* bond_type = nullptr;  double2* bond_par = nullptr;   // (k, r0)
+    int2* ub_atoms = nullptr;    int* ub_type = nullptr;    double2* ub_par = nullptr;
+    int4* angle_atoms = nullptr; int* angle_type = nullptr; double2* angle_par = nullptr; // (k, theta0)
+    int4* dih_atoms = nullptr;   int* dih_type = nullptr;   double4* dih_par = nullptr;   // (k, n, phase)
+    int4* imp_atoms = nullptr;   int* imp_type = nullptr;   double2* imp_par = nullptr;   // (k, psi0)
+    int* cmap_atoms = nullptr;   int* cmap_set = nullptr;                                // [5n], [n]
+    double* cmap_coeff = nullptr; float* cmap_coeff_f = nullptr;
+    int cmap_res = 0, cmap_n_sets = 0;
+    // exclusions
+    int* excl_start = nullptr;  int* excl_list = nullptr; int has_excl = 0;
+    int n_excl_pairs = 0; int2* excl_pairs = nullptr;
+    int n_pairs14 = 0; int2* pairs14 = nullptr; double* scee14 = nullptr;
+    double sum_q2 = 0.0;                   // for the self energy
+};
+
+// Work arrays that depend on the atom count / box.
+struct Work {
+    size_t cap_atoms = 0;
+    double* coords = nullptr;        // [3N] input (owned copy for host-pointer evals)
+    double* wrapped = nullptr;       // [3N] wrapped into the box
+    int* col_of = nullptr;           // [N] column of each atom
+    int* col_count = nullptr;        // [ncol + 1]
+    int* col_fill = nullptr;         // [ncol]
+    int* col_start = nullptr;        // [ncol + 1] atom offsets
+    int* col_cstart = nullptr;       // [ncol + 1] cluster offsets
+    int* col_tmp = nullptr;          // [N] atoms bucketed by column
+    size_t cap_cols = 0;
+    size_t cap_slots = 0;            // clusters * 8
+    int* slot_atom = nullptr;        // [slots] original atom or -1
+    int* slot_type = nullptr;        // [slots]
+    double4* slot_pos_d = nullptr;   // [slots] position relative to the cluster origin + charge
+    float4* slot_pos_f = nullptr;
+    int* cluster_col = nullptr;      // [clusters]
+    double4* cl_origin = nullptr;    // [clusters] bbox centre (w unused)
+    float4* cl_half = nullptr;       // [clusters] bbox half extents
+    // pair list (capacity per i-cluster)
+    int list_cap = 0;
+    size_t cap_list = 0;
+    int* list_count = nullptr;       // [clusters]
+    int* list_j = nullptr;           // [clusters * cap]
+    unsigned long long* list_mask = nullptr;
+    double4* list_delta_d = nullptr;
+    float4* list_delta_f = nullptr;
+    // forces
+    double* force_d = nullptr;                 // [3N]
+    unsigned long long* force_fx = nullptr;    // [3N] fixed point
+    // energies: partial rows, one per block of each producing kernel
+    size_t cap_partial = 0;
+    double* partial = nullptr;       // [rows * E_NSLOTS]
+    double* energies = nullptr;      // [E_NSLOTS] final
+    unsigned int* flags = nullptr;   // [4]: 0 fixed overflow, 1 list overflow, 2 column overflow, 3 spare
+    float* excl_rmax = nullptr;      // [1] largest excluded-pair distance (as float)
+    // PME
+    int pme_k[3] = {0, 0, 0};
+    size_t cap_grid = 0;
+    double* grid_d = nullptr;                   // [K^3] charge grid (FP64 form) / potential
+    unsigned long long* grid_fx = nullptr;      // [K^3] fixed-point charge grid (mixed form)
+    float* grid_f = nullptr;                    // [K^3] potential (mixed form)
+    double* grid_phi_d = nullptr;               // [K^3] potential (FP64 form)
+    void* cgrid = nullptr;                      // [Kx*Ky*(Kz/2+1)] complex (double2 / float2)
+    double* bsp_mod = nullptr;                  // [Kx + Ky + Kz] B-spline moduli 1/|b|^2
+    float* bsp_mod_f = nullptr;
+    double* twiddle_d = nullptr;                // [Kx + Ky + Kz] complex, e^{-2 pi i m/K}
+    float* twiddle_f = nullptr;
+};
+
+// Host-side per-evaluation geometry of the column grid.
+struct Grid {
+    int ncx = 1, ncy = 1;
+    double sx = 0, sy = 0;
+    double box[3] = {0, 0, 0};
+    int dcx = 0, dcy = 0;            // column search reach
+};
+
+struct Timing {
+    bool on = false;
+    double total_ms[MDGPU_K_COUNT] = {};
+    long long launches[MDGPU_K_COUNT] = {};
+};
+
+}  // namespace mdg
+
+struct mdgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    bool have_system = false;
+    bool have_params = false;
+    mdgpu_params params{};
+    mdg::Topo topo;
+    mdg::Work work;
+    mdg::Grid grid;
+    mdg::Timing timing;
+    bool counting = false;
+    long long counts[4] = {0, 0, 0, 0};
+    int n_clusters = 0;
+    int n_cols = 0;
+    int split = 4;                  // waves per i-cluster list in the pair kernel
+    std::vector<void*> topo_allocs;
+    // host copies of the per-type-pair LJ parameters (the device table also
+    // holds cutoff-dependent shift constants, rebuilt by mdgpu_set_params)
+    std::vector<double> host_lj_a, host_lj_b;
+    std::vector<unsigned char> host_listed;
+    // event pairs per kernel group; the extra pair times the inverse FFT passes
+    hipEvent_t ev[2 * (MDGPU_K_COUNT + 1)] = {};
+};
+
+namespace mdg {
+
+// ---- launchers (each returns hipSuccess or the first HIP error) ---------
+hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed);
+hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber);
+hipError_t launch_nonbonded(mdgpu_ctx* c, bool mixed, bool amber, int partial_row0);
+hipError_t launch_bonded(mdgpu_ctx* c, const double* coords, bool mixed, bool amber, int partial_row0);
+hipError_t launch_corrections(mdgpu_ctx* c, bool mixed, int partial_row0);
+hipError_t launch_excl_extent(mdgpu_ctx* c);
+hipError_t launch_pme(mdgpu_ctx* c, bool mixed, int partial_row0, int* rows_used);
+hipError_t launch_reduce(mdgpu_ctx* c, bool mixed, int n_rows, double* out_forces);
+
+// rows of the partial-energy buffer each kernel needs for a given launch
+int nonbonded_rows(const mdgpu_ctx* c);
+int bonded_rows(const mdgpu_ctx* c);
+int corrections_rows(const mdgpu_ctx* c);
+int pme_rows(const mdgpu_ctx* c);
+
+// fixed-point scale exponent of the mixed form for a semantics
+inline double fixed_scale(int semantics) { return semantics == MDGPU_SEM_REF ? 1048576.0 : 4294967296.0; }
+
+}  // namespace mdg
+
+#endif

@@ -1,0 +1,305 @@
+// Nonbonded cluster-pair kernel: LJ 12-6 (switched or force-shifted) and the
+// Coulomb forms, one 8 x 8 tile per wavefront step.
+//
+// Replaces the six pair-list passes and the O(N^2) pass of the reference:
+// calculate_LJ_energy_pairlist (NonBondedEnergy.cpp:79-131),
+// calculate_EE_energy (:133-174), calculate_EE_energy_pairlist (:176-204),
+// calculate_EE_energy_pairlist_with_cutoff (:206-234),
+// calculate_EE_ewald_direct_term (:236-268), calculate_forces_LJ_pairlist
+// (NonBondedForces.cpp:92-154) and calculate_forces_EE_pairlist (:207-242).
+//
+// Lane l of a wave computes the pair (i = 8c_i + l/8, j = 8c_j + l%8).  The
+// i-forces stay in registers over the wave's list entries; the j-forces of a
+// tile are summed over the 8 lanes sharing j with xor-shuffles and then added
+// by 24 lanes at once (one component each).  Forces go to 64-bit fixed point
+// (mixed form) or FP64 atomics (FP64 form); energies to per-block FP64 rows.
+//
+// REF semantics: every pair within the cutoff, LJ energy U*S and force
+// (S G - U dS/dr / r) d, energies of the plain, reaction-field and erfc
+// Coulomb forms, plain Coulomb force k qq d / r^3.
+// AMBER semantics: masked tiles (exclusions and 1-4 pairs removed), erfc
+// Ewald real-space energy and force.
+#include "ctx.h"
+
+namespace mdg {
+
+namespace {
+
+template <typename real>
+struct V4;
+template <>
+struct V4<float> { using T = float4; };
+template <>
+struct V4<double> { using T = double4; };
+
+constexpr int kWaves = 4;  // waves per block
+
+template <typename real>
+struct NbArgs {
+    const int* ncl_dev;
+    int max_clusters;
+    int split;
+    int cap;
+    const int* list_count;
+    const int* list_j;
+    const unsigned long long* list_mask;
+    const typename V4<real>::T* list_delta;
+    const typename V4<real>::T* pos;
+    const int* slot_type;
+    const int* slot_atom;
+    const typename V4<real>::T* lj;    // (T+1)^2 x (A, B, U(rc), G(rc))
+    const unsigned char* listed;       // (T+1)^2, 1 if the type pair has an LJ entry
+    int ntp;                           // T + 1
+    real rc2_lj, rc2_ee, rc_lj, rc_ee, rs, inv_sw;
+    real alpha, kc, krf_crf;           // krf_crf: 1/rc_ee (reaction field with eps = 1)
+    double* force_d;
+    FixedAcc fx;
+    double* partial;
+    int row0;
+    int count_pairs;
+};
+
+// erfc(x) for x >= 0 from e = exp(-x^2): Abramowitz & Stegun 7.1.26
+// (|error| <= 1.5e-7), t = 1/(1 + p x).
+__device__ __forceinline__ float erfc_as(float x, float e) {
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, x, 1.0f));
+    float p = 1.061405429f;
+    p = fmaf(p, t, -1.453152027f);
+    p = fmaf(p, t, 1.421413741f);
+    p = fmaf(p, t, -0.284496736f);
+    p = fmaf(p, t, 0.254829592f);
+    return p * t * e;
+}
+
+template <typename real>
+struct Math;
+template <>
+struct Math<float> {
+    static __device__ __forceinline__ float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ void erfc_exp(float x, float& ec, float& ex) {
+        ex = __expf(-x * x);
+        ec = erfc_as(x, ex);
+    }
+};
+template <>
+struct Math<double> {
+    static __device__ __forceinline__ double rsqrt(double x) { return 1.0 / sqrt(x); }
+    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+    static __device__ __forceinline__ void erfc_exp(double x, double& ec, double& ex) {
+        ex = exp(-x * x);
+        ec = erfc(x);
+    }
+};
+
+template <typename real>
+__device__ __forceinline__ void add_force(const NbArgs<real>& a, int idx, real v);
+template <>
+__device__ __forceinline__ void add_force<float>(const NbArgs<float>& a, int idx, float v) {
+    acc_add(a.fx, idx, static_cast<double>(v));
+}
+template <>
+__device__ __forceinline__ void add_force<double>(const NbArgs<double>& a, int idx, double v) {
+    acc_add(a.force_d, idx, v);
+}
+
+template <typename real, bool kAmber, bool kSwitch>
+__global__ __launch_bounds__(kWaves * kWave) void k_nonbonded(const NbArgs<real> a) {
+    using R4 = typename V4<real>::T;
+    using M = Math<real>;
+    extern __shared__ unsigned char smem[];
+    R4* s_lj = reinterpret_cast<R4*>(smem);
+    unsigned char* s_listed = smem + sizeof(R4) * a.ntp * a.ntp;
+    const int ntp2 = a.ntp * a.ntp;
+    for (int k = threadIdx.x; k < ntp2; k += blockDim.x) {
+        s_lj[k] = a.lj[k];
+        if (!kAmber) s_listed[k] = a.listed[k];
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ti = lane >> 3, tj = lane & 7;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int gw = blk * kWaves + wv;
+    const int ci = gw / a.split, part = gw % a.split;
+    const int ncl = min(*a.ncl_dev, a.max_clusters);
+    int k0 = 0, k1 = 0;
+    if (ci < ncl) {
+        const int cnt = a.list_count[ci];
+        k0 = static_cast<int>((static_cast<long long>(cnt) * part) / a.split);
+        k1 = static_cast<int>((static_cast<long long>(cnt) * (part + 1)) / a.split);
+    }
+    const int cis = ci < ncl ? ci : 0;
+    const R4 pi = a.pos[cis * kCluster + ti];
+    const int tyi = a.slot_type[cis * kCluster + ti] * a.ntp;
+    const real qi = pi.w * a.kc;
+
+    real fix = 0, fiy = 0, fiz = 0;
+    // energy accumulators: FP64 form in double; mixed form in float, flushed to
+    // double every 32 tiles
+    double e_vdw = 0, e_all = 0, e_list = 0, e_rf = 0, e_ew = 0, n_pairs = 0;
+    real s_vdw = 0, s_all = 0, s_list = 0, s_rf = 0, s_ew = 0;
+    const real tsp = static_cast<real>(1.1283791670955126) * a.alpha;  // 2 alpha / sqrt(pi)
+    const size_t base = static_cast<size_t>(cis) * a.cap;
+    for (int k = k0; k < k1; ++k) {
+        const int cj = a.list_j[base + k];
+        const unsigned long long mask = a.list_mask[base + k];
+        const R4 dl = a.list_delta[base + k];
+        const int sj = cj * kCluster + tj;
+        const R4 pj = a.pos[sj];
+        const real dx = pi.x - pj.x + dl.x;
+        const real dy = pi.y - pj.y + dl.y;
+        const real dz = pi.z - pj.z + dl.z;
+        const real r2 = dx * dx + dy * dy + dz * dz;
+        const bool in_mask = (mask >> lane) & 1ull;
+        const real rc2 = a.rc2_lj > a.rc2_ee ? a.rc2_lj : a.rc2_ee;
+        const bool act = in_mask && r2 <= rc2 && r2 >= static_cast<real>(1e-12);
+        real f = 0;
+        if (act) {
+            const real rinv = M::rsqrt(r2);
+            const real r = r2 * rinv;
+            const real r2inv = rinv * rinv;
+            const int tyj = a.slot_type[sj];
+            const int tp = tyi + tyj;
+            const R4 lj = s_lj[tp];
+            // work count: real (non-padding) pairs inside the cutoff
+            if (a.count_pairs && tyj != a.ntp - 1 && tyi != (a.ntp - 1) * a.ntp) n_pairs += 1.0;
+            if (r2 <= a.rc2_lj) {
+                const real r6 = r2inv * r2inv * r2inv;
+                const real ar6 = lj.x * r6;
+                const real u = (ar6 - lj.y) * r6;                       // A/r^12 - B/r^6
+                const real g = (static_cast<real>(12) * ar6 - static_cast<real>(6) * lj.y) * r6 * r2inv;  // -U'/r
+                real el, fl;
+                if (kSwitch) {
+                    real xs = (r - a.rs) * a.inv_sw;
+                    xs = xs < 0 ? static_cast<real>(0) : xs;
+                    const real x2 = xs * xs;
+                    const real sw = 1 + x2 * xs * (static_cast<real>(-10) + xs * (static_cast<real>(15) - 6 * xs));
+                    const real dsw = x2 * (static_cast<real>(-30) + xs * (static_cast<real>(60) - 30 * xs)) * a.inv_sw;
+                    el = u * sw;
+                    fl = sw * g - u * dsw * rinv;
+                } else {
+                    el = u - lj.z + (r - a.rc_lj) * (lj.w * a.rc_lj);
+                    fl = g - lj.w * a.rc_lj * rinv;
+                }
+                if (!kAmber && !s_listed[tp]) { el = 0; fl = 0; }
+                s_vdw += el;
+                f += fl;
+            }
+            if (r2 <= a.rc2_ee) {
+                const real qq = qi * pj.w;
+                real ec, ex;
+                M::erfc_exp(a.alpha * r, ec, ex);
+                const real e_c = qq * rinv;
+                if (kAmber) {
+                    s_ew += e_c * ec;
+                    f += qq * (ec * rinv + tsp * ex) * r2inv;
+                } else {
+                    const real lst = s_listed[tp] ? static_cast<real>(1) : static_cast<real>(0);
+                    s_all += e_c;
+                    s_list += lst * e_c;
+                    s_rf += lst * qq * (rinv - a.krf_crf);
+                    s_ew += lst * e_c * ec;
+                    f += lst * e_c * r2inv;
+                }
+            }
+        }
+        const real fx = f * dx, fy = f * dy, fz = f * dz;
+        fix += fx;
+        fiy += fy;
+        fiz += fz;
+        if (__ballot(act) != 0ull) {
+            real gx = fx, gy = fy, gz = fz;
+            gx += shfl_xor(gx, 8);  gy += shfl_xor(gy, 8);  gz += shfl_xor(gz, 8);
+            gx += shfl_xor(gx, 16); gy += shfl_xor(gy, 16); gz += shfl_xor(gz, 16);
+            gx += shfl_xor(gx, 32); gy += shfl_xor(gy, 32); gz += shfl_xor(gz, 32);
+            if (ti < 3) {
+                const int aj = a.slot_atom[sj];
+                const real v = ti == 0 ? gx : (ti == 1 ? gy : gz);
+                if (aj >= 0) add_force<real>(a, 3 * aj + ti, -v);
+            }
+        }
+        if (((k - k0) & 31) == 31) {
+            e_vdw += s_vdw; e_all += s_all; e_list += s_list; e_rf += s_rf; e_ew += s_ew;
+            s_vdw = s_all = s_list = s_rf = s_ew = 0;
+        }
+    }
+    e_vdw += s_vdw; e_all += s_all; e_list += s_list; e_rf += s_rf; e_ew += s_ew;
+    // i-forces: sum over the 8 lanes sharing i
+    fix += shfl_xor(fix, 1); fiy += shfl_xor(fiy, 1); fiz += shfl_xor(fiz, 1);
+    fix += shfl_xor(fix, 2); fiy += shfl_xor(fiy, 2); fiz += shfl_xor(fiz, 2);
+    fix += shfl_xor(fix, 4); fiy += shfl_xor(fiy, 4); fiz += shfl_xor(fiz, 4);
+    if (ci < ncl && tj < 3 && k1 > k0) {
+        const int ai = a.slot_atom[ci * kCluster + ti];
+        const real v = tj == 0 ? fix : (tj == 1 ? fiy : fiz);
+        if (ai >= 0) add_force<real>(a, 3 * ai + tj, v);
+    }
+    double v[6] = {e_vdw, e_all, e_list, e_rf, e_ew, n_pairs};
+    const int slots[6] = {E_VDW, E_EE_ALL, E_EE_LIST, E_EE_RF, E_EW_DIRECT, E_PAIRS};
+    block_sum_store<6>(v, a.partial + static_cast<size_t>(a.row0 + blockIdx.x) * E_NSLOTS, slots);
+}
+
+template <typename real>
+hipError_t run(mdgpu_ctx* c, bool amber, int row0) {
+    Work& w = c->work;
+    const Topo& t = c->topo;
+    const mdgpu_params& p = c->params;
+    NbArgs<real> a{};
+    const int ncol = c->grid.ncx * c->grid.ncy;
+    a.ncl_dev = w.col_cstart + ncol;
+    a.max_clusters = c->n_clusters;
+    a.split = c->split;
+    a.cap = w.list_cap;
+    a.list_count = w.list_count;
+    a.list_j = w.list_j;
+    a.list_mask = w.list_mask;
+    a.ntp = t.n_types + 1;
+    if constexpr (sizeof(real) == 4) {
+        a.list_delta = w.list_delta_f;
+        a.pos = w.slot_pos_f;
+        a.lj = t.lj_f;
+    } else {
+        a.list_delta = w.list_delta_d;
+        a.pos = w.slot_pos_d;
+        a.lj = t.lj_d;
+    }
+    a.listed = reinterpret_cast<const unsigned char*>(t.lj_listed);
+    a.slot_type = w.slot_type;
+    a.slot_atom = w.slot_atom;
+    a.rc2_lj = static_cast<real>(p.lj_cutoff * p.lj_cutoff);
+    a.rc2_ee = static_cast<real>(p.ee_cutoff * p.ee_cutoff);
+    a.rc_lj = static_cast<real>(p.lj_cutoff);
+    a.rc_ee = static_cast<real>(p.ee_cutoff);
+    const bool sw = p.lj_switch > 0.0;
+    a.rs = static_cast<real>(sw ? p.lj_switch : 0.0);
+    a.inv_sw = static_cast<real>(sw ? 1.0 / (p.lj_cutoff - p.lj_switch) : 0.0);
+    a.alpha = static_cast<real>(p.ewald_alpha);
+    a.kc = static_cast<real>(kCoulomb);
+    a.krf_crf = static_cast<real>(1.0 / p.ee_cutoff);
+    a.force_d = w.force_d;
+    a.fx = FixedAcc{w.force_fx, fixed_scale(p.semantics), w.flags};
+    a.partial = w.partial;
+    a.row0 = row0;
+    a.count_pairs = c->counting ? 1 : 0;
+    const int nblk = nonbonded_rows(c);
+    const size_t shm = (sizeof(typename V4<real>::T) + 1) * a.ntp * a.ntp;
+    dim3 grid(nblk), block(kWaves * kWave);
+    if (amber) {
+        if (sw) hipLaunchKernelGGL((k_nonbonded<real, true, true>), grid, block, shm, c->stream, a);
+        else hipLaunchKernelGGL((k_nonbonded<real, true, false>), grid, block, shm, c->stream, a);
+    } else {
+        if (sw) hipLaunchKernelGGL((k_nonbonded<real, false, true>), grid, block, shm, c->stream, a);
+        else hipLaunchKernelGGL((k_nonbonded<real, false, false>), grid, block, shm, c->stream, a);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int nonbonded_rows(const mdgpu_ctx* c) { return (c->n_clusters * c->split + kWaves - 1) / kWaves; }
+
+hipError_t launch_nonbonded(mdgpu_ctx* c, bool mixed, bool amber, int row0) {
+    return mixed ? run<float>(c, amber, row0) : run<double>(c, amber, row0);
+}
+
+}  // namespace mdg

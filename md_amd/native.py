@@ -1,0 +1,354 @@
+"""ctypes binding of the two native libraries built in-tree.
+
+* ``libmdhost.so`` - the C++ prmtop/rst7 reader (reference semantics) exposed
+  through include/mdhost.h;
+* ``libmdgpu.so``  - the HIP kernels behind the mdgpu C ABI (include/mdgpu.h).
+
+No computation happens in Python.  A missing library raises ImportError: there
+is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+SEM_REF, SEM_AMBER = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+
+ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "call out of order",
+          4: "fixed-point force overflow", 5: "unsupported parameters"}
+
+KERNELS = ["sort", "pairlist", "nonbonded", "bonded", "corrections", "spread", "fft", "conv",
+           "gather", "reduce"]
+
+P_I32 = C.POINTER(C.c_int32)
+P_F64 = C.POINTER(C.c_double)
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [
+        ("n_atoms", C.c_int32), ("n_types", C.c_int32),
+        ("charge", P_F64), ("type", P_I32), ("nb_index", P_I32),
+        ("n_lj", C.c_int32), ("lj_a", P_F64), ("lj_b", P_F64), ("lj14_a", P_F64), ("lj14_b", P_F64),
+        ("n_bonds", C.c_int32), ("bond_atoms", P_I32), ("bond_type", P_I32), ("n_bond_types", C.c_int32),
+        ("bond_k", P_F64), ("bond_r0", P_F64),
+        ("n_ubs", C.c_int32), ("ub_atoms", P_I32), ("ub_type", P_I32), ("n_ub_types", C.c_int32),
+        ("ub_k", P_F64), ("ub_r0", P_F64),
+        ("n_angles", C.c_int32), ("angle_atoms", P_I32), ("angle_type", P_I32), ("n_angle_types", C.c_int32),
+        ("angle_k", P_F64), ("angle_theta0", P_F64),
+        ("n_dihedrals", C.c_int32), ("dihedral_atoms", P_I32), ("dihedral_type", P_I32),
+        ("n_dihedral_types", C.c_int32), ("dihedral_k", P_F64), ("dihedral_n", P_F64),
+        ("dihedral_phase", P_F64),
+        ("n_impropers", C.c_int32), ("improper_atoms", P_I32), ("improper_type", P_I32),
+        ("n_improper_types", C.c_int32), ("improper_k", P_F64), ("improper_psi0", P_F64),
+        ("n_cmaps", C.c_int32), ("cmap_atoms", P_I32), ("cmap_set", P_I32), ("cmap_n_sets", C.c_int32),
+        ("cmap_resolution", C.c_int32), ("cmap_coeff", P_F64), ("cmap_grid", P_F64),
+        ("excl_start", P_I32), ("excl_list", P_I32),
+        ("n_excl_pairs", C.c_int32), ("excl_pairs", P_I32),
+        ("n_pairs14", C.c_int32), ("pairs14", P_I32), ("scee14", P_F64),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("semantics", C.c_int32), ("precision", C.c_int32),
+        ("lj_cutoff", C.c_double), ("ee_cutoff", C.c_double), ("lj_switch", C.c_double),
+        ("ewald_alpha", C.c_double), ("pme_grid", C.c_int32 * 3), ("pme_order", C.c_int32),
+    ]
+
+
+ENERGY_TERMS = ["bond", "angle", "dihedral", "urey_bradley", "improper", "cmap", "vdw", "ee_all",
+                "ee_pairlist", "ee_rf", "ewald_direct", "ewald_self", "ewald_recip", "ewald_excl",
+                "vdw14", "ee14", "total"]
+
+
+class Energies(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ENERGY_TERMS]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n in ENERGY_TERMS}
+
+
+def _load(name: str) -> C.CDLL:
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built; run __graft_entry__.build() (make -C md_amd/csrc)")
+    return C.CDLL(path)
+
+
+_gpu = None
+_host = None
+
+
+def gpu_lib() -> C.CDLL:
+    global _gpu
+    if _gpu is None:
+        lib = _load("libmdgpu.so")
+        lib.mdgpu_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+        lib.mdgpu_destroy.argtypes = [C.c_void_p]
+        lib.mdgpu_last_error.argtypes = [C.c_void_p]
+        lib.mdgpu_last_error.restype = C.c_char_p
+        lib.mdgpu_upload_system.argtypes = [C.c_void_p, C.POINTER(SystemDesc)]
+        lib.mdgpu_set_params.argtypes = [C.c_void_p, C.POINTER(Params)]
+        lib.mdgpu_eval.argtypes = [C.c_void_p, P_F64, P_F64, C.POINTER(Energies), P_F64, P_F64]
+        lib.mdgpu_eval_device.argtypes = [C.c_void_p, C.c_void_p, P_F64, C.POINTER(Energies), C.c_void_p,
+                                          P_F64]
+        lib.mdgpu_set_timing.argtypes = [C.c_void_p, C.c_int]
+        lib.mdgpu_get_timing.argtypes = [C.c_void_p, C.c_int, P_F64, C.POINTER(C.c_int64)]
+        lib.mdgpu_get_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        lib.mdgpu_set_counting.argtypes = [C.c_void_p, C.c_int]
+        _gpu = lib
+    return _gpu
+
+
+def host_lib() -> C.CDLL:
+    global _host
+    if _host is None:
+        gpu_lib()  # libmdhost.so links it
+        lib = _load("libmdhost.so")
+        lib.mdhost_load.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]
+        lib.mdhost_desc.argtypes = [C.c_void_p, C.POINTER(SystemDesc)]
+        lib.mdhost_coords.argtypes = [C.c_void_p]
+        lib.mdhost_coords.restype = P_F64
+        lib.mdhost_masses.argtypes = [C.c_void_p]
+        lib.mdhost_masses.restype = P_F64
+        lib.mdhost_box.argtypes = [C.c_void_p, P_F64]
+        lib.mdhost_dihedral_flags.argtypes = [C.c_void_p]
+        lib.mdhost_dihedral_flags.restype = P_I32
+        lib.mdhost_free.argtypes = [C.c_void_p]
+        lib.mdhost_last_error.restype = C.c_char_p
+        _host = lib
+    return _host
+
+
+def _arr(ptr, n, dtype):
+    if n == 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+class Topology:
+    """A parsed prmtop/rst7: host handle plus numpy copies of the arrays.
+
+    ``semantics`` is SEM_REF (the reference's term lists: last entries dropped,
+    no exclusions) or SEM_AMBER (complete lists and exclusions).
+    """
+
+    def __init__(self, prmtop: str, rst7: str | None = None, semantics: int = SEM_REF):
+        lib = host_lib()
+        h = C.c_void_p()
+        st = lib.mdhost_load(prmtop.encode(), (rst7 or "").encode(), semantics, C.byref(h))
+        if st != 0:
+            raise RuntimeError(f"mdhost_load: {lib.mdhost_last_error().decode()}")
+        self._h = h
+        self.semantics = semantics
+        d = SystemDesc()
+        lib.mdhost_desc(h, C.byref(d))
+        self._desc = d
+        n = d.n_atoms
+        self.n_atoms = n
+        self.coords = _arr(lib.mdhost_coords(h), 3 * n, np.float64).reshape(n, 3)
+        self.masses = _arr(lib.mdhost_masses(h), n, np.float64)
+        box = (C.c_double * 3)()
+        lib.mdhost_box(h, box)
+        self.box = np.array(box[:], dtype=np.float64)
+        self.dihedral_flags = _arr(lib.mdhost_dihedral_flags(h), d.n_dihedrals, np.int32)
+        self.arrays = self._copy_arrays(d)
+
+    @staticmethod
+    def _copy_arrays(d: SystemDesc) -> dict:
+        n, t = d.n_atoms, d.n_types
+        a = {
+            "charge": _arr(d.charge, n, np.float64), "type": _arr(d.type, n, np.int32),
+            "nb_index": _arr(d.nb_index, t * t, np.int32),
+            "lj_a": _arr(d.lj_a, d.n_lj, np.float64), "lj_b": _arr(d.lj_b, d.n_lj, np.float64),
+            "lj14_a": _arr(d.lj14_a, d.n_lj, np.float64), "lj14_b": _arr(d.lj14_b, d.n_lj, np.float64),
+            "bond_atoms": _arr(d.bond_atoms, 2 * d.n_bonds, np.int32),
+            "bond_type": _arr(d.bond_type, d.n_bonds, np.int32),
+            "bond_k": _arr(d.bond_k, d.n_bond_types, np.float64),
+            "bond_r0": _arr(d.bond_r0, d.n_bond_types, np.float64),
+            "ub_atoms": _arr(d.ub_atoms, 2 * d.n_ubs, np.int32), "ub_type": _arr(d.ub_type, d.n_ubs, np.int32),
+            "ub_k": _arr(d.ub_k, d.n_ub_types, np.float64), "ub_r0": _arr(d.ub_r0, d.n_ub_types, np.float64),
+            "angle_atoms": _arr(d.angle_atoms, 3 * d.n_angles, np.int32),
+            "angle_type": _arr(d.angle_type, d.n_angles, np.int32),
+            "angle_k": _arr(d.angle_k, d.n_angle_types, np.float64),
+            "angle_theta0": _arr(d.angle_theta0, d.n_angle_types, np.float64),
+            "dihedral_atoms": _arr(d.dihedral_atoms, 4 * d.n_dihedrals, np.int32),
+            "dihedral_type": _arr(d.dihedral_type, d.n_dihedrals, np.int32),
+            "dihedral_k": _arr(d.dihedral_k, d.n_dihedral_types, np.float64),
+            "dihedral_n": _arr(d.dihedral_n, d.n_dihedral_types, np.float64),
+            "dihedral_phase": _arr(d.dihedral_phase, d.n_dihedral_types, np.float64),
+            "improper_atoms": _arr(d.improper_atoms, 4 * d.n_impropers, np.int32),
+            "improper_type": _arr(d.improper_type, d.n_impropers, np.int32),
+            "improper_k": _arr(d.improper_k, d.n_improper_types, np.float64),
+            "improper_psi0": _arr(d.improper_psi0, d.n_improper_types, np.float64),
+            "cmap_atoms": _arr(d.cmap_atoms, 5 * d.n_cmaps, np.int32),
+            "cmap_set": _arr(d.cmap_set, d.n_cmaps, np.int32),
+            "cmap_coeff": _arr(d.cmap_coeff, d.cmap_n_sets * d.cmap_resolution ** 2 * 16, np.float64),
+            "cmap_grid": _arr(d.cmap_grid, d.cmap_n_sets * d.cmap_resolution ** 2, np.float64),
+            "excl_start": _arr(d.excl_start, n + 1, np.int32) if d.excl_start else None,
+            "excl_pairs": _arr(d.excl_pairs, 2 * d.n_excl_pairs, np.int32),
+            "pairs14": _arr(d.pairs14, 2 * d.n_pairs14, np.int32),
+            "scee14": _arr(d.scee14, d.n_pairs14, np.float64),
+        }
+        a["excl_list"] = (_arr(d.excl_list, int(a["excl_start"][-1]), np.int32)
+                          if a["excl_start"] is not None else None)
+        a["n_types"], a["cmap_n_sets"], a["cmap_resolution"] = t, d.cmap_n_sets, d.cmap_resolution
+        return a
+
+    @property
+    def desc(self) -> SystemDesc:
+        """The C description (pointers into the host handle)."""
+        return self._desc
+
+    def close(self):
+        if getattr(self, "_h", None):
+            host_lib().mdhost_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def desc_from_arrays(arr: dict, keep: list) -> SystemDesc:
+    """Build a SystemDesc over numpy arrays (kept alive through ``keep``)."""
+
+    def p(name, ctype):
+        v = arr.get(name)
+        if v is None:
+            return None
+        v = np.ascontiguousarray(v, dtype=np.int32 if ctype is C.c_int32 else np.float64)
+        keep.append(v)
+        return v.ctypes.data_as(C.POINTER(ctype))
+
+    d = SystemDesc()
+    n = int(arr["charge"].size)
+    d.n_atoms, d.n_types = n, int(arr["n_types"])
+    d.charge, d.type, d.nb_index = p("charge", C.c_double), p("type", C.c_int32), p("nb_index", C.c_int32)
+    d.n_lj = int(arr["lj_a"].size)
+    d.lj_a, d.lj_b = p("lj_a", C.c_double), p("lj_b", C.c_double)
+    d.lj14_a, d.lj14_b = p("lj14_a", C.c_double), p("lj14_b", C.c_double)
+    for name, width, ntab in [("bond", 2, ["bond_k", "bond_r0"]), ("ub", 2, ["ub_k", "ub_r0"]),
+                              ("angle", 3, ["angle_k", "angle_theta0"]),
+                              ("dihedral", 4, ["dihedral_k", "dihedral_n", "dihedral_phase"]),
+                              ("improper", 4, ["improper_k", "improper_psi0"])]:
+        cnt = int(arr[f"{name}_type"].size)
+        setattr(d, f"n_{name}s", cnt)
+        setattr(d, f"{name}_atoms", p(f"{name}_atoms", C.c_int32))
+        setattr(d, f"{name}_type", p(f"{name}_type", C.c_int32))
+        setattr(d, f"n_{name}_types", int(arr[ntab[0]].size))
+        for tname in ntab:
+            setattr(d, tname, p(tname, C.c_double))
+    d.n_cmaps = int(arr["cmap_set"].size)
+    d.cmap_atoms, d.cmap_set = p("cmap_atoms", C.c_int32), p("cmap_set", C.c_int32)
+    d.cmap_n_sets, d.cmap_resolution = int(arr["cmap_n_sets"]), int(arr["cmap_resolution"])
+    d.cmap_coeff, d.cmap_grid = p("cmap_coeff", C.c_double), p("cmap_grid", C.c_double)
+    d.excl_start, d.excl_list = p("excl_start", C.c_int32), p("excl_list", C.c_int32)
+    d.n_excl_pairs = int(arr["excl_pairs"].size // 2)
+    d.excl_pairs = p("excl_pairs", C.c_int32)
+    d.n_pairs14 = int(arr["scee14"].size)
+    d.pairs14, d.scee14 = p("pairs14", C.c_int32), p("scee14", C.c_double)
+    return d
+
+
+def make_params(semantics=SEM_REF, precision=PREC_FP64, lj_cutoff=10.0, ee_cutoff=None,
+                lj_switch=9.0, ewald_alpha=None, ewald_tol=1e-6, pme_grid=(0, 0, 0),
+                pme_order=4) -> Params:
+    """Parameters; alpha defaults to sqrt(-ln(2 tol))/rc_ee (PBC.cpp:88-91)."""
+    ee = lj_cutoff if ee_cutoff is None else ee_cutoff
+    if ewald_alpha is None:
+        ewald_alpha = float(np.sqrt(-np.log(2.0 * ewald_tol)) / ee)
+    p = Params()
+    p.semantics, p.precision = semantics, precision
+    p.lj_cutoff, p.ee_cutoff, p.lj_switch, p.ewald_alpha = lj_cutoff, ee, lj_switch, ewald_alpha
+    p.pme_grid[0], p.pme_grid[1], p.pme_grid[2] = pme_grid
+    p.pme_order = pme_order
+    return p
+
+
+class GpuError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{ERRORS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Evaluator:
+    """One mdgpu context (device + stream) holding one uploaded system."""
+
+    def __init__(self, device: int = 0):
+        self.lib = gpu_lib()
+        self.ctx = C.c_void_p()
+        st = self.lib.mdgpu_create(device, C.byref(self.ctx))
+        if st != 0:
+            raise GpuError(st, f"mdgpu_create(device={device}) failed")
+        self.n_atoms = 0
+
+    def _check(self, st):
+        if st != 0:
+            raise GpuError(st, self.lib.mdgpu_last_error(self.ctx).decode())
+
+    def upload(self, desc: SystemDesc):
+        self._check(self.lib.mdgpu_upload_system(self.ctx, C.byref(desc)))
+        self.n_atoms = desc.n_atoms
+
+    def set_params(self, params: Params):
+        self._check(self.lib.mdgpu_set_params(self.ctx, C.byref(params)))
+
+    def evaluate(self, coords, box):
+        """Host-buffer evaluation: returns (energies dict, forces (N,3), virial (3,3))."""
+        x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+        b = np.ascontiguousarray(box, dtype=np.float64)
+        f = np.zeros(x.size, dtype=np.float64)
+        v = np.zeros(9, dtype=np.float64)
+        e = Energies()
+        self._check(self.lib.mdgpu_eval(self.ctx, x.ctypes.data_as(P_F64), b.ctypes.data_as(P_F64),
+                                        C.byref(e), f.ctypes.data_as(P_F64), v.ctypes.data_as(P_F64)))
+        return e.as_dict(), f.reshape(-1, 3), v.reshape(3, 3)
+
+    def evaluate_device(self, coords_ptr: int, box, forces_ptr: int):
+        """Device-buffer evaluation (pointers to 3N doubles on the device)."""
+        b = np.ascontiguousarray(box, dtype=np.float64)
+        e = Energies()
+        v = np.zeros(9, dtype=np.float64)
+        self._check(self.lib.mdgpu_eval_device(self.ctx, C.c_void_p(coords_ptr), b.ctypes.data_as(P_F64),
+                                               C.byref(e), C.c_void_p(forces_ptr),
+                                               v.ctypes.data_as(P_F64)))
+        return e.as_dict(), v.reshape(3, 3)
+
+    def set_timing(self, on: bool):
+        self._check(self.lib.mdgpu_set_timing(self.ctx, 1 if on else 0))
+
+    def timing(self) -> dict:
+        out = {}
+        for k, name in enumerate(KERNELS):
+            ms = C.c_double()
+            n = C.c_int64()
+            self._check(self.lib.mdgpu_get_timing(self.ctx, k, C.byref(ms), C.byref(n)))
+            out[name] = (ms.value, n.value)
+        return out
+
+    def set_counting(self, on: bool):
+        self._check(self.lib.mdgpu_set_counting(self.ctx, 1 if on else 0))
+
+    def counts(self) -> dict:
+        c = (C.c_int64 * 4)()
+        self._check(self.lib.mdgpu_get_counts(self.ctx, c))
+        return {"pairs": c[0], "tiles": c[1], "clusters": c[2], "atoms": c[3]}
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.mdgpu_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

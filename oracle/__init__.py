@@ -1,0 +1,105 @@
+"""ctypes binding of the CPU oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from md_amd.native import Energies, Params, SystemDesc, P_F64
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+
+
+class Opts(C.Structure):
+    _fields_ = [("list_skin", C.c_double), ("skip_all_pairs", C.c_int)]
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is not built (make -C oracle)")
+        L = C.CDLL(path)
+        L.oracle_eval.argtypes = [C.POINTER(SystemDesc), C.POINTER(Params), P_F64, P_F64, C.POINTER(Opts),
+                                  C.POINTER(Energies), P_F64, P_F64]
+        L.oracle_ref_pairlist.argtypes = [C.POINTER(SystemDesc), P_F64, P_F64, C.c_double,
+                                          C.POINTER(C.c_int), C.c_long]
+        L.oracle_ref_pairlist.restype = C.c_long
+        L.oracle_ewald_recip_dense.argtypes = [C.c_int, P_F64, P_F64, P_F64, C.c_double, C.c_int, P_F64, P_F64]
+        L.oracle_ewald_recip_dense.restype = C.c_double
+        L.oracle_pme_recip.argtypes = [C.c_int, P_F64, P_F64, P_F64, C.c_double, C.POINTER(C.c_int), C.c_int,
+                                       P_F64, P_F64]
+        L.oracle_pme_recip.restype = C.c_double
+        L.oracle_bspline.argtypes = [C.c_int, C.c_double, P_F64, P_F64]
+        L.oracle_coulomb_constant.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(P_F64)
+
+
+def evaluate(desc: SystemDesc, params: Params, coords, box, list_skin=2.0, skip_all_pairs=False,
+             want_forces=True):
+    """Returns (energies dict, forces (N,3) or None, virial (3,3))."""
+    x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    f = np.zeros(x.size) if want_forces else None
+    v = np.zeros(9)
+    e = Energies()
+    o = Opts(list_skin, 1 if skip_all_pairs else 0)
+    st = lib().oracle_eval(C.byref(desc), C.byref(params), _p(x), _p(b), C.byref(o), C.byref(e),
+                           _p(f) if f is not None else None, _p(v))
+    if st != 0:
+        raise RuntimeError("oracle_eval failed")
+    return e.as_dict(), (f.reshape(-1, 3) if f is not None else None), v.reshape(3, 3)
+
+
+def ref_pairlist(desc: SystemDesc, coords, box, list_cutoff):
+    x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    n = lib().oracle_ref_pairlist(C.byref(desc), _p(x), _p(b), list_cutoff, None, 0)
+    out = np.zeros(3 * max(n, 1), dtype=np.int32)
+    lib().oracle_ref_pairlist(C.byref(desc), _p(x), _p(b), list_cutoff,
+                              out.ctypes.data_as(C.POINTER(C.c_int)), n)
+    return out[:3 * n].reshape(-1, 3)
+
+
+def ewald_recip_dense(q, coords, box, alpha, mmax):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    f = np.zeros(x.size)
+    v = np.zeros(9)
+    e = lib().oracle_ewald_recip_dense(q.size, _p(q), _p(x), _p(b), alpha, mmax, _p(f), _p(v))
+    return e, f.reshape(-1, 3), v.reshape(3, 3)
+
+
+def pme_recip(q, coords, box, alpha, grid, order=4):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    g = (C.c_int * 3)(*grid)
+    f = np.zeros(x.size)
+    v = np.zeros(9)
+    e = lib().oracle_pme_recip(q.size, _p(q), _p(x), _p(b), alpha, g, order, _p(f), _p(v))
+    return e, f.reshape(-1, 3), v.reshape(3, 3)
+
+
+def bspline(order, w):
+    th = np.zeros(order)
+    dth = np.zeros(order)
+    lib().oracle_bspline(order, w, _p(th), _p(dth))
+    return th, dth
+
+
+def coulomb_constant() -> float:
+    return lib().oracle_coulomb_constant()

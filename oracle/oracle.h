@@ -1,0 +1,55 @@
+/* oracle.h - CPU FP64 oracle of the energy-and-force evaluation.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (md_amd/, include/) links,
+ * loads or calls this code; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it, as the checker.
+ *
+ * It restates, single-threaded and in plain C, the reference's evaluation
+ * (files cited per function in oracle.c) in the two semantics of include/
+ * mdgpu.h, and adds two independent references used to pin it: a
+ * structure-factor Ewald sum and the cardinal B-splines.
+ */
+#ifndef MD_ORACLE_H
+#define MD_ORACLE_H
+
+#include "../include/mdgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    double list_skin;      /* REF pair-list skin (reference default 2 A)       */
+    int skip_all_pairs;    /* 1: skip the O(N^2) "EE" pass (ee_all = NaN)      */
+} oracle_opts;
+
+/* One evaluation.  Returns 0 on success. */
+int oracle_eval(const mdgpu_system_desc* sys, const mdgpu_params* p, const double* x,
+                const double box[3], const oracle_opts* opt, mdgpu_energies* e,
+                double* forces, double* virial);
+
+/* The REF pair list (NonBondedUtils.cpp:185-282) as (i, j, is14) triples.
+ * Returns the number of pairs; writes at most cap triples. */
+long oracle_ref_pairlist(const mdgpu_system_desc* sys, const double* x, const double box[3],
+                         double list_cutoff, int* out, long cap);
+
+/* Structure-factor Ewald reciprocal energy, forces (added) and virial over
+ * all integer vectors with |m_x|,|m_y|,|m_z| <= mmax, m != 0. */
+double oracle_ewald_recip_dense(int n, const double* q, const double* x, const double box[3],
+                                double alpha, int mmax, double* forces, double* virial);
+
+/* Smooth-PME reciprocal part alone (energy; forces added; virial). */
+double oracle_pme_recip(int n, const double* q, const double* x, const double box[3], double alpha,
+                        const int grid[3], int order, double* forces, double* virial);
+
+/* Cardinal B-spline weights M_p(w + j) and their derivatives, j = 0..p-1. */
+void oracle_bspline(int order, double w, double* theta, double* dtheta);
+
+/* kCoulomb = 18.2206899283247^2 as the reference forms it. */
+double oracle_coulomb_constant(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
