@@ -79,9 +79,13 @@ __device__ __forceinline__ real dihedral_angle(const R3<real>& b1, const R3<real
     return atan2(dot(b1, n2) * l2, dot(n1, n2));
 }
 
-// Torque tau = -dE/dphi on (a, b, c, d) distributed as in BondedForces.cpp:
-// 195-239; ba = x_a - x_b, cb = x_b - x_c, dc = x_c - x_d.  Returns false
-// (no force) when a cross product or the middle bond is degenerate.
+// Torque tau = -dE/dphi on (a, b, c, d); ba = x_a - x_b, cb = x_b - x_c,
+// dc = x_c - x_d.  REF semantics distributes it as the reference does
+// (BondedForces.cpp:195-239), which is not the gradient of the torsion
+// energy; AMBER semantics uses the gradient (Bekker's form):
+// F_a = -fa0, F_d = -fd0, F_b = (1 + pb) fa0 - pd fd0, F_c = (1 + pd) fd0 - pb fa0.
+// Returns false (no force) when a cross product or the middle bond is
+// degenerate.
 template <typename real>
 __device__ __forceinline__ bool torsion(const BArgs<real>& a, int ia, int ib, int ic, int id, const R3<real>& ba,
                                         const R3<real>& cb, const R3<real>& dc, real tau, bool apply) {
@@ -94,18 +98,27 @@ __device__ __forceinline__ bool torsion(const BArgs<real>& a, int ia, int ib, in
     const real sa = tau * lc / m2, sd = -tau * lc / n2;
     const R3<real> fa = {sa * m.x, sa * m.y, sa * m.z}, fd = {sd * n.x, sd * n.y, sd * n.z};
     const real pb = dot(ba, cb) / c2, pd = dot(dc, cb) / c2;
-    addf(a, ia, fa.x, fa.y, fa.z);
-    addf(a, id, fd.x, fd.y, fd.z);
-    addf(a, ib, (pb - 1) * fa.x - pd * fd.x, (pb - 1) * fa.y - pd * fd.y, (pb - 1) * fa.z - pd * fd.z);
-    addf(a, ic, (pd - 1) * fd.x - pb * fa.x, (pd - 1) * fd.y - pb * fa.y, (pd - 1) * fd.z - pb * fa.z);
+    if (a.amber) {
+        addf(a, ia, -fa.x, -fa.y, -fa.z);
+        addf(a, id, -fd.x, -fd.y, -fd.z);
+        addf(a, ib, (pb + 1) * fa.x - pd * fd.x, (pb + 1) * fa.y - pd * fd.y, (pb + 1) * fa.z - pd * fd.z);
+        addf(a, ic, (pd + 1) * fd.x - pb * fa.x, (pd + 1) * fd.y - pb * fa.y, (pd + 1) * fd.z - pb * fa.z);
+    } else {
+        addf(a, ia, fa.x, fa.y, fa.z);
+        addf(a, id, fd.x, fd.y, fd.z);
+        addf(a, ib, (pb - 1) * fa.x - pd * fd.x, (pb - 1) * fa.y - pd * fd.y, (pb - 1) * fa.z - pd * fd.z);
+        addf(a, ic, (pd - 1) * fd.x - pb * fa.x, (pd - 1) * fd.y - pb * fa.y, (pd - 1) * fd.z - pb * fa.z);
+    }
     return true;
 }
 
 template <typename real>
 __device__ __forceinline__ real wrap_pi(real d) {
     const real pi = static_cast<real>(kPi), two = static_cast<real>(2 * kPi);
-    while (d <= -pi) d += two;
-    while (d > pi) d -= two;
+    // the reference's loops (CMapGroup.cpp:88-91); bounded so that a
+    // non-finite angle cannot keep a thread spinning
+    for (int it = 0; it < 8 && d <= -pi; ++it) d += two;
+    for (int it = 0; it < 8 && d > pi; ++it) d -= two;
     return d;
 }
 

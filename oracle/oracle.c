@@ -156,10 +156,15 @@ static double angles(const mdgpu_system_desc* s, const double* x, double* f) {
 }
 
 /* Torsion force distribution shared by the cosine dihedral, the improper and
- * each CMAP angle (BondedForces.cpp:173-239, 261-330, 372-477): for the torque
- * tau = -dE/dphi on the quadruple (a, b, c, d). */
-static void torsion_forces(const double* xa, const double* xb, const double* xc, const double* xd, double tau,
-                           double* fa, double* fb, double* fc, double* fd, int* ok) {
+ * each CMAP angle, for the torque tau = -dE/dphi on (a, b, c, d).
+ * REF (amber = 0): the reference's distribution as written (BondedForces.cpp:
+ * 173-239, 261-330, 372-477).  It is not -grad E: relative to the gradient its
+ * forces on a and d have the opposite sign and b, c carry an extra 2 F_a,
+ * 2 F_d (checked by finite differences in tests/test_oracle.py).
+ * AMBER (amber = 1): the gradient, F_a = -fa0, F_d = -fd0,
+ * F_b = (1 + pb) fa0 - pd fd0, F_c = (1 + pd) fd0 - pb fa0 (Bekker's form). */
+static void torsion_forces_sem(const double* xa, const double* xb, const double* xc, const double* xd, double tau,
+                               double* fa, double* fb, double* fc, double* fd, int* ok, int amber) {
     double ba[3] = {xa[0] - xb[0], xa[1] - xb[1], xa[2] - xb[2]};
     double cb[3] = {xb[0] - xc[0], xb[1] - xc[1], xb[2] - xc[2]};
     double dc[3] = {xc[0] - xd[0], xc[1] - xd[1], xc[2] - xd[2]};
@@ -181,6 +186,20 @@ static void torsion_forces(const double* xa, const double* xb, const double* xc,
         fb[q] = (dba / cb2 - 1) * fa[q] - ddc / cb2 * fd[q];
         fc[q] = (ddc / cb2 - 1) * fd[q] - dba / cb2 * fa[q];
     }
+    if (amber)
+        for (int q = 0; q < 3; ++q) {
+            fb[q] += 2.0 * fa[q];
+            fc[q] += 2.0 * fd[q];
+            fa[q] = -fa[q];
+            fd[q] = -fd[q];
+        }
+}
+
+static int g_amber_torsion = 0;  /* set per evaluation by oracle_eval */
+
+static void torsion_forces(const double* xa, const double* xb, const double* xc, const double* xd, double tau,
+                           double* fa, double* fb, double* fc, double* fd, int* ok) {
+    torsion_forces_sem(xa, xb, xc, xd, tau, fa, fb, fc, fd, ok, g_amber_torsion);
 }
 
 /* BondedEnergy.cpp:51-72 and BondedForces.cpp:155-241 */
@@ -975,6 +994,7 @@ int oracle_eval(const mdgpu_system_desc* s, const mdgpu_params* p, const double*
     if (f) memset(f, 0, sizeof(double) * 3 * (size_t)n);
     if (vir) memset(vir, 0, sizeof(double) * 9);
     int amber = p->semantics == MDGPU_SEM_AMBER;
+    g_amber_torsion = amber;
     /* calculate_energies / calculate_forces (System.cpp:9-39) */
     e->bond = harmonic_pairs(s->n_bonds, s->bond_atoms, s->bond_type, s->bond_k, s->bond_r0, x, f);
     e->angle = angles(s, x, f);
