@@ -62,10 +62,10 @@ hipError_t ensure(T** p, size_t* cap, size_t n) {
 
 void free_work(mdgpu_ctx* c) {
     Work& w = c->work;
-    void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_count, w.col_fill, w.col_start, w.col_cstart, w.col_tmp,
-                    w.slot_atom, w.slot_type, w.slot_pos_d, w.slot_pos_f, w.cluster_col, w.cl_origin, w.cl_half,
+    void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.block_hist, w.block_off, w.col_start, w.col_cstart, w.col_tmp,
+                    w.slot_atom, w.slot_meta, w.slot_pos_d, w.slot_pos_f, w.cluster_col, w.cl_origin, w.cl_half,
                     w.list_count, w.list_j, w.list_mask, w.list_delta_d, w.list_delta_f, w.force_d, w.force_fx,
-                    w.partial, w.energies, w.flags, w.excl_rmax, w.grid_d, w.grid_fx, w.grid_f, w.grid_phi_d,
+                    w.partial, w.energies, w.force_out, w.flags, w.excl_rmax, w.grid_d, w.grid_fx, w.grid_f, w.grid_phi_d,
                     w.cgrid, w.bsp_mod, w.bsp_mod_f, w.twiddle_d, w.twiddle_f};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -128,8 +128,11 @@ int mdgpu_create(int device, mdgpu_ctx** out) {
     for (auto& ev : c->ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.flags), 4 * sizeof(unsigned int));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.energies), mdg::E_NSLOTS * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.energies), (mdg::E_NSLOTS + 4) * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.excl_rmax), sizeof(float));
+    // zero once; afterwards the final reduction clears them for the next evaluation
+    if (e == hipSuccess) e = hipMemset(c->work.flags, 0, 4 * sizeof(unsigned int));
+    if (e == hipSuccess) e = hipMemset(c->work.excl_rmax, 0, sizeof(float));
     if (e != hipSuccess) {
         mdgpu_destroy(c);
         return MDGPU_ERR_HIP;
@@ -470,10 +473,10 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     size_t cap;
     cap = w.cap_atoms;
     if (w.cap_atoms < static_cast<size_t>(n)) {
-        void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_tmp, w.force_d, w.force_fx};
+        void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_tmp, w.force_d, w.force_fx, w.force_out};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
-        w.coords = w.wrapped = w.force_d = nullptr;
+        w.coords = w.wrapped = w.force_d = w.force_out = nullptr;
         w.col_of = w.col_tmp = nullptr;
         w.force_fx = nullptr;
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.coords), 3 * n * sizeof(double)), "alloc");
@@ -482,28 +485,32 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_tmp), n * sizeof(int)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_d), 3 * n * sizeof(double)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_fx), 3 * n * sizeof(unsigned long long)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_out), 3 * n * sizeof(double)), "alloc");
+        // the accumulators start at zero; every evaluation's reduction re-zeroes them
+        HIP_TRY(hipMemset(w.force_d, 0, 3 * n * sizeof(double)), "clear");
+        HIP_TRY(hipMemset(w.force_fx, 0, 3 * n * sizeof(unsigned long long)), "clear");
         w.cap_atoms = n;
     }
     (void)cap;
-    if (w.cap_cols < static_cast<size_t>(ncol + 1)) {
-        void* ptrs[] = {w.col_count, w.col_fill, w.col_start, w.col_cstart};
+    const size_t nbc = static_cast<size_t>(mdg::sort_blocks(n)) * (ncol + 1);
+    if (w.cap_cols < nbc) {
+        void* ptrs[] = {w.block_hist, w.block_off, w.col_start, w.col_cstart};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
-        const size_t m = ncol + 1;
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_count), m * sizeof(int)), "alloc");
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_fill), m * sizeof(int)), "alloc");
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_start), m * sizeof(int)), "alloc");
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_cstart), m * sizeof(int)), "alloc");
-        w.cap_cols = m;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.block_hist), nbc * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.block_off), nbc * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_start), nbc * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_cstart), nbc * sizeof(int)), "alloc");
+        w.cap_cols = nbc;
     }
     const size_t ncl = c->n_clusters, nslots = ncl * mdg::kCluster;
     if (w.cap_slots < nslots) {
-        void* ptrs[] = {w.slot_atom, w.slot_type, w.slot_pos_d, w.slot_pos_f, w.cluster_col, w.cl_origin, w.cl_half,
+        void* ptrs[] = {w.slot_atom, w.slot_meta, w.slot_pos_d, w.slot_pos_f, w.cluster_col, w.cl_origin, w.cl_half,
                         w.list_count};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_atom), nslots * sizeof(int)), "alloc");
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_type), nslots * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_meta), nslots * sizeof(int2)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_pos_d), nslots * sizeof(double4)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_pos_f), nslots * sizeof(float4)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.cluster_col), ncl * sizeof(int)), "alloc");
@@ -545,25 +552,56 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     return MDGPU_OK;
 }
 
-// Final reduction: one thread per energy slot sums the rows in order; the
-// remaining threads convert the fixed-point forces (mixed form) or copy the
-// FP64 forces to the output.
-__global__ void k_reduce(const double* __restrict__ partial, int rows, double* __restrict__ energies, int mixed,
-                         const unsigned long long* __restrict__ fx, const double* __restrict__ fd, double inv_scale,
-                         int n3, double* __restrict__ out, const int* __restrict__ list_count,
-                         const int* __restrict__ ncl_dev) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x < mdg::E_NSLOTS) {
+// Final reduction.  Blocks [0, E_NSLOTS) each sum one energy slot over the
+// partial rows (fixed per-thread order and a fixed tree, so the result is
+// deterministic); block E_NSLOTS copies the flags to the output and clears
+// them; the remaining blocks convert the forces (fixed point -> FP64 in the
+// mixed form) into `out` and clear the accumulators for the next evaluation.
+constexpr int kReduceThreads = 256;
+__global__ __launch_bounds__(kReduceThreads) void k_reduce(const double* __restrict__ partial, int rows,
+                                                           double* __restrict__ out_buf, unsigned int* flags,
+                                                           float* excl_rmax, int mixed, unsigned long long* fx,
+                                                           double* fd, double inv_scale, int n3,
+                                                           double* __restrict__ out,
+                                                           const int* __restrict__ list_count,
+                                                           const int* __restrict__ ncl_dev) {
+    __shared__ double red[kReduceThreads];
+    const int b = blockIdx.x, t = threadIdx.x;
+    if (b < mdg::E_NSLOTS) {
         double s = 0.0;
-        if (threadIdx.x == mdg::E_TILES) {
-            for (int k = 0; k < *ncl_dev; ++k) s += list_count[k];
+        if (b == mdg::E_TILES) {
+            const int ncl = *ncl_dev;
+            for (int k = t; k < ncl; k += kReduceThreads) s += list_count[k];
         } else {
-            for (int r = 0; r < rows; ++r) s += partial[static_cast<size_t>(r) * mdg::E_NSLOTS + threadIdx.x];
+            for (int r = t; r < rows; r += kReduceThreads) s += partial[static_cast<size_t>(r) * mdg::E_NSLOTS + b];
         }
-        energies[threadIdx.x] = s;
+        red[t] = s;
+        __syncthreads();
+        for (int m = kReduceThreads / 2; m > 0; m >>= 1) {
+            if (t < m) red[t] += red[t + m];
+            __syncthreads();
+        }
+        if (t == 0) out_buf[b] = red[0];
+        return;
     }
-    for (int i = t; i < n3; i += gridDim.x * blockDim.x)
-        out[i] = mixed ? static_cast<double>(static_cast<long long>(fx[i])) * inv_scale : fd[i];
+    if (b == mdg::E_NSLOTS) {
+        if (t < 4) {
+            out_buf[mdg::E_NSLOTS + t] = static_cast<double>(flags[t]);
+            flags[t] = 0u;
+        }
+        if (t == 0) *excl_rmax = 0.f;
+        return;
+    }
+    const int nb = gridDim.x - mdg::E_NSLOTS - 1;
+    for (int i = (b - mdg::E_NSLOTS - 1) * kReduceThreads + t; i < n3; i += nb * kReduceThreads) {
+        if (mixed) {
+            out[i] = static_cast<double>(static_cast<long long>(fx[i])) * inv_scale;
+            fx[i] = 0ull;
+        } else {
+            out[i] = fd[i];
+            fd[i] = 0.0;
+        }
+    }
 }
 
 int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_energies* out_e, double* d_out,
@@ -578,18 +616,12 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
         int st = prepare(c, box);
         if (st != MDGPU_OK) return st;
         Work& w = c->work;
-        if (!coords_are_work)
-            HIP_TRY(hipMemcpyAsync(w.coords, d_coords, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
-                    "copy coords");
-        const double* x = w.coords;
+        const double* x = coords_are_work ? w.coords : d_coords;
+        c->cur_x = x;
         auto& T = c->timing;
         auto mark = [&](int k, bool start) {
             if (T.on) (void)hipEventRecord(c->ev[2 * k + (start ? 0 : 1)], c->stream);
         };
-        HIP_TRY(hipMemsetAsync(w.flags, 0, 4 * sizeof(unsigned int), c->stream), "clear flags");
-        HIP_TRY(hipMemsetAsync(w.partial, 0, w.cap_partial * mdg::E_NSLOTS * sizeof(double), c->stream), "clear");
-        if (mixed) HIP_TRY(hipMemsetAsync(w.force_fx, 0, 3 * n * sizeof(unsigned long long), c->stream), "clear");
-        else HIP_TRY(hipMemsetAsync(w.force_d, 0, 3 * n * sizeof(double), c->stream), "clear");
         mark(MDGPU_K_SORT, true);
         if (amber && c->topo.has_excl) HIP_TRY(mdg::launch_excl_extent(c), "excl extent");
         HIP_TRY(mdg::launch_sort(c, x, mixed), "sort");
@@ -616,17 +648,20 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
             row += used;
         }
         mark(MDGPU_K_REDUCE, true);
-        hipLaunchKernelGGL(k_reduce, dim3(std::max(1, std::min(1024, (3 * n + 255) / 256))), dim3(256), 0, c->stream,
-                           w.partial, row, w.energies, mixed ? 1 : 0, w.force_fx, w.force_d,
-                           1.0 / mdg::fixed_scale(p.semantics), 3 * n, d_out ? d_out : w.force_d, w.list_count,
+        const int fblocks = std::max(1, std::min(512, (3 * n + kReduceThreads - 1) / kReduceThreads));
+        hipLaunchKernelGGL(k_reduce, dim3(mdg::E_NSLOTS + 1 + fblocks), dim3(kReduceThreads), 0, c->stream,
+                           w.partial, row, w.energies, w.flags, w.excl_rmax, mixed ? 1 : 0, w.force_fx, w.force_d,
+                           1.0 / mdg::fixed_scale(p.semantics), 3 * n, d_out ? d_out : w.force_out, w.list_count,
                            w.col_cstart + c->n_cols);
         HIP_TRY(hipGetLastError(), "reduce");
         mark(MDGPU_K_REDUCE, false);
-        double e[mdg::E_NSLOTS];
-        unsigned int flags[4];
+        double e[mdg::E_NSLOTS + 4];
         HIP_TRY(hipMemcpyAsync(e, w.energies, sizeof e, hipMemcpyDeviceToHost, c->stream), "read energies");
-        HIP_TRY(hipMemcpyAsync(flags, w.flags, sizeof flags, hipMemcpyDeviceToHost, c->stream), "read flags");
         HIP_TRY(hipStreamSynchronize(c->stream), "evaluation");
+        const unsigned int flags[4] = {static_cast<unsigned int>(e[mdg::E_NSLOTS]),
+                                       static_cast<unsigned int>(e[mdg::E_NSLOTS + 1]),
+                                       static_cast<unsigned int>(e[mdg::E_NSLOTS + 2]),
+                                       static_cast<unsigned int>(e[mdg::E_NSLOTS + 3])};
         if (flags[2]) return set_error(c, MDGPU_ERR_UNSUPPORTED, "a sort column holds more than 2048 atoms");
         if (flags[3]) return set_error(c, MDGPU_ERR_UNSUPPORTED, "too many tiles near excluded pairs in one cluster");
         if (flags[1]) {
@@ -705,7 +740,7 @@ int mdgpu_eval(mdgpu_ctx* c, const double* coords, const double box[3], mdgpu_en
     st = eval_impl(c, nullptr, box, out_e, nullptr, out_virial, true);
     if (st != MDGPU_OK) return st;
     if (out_forces) {
-        HIP_TRY(hipMemcpy(out_forces, c->work.force_d, 3 * n * sizeof(double), hipMemcpyDeviceToHost), "read forces");
+        HIP_TRY(hipMemcpy(out_forces, c->work.force_out, 3 * n * sizeof(double), hipMemcpyDeviceToHost), "read forces");
     }
     return MDGPU_OK;
 }

@@ -353,7 +353,7 @@ hipError_t launch_corrections(mdgpu_ctx* c, bool mixed, int row0) {
     if (nblk == 0) return hipSuccess;
     const Topo& t = c->topo;
     CArgs a{};
-    a.x = c->work.coords;
+    a.x = c->cur_x;
     a.lx = c->grid.box[0];
     a.ly = c->grid.box[1];
     a.lz = c->grid.box[2];

@@ -70,11 +70,14 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Block-wide sum of several doubles into partial[slot] (one writer per
-// block); blockDim.x must be a multiple of 64 and <= 1024.
+// Block-wide sum of several doubles into one row of the partial-sum buffer:
+// slot slots[k] of the row receives the block total of v[k], every other
+// slot of the row receives 0 (so a row never keeps a value from an earlier
+// evaluation).  blockDim.x must be a multiple of 64, >= E_NSLOTS and <= 1024.
 template <int NV>
 __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* partial_row, const int* slots) {
     __shared__ double red[16][NV];
+    __shared__ double tot[NV];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
@@ -82,14 +85,19 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* partial
 #pragma unroll
         for (int k = 0; k < NV; ++k) red[w][k] = v[k];
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < NV) {
         const int nw = (blockDim.x + 63) >> 6;
+        double s = 0.0;
+        for (int q = 0; q < nw; ++q) s += red[q][threadIdx.x];
+        tot[threadIdx.x] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < E_NSLOTS) {
+        double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            double s = 0.0;
-            for (int q = 0; q < nw; ++q) s += red[q][k];
-            partial_row[slots[k]] = s;
-        }
+        for (int k = 0; k < NV; ++k)
+            if (slots[k] == static_cast<int>(threadIdx.x)) s = tot[k];
+        partial_row[threadIdx.x] = s;
     }
 }
 

@@ -51,15 +51,15 @@ struct Work {
     double* coords = nullptr;        // [3N] input (owned copy for host-pointer evals)
     double* wrapped = nullptr;       // [3N] wrapped into the box
     int* col_of = nullptr;           // [N] column of each atom
-    int* col_count = nullptr;        // [ncol + 1]
-    int* col_fill = nullptr;         // [ncol]
+    int* block_hist = nullptr;       // [sort blocks x ncol] column histogram per block
+    int* block_off = nullptr;        // [sort blocks x ncol] start of each block's run in a column
     int* col_start = nullptr;        // [ncol + 1] atom offsets
     int* col_cstart = nullptr;       // [ncol + 1] cluster offsets
     int* col_tmp = nullptr;          // [N] atoms bucketed by column
     size_t cap_cols = 0;
     size_t cap_slots = 0;            // clusters * 8
     int* slot_atom = nullptr;        // [slots] original atom or -1
-    int* slot_type = nullptr;        // [slots]
+    int2* slot_meta = nullptr;       // [slots] (LJ type, original atom or -1)
     double4* slot_pos_d = nullptr;   // [slots] position relative to the cluster origin + charge
     float4* slot_pos_f = nullptr;
     int* cluster_col = nullptr;      // [clusters]
@@ -79,7 +79,8 @@ struct Work {
     // energies: partial rows, one per block of each producing kernel
     size_t cap_partial = 0;
     double* partial = nullptr;       // [rows * E_NSLOTS]
-    double* energies = nullptr;      // [E_NSLOTS] final
+    double* energies = nullptr;      // [E_NSLOTS + 4] final energies, then the flags as doubles
+    double* force_out = nullptr;     // [3N] FP64 forces for host-buffer evaluations
     unsigned int* flags = nullptr;   // [4]: 0 fixed overflow, 1 list overflow, 2 column overflow, 3 spare
     float* excl_rmax = nullptr;      // [1] largest excluded-pair distance (as float)
     // PME
@@ -128,6 +129,7 @@ struct mdgpu_ctx {
     int n_clusters = 0;
     int n_cols = 0;
     int split = 4;                  // waves per i-cluster list in the pair kernel
+    const double* cur_x = nullptr;  // raw coordinates of the evaluation in progress (device)
     std::vector<void*> topo_allocs;
     // host copies of the per-type-pair LJ parameters (the device table also
     // holds cutoff-dependent shift constants, rebuilt by mdgpu_set_params)
@@ -154,6 +156,7 @@ int nonbonded_rows(const mdgpu_ctx* c);
 int bonded_rows(const mdgpu_ctx* c);
 int corrections_rows(const mdgpu_ctx* c);
 int pme_rows(const mdgpu_ctx* c);
+int sort_blocks(int n);
 
 // fixed-point scale exponent of the mixed form for a semantics
 inline double fixed_scale(int semantics) { return semantics == MDGPU_SEM_REF ? 1048576.0 : 4294967296.0; }

@@ -8,9 +8,9 @@
 // calculate_EE_ewald_direct_term (:236-268), calculate_forces_LJ_pairlist
 // (NonBondedForces.cpp:92-154) and calculate_forces_EE_pairlist (:207-242).
 //
-// Lane l of a wave computes the pair (i = 8c_i + l/8, j = 8c_j + l%8).  The
+// Lane l of a wave computes the pair (i = 8c_i + l%8, j = 8c_j + l/8).  The
 // i-forces stay in registers over the wave's list entries; the j-forces of a
-// tile are summed over the 8 lanes sharing j with xor-shuffles and then added
+// tile are summed over the 8 lanes sharing j with DPP moves and then added
 // by 24 lanes at once (one component each).  Forces go to 64-bit fixed point
 // (mixed form) or FP64 atomics (FP64 form); energies to per-block FP64 rows.
 //
@@ -45,8 +45,7 @@ struct NbArgs {
     const unsigned long long* list_mask;
     const typename V4<real>::T* list_delta;
     const typename V4<real>::T* pos;
-    const int* slot_type;
-    const int* slot_atom;
+    const int2* meta;                  // per slot (LJ type, original atom or -1)
     const typename V4<real>::T* lj;    // (T+1)^2 x (A, B, U(rc), G(rc))
     const unsigned char* listed;       // (T+1)^2, 1 if the type pair has an LJ entry
     int ntp;                           // T + 1
@@ -103,6 +102,53 @@ __device__ __forceinline__ void add_force<double>(const NbArgs<double>& a, int i
     acc_add(a.force_d, idx, v);
 }
 
+// Sum over the 8 consecutive lanes of a group with three DPP moves
+// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror); every lane of
+// the group ends with the group total.
+__device__ __forceinline__ int dpp_xor1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, true); }
+__device__ __forceinline__ int dpp_xor2(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, true); }
+__device__ __forceinline__ int dpp_hmir(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, true); }
+
+__device__ __forceinline__ float sum8(float v) {
+    v += __int_as_float(dpp_xor1(__float_as_int(v)));
+    v += __int_as_float(dpp_xor2(__float_as_int(v)));
+    v += __int_as_float(dpp_hmir(__float_as_int(v)));
+    return v;
+}
+__device__ __forceinline__ double dpp_d(double v, int which) {
+    const long long b = __double_as_longlong(v);
+    int lo = static_cast<int>(b & 0xffffffffll), hi = static_cast<int>(b >> 32);
+    if (which == 0) { lo = dpp_xor1(lo); hi = dpp_xor1(hi); }
+    else if (which == 1) { lo = dpp_xor2(lo); hi = dpp_xor2(hi); }
+    else { lo = dpp_hmir(lo); hi = dpp_hmir(hi); }
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ double sum8(double v) {
+    v += dpp_d(v, 0);
+    v += dpp_d(v, 1);
+    v += dpp_d(v, 2);
+    return v;
+}
+
+template <typename real>
+__device__ __forceinline__ real bcast(real v, int lane);
+template <>
+__device__ __forceinline__ float bcast<float>(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+template <>
+__device__ __forceinline__ double bcast<double>(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// Lane l handles i = l % 8 and j = l / 8 of the tile, so the per-tile j-force
+// sum runs over 8 consecutive lanes (DPP) and the i-force sum, done once per
+// wave, over lanes 8 apart.  List entries are fetched 64 at a time, one per
+// lane, and broadcast with readlane; the j-cluster data of the next tile is
+// loaded while the current tile is computed.
 template <typename real, bool kAmber, bool kSwitch>
 __global__ __launch_bounds__(kWaves * kWave) void k_nonbonded(const NbArgs<real> a) {
     using R4 = typename V4<real>::T;
@@ -117,10 +163,10 @@ __global__ __launch_bounds__(kWaves * kWave) void k_nonbonded(const NbArgs<real>
     }
     __syncthreads();
 
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int ti = lane >> 3, tj = lane & 7;
+    const int lane = threadIdx.x & 63;
+    const int ti = lane & 7, tj = lane >> 3;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int gw = blk * kWaves + wv;
+    const int gw = __builtin_amdgcn_readfirstlane(blk * kWaves + (threadIdx.x >> 6));
     const int ci = gw / a.split, part = gw % a.split;
     const int ncl = min(*a.ncl_dev, a.max_clusters);
     int k0 = 0, k1 = 0;
@@ -131,8 +177,10 @@ __global__ __launch_bounds__(kWaves * kWave) void k_nonbonded(const NbArgs<real>
     }
     const int cis = ci < ncl ? ci : 0;
     const R4 pi = a.pos[cis * kCluster + ti];
-    const int tyi = a.slot_type[cis * kCluster + ti] * a.ntp;
+    const int2 mi = a.meta[cis * kCluster + ti];
+    const int tyi = mi.x * a.ntp;
     const real qi = pi.w * a.kc;
+    const int mbit = 8 * ti + tj;  // mask bit of this lane's pair (i-major, as the search writes it)
 
     real fix = 0, fiy = 0, fiz = 0;
     // energy accumulators: FP64 form in double; mixed form in float, flushed to
@@ -140,100 +188,111 @@ __global__ __launch_bounds__(kWaves * kWave) void k_nonbonded(const NbArgs<real>
     double e_vdw = 0, e_all = 0, e_list = 0, e_rf = 0, e_ew = 0, n_pairs = 0;
     real s_vdw = 0, s_all = 0, s_list = 0, s_rf = 0, s_ew = 0;
     const real tsp = static_cast<real>(1.1283791670955126) * a.alpha;  // 2 alpha / sqrt(pi)
+    const real rc2 = a.rc2_lj > a.rc2_ee ? a.rc2_lj : a.rc2_ee;
     const size_t base = static_cast<size_t>(cis) * a.cap;
-    for (int k = k0; k < k1; ++k) {
-        const int cj = a.list_j[base + k];
-        const unsigned long long mask = a.list_mask[base + k];
-        const R4 dl = a.list_delta[base + k];
-        const int sj = cj * kCluster + tj;
-        const R4 pj = a.pos[sj];
-        const real dx = pi.x - pj.x + dl.x;
-        const real dy = pi.y - pj.y + dl.y;
-        const real dz = pi.z - pj.z + dl.z;
-        const real r2 = dx * dx + dy * dy + dz * dz;
-        const bool in_mask = (mask >> lane) & 1ull;
-        const real rc2 = a.rc2_lj > a.rc2_ee ? a.rc2_lj : a.rc2_ee;
-        const bool act = in_mask && r2 <= rc2 && r2 >= static_cast<real>(1e-12);
-        real f = 0;
-        if (act) {
-            const real rinv = M::rsqrt(r2);
-            const real r = r2 * rinv;
-            const real r2inv = rinv * rinv;
-            const int tyj = a.slot_type[sj];
-            const int tp = tyi + tyj;
-            const R4 lj = s_lj[tp];
-            // work count: real (non-padding) pairs inside the cutoff
-            if (a.count_pairs && tyj != a.ntp - 1 && tyi != (a.ntp - 1) * a.ntp) n_pairs += 1.0;
-            if (r2 <= a.rc2_lj) {
-                const real r6 = r2inv * r2inv * r2inv;
-                const real ar6 = lj.x * r6;
-                const real u = (ar6 - lj.y) * r6;                       // A/r^12 - B/r^6
-                const real g = (static_cast<real>(12) * ar6 - static_cast<real>(6) * lj.y) * r6 * r2inv;  // -U'/r
-                real el, fl;
-                if (kSwitch) {
-                    real xs = (r - a.rs) * a.inv_sw;
-                    xs = xs < 0 ? static_cast<real>(0) : xs;
-                    const real x2 = xs * xs;
-                    const real sw = 1 + x2 * xs * (static_cast<real>(-10) + xs * (static_cast<real>(15) - 6 * xs));
-                    const real dsw = x2 * (static_cast<real>(-30) + xs * (static_cast<real>(60) - 30 * xs)) * a.inv_sw;
-                    el = u * sw;
-                    fl = sw * g - u * dsw * rinv;
-                } else {
-                    el = u - lj.z + (r - a.rc_lj) * (lj.w * a.rc_lj);
-                    fl = g - lj.w * a.rc_lj * rinv;
-                }
-                if (!kAmber && !s_listed[tp]) { el = 0; fl = 0; }
-                s_vdw += el;
-                f += fl;
-            }
-            if (r2 <= a.rc2_ee) {
-                const real qq = qi * pj.w;
-                real ec, ex;
-                M::erfc_exp(a.alpha * r, ec, ex);
-                const real e_c = qq * rinv;
-                if (kAmber) {
-                    s_ew += e_c * ec;
-                    f += qq * (ec * rinv + tsp * ex) * r2inv;
-                } else {
-                    const real lst = s_listed[tp] ? static_cast<real>(1) : static_cast<real>(0);
-                    s_all += e_c;
-                    s_list += lst * e_c;
-                    s_rf += lst * qq * (rinv - a.krf_crf);
-                    s_ew += lst * e_c * ec;
-                    f += lst * e_c * r2inv;
-                }
-            }
+    int flush = 0;
+    for (int kb = k0; kb < k1; kb += kWave) {
+        const int nb = min(kWave, k1 - kb);
+        // this lane's list entry of the batch
+        int e_j = 0;
+        unsigned long long e_m = 0;
+        R4 e_d = {0, 0, 0, 0};
+        if (lane < nb) {
+            e_j = a.list_j[base + kb + lane];
+            e_m = a.list_mask[base + kb + lane];
+            e_d = a.list_delta[base + kb + lane];
         }
-        const real fx = f * dx, fy = f * dy, fz = f * dz;
-        fix += fx;
-        fiy += fy;
-        fiz += fz;
-        if (__ballot(act) != 0ull) {
-            real gx = fx, gy = fy, gz = fz;
-            gx += shfl_xor(gx, 8);  gy += shfl_xor(gy, 8);  gz += shfl_xor(gz, 8);
-            gx += shfl_xor(gx, 16); gy += shfl_xor(gy, 16); gz += shfl_xor(gz, 16);
-            gx += shfl_xor(gx, 32); gy += shfl_xor(gy, 32); gz += shfl_xor(gz, 32);
-            if (ti < 3) {
-                const int aj = a.slot_atom[sj];
-                const real v = ti == 0 ? gx : (ti == 1 ? gy : gz);
-                if (aj >= 0) add_force<real>(a, 3 * aj + ti, -v);
+        const int e_mlo = static_cast<int>(e_m & 0xffffffffull), e_mhi = static_cast<int>(e_m >> 32);
+        int cj = __builtin_amdgcn_readlane(e_j, 0);
+        R4 pj = a.pos[cj * kCluster + tj];
+        int2 mj = a.meta[cj * kCluster + tj];
+        for (int t = 0; t < nb; ++t) {
+            const R4 pcur = pj;
+            const int2 mcur = mj;
+            if (t + 1 < nb) {
+                cj = __builtin_amdgcn_readlane(e_j, t + 1);
+                pj = a.pos[cj * kCluster + tj];
+                mj = a.meta[cj * kCluster + tj];
             }
-        }
-        if (((k - k0) & 31) == 31) {
-            e_vdw += s_vdw; e_all += s_all; e_list += s_list; e_rf += s_rf; e_ew += s_ew;
-            s_vdw = s_all = s_list = s_rf = s_ew = 0;
+            const unsigned long long mask =
+                (static_cast<unsigned long long>(static_cast<unsigned int>(__builtin_amdgcn_readlane(e_mhi, t))) << 32) |
+                static_cast<unsigned int>(__builtin_amdgcn_readlane(e_mlo, t));
+            const real dx = pi.x - pcur.x + bcast<real>(e_d.x, t);
+            const real dy = pi.y - pcur.y + bcast<real>(e_d.y, t);
+            const real dz = pi.z - pcur.z + bcast<real>(e_d.z, t);
+            const real r2 = dx * dx + dy * dy + dz * dz;
+            const bool act = ((mask >> mbit) & 1ull) && r2 <= rc2 && r2 >= static_cast<real>(1e-12);
+            real f = 0;
+            if (act) {
+                const real rinv = M::rsqrt(r2);
+                const real r = r2 * rinv;
+                const real r2inv = rinv * rinv;
+                const int tp = tyi + mcur.x;
+                const R4 lj = s_lj[tp];
+                // work count: real (non-padding) pairs inside the cutoff
+                if (a.count_pairs && mcur.y >= 0 && mi.y >= 0) n_pairs += 1.0;
+                if (r2 <= a.rc2_lj) {
+                    const real r6 = r2inv * r2inv * r2inv;
+                    const real ar6 = lj.x * r6;
+                    const real u = (ar6 - lj.y) * r6;  // A/r^12 - B/r^6
+                    const real g = (static_cast<real>(12) * ar6 - static_cast<real>(6) * lj.y) * r6 * r2inv;  // -U'/r
+                    real el, fl;
+                    if (kSwitch) {
+                        real xs = (r - a.rs) * a.inv_sw;
+                        xs = xs < 0 ? static_cast<real>(0) : xs;
+                        const real x2 = xs * xs;
+                        const real sw = 1 + x2 * xs * (static_cast<real>(-10) + xs * (static_cast<real>(15) - 6 * xs));
+                        const real dsw = x2 * (static_cast<real>(-30) + xs * (static_cast<real>(60) - 30 * xs)) * a.inv_sw;
+                        el = u * sw;
+                        fl = sw * g - u * dsw * rinv;
+                    } else {
+                        el = u - lj.z + (r - a.rc_lj) * (lj.w * a.rc_lj);
+                        fl = g - lj.w * a.rc_lj * rinv;
+                    }
+                    if (!kAmber && !s_listed[tp]) { el = 0; fl = 0; }
+                    s_vdw += el;
+                    f += fl;
+                }
+                if (r2 <= a.rc2_ee) {
+                    const real qq = qi * pcur.w;
+                    real ec, ex;
+                    M::erfc_exp(a.alpha * r, ec, ex);
+                    const real e_c = qq * rinv;
+                    if (kAmber) {
+                        s_ew += e_c * ec;
+                        f += qq * (ec * rinv + tsp * ex) * r2inv;
+                    } else {
+                        const real lst = s_listed[tp] ? static_cast<real>(1) : static_cast<real>(0);
+                        s_all += e_c;
+                        s_list += lst * e_c;
+                        s_rf += lst * qq * (rinv - a.krf_crf);
+                        s_ew += lst * e_c * ec;
+                        f += lst * e_c * r2inv;
+                    }
+                }
+            }
+            const real fx = f * dx, fy = f * dy, fz = f * dz;
+            fix += fx;
+            fiy += fy;
+            fiz += fz;
+            if (__ballot(act) != 0ull) {
+                const real gx = sum8(fx), gy = sum8(fy), gz = sum8(fz);
+                if (ti < 3 && mcur.y >= 0) add_force<real>(a, 3 * mcur.y + ti, -(ti == 0 ? gx : (ti == 1 ? gy : gz)));
+            }
+            if (++flush == 32) {
+                flush = 0;
+                e_vdw += s_vdw; e_all += s_all; e_list += s_list; e_rf += s_rf; e_ew += s_ew;
+                s_vdw = s_all = s_list = s_rf = s_ew = 0;
+            }
         }
     }
     e_vdw += s_vdw; e_all += s_all; e_list += s_list; e_rf += s_rf; e_ew += s_ew;
-    // i-forces: sum over the 8 lanes sharing i
-    fix += shfl_xor(fix, 1); fiy += shfl_xor(fiy, 1); fiz += shfl_xor(fiz, 1);
-    fix += shfl_xor(fix, 2); fiy += shfl_xor(fiy, 2); fiz += shfl_xor(fiz, 2);
-    fix += shfl_xor(fix, 4); fiy += shfl_xor(fiy, 4); fiz += shfl_xor(fiz, 4);
-    if (ci < ncl && tj < 3 && k1 > k0) {
-        const int ai = a.slot_atom[ci * kCluster + ti];
-        const real v = tj == 0 ? fix : (tj == 1 ? fiy : fiz);
-        if (ai >= 0) add_force<real>(a, 3 * ai + tj, v);
-    }
+    // i-forces: sum over the 8 lanes sharing i (8 apart)
+    fix += shfl_xor(fix, 8);  fiy += shfl_xor(fiy, 8);  fiz += shfl_xor(fiz, 8);
+    fix += shfl_xor(fix, 16); fiy += shfl_xor(fiy, 16); fiz += shfl_xor(fiz, 16);
+    fix += shfl_xor(fix, 32); fiy += shfl_xor(fiy, 32); fiz += shfl_xor(fiz, 32);
+    if (ci < ncl && tj < 3 && k1 > k0 && mi.y >= 0)
+        add_force<real>(a, 3 * mi.y + tj, tj == 0 ? fix : (tj == 1 ? fiy : fiz));
     double v[6] = {e_vdw, e_all, e_list, e_rf, e_ew, n_pairs};
     const int slots[6] = {E_VDW, E_EE_ALL, E_EE_LIST, E_EE_RF, E_EW_DIRECT, E_PAIRS};
     block_sum_store<6>(v, a.partial + static_cast<size_t>(a.row0 + blockIdx.x) * E_NSLOTS, slots);
@@ -264,8 +323,7 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0) {
         a.lj = t.lj_d;
     }
     a.listed = reinterpret_cast<const unsigned char*>(t.lj_listed);
-    a.slot_type = w.slot_type;
-    a.slot_atom = w.slot_atom;
+    a.meta = w.slot_meta;
     a.rc2_lj = static_cast<real>(p.lj_cutoff * p.lj_cutoff);
     a.rc2_ee = static_cast<real>(p.ee_cutoff * p.ee_cutoff);
     a.rc_lj = static_cast<real>(p.lj_cutoff);

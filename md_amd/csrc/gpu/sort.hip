@@ -28,36 +28,55 @@ namespace {
 
 constexpr int kColMax = 2048;  // atoms per column the in-LDS sort handles
 
-__global__ void k_bin(const double* __restrict__ x, int n, double lx, double ly, double lz, int ncx, int ncy,
-                      double sx, double sy, double* __restrict__ wrapped, int* __restrict__ col_of,
-                      int* __restrict__ col_count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    // fractional wrapping in FP64: x - L floor(x / L)
-    const double wx = x[3 * i] - lx * floor(x[3 * i] / lx);
-    const double wy = x[3 * i + 1] - ly * floor(x[3 * i + 1] / ly);
-    const double wz = x[3 * i + 2] - lz * floor(x[3 * i + 2] / lz);
-    wrapped[3 * i] = wx;
-    wrapped[3 * i + 1] = wy;
-    wrapped[3 * i + 2] = wz;
-    int cx = static_cast<int>(wx / sx), cy = static_cast<int>(wy / sy);
-    cx = cx < 0 ? 0 : (cx >= ncx ? ncx - 1 : cx);
-    cy = cy < 0 ? 0 : (cy >= ncy ? ncy - 1 : cy);
-    const int c = cx + ncx * cy;
-    col_of[i] = c;
-    atomicAdd(col_count + c, 1);
+// Column binning without global atomics: each block takes a contiguous chunk
+// of atoms, histograms their columns in LDS and stores its histogram row;
+// the scan derives every (block, column) offset from those rows, and the
+// scatter repeats the per-block LDS counting to place the atoms.
+constexpr int kBinThreads = 1024;
+constexpr int kMaxCols = 8192;
+
+__global__ __launch_bounds__(kBinThreads) void k_bin(const double* __restrict__ x, int n, int chunk, double lx,
+                                                     double ly, double lz, int ncx, int ncy, double sx, double sy,
+                                                     double* __restrict__ wrapped, int* __restrict__ col_of,
+                                                     int* __restrict__ block_hist) {
+    __shared__ int hist[kMaxCols];
+    const int ncol = ncx * ncy;
+    for (int c = threadIdx.x; c < ncol; c += blockDim.x) hist[c] = 0;
+    __syncthreads();
+    const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        // fractional wrapping in FP64: x - L floor(x / L)
+        const double wx = x[3 * i] - lx * floor(x[3 * i] / lx);
+        const double wy = x[3 * i + 1] - ly * floor(x[3 * i + 1] / ly);
+        const double wz = x[3 * i + 2] - lz * floor(x[3 * i + 2] / lz);
+        wrapped[3 * i] = wx;
+        wrapped[3 * i + 1] = wy;
+        wrapped[3 * i + 2] = wz;
+        int cx = static_cast<int>(wx / sx), cy = static_cast<int>(wy / sy);
+        cx = cx < 0 ? 0 : (cx >= ncx ? ncx - 1 : cx);
+        cy = cy < 0 ? 0 : (cy >= ncy ? ncy - 1 : cy);
+        const int c = cx + ncx * cy;
+        col_of[i] = c;
+        atomicAdd(hist + c, 1);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < ncol; c += blockDim.x) block_hist[static_cast<size_t>(blockIdx.x) * ncol + c] = hist[c];
 }
 
-// Exclusive scans of the column atom counts and cluster counts (one block).
-__global__ void k_col_scan(const int* __restrict__ col_count, int ncol, int* __restrict__ col_start,
-                           int* __restrict__ col_cstart, unsigned int* flags) {
+// One block: column totals, exclusive scans of atom and cluster counts, and
+// the start of every (block, column) run.
+__global__ __launch_bounds__(1024) void k_col_scan(const int* __restrict__ block_hist, int nblk, int ncol,
+                                                   int* __restrict__ col_start, int* __restrict__ col_cstart,
+                                                   int* __restrict__ block_off, unsigned int* flags) {
     __shared__ int s_a[1024], s_c[1024];
     __shared__ int carry_a, carry_c;
     if (threadIdx.x == 0) { carry_a = 0; carry_c = 0; }
     __syncthreads();
     for (int base = 0; base < ncol; base += 1024) {
         const int i = base + threadIdx.x;
-        const int a = i < ncol ? col_count[i] : 0;
+        int a = 0;
+        if (i < ncol)
+            for (int b = 0; b < nblk; ++b) a += block_hist[static_cast<size_t>(b) * ncol + i];
         if (a > kColMax) atomicOr(flags + 2, 1u);
         s_a[threadIdx.x] = a;
         s_c[threadIdx.x] = (a + kCluster - 1) / kCluster;
@@ -73,6 +92,11 @@ __global__ void k_col_scan(const int* __restrict__ col_count, int ncol, int* __r
         if (i < ncol) {
             col_start[i + 1] = carry_a + s_a[threadIdx.x];
             col_cstart[i + 1] = carry_c + s_c[threadIdx.x];
+            int run = carry_a + s_a[threadIdx.x] - a;  // exclusive start of column i
+            for (int b = 0; b < nblk; ++b) {
+                block_off[static_cast<size_t>(b) * ncol + i] = run;
+                run += block_hist[static_cast<size_t>(b) * ncol + i];
+            }
         }
         __syncthreads();
         if (threadIdx.x == 1023) {
@@ -87,12 +111,14 @@ __global__ void k_col_scan(const int* __restrict__ col_count, int ncol, int* __r
     }
 }
 
-__global__ void k_scatter(int n, const int* __restrict__ col_of, const int* __restrict__ col_start,
-                          int* __restrict__ col_fill, int* __restrict__ col_tmp) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = col_of[i];
-    col_tmp[col_start[c] + atomicAdd(col_fill + c, 1)] = i;
+__global__ __launch_bounds__(kBinThreads) void k_scatter(int n, int chunk, int ncol, const int* __restrict__ col_of,
+                                                         const int* __restrict__ block_off,
+                                                         int* __restrict__ col_tmp) {
+    __shared__ int off[kMaxCols];
+    for (int c = threadIdx.x; c < ncol; c += blockDim.x) off[c] = block_off[static_cast<size_t>(blockIdx.x) * ncol + c];
+    __syncthreads();
+    const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) col_tmp[atomicAdd(off + col_of[i], 1)] = i;
 }
 
 // One block per column: rank the column's atoms by (z, index) and write the
@@ -139,7 +165,7 @@ template <typename real>
 __global__ void k_cluster_setup(const int* __restrict__ ncl_dev, int max_clusters, const int* __restrict__ slot_atom,
                                 const double* __restrict__ wrapped, const double* __restrict__ charge,
                                 const int* __restrict__ type, int pad_type, typename Vec4<real>::T* __restrict__ pos,
-                                int* __restrict__ slot_type, double4* __restrict__ origin,
+                                int2* __restrict__ slot_meta, double4* __restrict__ origin,
                                 float4* __restrict__ half) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int cl = slot / kCluster;
@@ -170,11 +196,11 @@ __global__ void k_cluster_setup(const int* __restrict__ ncl_dev, int max_cluster
         p.y = static_cast<real>(y - oy);
         p.z = static_cast<real>(z - oz);
         p.w = static_cast<real>(charge[atom]);
-        slot_type[slot] = type[atom];
+        slot_meta[slot] = make_int2(type[atom], atom);
     } else {
         p.x = p.y = p.z = static_cast<real>(kFarAway);
         p.w = 0;
-        slot_type[slot] = pad_type;
+        slot_meta[slot] = make_int2(pad_type, -1);
     }
     pos[slot] = p;
     if ((slot & (kCluster - 1)) == 0) {
@@ -349,12 +375,13 @@ __global__ void k_excl_extent(const double* __restrict__ x, int n1, const int2* 
 
 }  // namespace
 
+int sort_blocks(int n) { return std::max(1, std::min(64, (n + 2047) / 2048)); }
+
 hipError_t launch_excl_extent(mdgpu_ctx* c) {
     const Topo& t = c->topo;
     const int n = t.n_excl_pairs + t.n_pairs14;
-    hipError_t e = hipMemsetAsync(c->work.excl_rmax, 0, sizeof(float), c->stream);
-    if (e != hipSuccess || n == 0) return e;
-    const double* x = c->work.coords;
+    if (n == 0) return hipSuccess;
+    const double* x = c->cur_x;
     hipLaunchKernelGGL(k_excl_extent, dim3((n + 255) / 256), dim3(256), 0, c->stream, x, t.n_excl_pairs,
                        t.excl_pairs, t.n_pairs14, t.pairs14, c->grid.box[0], c->grid.box[1], c->grid.box[2],
                        c->work.excl_rmax);
@@ -366,15 +393,13 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed) {
     const Grid& g = c->grid;
     const int n = c->topo.n_atoms;
     const int ncol = g.ncx * g.ncy;
-    hipError_t e;
-    if ((e = hipMemsetAsync(w.col_count, 0, sizeof(int) * (ncol + 1), c->stream)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(w.col_fill, 0, sizeof(int) * (ncol + 1), c->stream)) != hipSuccess) return e;
-    const int nb = (n + 255) / 256;
-    hipLaunchKernelGGL(k_bin, dim3(nb), dim3(256), 0, c->stream, coords, n, g.box[0], g.box[1], g.box[2], g.ncx,
-                       g.ncy, g.sx, g.sy, w.wrapped, w.col_of, w.col_count);
-    hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, c->stream, w.col_count, ncol, w.col_start, w.col_cstart,
-                       w.flags);
-    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, c->stream, n, w.col_of, w.col_start, w.col_fill,
+    if (ncol > kMaxCols) return hipErrorInvalidValue;
+    const int nblk = sort_blocks(n), chunk = (n + nblk - 1) / nblk;
+    hipLaunchKernelGGL(k_bin, dim3(nblk), dim3(kBinThreads), 0, c->stream, coords, n, chunk, g.box[0], g.box[1],
+                       g.box[2], g.ncx, g.ncy, g.sx, g.sy, w.wrapped, w.col_of, w.block_hist);
+    hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, c->stream, w.block_hist, nblk, ncol, w.col_start,
+                       w.col_cstart, w.block_off, w.flags);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kBinThreads), 0, c->stream, n, chunk, ncol, w.col_of, w.block_off,
                        w.col_tmp);
     hipLaunchKernelGGL(k_col_sort, dim3(ncol), dim3(256), 0, c->stream, w.col_start, w.col_cstart, w.col_tmp,
                        w.wrapped, w.slot_atom, w.cluster_col);
@@ -384,11 +409,11 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed) {
     if (mixed) {
         hipLaunchKernelGGL(k_cluster_setup<float>, dim3((nslots + 255) / 256), dim3(256), 0, c->stream, ncl_dev,
                            max_cl, w.slot_atom, w.wrapped, c->topo.charge, c->topo.type, c->topo.n_types,
-                           w.slot_pos_f, w.slot_type, w.cl_origin, w.cl_half);
+                           w.slot_pos_f, w.slot_meta, w.cl_origin, w.cl_half);
     } else {
         hipLaunchKernelGGL(k_cluster_setup<double>, dim3((nslots + 255) / 256), dim3(256), 0, c->stream, ncl_dev,
                            max_cl, w.slot_atom, w.wrapped, c->topo.charge, c->topo.type, c->topo.n_types,
-                           w.slot_pos_d, w.slot_type, w.cl_origin, w.cl_half);
+                           w.slot_pos_d, w.slot_meta, w.cl_origin, w.cl_half);
     }
     return hipGetLastError();
 }
