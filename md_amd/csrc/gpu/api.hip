@@ -127,11 +127,11 @@ int mdgpu_create(int device, mdgpu_ctx** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (auto& ev : c->ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.flags), 4 * sizeof(unsigned int));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.energies), (mdg::E_NSLOTS + 4) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.flags), mdg::kFlags * sizeof(unsigned int));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.energies), (mdg::E_NSLOTS + mdg::kFlags) * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.excl_rmax), sizeof(float));
     // zero once; afterwards the final reduction clears them for the next evaluation
-    if (e == hipSuccess) e = hipMemset(c->work.flags, 0, 4 * sizeof(unsigned int));
+    if (e == hipSuccess) e = hipMemset(c->work.flags, 0, mdg::kFlags * sizeof(unsigned int));
     if (e == hipSuccess) e = hipMemset(c->work.excl_rmax, 0, sizeof(float));
     if (e != hipSuccess) {
         mdgpu_destroy(c);
@@ -585,7 +585,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double* __restr
         return;
     }
     if (b == mdg::E_NSLOTS) {
-        if (t < 4) {
+        if (t < mdg::kFlags) {
             out_buf[mdg::E_NSLOTS + t] = static_cast<double>(flags[t]);
             flags[t] = 0u;
         }
@@ -655,13 +655,16 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
                            w.col_cstart + c->n_cols);
         HIP_TRY(hipGetLastError(), "reduce");
         mark(MDGPU_K_REDUCE, false);
-        double e[mdg::E_NSLOTS + 4];
+        double e[mdg::E_NSLOTS + mdg::kFlags];
         HIP_TRY(hipMemcpyAsync(e, w.energies, sizeof e, hipMemcpyDeviceToHost, c->stream), "read energies");
         HIP_TRY(hipStreamSynchronize(c->stream), "evaluation");
-        const unsigned int flags[4] = {static_cast<unsigned int>(e[mdg::E_NSLOTS]),
-                                       static_cast<unsigned int>(e[mdg::E_NSLOTS + 1]),
-                                       static_cast<unsigned int>(e[mdg::E_NSLOTS + 2]),
-                                       static_cast<unsigned int>(e[mdg::E_NSLOTS + 3])};
+        unsigned int flags[mdg::kFlags];
+        for (int k = 0; k < mdg::kFlags; ++k) flags[k] = static_cast<unsigned int>(e[mdg::E_NSLOTS + k]);
+        if (flags[4]) {
+            // a cluster is too wide for the minimum-image search: redo with the general one
+            c->search_general = true;
+            continue;
+        }
         if (flags[2]) return set_error(c, MDGPU_ERR_UNSUPPORTED, "a sort column holds more than 2048 atoms");
         if (flags[3]) return set_error(c, MDGPU_ERR_UNSUPPORTED, "too many tiles near excluded pairs in one cluster");
         if (flags[1]) {

@@ -10,6 +10,8 @@
 
 namespace mdg {
 
+constexpr int kFlags = 8;
+
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
@@ -79,9 +81,10 @@ struct Work {
     // energies: partial rows, one per block of each producing kernel
     size_t cap_partial = 0;
     double* partial = nullptr;       // [rows * E_NSLOTS]
-    double* energies = nullptr;      // [E_NSLOTS + 4] final energies, then the flags as doubles
+    double* energies = nullptr;      // [E_NSLOTS + kFlags] final energies, then the flags as doubles
     double* force_out = nullptr;     // [3N] FP64 forces for host-buffer evaluations
-    unsigned int* flags = nullptr;   // [4]: 0 fixed overflow, 1 list overflow, 2 column overflow, 3 spare
+    unsigned int* flags = nullptr;   // [kFlags]: 0 fixed overflow, 1 list overflow, 2 column overflow,
+                                     // 3 near-tile overflow, 4 cluster too wide for the fast search
     float* excl_rmax = nullptr;      // [1] largest excluded-pair distance (as float)
     // PME
     int pme_k[3] = {0, 0, 0};
@@ -130,6 +133,7 @@ struct mdgpu_ctx {
     int n_cols = 0;
     int split = 4;                  // waves per i-cluster list in the pair kernel
     const double* cur_x = nullptr;  // raw coordinates of the evaluation in progress (device)
+    bool search_general = false;    // use the general (per-shift) cluster search
     std::vector<void*> topo_allocs;
     // host copies of the per-type-pair LJ parameters (the device table also
     // holds cutoff-dependent shift constants, rebuilt by mdgpu_set_params)

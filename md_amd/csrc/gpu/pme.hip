@@ -102,26 +102,38 @@ __global__ void k_spread(int n, const double* __restrict__ wrapped, const double
 }
 
 // ------------------------------------------------------------ LDS FFT -----
-// Batched Stockham FFT of nl lines of length n held in LDS as buf[i*nl + l].
-// Passes of radix 4, then 2, then 3; p is the length already transformed.
-// Each pass reads src and writes dst; the result ends in the returned buffer.
+// Batched Stockham FFT of nl lines (nl a power of two) of length n held in
+// LDS as buf[i*nl + l].  Passes of radix 4, then 2, then 3; p is the length
+// already transformed.  The twiddle table (e^{-2 pi i m/n}, m < n) is in LDS.
+// Divisions by the runtime p use a float reciprocal with an exact integer
+// correction (i < 2^16).  Each pass reads src and writes dst; the result ends
+// in the returned buffer.
+__device__ __forceinline__ int div_small(int i, int p, float inv_p) {
+    int q = static_cast<int>(static_cast<float>(i) * inv_p);
+    q -= (q * p > i);
+    q += ((q + 1) * p <= i);
+    return q;
+}
+
 template <typename real>
-__device__ typename C2<real>::T* lds_fft(typename C2<real>::T* src, typename C2<real>::T* dst, int n, int nl,
+__device__ typename C2<real>::T* lds_fft(typename C2<real>::T* src, typename C2<real>::T* dst, int n, int lognl,
                                          const typename C2<real>::T* __restrict__ tw, int sign) {
     using c2 = typename C2<real>::T;
+    const int nl = 1 << lognl;
     int rem = n, p = 1;
     while (rem > 1) {
         const int r = (rem % 4 == 0) ? 4 : ((rem % 2 == 0) ? 2 : 3);
         const int nr = n / r, stride_tw = n / (p * r);
+        const float inv_p = 1.0f / static_cast<float>(p);
         for (int t = threadIdx.x; t < nr * nl; t += blockDim.x) {
-            const int l = t % nl, i = t / nl;
-            const int k = i % p;
+            const int l = t & (nl - 1), i = t >> lognl;
+            const int k = i - div_small(i, p, inv_p) * p;
             c2 u[4];
             for (int q = 0; q < r; ++q) {
                 c2 v = src[(i + q * nr) * nl + l];
                 if (q > 0 && k > 0) {
-                    const c2 w = tw[(q * k * stride_tw) % n];
-                    const real wi = sign < 0 ? w.y : -w.y;   // table holds e^{-2 pi i m/n}
+                    const c2 w = tw[q * k * stride_tw];   // q k n/(p r) < n
+                    const real wi = sign < 0 ? w.y : -w.y;
                     v = {v.x * w.x - v.y * wi, v.x * wi + v.y * w.x};
                 }
                 u[q] = v;
@@ -145,8 +157,7 @@ __device__ typename C2<real>::T* lds_fft(typename C2<real>::T* src, typename C2<
                 const c2 a = {u[1].x + u[2].x, u[1].y + u[2].y}, b = {u[1].x - u[2].x, u[1].y - u[2].y};
                 o[0] = {u[0].x + a.x, u[0].y + a.y};
                 const c2 m = {u[0].x + c * a.x, u[0].y + c * a.y};
-                // i s b
-                const c2 isb = {-s * b.y, s * b.x};
+                const c2 isb = {-s * b.y, s * b.x};  // i s b
                 o[1] = {m.x + isb.x, m.y + isb.y};
                 o[2] = {m.x - isb.x, m.y - isb.y};
             }
@@ -163,14 +174,25 @@ __device__ typename C2<real>::T* lds_fft(typename C2<real>::T* src, typename C2<
     return src;
 }
 
+// Twiddles of one axis into LDS (after the two line buffers).
+template <typename real>
+__device__ __forceinline__ const typename C2<real>::T* stage_twiddles(typename C2<real>::T* s_tw,
+                                                                      const typename C2<real>::T* __restrict__ tw,
+                                                                      int n) {
+    for (int m = threadIdx.x; m < n; m += blockDim.x) s_tw[m] = tw[m];
+    return s_tw;
+}
+
 // --------------------------------------------------------------- z r2c ----
 template <typename real>
-__global__ void k_fft_z_r2c(PmeGeom g, int nl, double* grid_d, unsigned long long* grid_fx,
-                            typename C2<real>::T* __restrict__ cg, const typename C2<real>::T* __restrict__ tw) {
+__global__ void k_fft_z_r2c(PmeGeom g, int lognl, double* grid_d, unsigned long long* grid_fx,
+                            typename C2<real>::T* __restrict__ cg, const typename C2<real>::T* __restrict__ tw_g) {
     using c2 = typename C2<real>::T;
     extern __shared__ unsigned char smem[];
+    const int nl = 1 << lognl;
     c2* A = reinterpret_cast<c2*>(smem);
     c2* B = A + g.kz * nl;
+    const c2* tw = stage_twiddles<real>(B + g.kz * nl, tw_g, g.kz);
     const int nlines = g.kx * g.ky;
     const int l0 = blockIdx.x * nl;
     const int nv = min(nl, nlines - l0);
@@ -190,7 +212,7 @@ __global__ void k_fft_z_r2c(PmeGeom g, int nl, double* grid_d, unsigned long lon
         A[i * nl + l] = {v, static_cast<real>(0)};
     }
     __syncthreads();
-    c2* R = lds_fft<real>(A, B, g.kz, nl, tw, -1);
+    c2* R = lds_fft<real>(A, B, g.kz, lognl, tw, -1);
     for (int t = threadIdx.x; t < nl * g.hz; t += blockDim.x) {
         const int l = t / g.hz, k = t % g.hz;
         if (l < nv) cg[static_cast<size_t>(l0 + l) * g.hz + k] = R[k * nl + l];
@@ -199,12 +221,14 @@ __global__ void k_fft_z_r2c(PmeGeom g, int nl, double* grid_d, unsigned long lon
 
 // --------------------------------------------------------------- y pass ---
 template <typename real>
-__global__ void k_fft_y(PmeGeom g, int nl, typename C2<real>::T* __restrict__ cg,
-                        const typename C2<real>::T* __restrict__ tw, int sign) {
+__global__ void k_fft_y(PmeGeom g, int lognl, typename C2<real>::T* __restrict__ cg,
+                        const typename C2<real>::T* __restrict__ tw_g, int sign) {
     using c2 = typename C2<real>::T;
     extern __shared__ unsigned char smem[];
+    const int nl = 1 << lognl;
     c2* A = reinterpret_cast<c2*>(smem);
     c2* B = A + g.ky * nl;
+    const c2* tw = stage_twiddles<real>(B + g.ky * nl, tw_g, g.ky);
     const int x = blockIdx.y, kz0 = blockIdx.x * nl;
     const int nv = min(nl, g.hz - kz0);
     for (int t = threadIdx.x; t < nl * g.ky; t += blockDim.x) {
@@ -212,7 +236,7 @@ __global__ void k_fft_y(PmeGeom g, int nl, typename C2<real>::T* __restrict__ cg
         A[y * nl + l] = l < nv ? cg[(static_cast<size_t>(x) * g.ky + y) * g.hz + kz0 + l] : c2{0, 0};
     }
     __syncthreads();
-    c2* R = lds_fft<real>(A, B, g.ky, nl, tw, sign);
+    c2* R = lds_fft<real>(A, B, g.ky, lognl, tw, sign);
     for (int t = threadIdx.x; t < nl * g.ky; t += blockDim.x) {
         const int l = t % nl, y = t / nl;
         if (l < nv) cg[(static_cast<size_t>(x) * g.ky + y) * g.hz + kz0 + l] = R[y * nl + l];
@@ -221,13 +245,15 @@ __global__ void k_fft_y(PmeGeom g, int nl, typename C2<real>::T* __restrict__ cg
 
 // ----------------------------------------------- x pass + convolution -----
 template <typename real>
-__global__ void k_fft_x_conv(PmeGeom g, int nl, typename C2<real>::T* __restrict__ cg,
-                             const typename C2<real>::T* __restrict__ tw, const double* __restrict__ bmod,
+__global__ void k_fft_x_conv(PmeGeom g, int lognl, typename C2<real>::T* __restrict__ cg,
+                             const typename C2<real>::T* __restrict__ tw_g, const double* __restrict__ bmod,
                              double alpha, double* partial, int row0) {
     using c2 = typename C2<real>::T;
     extern __shared__ unsigned char smem[];
+    const int nl = 1 << lognl;
     c2* A = reinterpret_cast<c2*>(smem);
     c2* B = A + g.kx * nl;
+    const c2* tw = stage_twiddles<real>(B + g.kx * nl, tw_g, g.kx);
     const int y = blockIdx.y, kz0 = blockIdx.x * nl;
     const int nv = min(nl, g.hz - kz0);
     for (int t = threadIdx.x; t < nl * g.kx; t += blockDim.x) {
@@ -235,7 +261,7 @@ __global__ void k_fft_x_conv(PmeGeom g, int nl, typename C2<real>::T* __restrict
         A[x * nl + l] = l < nv ? cg[(static_cast<size_t>(x) * g.ky + y) * g.hz + kz0 + l] : c2{0, 0};
     }
     __syncthreads();
-    c2* S = lds_fft<real>(A, B, g.kx, nl, tw, -1);
+    c2* S = lds_fft<real>(A, B, g.kx, lognl, tw, -1);
     c2* other = S == A ? B : A;
     // G(m) |S|^2, virial, and S <- G S
     const double V = g.lx * g.ly * g.lz;
@@ -273,7 +299,7 @@ __global__ void k_fft_x_conv(PmeGeom g, int nl, typename C2<real>::T* __restrict
         S[x * nl + l] = s;
     }
     __syncthreads();
-    c2* R = lds_fft<real>(S, other, g.kx, nl, tw, +1);
+    c2* R = lds_fft<real>(S, other, g.kx, lognl, tw, +1);
     for (int t = threadIdx.x; t < nl * g.kx; t += blockDim.x) {
         const int l = t % nl, x = t / nl;
         if (l < nv) cg[(static_cast<size_t>(x) * g.ky + y) * g.hz + kz0 + l] = R[x * nl + l];
@@ -285,12 +311,14 @@ __global__ void k_fft_x_conv(PmeGeom g, int nl, typename C2<real>::T* __restrict
 
 // --------------------------------------------------------------- z c2r ----
 template <typename real>
-__global__ void k_fft_z_c2r(PmeGeom g, int nl, const typename C2<real>::T* __restrict__ cg,
-                            const typename C2<real>::T* __restrict__ tw, real* __restrict__ phi) {
+__global__ void k_fft_z_c2r(PmeGeom g, int lognl, const typename C2<real>::T* __restrict__ cg,
+                            const typename C2<real>::T* __restrict__ tw_g, real* __restrict__ phi) {
     using c2 = typename C2<real>::T;
     extern __shared__ unsigned char smem[];
+    const int nl = 1 << lognl;
     c2* A = reinterpret_cast<c2*>(smem);
     c2* B = A + g.kz * nl;
+    const c2* tw = stage_twiddles<real>(B + g.kz * nl, tw_g, g.kz);
     const int nlines = g.kx * g.ky;
     const int l0 = blockIdx.x * nl;
     const int nv = min(nl, nlines - l0);
@@ -309,7 +337,7 @@ __global__ void k_fft_z_c2r(PmeGeom g, int nl, const typename C2<real>::T* __res
         A[k * nl + l] = v;
     }
     __syncthreads();
-    c2* R = lds_fft<real>(A, B, g.kz, nl, tw, +1);
+    c2* R = lds_fft<real>(A, B, g.kz, lognl, tw, +1);
     for (int t = threadIdx.x; t < nl * g.kz; t += blockDim.x) {
         const int l = t / g.kz, i = t % g.kz;
         if (l < nv) phi[static_cast<size_t>(l0) * g.kz + t] = R[i * nl + l].x;
@@ -358,10 +386,16 @@ __global__ void k_gather(int n, const double* __restrict__ wrapped, const double
 }
 
 int lines_per_block(int n, size_t elem) {
-    // two LDS buffers of n x nl complex values within 64 KB
+    // two LDS buffers of n x nl complex values plus n twiddles within 64 KB
     int nl = 16;
-    while (nl > 1 && 2 * static_cast<size_t>(n) * nl * elem > 65536) nl >>= 1;
+    while (nl > 1 && (2 * static_cast<size_t>(n) * nl + n) * elem > 65536) nl >>= 1;
     return nl;
+}
+
+int ilog2(int v) {
+    int l = 0;
+    while ((1 << (l + 1)) <= v) ++l;
+    return l;
 }
 
 template <typename real>
@@ -389,32 +423,32 @@ hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used) {
     mark(MDGPU_K_FFT, true);
     {
         const int nl = lines_per_block(g.kz, sizeof(c2));
-        const size_t shm = 2 * static_cast<size_t>(g.kz) * nl * sizeof(c2);
+        const size_t shm = (2 * static_cast<size_t>(g.kz) * nl + g.kz) * sizeof(c2);
         hipLaunchKernelGGL(k_fft_z_r2c<real>, dim3((g.kx * g.ky + nl - 1) / nl), dim3(kThreads), shm, c->stream, g,
-                           nl, w.grid_d, w.grid_fx, cg, twz);
+                           ilog2(nl), w.grid_d, w.grid_fx, cg, twz);
     }
     const int nly = lines_per_block(g.ky, sizeof(c2));
-    const size_t shy = 2 * static_cast<size_t>(g.ky) * nly * sizeof(c2);
-    hipLaunchKernelGGL(k_fft_y<real>, dim3((g.hz + nly - 1) / nly, g.kx), dim3(kThreads), shy, c->stream, g, nly, cg,
-                       twy, -1);
+    const size_t shy = (2 * static_cast<size_t>(g.ky) * nly + g.ky) * sizeof(c2);
+    hipLaunchKernelGGL(k_fft_y<real>, dim3((g.hz + nly - 1) / nly, g.kx), dim3(kThreads), shy, c->stream, g,
+                       ilog2(nly), cg, twy, -1);
     mark(MDGPU_K_FFT, false);
     mark(MDGPU_K_CONV, true);
     const int nlx = lines_per_block(g.kx, sizeof(c2));
-    const size_t shx = 2 * static_cast<size_t>(g.kx) * nlx * sizeof(c2);
+    const size_t shx = (2 * static_cast<size_t>(g.kx) * nlx + g.kx) * sizeof(c2);
     const dim3 gx((g.hz + nlx - 1) / nlx, g.ky);
-    hipLaunchKernelGGL(k_fft_x_conv<real>, gx, dim3(kThreads), shx, c->stream, g, nlx, cg, twx, w.bsp_mod,
+    hipLaunchKernelGGL(k_fft_x_conv<real>, gx, dim3(kThreads), shx, c->stream, g, ilog2(nlx), cg, twx, w.bsp_mod,
                        c->params.ewald_alpha, w.partial, row0);
     mark(MDGPU_K_CONV, false);
     *rows_used = static_cast<int>(gx.x * gx.y);
     mark(MDGPU_K_COUNT, true);  // inverse passes: the extra event pair
-    hipLaunchKernelGGL(k_fft_y<real>, dim3((g.hz + nly - 1) / nly, g.kx), dim3(kThreads), shy, c->stream, g, nly, cg,
-                       twy, +1);
+    hipLaunchKernelGGL(k_fft_y<real>, dim3((g.hz + nly - 1) / nly, g.kx), dim3(kThreads), shy, c->stream, g,
+                       ilog2(nly), cg, twy, +1);
     real* phi = dbl ? reinterpret_cast<real*>(w.grid_phi_d) : reinterpret_cast<real*>(w.grid_f);
     {
         const int nl = lines_per_block(g.kz, sizeof(c2));
-        const size_t shm = 2 * static_cast<size_t>(g.kz) * nl * sizeof(c2);
+        const size_t shm = (2 * static_cast<size_t>(g.kz) * nl + g.kz) * sizeof(c2);
         hipLaunchKernelGGL(k_fft_z_c2r<real>, dim3((g.kx * g.ky + nl - 1) / nl), dim3(kThreads), shm, c->stream, g,
-                           nl, cg, twz, phi);
+                           ilog2(nl), cg, twz, phi);
     }
     mark(MDGPU_K_COUNT, false);
     mark(MDGPU_K_GATHER, true);

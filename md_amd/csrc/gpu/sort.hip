@@ -355,6 +355,135 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
     }
 }
 
+// Fast search: every candidate j-cluster is taken at its minimum-image shift
+// (per lane, FP64 rint), so one pass over the clusters of each row of
+// columns in reach replaces the per-column, per-shift enumeration.  It is
+// exact when every cluster half-extent h satisfies 2h <= L/2 - r_list in each
+// dimension (then no second image of a cluster can come within r_list);
+// a cluster that breaks the bound sets flags[4] and the host falls back to
+// k_search.
+template <typename real>
+__global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const SearchArgs a, double hlx, double hly,
+                                                                     double hlz) {
+    using R4 = typename Vec4<real>::T;
+    __shared__ int2 s_near[kSearchWaves][kNearCap];
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
+    const int ci = __builtin_amdgcn_readfirstlane(blockIdx.x * kSearchWaves + wv);
+    const int ncl = min(*a.ncl_dev, a.max_clusters);
+    if (ci >= ncl) return;  // whole wave leaves; the kernel has no block barrier
+    const int col = a.cluster_col[ci];
+    const int cx = col % a.ncx, cy = col / a.ncx;
+    const double4 oi = a.origin[ci];
+    const float4 hi = a.half[ci];
+    if (hi.x > hlx || hi.y > hly || hi.z > hlz) {
+        if (lane == 0) {
+            atomicOr(a.flags + 4, 1u);
+            a.list_count[ci] = 0;  // the pair kernel of this attempt reads an empty list
+        }
+        return;
+    }
+    const double ilx = 1.0 / a.lx, ily = 1.0 / a.ly, ilz = 1.0 / a.lz;
+    const size_t base_i = static_cast<size_t>(ci) * a.cap;
+    R4* delta = reinterpret_cast<R4*>(a.list_delta);
+    double rex = 0.0;
+    if (a.amber) rex = static_cast<double>(*a.excl_rmax) + 0.01;
+    const double rex2 = rex * rex;
+    int count = 0, nnear = 0;
+    const unsigned long long self_mask = 0x0080C0E0F0F8FCFEull;  // bit 8*ti+tj set iff tj > ti
+    const int ny = min(2 * a.dcy + 1, a.ncy);
+    const bool yall = ny == a.ncy;
+    const int y0 = yall ? 0 : cy - a.dcy;
+    for (int ry = 0; ry < ny; ++ry) {
+        int jy = (y0 + ry) % a.ncy;
+        jy += jy < 0 ? a.ncy : 0;
+        // columns of this row within reach: |dx| <= dxr, where the x gap
+        // (|dx| - 1) sx and the y gap (|dy| - 1) sy stay inside r_list
+        int dxr = a.dcx;
+        if (!yall) {
+            const int dy = ry - a.dcy;
+            const double gy = fmax(0.0, (abs(dy) - 1) * a.sy);
+            const double rem = a.rl2 - gy * gy;
+            if (rem <= 0.0) continue;
+            dxr = min(a.dcx, static_cast<int>(sqrt(rem) / a.sx) + 1);
+        }
+        const int nx = min(2 * dxr + 1, a.ncx);
+        const int x0 = nx == a.ncx ? 0 : cx - dxr;
+        // the x window of this row as at most two contiguous column segments
+        int xa = x0 % a.ncx;
+        xa += xa < 0 ? a.ncx : 0;
+        const int first = min(nx, a.ncx - xa);
+        for (int seg = 0; seg < 2; ++seg) {
+            const int c_lo = seg == 0 ? xa : 0;
+            const int c_n = seg == 0 ? first : nx - first;
+            if (c_n <= 0) continue;
+            const int rowc = a.ncx * jy;
+            const int b0 = a.col_cstart[rowc + c_lo], b1 = a.col_cstart[rowc + c_lo + c_n];
+            for (int b = b0; b < b1; b += kWave) {
+                const int cj = b + lane;
+                const bool valid = cj < b1 && cj >= ci;
+                double ddx = 0, ddy = 0, ddz = 0, d2 = 1e300;
+                if (valid) {
+                    const double4 oj = a.origin[cj];
+                    const float4 hj = a.half[cj];
+                    ddx = oj.x - oi.x;
+                    ddy = oj.y - oi.y;
+                    ddz = oj.z - oi.z;
+                    ddx -= a.lx * rint(ddx * ilx);
+                    ddy -= a.ly * rint(ddy * ily);
+                    ddz -= a.lz * rint(ddz * ilz);
+                    const double qx = fmax(0.0, fabs(ddx) - (double)(hi.x + hj.x));
+                    const double qy = fmax(0.0, fabs(ddy) - (double)(hi.y + hj.y));
+                    const double qz = fmax(0.0, fabs(ddz) - (double)(hi.z + hj.z));
+                    d2 = qx * qx + qy * qy + qz * qz;
+                }
+                const bool cand = valid && d2 < a.rl2;
+                const unsigned long long bal = __ballot(cand);
+                if (bal == 0) continue;
+                const int pos = count + __popcll(bal & ((1ull << lane) - 1ull));
+                const bool near = a.amber && cand && d2 < rex2;
+                if (cand && pos < a.cap) {
+                    const size_t e = base_i + pos;
+                    a.list_j[e] = cj;
+                    if (!near) a.list_mask[e] = (cj == ci) ? self_mask : ~0ull;
+                    R4 dl;
+                    dl.x = static_cast<real>(-ddx);
+                    dl.y = static_cast<real>(-ddy);
+                    dl.z = static_cast<real>(-ddz);
+                    dl.w = 0;
+                    delta[e] = dl;
+                }
+                const unsigned long long nbal = __ballot(near && pos < a.cap);
+                if (near && pos < a.cap) {
+                    const int nr = nnear + __popcll(nbal & ((1ull << lane) - 1ull));
+                    if (nr < kNearCap) s_near[wv][nr] = make_int2(pos, cj);
+                }
+                nnear += __popcll(nbal);
+                count += __popcll(bal);
+            }
+        }
+    }
+    if (count > a.cap) {
+        if (lane == 0) atomicOr(a.flags + 1, 1u);
+        count = a.cap;
+    }
+    if (lane == 0) a.list_count[ci] = count;
+    if (!a.amber) return;
+    if (nnear > kNearCap) {
+        if (lane == 0) atomicOr(a.flags + 3, 1u);
+        nnear = kNearCap;
+    }
+    const int ti = lane >> 3, tj = lane & 7;
+    const int ai = a.slot_atom[ci * kCluster + ti];
+    __builtin_amdgcn_wave_barrier();
+    for (int q = 0; q < nnear; ++q) {
+        const int2 en = s_near[wv][q];
+        const int aj = a.slot_atom[en.y * kCluster + tj];
+        const bool ex = ai >= 0 && aj >= 0 && pair_excluded(a.excl_start, a.excl_list, ai, aj);
+        const unsigned long long bal = __ballot(ex);
+        if (lane == 0) a.list_mask[base_i + en.x] = ((en.y == ci) ? self_mask : ~0ull) & ~bal;
+    }
+}
+
 // Largest minimum-image distance over the excluded and 1-4 pairs (bounds the
 // tiles that can hold an excluded pair).
 __global__ void k_excl_extent(const double* __restrict__ x, int n1, const int2* __restrict__ p1, int n2,
@@ -452,8 +581,19 @@ hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber) {
     a.excl_list = c->topo.excl_list;
     a.excl_rmax = w.excl_rmax;
     const int nb = (c->n_clusters + kSearchWaves - 1) / kSearchWaves;
-    if (mixed) hipLaunchKernelGGL(k_search<float>, dim3(nb), dim3(kSearchWaves * kWave), 0, c->stream, a);
-    else hipLaunchKernelGGL(k_search<double>, dim3(nb), dim3(kSearchWaves * kWave), 0, c->stream, a);
+    // half-extent bound of the minimum-image search in each dimension
+    auto hlim = [&](double L) { return 0.5 * (0.5 * L - rl); };
+    const double hx = hlim(g.box[0]), hy = hlim(g.box[1]), hz = hlim(g.box[2]);
+    const bool fast = !c->search_general && hx > 1.0 && hy > 1.0 && hz > 1.0;
+    if (fast) {
+        if (mixed)
+            hipLaunchKernelGGL(k_search_mi<float>, dim3(nb), dim3(kSearchWaves * kWave), 0, c->stream, a, hx, hy, hz);
+        else
+            hipLaunchKernelGGL(k_search_mi<double>, dim3(nb), dim3(kSearchWaves * kWave), 0, c->stream, a, hx, hy, hz);
+    } else {
+        if (mixed) hipLaunchKernelGGL(k_search<float>, dim3(nb), dim3(kSearchWaves * kWave), 0, c->stream, a);
+        else hipLaunchKernelGGL(k_search<double>, dim3(nb), dim3(kSearchWaves * kWave), 0, c->stream, a);
+    }
     return hipGetLastError();
 }
 
