@@ -55,6 +55,17 @@ def read_traffic(profile_dir):
     return None
 
 
+def copy_calibration():
+    import torch
+    n = 64 * 1024 * 1024  # float32 elements: 256 MiB read + 256 MiB written per copy
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,7 +73,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device-to-device copy three times (the reference "
+                         "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X; "
               "multi-GPU runs are not part of this build (see DESIGN.md on sharding).", file=sys.stderr)
@@ -116,22 +133,37 @@ def main():
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
+    # the timed region: the pair kernel is bracketed by two HIP events on the
+    # evaluator's stream in every step, so its launch duration is measured live
+    ev32.set_timing(2)
     t32 = timed(ev32, f32, args.steps, args.warmup)
+    nb_total, nb_launches = ev32.timing()["nonbonded"]
+    ev32.set_timing(0)
+    nb_ms = nb_total / max(nb_launches, 1)
     steps64 = args.fp64_steps or max(10, args.steps // 5)
     t64 = timed(ev64, f64, steps64, max(2, args.warmup // 5))
     rate32 = args.steps / t32
     rate64 = steps64 / t64
 
-    # per-kernel HIP-event timings (separate pass, so events do not perturb
-    # the throughput measurement)
-    ev32.set_timing(True)
+    # host-buffer path (mdgpu_eval: coordinates in and forces out over PCIe)
+    xh = top.coords.copy()
+    for _ in range(3):
+        ev32.evaluate(xh, box)
+    nh = max(10, args.steps // 5)
+    t0 = time.perf_counter()
+    for _ in range(nh):
+        ev32.evaluate(xh, box)
+    rate_host = nh / (time.perf_counter() - t0)
+
+    # per-kernel-group HIP-event breakdown (separate pass: twenty events per
+    # step would perturb the throughput figure)
+    ev32.set_timing(1)
     nt = max(20, args.steps // 4)
     for _ in range(nt):
         ev32.evaluate_device(x.data_ptr(), box, f32.data_ptr())
     torch.cuda.synchronize()
     tim = ev32.timing()
-    ev32.set_timing(False)
-    nb_ms = tim["nonbonded"][0] / max(tim["nonbonded"][1], 1)
+    ev32.set_timing(0)
     flops = PAIR_FLOP * counts["pairs"]
     achieved = flops / (nb_ms * 1e-3) / 1e12
     traffic = read_traffic(os.path.join(ROOT, "profiles"))
@@ -145,7 +177,7 @@ def main():
         while True:
             oracle.evaluate(top.desc, p64, top.coords, box)
             nrep += 1
-            if time.perf_counter() - t0 > 10.0 or nrep >= 3:
+            if time.perf_counter() - t0 > 12.0 or nrep >= 30:
                 break
         dt = time.perf_counter() - t0
         cpu = {"value": nrep / dt, "unit": "evals/s", "cores": 1, "kind": "port",
@@ -168,6 +200,7 @@ def main():
         "config": WORKLOAD,
         "fp64_evals_per_s": rate64,
         "speedup_mixed_vs_fp64": rate32 / rate64,
+        "host_buffer_evals_per_s": rate_host,
         "force_rel_rms_mixed_vs_fp64": rms,
         "energy_rel_err_mixed_vs_fp64": term_err,
         "work": counts,
@@ -181,6 +214,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": traffic,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "avg_launch_ms": nb_ms,
             "algorithmic": f"{PAIR_FLOP} flop x {counts['pairs']} pairs within 12 A per launch",
         },
         "cpu_baseline": cpu,

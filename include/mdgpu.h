@@ -206,9 +206,11 @@ int mdgpu_eval_device(mdgpu_ctx* ctx, const double* d_coords, const double box[3
                       mdgpu_energies* out_e, double* d_forces, double* out_virial);
 
 /* Per-kernel timing with HIP events on the context stream (for roofline
- * reporting).  mdgpu_set_timing(ctx, 1) resets and enables; each later
- * evaluation adds the event-measured duration of every kernel group. */
-int mdgpu_set_timing(mdgpu_ctx* ctx, int enable);
+ * reporting).  mdgpu_set_timing(ctx, mode) resets the totals; mode 1 makes
+ * each later evaluation add the event-measured duration of every kernel
+ * group, mode 2 only that of the pair kernel (two events per evaluation),
+ * mode 0 turns timing off. */
+int mdgpu_set_timing(mdgpu_ctx* ctx, int mode);
 int mdgpu_get_timing(mdgpu_ctx* ctx, int kernel_id, double* total_ms, int64_t* launches);
 /* Work counts of the last evaluation: [0] atom pairs within cutoff (set when
  * counting is enabled), [1] cluster-pair tiles, [2] clusters, [3] atoms. */

@@ -619,8 +619,9 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
         const double* x = coords_are_work ? w.coords : d_coords;
         c->cur_x = x;
         auto& T = c->timing;
+        auto timed = [&](int k) { return T.mode == 1 || (T.mode == 2 && k == MDGPU_K_NONBONDED); };
         auto mark = [&](int k, bool start) {
-            if (T.on) (void)hipEventRecord(c->ev[2 * k + (start ? 0 : 1)], c->stream);
+            if (timed(k)) (void)hipEventRecord(c->ev[2 * k + (start ? 0 : 1)], c->stream);
         };
         mark(MDGPU_K_SORT, true);
         if (amber && c->topo.has_excl) HIP_TRY(mdg::launch_excl_extent(c), "excl extent");
@@ -673,8 +674,9 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
             continue;
         }
         if (flags[0]) return set_error(c, MDGPU_ERR_OVERFLOW, "fixed-point force accumulator overflow");
-        if (T.on) {
+        if (T.mode != 0) {
             for (int k = 0; k <= MDGPU_K_COUNT; ++k) {
+                if (!timed(k == MDGPU_K_COUNT ? MDGPU_K_FFT : k)) continue;
                 if (k == MDGPU_K_SPREAD || k == MDGPU_K_FFT || k == MDGPU_K_CONV || k == MDGPU_K_GATHER ||
                     k == MDGPU_K_COUNT) {
                     if (!(amber && w.pme_k[0] > 0)) continue;
@@ -754,10 +756,11 @@ int mdgpu_eval_device(mdgpu_ctx* c, const double* d_coords, const double box[3],
     return eval_impl(c, d_coords, box, out_e, d_forces, out_virial, false);
 }
 
-int mdgpu_set_timing(mdgpu_ctx* c, int enable) {
+int mdgpu_set_timing(mdgpu_ctx* c, int mode) {
     if (!c) return MDGPU_ERR_ARG;
+    if (mode < 0 || mode > 2) return set_error(c, MDGPU_ERR_ARG, "timing mode must be 0, 1 or 2");
     c->timing = mdg::Timing{};
-    c->timing.on = enable != 0;
+    c->timing.mode = mode;
     return MDGPU_OK;
 }
 

@@ -109,7 +109,7 @@ struct Grid {
 };
 
 struct Timing {
-    bool on = false;
+    int mode = 0;  // 0 off, 1 every kernel group, 2 the pair kernel only
     double total_ms[MDGPU_K_COUNT] = {};
     long long launches[MDGPU_K_COUNT] = {};
 };

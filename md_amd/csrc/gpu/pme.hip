@@ -413,7 +413,7 @@ hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used) {
     const c2* twz = tw + g.kx + g.ky;
     auto& T = c->timing;
     auto mark = [&](int k, bool start) {
-        if (T.on) (void)hipEventRecord(c->ev[2 * k + (start ? 0 : 1)], c->stream);
+        if (T.mode == 1) (void)hipEventRecord(c->ev[2 * k + (start ? 0 : 1)], c->stream);
     };
     const int nb = (n + kThreads - 1) / kThreads;
     mark(MDGPU_K_SPREAD, true);

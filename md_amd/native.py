@@ -322,8 +322,9 @@ class Evaluator:
                                                v.ctypes.data_as(P_F64)))
         return e.as_dict(), v.reshape(3, 3)
 
-    def set_timing(self, on: bool):
-        self._check(self.lib.mdgpu_set_timing(self.ctx, 1 if on else 0))
+    def set_timing(self, mode):
+        """0/False off, 1/True every kernel group, 2 the pair kernel only."""
+        self._check(self.lib.mdgpu_set_timing(self.ctx, int(mode)))
 
     def timing(self) -> dict:
         out = {}
