@@ -139,6 +139,13 @@ private:
     double box_alpha_ = 90.0, box_beta_ = 90.0, box_gamma_ = 90.0;
 };
 
+// CMAP patch coefficients of n_sets grids of res x res values (16 per cell,
+// the reference's table map, CreateNonBondedFromParsed.cpp:46-117), and the
+// transpose of that linear map (gradient with respect to grid values from the
+// gradient with respect to coefficients).  Not in the reference's API.
+void cmap_patch_coefficients(int res, int n_sets, const double* grid, double* coeff);
+void cmap_patch_coefficients_adjoint(int res, int n_sets, const double* d_coeff, double* d_grid);
+
 }  // namespace AmberTopology
 }  // namespace md
 

@@ -205,6 +205,46 @@ int mdgpu_eval(mdgpu_ctx* ctx, const double* coords, const double box[3],
 int mdgpu_eval_device(mdgpu_ctx* ctx, const double* d_coords, const double box[3],
                       mdgpu_energies* out_e, double* d_forces, double* out_virial);
 
+/* Parameter gradients of a weighted energy sum (see mdgpu_adjoint).  Host
+ * arrays owned by the caller; a NULL pointer skips that block, every other
+ * array is overwritten.  Lengths follow the uploaded mdgpu_system_desc:
+ * charge [n_atoms]; lj_* and lj14_* [n_lj] (per LJ table entry, i.e. per
+ * type pair); bond_* [n_bond_types]; ub_* [n_ub_types]; angle_*
+ * [n_angle_types]; dihedral_* [n_dihedral_types]; improper_*
+ * [n_improper_types]; cmap_coeff [cmap_n_sets * res * res * 16], the
+ * gradient with respect to the patch coefficients (mdhost_cmap_grid_grad maps
+ * it to the grid values through the transpose of the patch table). */
+typedef struct {
+    double* charge;
+    double* lj_a;
+    double* lj_b;
+    double* lj14_a;
+    double* lj14_b;
+    double* bond_k;
+    double* bond_r0;
+    double* ub_k;
+    double* ub_r0;
+    double* angle_k;
+    double* angle_theta0;
+    double* dihedral_k;
+    double* dihedral_phase;
+    double* improper_k;
+    double* improper_psi0;
+    double* cmap_coeff;
+} mdgpu_param_grads;
+
+/* Adjoint: sum_t w_t dE_t/dtheta at one configuration, for the parameters
+ * of mdgpu_param_grads, in AMBER semantics (the reference has no gradient
+ * with respect to parameters; this is new).  weights holds one weight per
+ * energy term (fields of mdgpu_energies); weights->total is added to every
+ * term of the total.  The pair list and every pair contribution are
+ * recomputed (nothing is kept from an evaluation).  Arithmetic follows the
+ * context's precision form; sums are accumulated in 64-bit fixed point and
+ * converted in FP64 (deterministic).
+ *   coords [3N] host coordinates; box [3]. */
+int mdgpu_adjoint(mdgpu_ctx* ctx, const double* coords, const double box[3], const mdgpu_energies* weights,
+                  mdgpu_param_grads* out);
+
 /* Per-kernel timing with HIP events on the context stream (for roofline
  * reporting).  mdgpu_set_timing(ctx, mode) resets the totals; mode 1 makes
  * each later evaluation add the event-measured duration of every kernel

@@ -31,6 +31,12 @@ int mdhost_box(const mdhost_system* h, double box[3]);
 /* Per-dihedral flags: bit 0 = no 1-4 (negative 3rd index), bit 1 = improper. */
 const int32_t* mdhost_dihedral_flags(const mdhost_system* h);
 void mdhost_free(mdhost_system* h);
+/* CMAP patch coefficients [n_sets*res*res*16] from grids [n_sets*res*res]
+ * (the reference's 16 x 16 table map, CreateNonBondedFromParsed.cpp:46-117),
+ * and the transposed map: d_grid from d_coeff (for mdgpu_adjoint's CMAP
+ * output).  No counterpart in the reference's API. */
+int mdhost_cmap_coefficients(int res, int n_sets, const double* grid, double* coeff);
+int mdhost_cmap_grid_grad(int res, int n_sets, const double* d_coeff, double* d_grid);
 /* Message of the last failure on this thread. */
 const char* mdhost_last_error(void);
 

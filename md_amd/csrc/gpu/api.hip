@@ -147,6 +147,8 @@ int mdgpu_destroy(mdgpu_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_topo(c);
     free_work(c);
+    if (c->adj.acc) (void)hipFree(c->adj.acc);
+    if (c->adj.pot_rec) (void)hipFree(c->adj.pot_rec);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -352,6 +354,19 @@ extern "C" int mdgpu_upload_system(mdgpu_ctx* c, const mdgpu_system_desc* s) {
     double q2 = 0.0;
     for (int i = 0; i < n; ++i) q2 += s->charge[i] * s->charge[i];
     t.sum_q2 = q2;
+    c->host_charge.assign(s->charge, s->charge + n);
+    t.n_lj = s->n_lj;
+    t.n_bond_types = s->n_bond_types;
+    t.n_ub_types = s->n_ub_types;
+    t.n_angle_types = s->n_angle_types;
+    t.n_dih_types = s->n_dihedral_types;
+    t.n_imp_types = s->n_improper_types;
+    {
+        std::vector<int> pidx(tp2, -1);
+        for (int i = 0; i < T; ++i)
+            for (int j = 0; j < T; ++j) pidx[static_cast<size_t>(i) * (T + 1) + j] = s->nb_index[i * T + j] - 1;
+        HIP_TRY(upload(c, &t.lj_pidx, pidx.data(), pidx.size()), "upload LJ index map");
+    }
     c->have_system = true;
     if (c->have_params) return build_lj_table(c);
     return MDGPU_OK;
@@ -754,6 +769,150 @@ int mdgpu_eval_device(mdgpu_ctx* c, const double* d_coords, const double box[3],
                       double* d_forces, double* out_virial) {
     if (!c || !d_coords || !box) return set_error(c, MDGPU_ERR_ARG, "null argument");
     return eval_impl(c, d_coords, box, out_e, d_forces, out_virial, false);
+}
+
+namespace {
+
+// Offsets of the adjoint accumulators for the uploaded system; allocates on
+// first use or when the system grew.
+int ensure_adjoint(mdgpu_ctx* c) {
+    const Topo& t = c->topo;
+    mdg::AdjointBuf& b = c->adj;
+    const size_t n = static_cast<size_t>(t.n_atoms), nlj = static_cast<size_t>(std::max(t.n_lj, 0));
+    size_t o = 0;
+    b.q_dir = o; o += n;
+    b.q_excl = o; o += n;
+    b.q_14 = o; o += n;
+    b.lj_a = o; o += nlj;
+    b.lj_b = o; o += nlj;
+    b.lj14_a = o; o += nlj;
+    b.lj14_b = o; o += nlj;
+    b.bonded = o;
+    o += 2 * static_cast<size_t>(t.n_bond_types + t.n_ub_types + t.n_angle_types + t.n_dih_types + t.n_imp_types);
+    b.cmap = o;
+    if (t.n_cmaps > 0) o += static_cast<size_t>(t.cmap_n_sets) * t.cmap_res * t.cmap_res * 16;
+    b.flag = o; o += 1;
+    b.total = o;
+    if (b.cap < o) {
+        if (b.acc) (void)hipFree(b.acc);
+        b.acc = nullptr;
+        b.cap = 0;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&b.acc), o * sizeof(unsigned long long)), "alloc adjoint");
+        b.cap = o;
+    }
+    if (b.cap_pot < n) {
+        if (b.pot_rec) (void)hipFree(b.pot_rec);
+        b.pot_rec = nullptr;
+        b.cap_pot = 0;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&b.pot_rec), std::max<size_t>(n, 1) * sizeof(double)),
+                "alloc adjoint");
+        b.cap_pot = n;
+    }
+    return MDGPU_OK;
+}
+
+}  // namespace
+
+int mdgpu_adjoint(mdgpu_ctx* c, const double* coords, const double box[3], const mdgpu_energies* weights,
+                  mdgpu_param_grads* out) {
+    if (!c || !coords || !box || !weights || !out) return set_error(c, MDGPU_ERR_ARG, "null argument");
+    if (!c->have_system || !c->have_params) return set_error(c, MDGPU_ERR_STATE, "upload a system and set params first");
+    const mdgpu_params& p = c->params;
+    if (p.semantics != MDGPU_SEM_AMBER)
+        return set_error(c, MDGPU_ERR_UNSUPPORTED, "the adjoint is defined for AMBER semantics only");
+    HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
+    const bool mixed = p.precision == MDGPU_PREC_MIXED;
+    const Topo& t = c->topo;
+    const int n = t.n_atoms;
+    int st = ensure_adjoint(c);
+    if (st != MDGPU_OK) return st;
+    mdg::AdjointBuf& b = c->adj;
+    std::vector<unsigned long long> acc(b.total);
+    std::vector<double> pot(n, 0.0);
+    const bool pme = c->work.pme_k[0] > 0 || p.pme_grid[0] > 0;
+    bool done = false;
+    for (int attempt = 0; attempt < 4 && !done; ++attempt) {
+        st = prepare(c, box);
+        if (st != MDGPU_OK) return st;
+        Work& w = c->work;
+        HIP_TRY(hipMemcpyAsync(w.coords, coords, 3 * static_cast<size_t>(n) * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream), "upload coords");
+        c->cur_x = w.coords;
+        HIP_TRY(hipMemsetAsync(b.acc, 0, b.total * sizeof(unsigned long long), c->stream), "clear adjoint");
+        if (t.has_excl) HIP_TRY(mdg::launch_excl_extent(c), "excl extent");
+        HIP_TRY(mdg::launch_sort(c, w.coords, mixed), "sort");
+        HIP_TRY(mdg::launch_pairsearch(c, mixed, true), "pair search");
+        HIP_TRY(mdg::launch_pair_adjoint(c, mixed), "pair adjoint");
+        HIP_TRY(mdg::launch_corrections_adjoint(c), "corrections adjoint");
+        HIP_TRY(mdg::launch_bonded_adjoint(c, w.coords, mixed), "bonded adjoint");
+        if (pme) HIP_TRY(mdg::launch_pme_potential(c, mixed, 0, b.pot_rec), "pme potential");
+        unsigned int flags[mdg::kFlags];
+        HIP_TRY(hipMemcpyAsync(flags, w.flags, sizeof flags, hipMemcpyDeviceToHost, c->stream), "read flags");
+        HIP_TRY(hipMemcpyAsync(acc.data(), b.acc, b.total * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                               c->stream), "read adjoint");
+        if (pme)
+            HIP_TRY(hipMemcpyAsync(pot.data(), b.pot_rec, static_cast<size_t>(n) * sizeof(double),
+                                   hipMemcpyDeviceToHost, c->stream), "read potential");
+        HIP_TRY(hipMemsetAsync(w.flags, 0, sizeof flags, c->stream), "clear flags");
+        HIP_TRY(hipStreamSynchronize(c->stream), "adjoint");
+        if (flags[4]) {
+            c->search_general = true;
+            continue;
+        }
+        if (flags[2]) return set_error(c, MDGPU_ERR_UNSUPPORTED, "a sort column holds more than 2048 atoms");
+        if (flags[3]) return set_error(c, MDGPU_ERR_UNSUPPORTED, "too many tiles near excluded pairs in one cluster");
+        if (flags[1]) {
+            w.list_cap *= 2;
+            continue;
+        }
+        done = true;
+    }
+    if (!done) return set_error(c, MDGPU_ERR_STATE, "pair list did not settle");
+    if (acc[b.flag]) return set_error(c, MDGPU_ERR_OVERFLOW, "adjoint fixed-point accumulator overflow");
+
+    // weights: each term's own weight plus the weight of the total
+    mdgpu_energies wt = *weights;
+    const double wtot = weights->total;
+    wt.bond += wtot; wt.angle += wtot; wt.dihedral += wtot; wt.urey_bradley += wtot; wt.improper += wtot;
+    wt.cmap += wtot; wt.vdw += wtot; wt.ewald_direct += wtot; wt.ewald_self += wtot; wt.ewald_recip += wtot;
+    wt.ewald_excl += wtot; wt.vdw14 += wtot; wt.ee14 += wtot;
+    auto val = [&](size_t k, double scale) { return static_cast<double>(static_cast<long long>(acc[k])) / scale; };
+    if (out->charge) {
+        const double self = -2.0 * mdg::kCoulomb * p.ewald_alpha / std::sqrt(mdg::kPi);
+        for (int i = 0; i < n; ++i)
+            out->charge[i] = wt.ewald_direct * val(b.q_dir + i, mdg::kAdjScaleQ) +
+                             wt.ewald_excl * val(b.q_excl + i, mdg::kAdjScaleQ) +
+                             wt.ee14 * val(b.q_14 + i, mdg::kAdjScaleQ) + wt.ewald_self * self * c->host_charge[i] +
+                             (pme ? wt.ewald_recip * pot[i] : 0.0);
+    }
+    auto fill = [&](double* dst, size_t off, int cnt, int stride, double wgt, double scale) {
+        if (!dst) return;
+        for (int k = 0; k < cnt; ++k) dst[k] = wgt * val(off + static_cast<size_t>(stride) * k, scale);
+    };
+    fill(out->lj_a, b.lj_a, t.n_lj, 1, wt.vdw, mdg::kAdjScaleLJ);
+    fill(out->lj_b, b.lj_b, t.n_lj, 1, wt.vdw, mdg::kAdjScaleLJ);
+    fill(out->lj14_a, b.lj14_a, t.n_lj, 1, wt.vdw14, mdg::kAdjScaleLJ);
+    fill(out->lj14_b, b.lj14_b, t.n_lj, 1, wt.vdw14, mdg::kAdjScaleLJ);
+    size_t o = b.bonded;
+    fill(out->bond_k, o, t.n_bond_types, 2, wt.bond, mdg::kAdjScaleB);
+    fill(out->bond_r0, o + 1, t.n_bond_types, 2, wt.bond, mdg::kAdjScaleB);
+    o += 2 * static_cast<size_t>(t.n_bond_types);
+    fill(out->ub_k, o, t.n_ub_types, 2, wt.urey_bradley, mdg::kAdjScaleB);
+    fill(out->ub_r0, o + 1, t.n_ub_types, 2, wt.urey_bradley, mdg::kAdjScaleB);
+    o += 2 * static_cast<size_t>(t.n_ub_types);
+    fill(out->angle_k, o, t.n_angle_types, 2, wt.angle, mdg::kAdjScaleB);
+    fill(out->angle_theta0, o + 1, t.n_angle_types, 2, wt.angle, mdg::kAdjScaleB);
+    o += 2 * static_cast<size_t>(t.n_angle_types);
+    fill(out->dihedral_k, o, t.n_dih_types, 2, wt.dihedral, mdg::kAdjScaleB);
+    fill(out->dihedral_phase, o + 1, t.n_dih_types, 2, wt.dihedral, mdg::kAdjScaleB);
+    o += 2 * static_cast<size_t>(t.n_dih_types);
+    fill(out->improper_k, o, t.n_imp_types, 2, wt.improper, mdg::kAdjScaleB);
+    fill(out->improper_psi0, o + 1, t.n_imp_types, 2, wt.improper, mdg::kAdjScaleB);
+    if (out->cmap_coeff) {
+        const int nc = t.n_cmaps > 0 ? t.cmap_n_sets * t.cmap_res * t.cmap_res * 16 : 0;
+        fill(out->cmap_coeff, b.cmap, nc, 1, wt.cmap, mdg::kAdjScaleLJ);
+    }
+    return MDGPU_OK;
 }
 
 int mdgpu_set_timing(mdgpu_ctx* c, int mode) {

@@ -45,7 +45,29 @@ struct Topo {
     int n_excl_pairs = 0; int2* excl_pairs = nullptr;
     int n_pairs14 = 0; int2* pairs14 = nullptr; double* scee14 = nullptr;
     double sum_q2 = 0.0;                   // for the self energy
+    // parameter counts and the pair-table index map, for the adjoint
+    int n_lj = 0, n_bond_types = 0, n_ub_types = 0, n_angle_types = 0, n_dih_types = 0, n_imp_types = 0;
+    int* lj_pidx = nullptr;                // (T+1)^2: LJ table entry of a type pair, -1 if none
 };
+
+// Fixed-point accumulators of the adjoint (one allocation; offsets in
+// elements).  Each block is the gradient of one energy term before its
+// weight is applied.
+struct AdjointBuf {
+    size_t cap = 0;                        // elements allocated
+    unsigned long long* acc = nullptr;
+    double* pot_rec = nullptr;             // [N] reciprocal potential (FP64, plain stores)
+    size_t cap_pot = 0;
+    size_t q_dir = 0, q_excl = 0, q_14 = 0;          // [N] each
+    size_t lj_a = 0, lj_b = 0, lj14_a = 0, lj14_b = 0;  // [n_lj] each
+    size_t bonded = 0;                               // [2 x (bond, UB, angle, dihedral, improper types)]
+    size_t cmap = 0;                                 // [sets x res^2 x 16]
+    size_t flag = 0;                                 // overflow flag (as an element)
+    size_t total = 0;
+};
+constexpr double kAdjScaleQ = 68719476736.0;        // 2^36: potentials
+constexpr double kAdjScaleLJ = 17592186044416.0;    // 2^44: LJ and CMAP coefficient sums
+constexpr double kAdjScaleB = 1099511627776.0;      // 2^40: bonded constants and equilibria
 
 // Work arrays that depend on the atom count / box.
 struct Work {
@@ -127,6 +149,8 @@ struct mdgpu_ctx {
     mdg::Work work;
     mdg::Grid grid;
     mdg::Timing timing;
+    mdg::AdjointBuf adj;
+    std::vector<double> host_charge;
     bool counting = false;
     long long counts[4] = {0, 0, 0, 0};
     int n_clusters = 0;
@@ -154,6 +178,11 @@ hipError_t launch_corrections(mdgpu_ctx* c, bool mixed, int partial_row0);
 hipError_t launch_excl_extent(mdgpu_ctx* c);
 hipError_t launch_pme(mdgpu_ctx* c, bool mixed, int partial_row0, int* rows_used);
 hipError_t launch_reduce(mdgpu_ctx* c, bool mixed, int n_rows, double* out_forces);
+// adjoint kernels (AMBER semantics)
+hipError_t launch_pair_adjoint(mdgpu_ctx* c, bool mixed);
+hipError_t launch_bonded_adjoint(mdgpu_ctx* c, const double* coords, bool mixed);
+hipError_t launch_corrections_adjoint(mdgpu_ctx* c);
+hipError_t launch_pme_potential(mdgpu_ctx* c, bool mixed, int partial_row0, double* pot);
 
 // rows of the partial-energy buffer each kernel needs for a given launch
 int nonbonded_rows(const mdgpu_ctx* c);

@@ -385,6 +385,32 @@ __global__ void k_gather(int n, const double* __restrict__ wrapped, const double
     }
 }
 
+// Reciprocal potential at each atom, dE_rec/dq_i = sum_k theta_i(k) phi(k)
+// (the adjoint of the spreading), one thread per atom, plain stores.
+template <typename real>
+__global__ void k_potential(int n, const double* __restrict__ wrapped, PmeGeom g, const real* __restrict__ phi,
+                            double* __restrict__ pot) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int fl[3];
+    double w[3];
+    frac(wrapped, i, g, fl, w);
+    real th[3][4], dth[3][4];
+    for (int d = 0; d < 3; ++d) bspline4<real>(static_cast<real>(w[d]), th[d], dth[d]);
+    double s = 0.0;
+    for (int a = 0; a < 4; ++a) {
+        const int ix = wrapk(fl[0] - a, g.kx);
+        for (int b = 0; b < 4; ++b) {
+            const int iy = wrapk(fl[1] - b, g.ky);
+            const real* row = phi + (static_cast<size_t>(ix) * g.ky + iy) * g.kz;
+            real sab = 0;
+            for (int c = 0; c < 4; ++c) sab += th[2][c] * row[wrapk(fl[2] - c, g.kz)];
+            s += static_cast<double>(th[0][a] * th[1][b] * sab);
+        }
+    }
+    pot[i] = s;
+}
+
 int lines_per_block(int n, size_t elem) {
     // two LDS buffers of n x nl complex values plus n twiddles within 64 KB
     int nl = 16;
@@ -399,7 +425,7 @@ int ilog2(int v) {
 }
 
 template <typename real>
-hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used) {
+hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used, double* pot) {
     using c2 = typename C2<real>::T;
     Work& w = c->work;
     const int n = c->topo.n_atoms;
@@ -452,8 +478,11 @@ hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used) {
     }
     mark(MDGPU_K_COUNT, false);
     mark(MDGPU_K_GATHER, true);
-    hipLaunchKernelGGL(k_gather<real>, dim3(nb), dim3(kThreads), 0, c->stream, n, w.wrapped, c->topo.charge, g, phi,
-                       w.force_d, FixedAcc{w.force_fx, fixed_scale(c->params.semantics), w.flags});
+    if (pot)
+        hipLaunchKernelGGL(k_potential<real>, dim3(nb), dim3(kThreads), 0, c->stream, n, w.wrapped, g, phi, pot);
+    else
+        hipLaunchKernelGGL(k_gather<real>, dim3(nb), dim3(kThreads), 0, c->stream, n, w.wrapped, c->topo.charge, g,
+                           phi, w.force_d, FixedAcc{w.force_fx, fixed_scale(c->params.semantics), w.flags});
     mark(MDGPU_K_GATHER, false);
     return hipGetLastError();
 }
@@ -471,7 +500,13 @@ int pme_rows(const mdgpu_ctx* c) {
 hipError_t launch_pme(mdgpu_ctx* c, bool mixed, int row0, int* rows_used) {
     *rows_used = 0;
     if (c->work.pme_k[0] == 0) return hipSuccess;
-    return mixed ? run_pme<float>(c, row0, rows_used) : run_pme<double>(c, row0, rows_used);
+    return mixed ? run_pme<float>(c, row0, rows_used, nullptr) : run_pme<double>(c, row0, rows_used, nullptr);
+}
+
+hipError_t launch_pme_potential(mdgpu_ctx* c, bool mixed, int row0, double* pot) {
+    int used = 0;
+    if (c->work.pme_k[0] == 0) return hipSuccess;
+    return mixed ? run_pme<float>(c, row0, &used, pot) : run_pme<double>(c, row0, &used, pot);
 }
 
 }  // namespace mdg

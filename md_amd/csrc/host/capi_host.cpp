@@ -60,6 +60,18 @@ const int32_t* mdhost_dihedral_flags(const mdhost_system* h) { return h ? h->d.d
 
 void mdhost_free(mdhost_system* h) { delete h; }
 
+int mdhost_cmap_coefficients(int res, int n_sets, const double* grid, double* coeff) {
+    if (res <= 0 || n_sets < 0 || (!grid && n_sets) || (!coeff && n_sets)) return MDGPU_ERR_ARG;
+    md::AmberTopology::cmap_patch_coefficients(res, n_sets, grid, coeff);
+    return MDGPU_OK;
+}
+
+int mdhost_cmap_grid_grad(int res, int n_sets, const double* d_coeff, double* d_grid) {
+    if (res <= 0 || n_sets < 0 || (!d_coeff && n_sets) || (!d_grid && n_sets)) return MDGPU_ERR_ARG;
+    md::AmberTopology::cmap_patch_coefficients_adjoint(res, n_sets, d_coeff, d_grid);
+    return MDGPU_OK;
+}
+
 const char* mdhost_last_error(void) { return g_error.c_str(); }
 
 }  // extern "C"

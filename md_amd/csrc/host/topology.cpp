@@ -357,20 +357,51 @@ void AmberTopology::build_cmap_coefficients() {
         const auto& g = cmap_grids_[s];
         if (res <= 0 || g.size() < static_cast<size_t>(res) * res)
             throw std::runtime_error("CMAP grid shorter than resolution^2");
-        auto wrap = [res](int i) { return i < 0 ? i + res : (i >= res ? i - res : i); };
-        for (int row = 0; row < res; ++row) {
+        const size_t off = cmap_coeff_.size();
+        cmap_coeff_.resize(off + static_cast<size_t>(res) * res * 16);
+        cmap_patch_coefficients(res, 1, g.data(), cmap_coeff_.data() + off);
+    }
+}
+
+// Cell (row, col) of a res x res grid: a = T e / 16 with e[4c + r] =
+// grid(row + r - 1, col + c - 1), indices periodic.
+void cmap_patch_coefficients(int res, int n_sets, const double* grid, double* coeff) {
+    auto wrap = [res](int i) { return i < 0 ? i + res : (i >= res ? i - res : i); };
+    for (int s = 0; s < n_sets; ++s) {
+        const double* g = grid + static_cast<size_t>(s) * res * res;
+        for (int row = 0; row < res; ++row)
             for (int col = 0; col < res; ++col) {
                 double e[16];
                 for (int c = 0; c < 4; ++c)
                     for (int r = 0; r < 4; ++r)
                         e[4 * c + r] = g[static_cast<size_t>(wrap(row + r - 1)) * res + wrap(col + c - 1)];
+                double* a = coeff + ((static_cast<size_t>(s) * res + row) * res + col) * 16;
                 for (int k = 0; k < 16; ++k) {
                     double acc = 0.0;
                     for (int n = 0; n < 16; ++n) acc += kCmapTable[k][n] * e[n] / 16;
-                    cmap_coeff_.push_back(acc);
+                    a[k] = acc;
                 }
             }
-        }
+    }
+}
+
+// Transpose of cmap_patch_coefficients: d_grid = sum over cells of T^T d_a / 16
+// scattered back to the 4 x 4 neighbourhood of each cell.
+void cmap_patch_coefficients_adjoint(int res, int n_sets, const double* d_coeff, double* d_grid) {
+    auto wrap = [res](int i) { return i < 0 ? i + res : (i >= res ? i - res : i); };
+    for (int s = 0; s < n_sets; ++s) {
+        double* g = d_grid + static_cast<size_t>(s) * res * res;
+        for (int i = 0; i < res * res; ++i) g[i] = 0.0;
+        for (int row = 0; row < res; ++row)
+            for (int col = 0; col < res; ++col) {
+                const double* da = d_coeff + ((static_cast<size_t>(s) * res + row) * res + col) * 16;
+                for (int n = 0; n < 16; ++n) {
+                    double acc = 0.0;
+                    for (int k = 0; k < 16; ++k) acc += kCmapTable[k][n] * da[k] / 16;
+                    const int c = n / 4, r = n % 4;
+                    g[static_cast<size_t>(wrap(row + r - 1)) * res + wrap(col + c - 1)] += acc;
+                }
+            }
     }
 }
 

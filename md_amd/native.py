@@ -73,6 +73,45 @@ class Energies(C.Structure):
         return {n: getattr(self, n) for n in ENERGY_TERMS}
 
 
+# parameter-gradient blocks of mdgpu_param_grads and the desc count giving each length
+GRAD_FIELDS = [("charge", "n_atoms"), ("lj_a", "n_lj"), ("lj_b", "n_lj"), ("lj14_a", "n_lj"),
+               ("lj14_b", "n_lj"), ("bond_k", "n_bond_types"), ("bond_r0", "n_bond_types"),
+               ("ub_k", "n_ub_types"), ("ub_r0", "n_ub_types"), ("angle_k", "n_angle_types"),
+               ("angle_theta0", "n_angle_types"), ("dihedral_k", "n_dihedral_types"),
+               ("dihedral_phase", "n_dihedral_types"), ("improper_k", "n_improper_types"),
+               ("improper_psi0", "n_improper_types"), ("cmap_coeff", "cmap_coeff")]
+
+
+class ParamGrads(C.Structure):
+    _fields_ = [(name, P_F64) for name, _ in GRAD_FIELDS]
+
+
+def grad_lengths(desc: SystemDesc) -> dict:
+    out = {}
+    for name, cnt in GRAD_FIELDS:
+        if cnt == "cmap_coeff":
+            out[name] = desc.cmap_n_sets * desc.cmap_resolution ** 2 * 16 if desc.n_cmaps > 0 else 0
+        else:
+            out[name] = int(getattr(desc, cnt))
+    return out
+
+
+def alloc_grads(desc: SystemDesc):
+    """numpy arrays for every gradient block and a ParamGrads pointing at them."""
+    arrs = {k: np.zeros(max(n, 0)) for k, n in grad_lengths(desc).items()}
+    g = ParamGrads()
+    for k, a in arrs.items():
+        setattr(g, k, a.ctypes.data_as(P_F64) if a.size else None)
+    return arrs, g
+
+
+def energies_struct(d: dict) -> Energies:
+    e = Energies()
+    for k in ENERGY_TERMS:
+        setattr(e, k, float(d.get(k, 0.0)))
+    return e
+
+
 def _load(name: str) -> C.CDLL:
     path = os.path.join(_HERE, name)
     if not os.path.exists(path):
@@ -101,6 +140,7 @@ def gpu_lib() -> C.CDLL:
         lib.mdgpu_get_timing.argtypes = [C.c_void_p, C.c_int, P_F64, C.POINTER(C.c_int64)]
         lib.mdgpu_get_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         lib.mdgpu_set_counting.argtypes = [C.c_void_p, C.c_int]
+        lib.mdgpu_adjoint.argtypes = [C.c_void_p, P_F64, P_F64, C.POINTER(Energies), C.POINTER(ParamGrads)]
         _gpu = lib
     return _gpu
 
@@ -121,8 +161,32 @@ def host_lib() -> C.CDLL:
         lib.mdhost_dihedral_flags.restype = P_I32
         lib.mdhost_free.argtypes = [C.c_void_p]
         lib.mdhost_last_error.restype = C.c_char_p
+        lib.mdhost_cmap_coefficients.argtypes = [C.c_int, C.c_int, P_F64, P_F64]
+        lib.mdhost_cmap_grid_grad.argtypes = [C.c_int, C.c_int, P_F64, P_F64]
         _host = lib
     return _host
+
+
+def cmap_coefficients(grid, res: int, n_sets: int) -> np.ndarray:
+    """Patch coefficients [n_sets*res*res*16] of CMAP grids [n_sets*res*res]."""
+    g = np.ascontiguousarray(grid, dtype=np.float64).reshape(-1)
+    out = np.zeros(n_sets * res * res * 16)
+    if n_sets:
+        st = host_lib().mdhost_cmap_coefficients(res, n_sets, g.ctypes.data_as(P_F64), out.ctypes.data_as(P_F64))
+        if st != 0:
+            raise ValueError("mdhost_cmap_coefficients: bad arguments")
+    return out
+
+
+def cmap_grid_grad(d_coeff, res: int, n_sets: int) -> np.ndarray:
+    """Gradient with respect to the grid values from that of the coefficients."""
+    dc = np.ascontiguousarray(d_coeff, dtype=np.float64).reshape(-1)
+    out = np.zeros(n_sets * res * res)
+    if n_sets:
+        st = host_lib().mdhost_cmap_grid_grad(res, n_sets, dc.ctypes.data_as(P_F64), out.ctypes.data_as(P_F64))
+        if st != 0:
+            raise ValueError("mdhost_cmap_grid_grad: bad arguments")
+    return out
 
 
 def _arr(ptr, n, dtype):
@@ -321,6 +385,20 @@ class Evaluator:
                                                C.byref(e), C.c_void_p(forces_ptr),
                                                v.ctypes.data_as(P_F64)))
         return e.as_dict(), v.reshape(3, 3)
+
+    def adjoint(self, coords, box, weights: dict, desc: SystemDesc) -> dict:
+        """sum_t w_t dE_t/dtheta at one configuration (AMBER semantics).
+
+        ``weights`` maps energy-term names to weights (``total`` adds to every
+        term of the total); returns a dict of gradient arrays (GRAD_FIELDS).
+        """
+        x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+        b = np.ascontiguousarray(box, dtype=np.float64)
+        w = energies_struct(weights)
+        arrs, g = alloc_grads(desc)
+        self._check(self.lib.mdgpu_adjoint(self.ctx, x.ctypes.data_as(P_F64), b.ctypes.data_as(P_F64),
+                                           C.byref(w), C.byref(g)))
+        return arrs
 
     def set_timing(self, mode):
         """0/False off, 1/True every kernel group, 2 the pair kernel only."""

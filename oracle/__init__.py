@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from md_amd.native import Energies, Params, SystemDesc, P_F64
+from md_amd.native import Energies, ParamGrads, Params, SystemDesc, P_F64, alloc_grads, energies_struct
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _lib = None
@@ -37,6 +37,11 @@ def lib() -> C.CDLL:
         L.oracle_pme_recip.argtypes = [C.c_int, P_F64, P_F64, P_F64, C.c_double, C.POINTER(C.c_int), C.c_int,
                                        P_F64, P_F64]
         L.oracle_pme_recip.restype = C.c_double
+        L.oracle_pme_potential.argtypes = [C.c_int, P_F64, P_F64, P_F64, C.c_double, C.POINTER(C.c_int),
+                                           C.c_int, P_F64]
+        L.oracle_pme_potential.restype = C.c_double
+        L.oracle_param_grads.argtypes = [C.POINTER(SystemDesc), C.POINTER(Params), P_F64, P_F64,
+                                         C.POINTER(Energies), C.POINTER(ParamGrads)]
         L.oracle_bspline.argtypes = [C.c_int, C.c_double, P_F64, P_F64]
         L.oracle_coulomb_constant.restype = C.c_double
         _lib = L
@@ -103,3 +108,26 @@ def bspline(order, w):
 
 def coulomb_constant() -> float:
     return lib().oracle_coulomb_constant()
+
+
+def pme_potential(q, coords, box, alpha, grid, order=4):
+    """(E_rec, dE_rec/dq_i)."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    g = (C.c_int * 3)(*grid)
+    pot = np.zeros(q.size)
+    e = lib().oracle_pme_potential(q.size, _p(q), _p(x), _p(b), alpha, g, order, _p(pot))
+    return e, pot
+
+
+def param_grads(desc: SystemDesc, params: Params, coords, box, weights: dict) -> dict:
+    """sum_t w_t dE_t/dtheta (AMBER semantics) as a dict of arrays."""
+    x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    arrs, g = alloc_grads(desc)
+    w = energies_struct(weights)
+    st = lib().oracle_param_grads(C.byref(desc), C.byref(params), _p(x), _p(b), C.byref(w), C.byref(g))
+    if st != 0:
+        raise RuntimeError("oracle_param_grads failed")
+    return arrs

@@ -819,8 +819,24 @@ static void dft_axis(double* re, double* im, const int K[3], int axis, int sign)
  * E = 1/2 sum_m G(m) |F(Q)(m)|^2, G(m) = kC exp(-pi^2 m^2/a^2) / (pi V m^2) B(m),
  * B(m) = prod_d 1/|sum_{k=0}^{p-2} M_p(k+1) e^{2 pi i m_d k / K_d}|^2;
  * F_i = -q_i sum_k grad theta(k) (G * Q)(k). */
+static double pme_core(int n, const double* q, const double* x, const double box[3], double alpha, const int K[3],
+                       int order, double* f, double* vir, double* pot);
+
 double oracle_pme_recip(int n, const double* q, const double* x, const double box[3], double alpha, const int K[3],
                         int order, double* f, double* vir) {
+    return pme_core(n, q, x, box, alpha, K, order, f, vir, NULL);
+}
+
+/* Reciprocal potential at each atom, dE_rec/dq_i = sum_k theta_i(k) (G * Q)(k):
+ * E_rec = 1/2 Q^T C Q with C the convolution by G, so its q-gradient is the
+ * convolved grid sampled with the spreading weights. */
+double oracle_pme_potential(int n, const double* q, const double* x, const double box[3], double alpha, const int K[3],
+                            int order, double* pot) {
+    return pme_core(n, q, x, box, alpha, K, order, NULL, NULL, pot);
+}
+
+static double pme_core(int n, const double* q, const double* x, const double box[3], double alpha, const int K[3],
+                       int order, double* f, double* vir, double* pot) {
     const double kc = (double)(SQRT_KC * SQRT_KC);
     size_t G = (size_t)K[0] * K[1] * K[2];
     double* re = (double*)calloc(G, sizeof(double));
@@ -897,7 +913,7 @@ double oracle_pme_recip(int n, const double* q, const double* x, const double bo
         vir[2] = vir[6] = W[4];
         vir[5] = vir[7] = W[5];
     }
-    if (f) {
+    if (f || pot) {
         for (int ax = 0; ax < 3; ++ax) dft_axis(re, im, K, ax, +1);
         for (int i = 0; i < n; ++i) {
             for (int d = 0; d < 3; ++d) {
@@ -906,7 +922,7 @@ double oracle_pme_recip(int n, const double* q, const double* x, const double bo
                 oracle_bspline(order, u - fl, th[d], dth[d]);
                 base[d] = fl;
             }
-            double fx = 0, fy = 0, fz = 0;
+            double fx = 0, fy = 0, fz = 0, ph = 0;
             for (int a = 0; a < order; ++a) {
                 int kx = ((base[0] - a) % K[0] + K[0]) % K[0];
                 for (int b = 0; b < order; ++b) {
@@ -917,12 +933,16 @@ double oracle_pme_recip(int n, const double* q, const double* x, const double bo
                         fx += dth[0][a] * th[1][b] * th[2][c] * phi;
                         fy += th[0][a] * dth[1][b] * th[2][c] * phi;
                         fz += th[0][a] * th[1][b] * dth[2][c] * phi;
+                        ph += th[0][a] * th[1][b] * th[2][c] * phi;
                     }
                 }
             }
-            f[3 * i] -= q[i] * fx * K[0] / box[0];
-            f[3 * i + 1] -= q[i] * fy * K[1] / box[1];
-            f[3 * i + 2] -= q[i] * fz * K[2] / box[2];
+            if (pot) pot[i] = ph;
+            if (f) {
+                f[3 * i] -= q[i] * fx * K[0] / box[0];
+                f[3 * i + 1] -= q[i] * fy * K[1] / box[1];
+                f[3 * i + 2] -= q[i] * fz * K[2] / box[2];
+            }
         }
     }
     for (int d = 0; d < 3; ++d) free(bm[d]);
@@ -1020,5 +1040,183 @@ int oracle_eval(const mdgpu_system_desc* s, const mdgpu_params* p, const double*
         e->ewald_recip = oracle_pme_recip(n, s->charge, x, box, p->ewald_alpha, p->pme_grid, p->pme_order, f, vir);
     }
     e->total = bonded + e->vdw + e->vdw14 + e->ewald_direct + e->ewald_recip + e->ewald_self + e->ewald_excl + e->ee14;
+    return 0;
+}
+
+/* --------------------------------------------------- parameter gradients - */
+
+/* Weight of term t in a weighted sum: w.t plus w.total for every term of the
+ * AMBER total. */
+static void eff_weights(const mdgpu_energies* w, mdgpu_energies* o) {
+    *o = *w;
+    o->bond += w->total; o->angle += w->total; o->dihedral += w->total; o->urey_bradley += w->total;
+    o->improper += w->total; o->cmap += w->total; o->vdw += w->total; o->ewald_direct += w->total;
+    o->ewald_self += w->total; o->ewald_recip += w->total; o->ewald_excl += w->total;
+    o->vdw14 += w->total; o->ee14 += w->total;
+}
+
+static void zero_if(double* a, size_t n) { if (a) memset(a, 0, n * sizeof(double)); }
+
+/* Switched or force-shifted LJ energy of one pair for unit (A, B): the
+ * energy is linear in A and B (NonBondedEnergy.cpp:79-131), so dE/dA and
+ * dE/dB are the energies at (1, 0) and (0, 1). */
+static void lj_unit(const mdgpu_params* p, double r2, double* ea, double* eb) {
+    double r = sqrt(r2), i6 = 1.0 / (r2 * r2 * r2), i12 = i6 * i6;
+    if (p->lj_switch > 0.0) {
+        double S, dS;
+        lj_switch(r, p->lj_switch, p->lj_cutoff, 1, &S, &dS);
+        *ea = S * i12;
+        *eb = -S * i6;
+    } else {
+        double rc = p->lj_cutoff, rc2 = rc * rc;
+        double c6 = 1.0 / (rc2 * rc2 * rc2), c12 = c6 * c6;
+        *ea = i12 - c12 + (r - rc) * rc * (12.0 * c12 / rc2);
+        *eb = -(i6 - c6) - (r - rc) * rc * (6.0 * c6 / rc2);
+    }
+}
+
+int oracle_param_grads(const mdgpu_system_desc* s, const mdgpu_params* p, const double* x, const double box[3],
+                       const mdgpu_energies* weights, mdgpu_param_grads* g) {
+    if (!s || !p || !x || !weights || !g || p->semantics != MDGPU_SEM_AMBER) return 1;
+    const double kc = (double)(SQRT_KC * SQRT_KC);
+    mdgpu_energies w;
+    eff_weights(weights, &w);
+    int n = s->n_atoms;
+    zero_if(g->charge, (size_t)n);
+    zero_if(g->lj_a, (size_t)s->n_lj); zero_if(g->lj_b, (size_t)s->n_lj);
+    zero_if(g->lj14_a, (size_t)s->n_lj); zero_if(g->lj14_b, (size_t)s->n_lj);
+    zero_if(g->bond_k, (size_t)s->n_bond_types); zero_if(g->bond_r0, (size_t)s->n_bond_types);
+    zero_if(g->ub_k, (size_t)s->n_ub_types); zero_if(g->ub_r0, (size_t)s->n_ub_types);
+    zero_if(g->angle_k, (size_t)s->n_angle_types); zero_if(g->angle_theta0, (size_t)s->n_angle_types);
+    zero_if(g->dihedral_k, (size_t)s->n_dihedral_types); zero_if(g->dihedral_phase, (size_t)s->n_dihedral_types);
+    zero_if(g->improper_k, (size_t)s->n_improper_types); zero_if(g->improper_psi0, (size_t)s->n_improper_types);
+    size_t ncoef = (size_t)s->cmap_n_sets * s->cmap_resolution * s->cmap_resolution * 16;
+    zero_if(g->cmap_coeff, ncoef);
+
+    /* bonds and Urey-Bradley: E = k (r - r0)^2 */
+    for (int pass = 0; pass < 2; ++pass) {
+        int nt = pass ? s->n_ubs : s->n_bonds;
+        const int32_t* at = pass ? s->ub_atoms : s->bond_atoms;
+        const int32_t* ty = pass ? s->ub_type : s->bond_type;
+        const double* kk = pass ? s->ub_k : s->bond_k;
+        const double* r0 = pass ? s->ub_r0 : s->bond_r0;
+        double* gk = pass ? g->ub_k : g->bond_k;
+        double* gr = pass ? g->ub_r0 : g->bond_r0;
+        double wt = pass ? w.urey_bradley : w.bond;
+        for (int t = 0; t < nt; ++t) {
+            int q = ty[t];
+            double d = m_distance(x + 3 * at[2 * t], x + 3 * at[2 * t + 1]) - r0[q];
+            if (gk) gk[q] += wt * d * d;
+            if (gr) gr[q] += wt * (-2.0 * kk[q] * d);
+        }
+    }
+    /* angles: E = k (theta - theta0)^2 */
+    for (int t = 0; t < s->n_angles; ++t) {
+        const int32_t* at = s->angle_atoms + 3 * t;
+        int q = s->angle_type[t];
+        double d = m_angle(x + 3 * at[0], x + 3 * at[1], x + 3 * at[2]) - s->angle_theta0[q];
+        if (g->angle_k) g->angle_k[q] += w.angle * d * d;
+        if (g->angle_theta0) g->angle_theta0[q] += w.angle * (-2.0 * s->angle_k[q] * d);
+    }
+    /* dihedrals: E = k (1 + cos(n phi - phase)) */
+    for (int t = 0; t < s->n_dihedrals; ++t) {
+        const int32_t* at = s->dihedral_atoms + 4 * t;
+        int q = s->dihedral_type[t];
+        double phi = m_dihedral(x + 3 * at[0], x + 3 * at[1], x + 3 * at[2], x + 3 * at[3]);
+        double arg = s->dihedral_n[q] * phi - s->dihedral_phase[q];
+        if (g->dihedral_k) g->dihedral_k[q] += w.dihedral * (1.0 + cos(arg));
+        if (g->dihedral_phase) g->dihedral_phase[q] += w.dihedral * s->dihedral_k[q] * sin(arg);
+    }
+    /* impropers (AMBER: wrapped difference): E = k d^2 */
+    for (int t = 0; t < s->n_impropers; ++t) {
+        const int32_t* at = s->improper_atoms + 4 * t;
+        int q = s->improper_type[t];
+        double psi = m_dihedral(x + 3 * at[0], x + 3 * at[1], x + 3 * at[2], x + 3 * at[3]);
+        double d = wrap_pi(psi - s->improper_psi0[q]);
+        if (g->improper_k) g->improper_k[q] += w.improper * d * d;
+        if (g->improper_psi0) g->improper_psi0[q] += w.improper * (-2.0 * s->improper_k[q] * d);
+    }
+    /* CMAP: E = sum_ij a_ij u^i v^j in the cell of (phi, psi) */
+    if (g->cmap_coeff) {
+        int res = s->cmap_resolution;
+        for (int t = 0; t < s->n_cmaps; ++t) {
+            const int32_t* at = s->cmap_atoms + 5 * t;
+            double phi = m_dihedral(x + 3 * at[0], x + 3 * at[1], x + 3 * at[2], x + 3 * at[3]);
+            double psi = m_dihedral(x + 3 * at[1], x + 3 * at[2], x + 3 * at[3], x + 3 * at[4]);
+            phi = wrap_pi(phi) + M_PI;
+            psi = wrap_pi(psi) + M_PI;
+            double sp = (2.0 * M_PI) / res, gp = phi / sp, gs = psi / sp;
+            int row = (int)gp, col = (int)gs;
+            double u = gp - row, v = gs - col;
+            row %= res; col %= res;
+            if (row < 0) row += res;
+            if (col < 0) col += res;
+            double* gc = g->cmap_coeff + (size_t)(s->cmap_set[t] - 1) * res * res * 16 + (size_t)(row * res + col) * 16;
+            double up[4] = {1.0, u, u * u, u * u * u}, vp[4] = {1.0, v, v * v, v * v * v};
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) gc[4 * i + j] += w.cmap * up[i] * vp[j];
+        }
+    }
+
+    /* pairs within the cutoffs (brute force over i < j with minimum image;
+     * the oracle is used on small systems) */
+    double alpha = p->ewald_alpha;
+    double rc2_lj = p->lj_cutoff * p->lj_cutoff, rc2_ee = p->ee_cutoff * p->ee_cutoff;
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            if (excluded(s, i, j)) continue;
+            double d[3] = {x[3 * i] - x[3 * j], x[3 * i + 1] - x[3 * j + 1], x[3 * i + 2] - x[3 * j + 2]};
+            min_image(d, box, 1);
+            double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (r2 < 1e-12) continue;
+            if (r2 <= rc2_lj) {
+                int pidx = s->nb_index[s->type[i] * s->n_types + s->type[j]] - 1;
+                if (pidx >= 0) {
+                    double ea, eb;
+                    lj_unit(p, r2, &ea, &eb);
+                    if (g->lj_a) g->lj_a[pidx] += w.vdw * ea;
+                    if (g->lj_b) g->lj_b[pidx] += w.vdw * eb;
+                }
+            }
+            if (r2 <= rc2_ee && g->charge) {
+                double r = sqrt(r2), ec = kc * erfc(alpha * r) / r * w.ewald_direct;
+                g->charge[i] += ec * s->charge[j];
+                g->charge[j] += ec * s->charge[i];
+            }
+        }
+    /* pure exclusions: E = -k qq erf(a r)/r */
+    for (int t = 0; t < s->n_excl_pairs; ++t) {
+        int i = s->excl_pairs[2 * t], j = s->excl_pairs[2 * t + 1];
+        double d[3] = {x[3 * i] - x[3 * j], x[3 * i + 1] - x[3 * j + 1], x[3 * i + 2] - x[3 * j + 2]};
+        min_image(d, box, 1);
+        double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        double c = -kc * erf(alpha * r) / r * w.ewald_excl;
+        if (g->charge) { g->charge[i] += c * s->charge[j]; g->charge[j] += c * s->charge[i]; }
+    }
+    /* 1-4 pairs: LJ-14 tables and k qq (1/(scee r) - erf(a r)/r) */
+    for (int t = 0; t < s->n_pairs14; ++t) {
+        int i = s->pairs14[2 * t], j = s->pairs14[2 * t + 1];
+        double d[3] = {x[3 * i] - x[3 * j], x[3 * i + 1] - x[3 * j + 1], x[3 * i + 2] - x[3 * j + 2]};
+        min_image(d, box, 1);
+        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2], r = sqrt(r2);
+        int pidx = s->nb_index[s->type[i] * s->n_types + s->type[j]] - 1;
+        if (pidx >= 0) {
+            double i6 = 1.0 / (r2 * r2 * r2);
+            if (g->lj14_a) g->lj14_a[pidx] += w.vdw14 * i6 * i6;
+            if (g->lj14_b) g->lj14_b[pidx] += -w.vdw14 * i6;
+        }
+        double c = kc * (1.0 / (s->scee14[t] * r) - erf(alpha * r) / r) * w.ee14;
+        if (g->charge) { g->charge[i] += c * s->charge[j]; g->charge[j] += c * s->charge[i]; }
+    }
+    if (g->charge) {
+        /* self: E = -k a/sqrt(pi) sum q^2 */
+        for (int i = 0; i < n; ++i) g->charge[i] += w.ewald_self * (-2.0 * kc * alpha / sqrt(M_PI)) * s->charge[i];
+        if (p->pme_grid[0] > 0 && w.ewald_recip != 0.0) {
+            double* pot = (double*)malloc(sizeof(double) * (size_t)n);
+            pme_core(n, s->charge, x, box, alpha, p->pme_grid, p->pme_order, NULL, NULL, pot);
+            for (int i = 0; i < n; ++i) g->charge[i] += w.ewald_recip * pot[i];
+            free(pot);
+        }
+    }
     return 0;
 }

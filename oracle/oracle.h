@@ -42,6 +42,17 @@ double oracle_ewald_recip_dense(int n, const double* q, const double* x, const d
 double oracle_pme_recip(int n, const double* q, const double* x, const double box[3], double alpha,
                         const int grid[3], int order, double* forces, double* virial);
 
+/* Reciprocal potential dE_rec/dq_i at each atom (returns E_rec). */
+double oracle_pme_potential(int n, const double* q, const double* x, const double box[3], double alpha,
+                            const int grid[3], int order, double* pot);
+
+/* Parameter gradients sum_t w_t dE_t/dtheta at one configuration (AMBER
+ * semantics): per-term weights in *w (w->total adds to every term of the
+ * total); arrays of *g as in mdgpu_param_grads (NULL skips, others are
+ * overwritten).  Pairs are enumerated by brute force: small systems only. */
+int oracle_param_grads(const mdgpu_system_desc* sys, const mdgpu_params* p, const double* x,
+                       const double box[3], const mdgpu_energies* w, mdgpu_param_grads* g);
+
 /* Cardinal B-spline weights M_p(w + j) and their derivatives, j = 0..p-1. */
 void oracle_bspline(int order, double w, double* theta, double* dtheta);
 
