@@ -245,6 +245,53 @@ typedef struct {
 int mdgpu_adjoint(mdgpu_ctx* ctx, const double* coords, const double box[3], const mdgpu_energies* weights,
                   mdgpu_param_grads* out);
 
+/* ---- Batches: many independent systems in one graph launch ------------
+ * The reference evaluates one system per process; this is new.  A batch owns
+ * one context per system (own topology, parameters and fixed box) and
+ * evaluates all of them with ONE hipGraph launch: every system's kernel
+ * sequence is captured once, spread over a few concurrent streams, and
+ * replayed.  Coordinates and forces of all systems are concatenated in the
+ * order the systems were added (3 * sum N doubles).  The first graph
+ * evaluation runs every system once on its own (sizing the pair lists), then
+ * captures the graph, once per batch.  A later call whose buffer addresses
+ * differ from the captured ones, or in which a system's pair list outgrows its
+ * capacity (that system is redone on its own), leaves the graph stale, and the
+ * batch then evaluates its systems one after another (see
+ * mdgpu_batch_graph_state). */
+typedef struct mdgpu_batch mdgpu_batch;
+
+int mdgpu_batch_create(int device, mdgpu_batch** out);
+int mdgpu_batch_destroy(mdgpu_batch* b);
+const char* mdgpu_batch_last_error(const mdgpu_batch* b);
+/* Add a system (copied to the device now); *index receives its position. */
+int mdgpu_batch_add(mdgpu_batch* b, const mdgpu_system_desc* sys, const mdgpu_params* p, const double box[3],
+                    int* index);
+/* Evaluate every system.  d_coords / d_forces: device buffers of 3 * sum N
+ * doubles (d_forces may be NULL); out_e: [n_systems] host energies.  The
+ * graph is (re)captured when a buffer address changes. */
+int mdgpu_batch_eval_device(mdgpu_batch* b, const double* d_coords, mdgpu_energies* out_e, double* d_forces);
+/* Same from host buffers (copied in and out around the graph launch). */
+int mdgpu_batch_eval(mdgpu_batch* b, const double* coords, mdgpu_energies* out_e, double* out_forces);
+/* Execution settings: mode 0 = one graph launch (default), 1 = one
+ * synchronous evaluation per system in turn (the serial baseline);
+ * streams = concurrent streams in the graph (1..8); split = waves per
+ * i-cluster list in the pair kernel (1..8); fft_lines = cap on FFT lines per
+ * block (0 = automatic).  streams, split and fft_lines are fixed once the
+ * graph exists (MDGPU_ERR_STATE); mode can always change. */
+int mdgpu_batch_configure(mdgpu_batch* b, int mode, int streams, int split, int fft_lines);
+/* Autotuner, before the first graph evaluation: times `reps` serial passes
+ * over all systems for each candidate pair-tile split (1, 2, 4, 8) and FFT
+ * block shape (16, 8, 4 lines) on the given device buffers and keeps the
+ * fastest; streams become min(8, systems).  Returns the settings and the mean
+ * serial pass time (ms). */
+int mdgpu_batch_tune(mdgpu_batch* b, const double* d_coords, double* d_forces, int reps, int* streams, int* split,
+                     int* fft_lines, double* ms);
+/* 0: no graph yet; 1: the captured graph is in use; 2: the graph went stale
+ * and the batch evaluates serially. */
+int mdgpu_batch_graph_state(const mdgpu_batch* b, int* state);
+/* Number of systems and total atoms. */
+int mdgpu_batch_size(const mdgpu_batch* b, int* n_systems, int64_t* n_atoms);
+
 /* Per-kernel timing with HIP events on the context stream (for roofline
  * reporting).  mdgpu_set_timing(ctx, mode) resets the totals; mode 1 makes
  * each later evaluation add the event-measured duration of every kernel

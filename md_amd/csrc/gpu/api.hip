@@ -619,77 +619,140 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double* __restr
     }
 }
 
-int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_energies* out_e, double* d_out,
-              double* out_virial, bool coords_are_work) {
-    if (!c->have_system || !c->have_params) return set_error(c, MDGPU_ERR_STATE, "upload a system and set params first");
-    HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
+// Launches one evaluation on c->stream after prepare(): no host
+// synchronisation and no allocation, so a caller can capture it into a graph.
+// Energies and flags land in w.energies, forces in d_out.
+int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out) {
     const mdgpu_params& p = c->params;
     const bool mixed = p.precision == MDGPU_PREC_MIXED;
     const bool amber = p.semantics == MDGPU_SEM_AMBER;
     const int n = c->topo.n_atoms;
+    Work& w = c->work;
+    c->cur_x = x;
+    auto& T = c->timing;
+    auto timed = [&](int k) { return T.mode == 1 || (T.mode == 2 && k == MDGPU_K_NONBONDED); };
+    auto mark = [&](int k, bool start) {
+        if (timed(k)) (void)hipEventRecord(c->ev[2 * k + (start ? 0 : 1)], c->stream);
+    };
+    mark(MDGPU_K_SORT, true);
+    if (amber && c->topo.has_excl) HIP_TRY(mdg::launch_excl_extent(c), "excl extent");
+    HIP_TRY(mdg::launch_sort(c, x, mixed), "sort");
+    mark(MDGPU_K_SORT, false);
+    mark(MDGPU_K_PAIRLIST, true);
+    HIP_TRY(mdg::launch_pairsearch(c, mixed, amber), "pair search");
+    mark(MDGPU_K_PAIRLIST, false);
+    int row = 0;
+    mark(MDGPU_K_NONBONDED, true);
+    HIP_TRY(mdg::launch_nonbonded(c, mixed, amber, row), "nonbonded");
+    mark(MDGPU_K_NONBONDED, false);
+    row += mdg::nonbonded_rows(c);
+    mark(MDGPU_K_BONDED, true);
+    HIP_TRY(mdg::launch_bonded(c, x, mixed, amber, row), "bonded");
+    mark(MDGPU_K_BONDED, false);
+    row += mdg::bonded_rows(c);
+    if (amber) {
+        mark(MDGPU_K_CORRECT, true);
+        HIP_TRY(mdg::launch_corrections(c, mixed, row), "corrections");
+        mark(MDGPU_K_CORRECT, false);
+        row += mdg::corrections_rows(c);
+        int used = 0;
+        HIP_TRY(mdg::launch_pme(c, mixed, row, &used), "pme");
+        row += used;
+    }
+    mark(MDGPU_K_REDUCE, true);
+    const int fblocks = std::max(1, std::min(512, (3 * n + kReduceThreads - 1) / kReduceThreads));
+    hipLaunchKernelGGL(k_reduce, dim3(mdg::E_NSLOTS + 1 + fblocks), dim3(kReduceThreads), 0, c->stream, w.partial,
+                       row, w.energies, w.flags, w.excl_rmax, mixed ? 1 : 0, w.force_fx, w.force_d,
+                       1.0 / mdg::fixed_scale(p.semantics), 3 * n, d_out, w.list_count, w.col_cstart + c->n_cols);
+    HIP_TRY(hipGetLastError(), "reduce");
+    mark(MDGPU_K_REDUCE, false);
+    return MDGPU_OK;
+}
+
+constexpr int kRedo = mdg::kEvalRedo;
+
+// Acts on the flags of a finished evaluation (e: host copy of w.energies).
+// Returns MDGPU_OK, kRedo after adjusting the context so that a repeat
+// succeeds (larger list, general search), or an error status.
+int check_flags(mdgpu_ctx* c, const double* e) {
+    unsigned int flags[mdg::kFlags];
+    for (int k = 0; k < mdg::kFlags; ++k) flags[k] = static_cast<unsigned int>(e[mdg::E_NSLOTS + k]);
+    if (flags[4]) {
+        // a cluster is too wide for the minimum-image search: redo with the general one
+        c->search_general = true;
+        return kRedo;
+    }
+    if (flags[2]) return set_error(c, MDGPU_ERR_UNSUPPORTED, "a sort column holds more than 2048 atoms");
+    if (flags[3]) return set_error(c, MDGPU_ERR_UNSUPPORTED, "too many tiles near excluded pairs in one cluster");
+    if (flags[1]) {
+        // list capacity exceeded: grow and evaluate again
+        c->work.list_cap *= 2;
+        return kRedo;
+    }
+    if (flags[0]) return set_error(c, MDGPU_ERR_OVERFLOW, "fixed-point force accumulator overflow");
+    return MDGPU_OK;
+}
+
+// Energies, virial and work counts of a finished evaluation.
+void fill_outputs(mdgpu_ctx* c, const double* e, mdgpu_energies* out_e, double* out_virial) {
+    const mdgpu_params& p = c->params;
+    const bool amber = p.semantics == MDGPU_SEM_AMBER;
+    c->counts[0] = static_cast<long long>(llround(e[mdg::E_PAIRS]));
+    c->counts[1] = static_cast<long long>(llround(e[mdg::E_TILES]));
+    c->counts[2] = c->n_clusters;
+    c->counts[3] = c->topo.n_atoms;
+    if (out_e) {
+        mdgpu_energies& o = *out_e;
+        o.bond = e[mdg::E_BOND];
+        o.angle = e[mdg::E_ANGLE];
+        o.dihedral = e[mdg::E_DIHEDRAL];
+        o.urey_bradley = e[mdg::E_UB];
+        o.improper = e[mdg::E_IMPROPER];
+        o.cmap = e[mdg::E_CMAP];
+        o.vdw = e[mdg::E_VDW];
+        o.ee_all = amber ? 0.0 : e[mdg::E_EE_ALL];
+        o.ee_pairlist = amber ? 0.0 : e[mdg::E_EE_LIST];
+        o.ee_rf = amber ? 0.0 : e[mdg::E_EE_RF];
+        o.ewald_direct = e[mdg::E_EW_DIRECT];
+        o.ewald_self = -mdg::kCoulomb * (p.ewald_alpha / std::sqrt(mdg::kPi)) * c->topo.sum_q2;
+        o.ewald_recip = amber ? e[mdg::E_EW_RECIP] : 0.0;
+        o.ewald_excl = amber ? e[mdg::E_EW_EXCL] : 0.0;
+        o.vdw14 = amber ? e[mdg::E_VDW14] : 0.0;
+        o.ee14 = amber ? e[mdg::E_EE14] : 0.0;
+        const double bonded = o.bond + o.angle + o.dihedral + o.urey_bradley + o.improper + o.cmap;
+        o.total = amber ? bonded + o.vdw + o.vdw14 + o.ewald_direct + o.ewald_recip + o.ewald_self + o.ewald_excl +
+                              o.ee14
+                        : bonded + o.vdw + o.ee_pairlist;
+    }
+    if (out_virial) {
+        const double xx = e[mdg::E_VIR_XX], yy = e[mdg::E_VIR_YY], zz = e[mdg::E_VIR_ZZ];
+        const double xy = e[mdg::E_VIR_XY], xz = e[mdg::E_VIR_XZ], yz = e[mdg::E_VIR_YZ];
+        const double v[9] = {xx, xy, xz, xy, yy, yz, xz, yz, zz};
+        std::memcpy(out_virial, v, sizeof v);
+    }
+}
+
+int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_energies* out_e, double* d_out,
+              double* out_virial, bool coords_are_work) {
+    if (!c->have_system || !c->have_params) return set_error(c, MDGPU_ERR_STATE, "upload a system and set params first");
+    HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
+    const bool amber = c->params.semantics == MDGPU_SEM_AMBER;
     for (int attempt = 0; attempt < 4; ++attempt) {
         int st = prepare(c, box);
         if (st != MDGPU_OK) return st;
         Work& w = c->work;
         const double* x = coords_are_work ? w.coords : d_coords;
-        c->cur_x = x;
-        auto& T = c->timing;
-        auto timed = [&](int k) { return T.mode == 1 || (T.mode == 2 && k == MDGPU_K_NONBONDED); };
-        auto mark = [&](int k, bool start) {
-            if (timed(k)) (void)hipEventRecord(c->ev[2 * k + (start ? 0 : 1)], c->stream);
-        };
-        mark(MDGPU_K_SORT, true);
-        if (amber && c->topo.has_excl) HIP_TRY(mdg::launch_excl_extent(c), "excl extent");
-        HIP_TRY(mdg::launch_sort(c, x, mixed), "sort");
-        mark(MDGPU_K_SORT, false);
-        mark(MDGPU_K_PAIRLIST, true);
-        HIP_TRY(mdg::launch_pairsearch(c, mixed, amber), "pair search");
-        mark(MDGPU_K_PAIRLIST, false);
-        int row = 0;
-        mark(MDGPU_K_NONBONDED, true);
-        HIP_TRY(mdg::launch_nonbonded(c, mixed, amber, row), "nonbonded");
-        mark(MDGPU_K_NONBONDED, false);
-        row += mdg::nonbonded_rows(c);
-        mark(MDGPU_K_BONDED, true);
-        HIP_TRY(mdg::launch_bonded(c, x, mixed, amber, row), "bonded");
-        mark(MDGPU_K_BONDED, false);
-        row += mdg::bonded_rows(c);
-        if (amber) {
-            mark(MDGPU_K_CORRECT, true);
-            HIP_TRY(mdg::launch_corrections(c, mixed, row), "corrections");
-            mark(MDGPU_K_CORRECT, false);
-            row += mdg::corrections_rows(c);
-            int used = 0;
-            HIP_TRY(mdg::launch_pme(c, mixed, row, &used), "pme");
-            row += used;
-        }
-        mark(MDGPU_K_REDUCE, true);
-        const int fblocks = std::max(1, std::min(512, (3 * n + kReduceThreads - 1) / kReduceThreads));
-        hipLaunchKernelGGL(k_reduce, dim3(mdg::E_NSLOTS + 1 + fblocks), dim3(kReduceThreads), 0, c->stream,
-                           w.partial, row, w.energies, w.flags, w.excl_rmax, mixed ? 1 : 0, w.force_fx, w.force_d,
-                           1.0 / mdg::fixed_scale(p.semantics), 3 * n, d_out ? d_out : w.force_out, w.list_count,
-                           w.col_cstart + c->n_cols);
-        HIP_TRY(hipGetLastError(), "reduce");
-        mark(MDGPU_K_REDUCE, false);
+        st = enqueue_eval(c, x, d_out ? d_out : w.force_out);
+        if (st != MDGPU_OK) return st;
         double e[mdg::E_NSLOTS + mdg::kFlags];
         HIP_TRY(hipMemcpyAsync(e, w.energies, sizeof e, hipMemcpyDeviceToHost, c->stream), "read energies");
         HIP_TRY(hipStreamSynchronize(c->stream), "evaluation");
-        unsigned int flags[mdg::kFlags];
-        for (int k = 0; k < mdg::kFlags; ++k) flags[k] = static_cast<unsigned int>(e[mdg::E_NSLOTS + k]);
-        if (flags[4]) {
-            // a cluster is too wide for the minimum-image search: redo with the general one
-            c->search_general = true;
-            continue;
-        }
-        if (flags[2]) return set_error(c, MDGPU_ERR_UNSUPPORTED, "a sort column holds more than 2048 atoms");
-        if (flags[3]) return set_error(c, MDGPU_ERR_UNSUPPORTED, "too many tiles near excluded pairs in one cluster");
-        if (flags[1]) {
-            // list capacity exceeded: grow and evaluate again
-            w.list_cap *= 2;
-            continue;
-        }
-        if (flags[0]) return set_error(c, MDGPU_ERR_OVERFLOW, "fixed-point force accumulator overflow");
+        st = check_flags(c, e);
+        if (st == kRedo) continue;
+        if (st != MDGPU_OK) return st;
+        auto& T = c->timing;
         if (T.mode != 0) {
+            auto timed = [&](int k) { return T.mode == 1 || (T.mode == 2 && k == MDGPU_K_NONBONDED); };
             for (int k = 0; k <= MDGPU_K_COUNT; ++k) {
                 if (!timed(k == MDGPU_K_COUNT ? MDGPU_K_FFT : k)) continue;
                 if (k == MDGPU_K_SPREAD || k == MDGPU_K_FFT || k == MDGPU_K_CONV || k == MDGPU_K_GATHER ||
@@ -705,45 +768,29 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
                 }
             }
         }
-        c->counts[0] = static_cast<long long>(llround(e[mdg::E_PAIRS]));
-        c->counts[1] = static_cast<long long>(llround(e[mdg::E_TILES]));
-        c->counts[2] = c->n_clusters;
-        c->counts[3] = n;
-        if (out_e) {
-            mdgpu_energies& o = *out_e;
-            o.bond = e[mdg::E_BOND];
-            o.angle = e[mdg::E_ANGLE];
-            o.dihedral = e[mdg::E_DIHEDRAL];
-            o.urey_bradley = e[mdg::E_UB];
-            o.improper = e[mdg::E_IMPROPER];
-            o.cmap = e[mdg::E_CMAP];
-            o.vdw = e[mdg::E_VDW];
-            o.ee_all = amber ? 0.0 : e[mdg::E_EE_ALL];
-            o.ee_pairlist = amber ? 0.0 : e[mdg::E_EE_LIST];
-            o.ee_rf = amber ? 0.0 : e[mdg::E_EE_RF];
-            o.ewald_direct = e[mdg::E_EW_DIRECT];
-            o.ewald_self = -mdg::kCoulomb * (p.ewald_alpha / std::sqrt(mdg::kPi)) * c->topo.sum_q2;
-            o.ewald_recip = amber ? e[mdg::E_EW_RECIP] : 0.0;
-            o.ewald_excl = amber ? e[mdg::E_EW_EXCL] : 0.0;
-            o.vdw14 = amber ? e[mdg::E_VDW14] : 0.0;
-            o.ee14 = amber ? e[mdg::E_EE14] : 0.0;
-            const double bonded = o.bond + o.angle + o.dihedral + o.urey_bradley + o.improper + o.cmap;
-            o.total = amber ? bonded + o.vdw + o.vdw14 + o.ewald_direct + o.ewald_recip + o.ewald_self + o.ewald_excl +
-                                  o.ee14
-                            : bonded + o.vdw + o.ee_pairlist;
-        }
-        if (out_virial) {
-            const double xx = e[mdg::E_VIR_XX], yy = e[mdg::E_VIR_YY], zz = e[mdg::E_VIR_ZZ];
-            const double xy = e[mdg::E_VIR_XY], xz = e[mdg::E_VIR_XZ], yz = e[mdg::E_VIR_YZ];
-            const double v[9] = {xx, xy, xz, xy, yy, yz, xz, yz, zz};
-            std::memcpy(out_virial, v, sizeof v);
-        }
+        fill_outputs(c, e, out_e, out_virial);
         return MDGPU_OK;
     }
     return set_error(c, MDGPU_ERR_HIP, "pair list did not fit after regrowing");
 }
 
 }  // namespace
+
+namespace mdg {
+int ctx_prepare(mdgpu_ctx* c, const double box[3]) { return prepare(c, box); }
+int ctx_enqueue(mdgpu_ctx* c, const double* d_coords, double* d_forces) {
+    return enqueue_eval(c, d_coords, d_forces ? d_forces : c->work.force_out);
+}
+int ctx_check(mdgpu_ctx* c, const double* e) { return check_flags(c, e); }
+void ctx_outputs(mdgpu_ctx* c, const double* e, mdgpu_energies* out_e, double* out_virial) {
+    fill_outputs(c, e, out_e, out_virial);
+}
+int ctx_eval_device(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_energies* out_e,
+                    double* d_forces) {
+    return eval_impl(c, d_coords, box, out_e, d_forces, nullptr, false);
+}
+int ctx_error(mdgpu_ctx* c, int status, const std::string& msg) { return set_error(c, status, msg.c_str()); }
+}  // namespace mdg
 
 extern "C" {
 
@@ -854,6 +901,7 @@ int mdgpu_adjoint(mdgpu_ctx* c, const double* coords, const double box[3], const
             HIP_TRY(hipMemcpyAsync(pot.data(), b.pot_rec, static_cast<size_t>(n) * sizeof(double),
                                    hipMemcpyDeviceToHost, c->stream), "read potential");
         HIP_TRY(hipMemsetAsync(w.flags, 0, sizeof flags, c->stream), "clear flags");
+        HIP_TRY(hipMemsetAsync(w.excl_rmax, 0, sizeof(float), c->stream), "clear flags");
         HIP_TRY(hipStreamSynchronize(c->stream), "adjoint");
         if (flags[4]) {
             c->search_general = true;

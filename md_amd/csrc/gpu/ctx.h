@@ -5,6 +5,7 @@
 #include "common.h"
 #include "mdgpu.h"
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -156,6 +157,7 @@ struct mdgpu_ctx {
     int n_clusters = 0;
     int n_cols = 0;
     int split = 4;                  // waves per i-cluster list in the pair kernel
+    int fft_lines = 0;              // cap on FFT lines per block (power of two; 0 = 16)
     const double* cur_x = nullptr;  // raw coordinates of the evaluation in progress (device)
     bool search_general = false;    // use the general (per-shift) cluster search
     std::vector<void*> topo_allocs;
@@ -183,6 +185,19 @@ hipError_t launch_pair_adjoint(mdgpu_ctx* c, bool mixed);
 hipError_t launch_bonded_adjoint(mdgpu_ctx* c, const double* coords, bool mixed);
 hipError_t launch_corrections_adjoint(mdgpu_ctx* c);
 hipError_t launch_pme_potential(mdgpu_ctx* c, bool mixed, int partial_row0, double* pot);
+
+// one evaluation of one context, in steps (api.hip): prepare buffers for a
+// box; enqueue the launches (no synchronisation or allocation); act on the
+// flags of a finished evaluation (MDGPU_OK, kEvalRedo after adjusting the
+// context, or an error status); fill the outputs; or all of it, synchronously
+constexpr int kEvalRedo = -1;
+int ctx_prepare(mdgpu_ctx* c, const double box[3]);
+int ctx_enqueue(mdgpu_ctx* c, const double* d_coords, double* d_forces);
+int ctx_check(mdgpu_ctx* c, const double* energies_and_flags);
+void ctx_outputs(mdgpu_ctx* c, const double* energies_and_flags, mdgpu_energies* out_e, double* out_virial);
+int ctx_eval_device(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_energies* out_e,
+                    double* d_forces);
+int ctx_error(mdgpu_ctx* c, int status, const std::string& msg);
 
 // rows of the partial-energy buffer each kernel needs for a given launch
 int nonbonded_rows(const mdgpu_ctx* c);

@@ -411,9 +411,10 @@ __global__ void k_potential(int n, const double* __restrict__ wrapped, PmeGeom g
     pot[i] = s;
 }
 
-int lines_per_block(int n, size_t elem) {
-    // two LDS buffers of n x nl complex values plus n twiddles within 64 KB
-    int nl = 16;
+// Lines per FFT block: two LDS buffers of n x nl complex values plus n
+// twiddles within 64 KB, at most `cap` (a power of two; 0 = 16).
+int lines_per_block(int n, size_t elem, int cap = 0) {
+    int nl = cap > 0 ? cap : 16;
     while (nl > 1 && (2 * static_cast<size_t>(n) * nl + n) * elem > 65536) nl >>= 1;
     return nl;
 }
@@ -448,18 +449,18 @@ hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used, double* pot) {
     mark(MDGPU_K_SPREAD, false);
     mark(MDGPU_K_FFT, true);
     {
-        const int nl = lines_per_block(g.kz, sizeof(c2));
+        const int nl = lines_per_block(g.kz, sizeof(c2), c->fft_lines);
         const size_t shm = (2 * static_cast<size_t>(g.kz) * nl + g.kz) * sizeof(c2);
         hipLaunchKernelGGL(k_fft_z_r2c<real>, dim3((g.kx * g.ky + nl - 1) / nl), dim3(kThreads), shm, c->stream, g,
                            ilog2(nl), w.grid_d, w.grid_fx, cg, twz);
     }
-    const int nly = lines_per_block(g.ky, sizeof(c2));
+    const int nly = lines_per_block(g.ky, sizeof(c2), c->fft_lines);
     const size_t shy = (2 * static_cast<size_t>(g.ky) * nly + g.ky) * sizeof(c2);
     hipLaunchKernelGGL(k_fft_y<real>, dim3((g.hz + nly - 1) / nly, g.kx), dim3(kThreads), shy, c->stream, g,
                        ilog2(nly), cg, twy, -1);
     mark(MDGPU_K_FFT, false);
     mark(MDGPU_K_CONV, true);
-    const int nlx = lines_per_block(g.kx, sizeof(c2));
+    const int nlx = lines_per_block(g.kx, sizeof(c2), c->fft_lines);
     const size_t shx = (2 * static_cast<size_t>(g.kx) * nlx + g.kx) * sizeof(c2);
     const dim3 gx((g.hz + nlx - 1) / nlx, g.ky);
     hipLaunchKernelGGL(k_fft_x_conv<real>, gx, dim3(kThreads), shx, c->stream, g, ilog2(nlx), cg, twx, w.bsp_mod,
@@ -471,7 +472,7 @@ hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used, double* pot) {
                        ilog2(nly), cg, twy, +1);
     real* phi = dbl ? reinterpret_cast<real*>(w.grid_phi_d) : reinterpret_cast<real*>(w.grid_f);
     {
-        const int nl = lines_per_block(g.kz, sizeof(c2));
+        const int nl = lines_per_block(g.kz, sizeof(c2), c->fft_lines);
         const size_t shm = (2 * static_cast<size_t>(g.kz) * nl + g.kz) * sizeof(c2);
         hipLaunchKernelGGL(k_fft_z_c2r<real>, dim3((g.kx * g.ky + nl - 1) / nl), dim3(kThreads), shm, c->stream, g,
                            ilog2(nl), cg, twz, phi);
@@ -493,7 +494,8 @@ int pme_rows(const mdgpu_ctx* c) {
     const int* k = c->work.pme_k;
     if (k[0] == 0) return 0;
     const int hz = k[2] / 2 + 1;
-    const int nlx = lines_per_block(k[0], 16);  // the FP64 complex size bounds both forms
+    // rows of the smaller of the two complex sizes bound both forms
+    const int nlx = std::min(lines_per_block(k[0], 16, c->fft_lines), lines_per_block(k[0], 8, c->fft_lines));
     return ((hz + nlx - 1) / nlx) * k[1] * 2;
 }
 

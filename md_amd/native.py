@@ -141,6 +141,19 @@ def gpu_lib() -> C.CDLL:
         lib.mdgpu_get_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         lib.mdgpu_set_counting.argtypes = [C.c_void_p, C.c_int]
         lib.mdgpu_adjoint.argtypes = [C.c_void_p, P_F64, P_F64, C.POINTER(Energies), C.POINTER(ParamGrads)]
+        lib.mdgpu_batch_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+        lib.mdgpu_batch_destroy.argtypes = [C.c_void_p]
+        lib.mdgpu_batch_last_error.argtypes = [C.c_void_p]
+        lib.mdgpu_batch_last_error.restype = C.c_char_p
+        lib.mdgpu_batch_add.argtypes = [C.c_void_p, C.POINTER(SystemDesc), C.POINTER(Params), P_F64,
+                                        C.POINTER(C.c_int)]
+        lib.mdgpu_batch_eval_device.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Energies), C.c_void_p]
+        lib.mdgpu_batch_eval.argtypes = [C.c_void_p, P_F64, C.POINTER(Energies), P_F64]
+        lib.mdgpu_batch_configure.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+        lib.mdgpu_batch_tune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int), C.POINTER(C.c_int), P_F64]
+        lib.mdgpu_batch_size.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+        lib.mdgpu_batch_graph_state.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         _gpu = lib
     return _gpu
 
@@ -425,6 +438,75 @@ class Evaluator:
         if getattr(self, "ctx", None):
             self.lib.mdgpu_destroy(self.ctx)
             self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many independent systems evaluated by one graph launch (mdgpu_batch_*)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = gpu_lib()
+        self.b = C.c_void_p()
+        st = self.lib.mdgpu_batch_create(device, C.byref(self.b))
+        if st != 0:
+            raise GpuError(st, f"mdgpu_batch_create(device={device}) failed")
+        self.n_atoms = []
+
+    def _check(self, st):
+        if st != 0:
+            raise GpuError(st, self.lib.mdgpu_batch_last_error(self.b).decode())
+
+    def add(self, desc: SystemDesc, params: Params, box) -> int:
+        bx = np.ascontiguousarray(np.broadcast_to(np.asarray(box, dtype=np.float64), (3,)))
+        idx = C.c_int()
+        self._check(self.lib.mdgpu_batch_add(self.b, C.byref(desc), C.byref(params), bx.ctypes.data_as(P_F64),
+                                             C.byref(idx)))
+        self.n_atoms.append(int(desc.n_atoms))
+        return idx.value
+
+    def __len__(self):
+        return len(self.n_atoms)
+
+    def evaluate_device(self, coords_ptr: int, forces_ptr: int = 0) -> list:
+        e = (Energies * len(self))()
+        self._check(self.lib.mdgpu_batch_eval_device(self.b, C.c_void_p(coords_ptr), e,
+                                                     C.c_void_p(forces_ptr) if forces_ptr else None))
+        return [x.as_dict() for x in e]
+
+    def evaluate(self, coords, want_forces: bool = True):
+        x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+        f = np.zeros_like(x) if want_forces else None
+        e = (Energies * len(self))()
+        self._check(self.lib.mdgpu_batch_eval(self.b, x.ctypes.data_as(P_F64), e,
+                                              f.ctypes.data_as(P_F64) if f is not None else None))
+        return [y.as_dict() for y in e], (f.reshape(-1, 3) if f is not None else None)
+
+    def configure(self, mode=0, streams=4, split=4, fft_lines=0):
+        self._check(self.lib.mdgpu_batch_configure(self.b, mode, streams, split, fft_lines))
+
+    def tune(self, coords_ptr: int, forces_ptr: int = 0, reps: int = 3) -> dict:
+        s, p, fl = C.c_int(), C.c_int(), C.c_int()
+        ms = C.c_double()
+        self._check(self.lib.mdgpu_batch_tune(self.b, C.c_void_p(coords_ptr),
+                                              C.c_void_p(forces_ptr) if forces_ptr else None, reps,
+                                              C.byref(s), C.byref(p), C.byref(fl), C.byref(ms)))
+        return {"streams": s.value, "split": p.value, "fft_lines": fl.value, "ms_serial_pass": ms.value}
+
+    def graph_state(self) -> int:
+        """0 no graph yet, 1 graph in use, 2 graph stale (serial evaluation)."""
+        st = C.c_int()
+        self._check(self.lib.mdgpu_batch_graph_state(self.b, C.byref(st)))
+        return st.value
+
+    def close(self):
+        if getattr(self, "b", None) and self.b.value:
+            self.lib.mdgpu_batch_destroy(self.b)
+            self.b = C.c_void_p()
 
     def __del__(self):
         try:

@@ -600,3 +600,98 @@ def cached_system(name: str, cache_dir: str | None = None) -> tuple:
     if not (os.path.exists(prefix + ".prmtop") and os.path.exists(prefix + ".rst7")):
         write_system(WORKLOADS[name](), prefix)
     return prefix + ".prmtop", prefix + ".rst7"
+
+
+# --------------------------------------------------------------------------
+# many water boxes as arrays (batched workloads)
+# --------------------------------------------------------------------------
+
+def smallest_2a3b(n: int) -> int:
+    """Smallest 2^a 3^b >= n."""
+    best = None
+    p2 = 1
+    while p2 < 2 * n:
+        p = p2
+        while p < 3 * n:
+            if p >= n and (best is None or p < best):
+                best = p
+            p *= 3
+        p2 *= 2
+    return best
+
+
+def _rotations(rng, m):
+    """m uniformly random rotation matrices (from unit quaternions)."""
+    q = rng.normal(size=(m, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+def water_arrays(seed: int, n_atoms: int, density: float = 0.1, jitter: float = 0.3):
+    """A water box of about ``n_atoms`` atoms as system arrays (AMBER
+    semantics, the layout md_amd.native.desc_from_arrays takes), without a
+    prmtop round trip: the water model, parameters and placement rules of
+    ``water_box`` (jittered lattice sites, H at 0.9572 A and 104.52 deg with a
+    random orientation), vectorised.  Returns (arrays, coords (N,3), box L)."""
+    rng = np.random.default_rng(seed)
+    n_water = max(1, n_atoms // 3)
+    n = 3 * n_water
+    box = (n / density) ** (1.0 / 3.0)
+    n_lat = int(math.ceil(n_water ** (1.0 / 3.0)))
+    sites = _water_sites(rng, box, n_lat, jitter)
+    pick = np.sort(rng.permutation(sites.shape[0])[:n_water])
+    o = sites[pick]
+    s, c = math.sin(WATER_THETA / 2), math.cos(WATER_THETA / 2)
+    local = np.array([[WATER_R_OH * s, 0.0, WATER_R_OH * c], [-WATER_R_OH * s, 0.0, WATER_R_OH * c]])
+    rot = _rotations(rng, n_water)
+    h = o[:, None, :] + np.einsum("wij,kj->wki", rot, local)
+    coords = np.concatenate([o[:, None, :], h], axis=1).reshape(n, 3)
+    qscale = AMBER_CHARGE_SCALE / 18.2206899283247
+    a_oo = 4.0 * O_EPS * O_SIGMA ** 12
+    b_oo = 4.0 * O_EPS * O_SIGMA ** 6
+    base = 3 * np.arange(n_water, dtype=np.int32)
+    oh1 = np.stack([base, base + 1], 1)
+    oh2 = np.stack([base, base + 2], 1)
+    bond_atoms = np.stack([oh1, oh2], 1).reshape(-1)
+    angle_atoms = np.stack([base + 1, base, base + 2], 1).reshape(-1)
+    excl_list = np.stack([base + 1, base + 2, base, base + 2, base, base + 1], 1).reshape(-1).astype(np.int32)
+    excl_pairs = np.stack([base, base + 1, base, base + 2, base + 1, base + 2], 1).reshape(-1).astype(np.int32)
+    empty_i, empty_d = np.zeros(0, np.int32), np.zeros(0)
+    arrays = {
+        "charge": np.tile([-0.834 * qscale, 0.417 * qscale, 0.417 * qscale], n_water),
+        "type": np.tile(np.array([0, 1, 1], np.int32), n_water), "n_types": 2,
+        "nb_index": np.array([1, 2, 2, 3], np.int32),
+        "lj_a": np.array([a_oo, 0.0, 0.0]), "lj_b": np.array([b_oo, 0.0, 0.0]),
+        "lj14_a": np.array([a_oo / 2, 0.0, 0.0]), "lj14_b": np.array([b_oo / 2, 0.0, 0.0]),
+        "bond_atoms": bond_atoms.astype(np.int32), "bond_type": np.zeros(2 * n_water, np.int32),
+        "bond_k": np.array([553.0]), "bond_r0": np.array([WATER_R_OH]),
+        "ub_atoms": empty_i, "ub_type": empty_i, "ub_k": empty_d, "ub_r0": empty_d,
+        "angle_atoms": angle_atoms.astype(np.int32), "angle_type": np.zeros(n_water, np.int32),
+        "angle_k": np.array([100.0]), "angle_theta0": np.array([WATER_THETA]),
+        "dihedral_atoms": empty_i, "dihedral_type": empty_i, "dihedral_k": empty_d, "dihedral_n": empty_d,
+        "dihedral_phase": empty_d,
+        "improper_atoms": empty_i, "improper_type": empty_i, "improper_k": empty_d, "improper_psi0": empty_d,
+        "cmap_atoms": empty_i, "cmap_set": empty_i, "cmap_coeff": empty_d, "cmap_grid": empty_d,
+        "cmap_n_sets": 0, "cmap_resolution": 0,
+        "excl_start": (2 * np.arange(n + 1)).astype(np.int32), "excl_list": excl_list,
+        "excl_pairs": excl_pairs, "pairs14": empty_i, "scee14": empty_d,
+    }
+    return arrays, coords, box
+
+
+def batch_workload(n_systems: int = 256, seed0: int = 4000, lo: int = 3000, hi: int = 30000):
+    """Uneven water boxes: atom counts uniform in [lo, hi] (seed 4000 + i for
+    system i), cubic boxes at 0.1 atoms/A^3, PME grid = smallest 2^a 3^b >= L/1 A.
+    Returns a list of (arrays, coords, box, grid)."""
+    out = []
+    for i in range(n_systems):
+        rng = np.random.default_rng(seed0 + i)
+        n = int(rng.integers(lo, hi + 1))
+        arrays, coords, box = water_arrays(seed0 + i, n)
+        k = smallest_2a3b(int(math.ceil(box / 1.0)))
+        out.append((arrays, coords, box, k))
+    return out
