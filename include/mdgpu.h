@@ -239,8 +239,8 @@ typedef struct {
  * energy term (fields of mdgpu_energies); weights->total is added to every
  * term of the total.  The pair list and every pair contribution are
  * recomputed (nothing is kept from an evaluation).  Arithmetic follows the
- * context's precision form; sums are accumulated in 64-bit fixed point and
- * converted in FP64 (deterministic).
+ * context's precision form; sums are accumulated in 64-bit fixed point
+ * (deterministic), except the LJ sums of the FP64 form (FP64 atomics).
  *   coords [3N] host coordinates; box [3]. */
 int mdgpu_adjoint(mdgpu_ctx* ctx, const double* coords, const double box[3], const mdgpu_energies* weights,
                   mdgpu_param_grads* out);

@@ -13,7 +13,11 @@
 //   CMAP gives dE/da per patch coefficient (u^i v^j of the cell).
 // The pair list and the pair values are recomputed, nothing is stored by an
 // evaluation.  Sums go to 64-bit fixed-point accumulators (integer adds are
-// associative: the result does not depend on the order of the atomics).
+// associative: the result does not depend on the order of the atomics),
+// except the LJ sums of the FP64 form, which use FP64 atomics: one pair of
+// atoms without LJ parameters can sit a fraction of an angstrom apart, and
+// its 1/r^12 then exceeds every fixed-point range fine enough for the FP64
+// bar (the mixed form uses 2^32).
 #include "ctx.h"
 
 namespace mdg {
@@ -75,9 +79,34 @@ struct PairAdjArgs {
     real c12, g12, c6, g6;             // force-shifted LJ: rc^-12, 12 rc^-13, rc^-6, 6 rc^-7
     real alpha, kc;
     unsigned long long* q_dir;         // [N] sum_j k erfc(a r)/r q_j
-    unsigned long long* lj_a;          // [n_lj]
+    unsigned long long* lj_a;          // [n_lj] fixed point (mixed form)
     unsigned long long* lj_b;
+    double* lj_ad;                     // [n_lj] FP64 (FP64 form)
+    double* lj_bd;
     unsigned int* ovf;
+};
+
+// LJ derivative bins: FP64 in the FP64 form, 64-bit fixed point (2^32) in the
+// mixed form.
+template <typename real>
+struct LjBin;
+template <>
+struct LjBin<double> {
+    using T = double;
+    static __device__ __forceinline__ void add(double* s, double v, unsigned int*) { atomicAdd(s, v); }
+    static __device__ __forceinline__ void flush(double v, double* gd, unsigned long long*) {
+        if (v != 0.0) atomicAdd(gd, v);
+    }
+};
+template <>
+struct LjBin<float> {
+    using T = unsigned long long;
+    static __device__ __forceinline__ void add(unsigned long long* s, double v, unsigned int* ovf) {
+        fx_add(s, v, kAdjScaleLJm, ovf);
+    }
+    static __device__ __forceinline__ void flush(unsigned long long v, double*, unsigned long long* gf) {
+        if (v) atomicAdd(gf, v);
+    }
 };
 
 // One wave per (i-cluster, list part) as in the pair kernel, lane = 8 tj + ti.
@@ -85,14 +114,16 @@ struct PairAdjArgs {
 template <typename real, bool kSwitch>
 __global__ __launch_bounds__(kWaves * kWave) void k_pair_adjoint(const PairAdjArgs<real> a) {
     using R4 = typename V4<real>::T;
+    using Bin = LjBin<real>;
+    using BT = typename Bin::T;
     extern __shared__ unsigned char smem[];
-    unsigned long long* s_a = reinterpret_cast<unsigned long long*>(smem);
-    unsigned long long* s_b = s_a + a.n_lj;
+    BT* s_a = reinterpret_cast<BT*>(smem);
+    BT* s_b = s_a + a.n_lj;
     int* s_p = reinterpret_cast<int*>(s_b + a.n_lj);
     const int ntp2 = a.ntp * a.ntp;
     for (int k = threadIdx.x; k < a.n_lj; k += blockDim.x) {
-        s_a[k] = 0ull;
-        s_b[k] = 0ull;
+        s_a[k] = 0;
+        s_b[k] = 0;
     }
     for (int k = threadIdx.x; k < ntp2; k += blockDim.x) s_p[k] = a.pidx[k];
     __syncthreads();
@@ -140,8 +171,8 @@ __global__ __launch_bounds__(kWaves * kWave) void k_pair_adjoint(const PairAdjAr
                             ea = i12 - a.c12 + (r - a.rc_lj) * a.g12;
                             eb = -(i6 - a.c6) - (r - a.rc_lj) * a.g6;
                         }
-                        fx_add(s_a + p, static_cast<double>(ea), kAdjScaleLJ, a.ovf);
-                        fx_add(s_b + p, static_cast<double>(eb), kAdjScaleLJ, a.ovf);
+                        Bin::add(s_a + p, static_cast<double>(ea), a.ovf);
+                        Bin::add(s_b + p, static_cast<double>(eb), a.ovf);
                     }
                 }
                 if (r2 <= a.rc2_ee) {
@@ -159,8 +190,8 @@ __global__ __launch_bounds__(kWaves * kWave) void k_pair_adjoint(const PairAdjAr
     }
     __syncthreads();
     for (int k = threadIdx.x; k < a.n_lj; k += blockDim.x) {
-        if (s_a[k]) atomicAdd(a.lj_a + k, s_a[k]);
-        if (s_b[k]) atomicAdd(a.lj_b + k, s_b[k]);
+        Bin::flush(s_a[k], a.lj_ad + k, a.lj_a + k);
+        Bin::flush(s_b[k], a.lj_bd + k, a.lj_b + k);
     }
 }
 
@@ -401,6 +432,8 @@ hipError_t run_pair(mdgpu_ctx* c) {
     a.q_dir = b.acc + b.q_dir;
     a.lj_a = b.acc + b.lj_a;
     a.lj_b = b.acc + b.lj_b;
+    a.lj_ad = b.lj_d;
+    a.lj_bd = b.lj_d + t.n_lj;
     a.ovf = reinterpret_cast<unsigned int*>(b.acc + b.flag);
     const size_t shm = 2 * sizeof(unsigned long long) * t.n_lj + sizeof(int) * a.ntp * a.ntp;
     if (shm > 60 * 1024) return hipErrorInvalidValue;

@@ -149,6 +149,7 @@ int mdgpu_destroy(mdgpu_ctx* c) {
     free_work(c);
     if (c->adj.acc) (void)hipFree(c->adj.acc);
     if (c->adj.pot_rec) (void)hipFree(c->adj.pot_rec);
+    if (c->adj.lj_d) (void)hipFree(c->adj.lj_d);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -847,6 +848,13 @@ int ensure_adjoint(mdgpu_ctx* c) {
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&b.acc), o * sizeof(unsigned long long)), "alloc adjoint");
         b.cap = o;
     }
+    if (b.cap_lj_d < 2 * nlj + 1) {
+        if (b.lj_d) (void)hipFree(b.lj_d);
+        b.lj_d = nullptr;
+        b.cap_lj_d = 0;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&b.lj_d), (2 * nlj + 1) * sizeof(double)), "alloc adjoint");
+        b.cap_lj_d = 2 * nlj + 1;
+    }
     if (b.cap_pot < n) {
         if (b.pot_rec) (void)hipFree(b.pot_rec);
         b.pot_rec = nullptr;
@@ -876,6 +884,7 @@ int mdgpu_adjoint(mdgpu_ctx* c, const double* coords, const double box[3], const
     mdg::AdjointBuf& b = c->adj;
     std::vector<unsigned long long> acc(b.total);
     std::vector<double> pot(n, 0.0);
+    std::vector<double> ljd(2 * static_cast<size_t>(std::max(t.n_lj, 0)), 0.0);
     const bool pme = c->work.pme_k[0] > 0 || p.pme_grid[0] > 0;
     bool done = false;
     for (int attempt = 0; attempt < 4 && !done; ++attempt) {
@@ -886,6 +895,7 @@ int mdgpu_adjoint(mdgpu_ctx* c, const double* coords, const double box[3], const
                                c->stream), "upload coords");
         c->cur_x = w.coords;
         HIP_TRY(hipMemsetAsync(b.acc, 0, b.total * sizeof(unsigned long long), c->stream), "clear adjoint");
+        HIP_TRY(hipMemsetAsync(b.lj_d, 0, b.cap_lj_d * sizeof(double), c->stream), "clear adjoint");
         if (t.has_excl) HIP_TRY(mdg::launch_excl_extent(c), "excl extent");
         HIP_TRY(mdg::launch_sort(c, w.coords, mixed), "sort");
         HIP_TRY(mdg::launch_pairsearch(c, mixed, true), "pair search");
@@ -900,6 +910,9 @@ int mdgpu_adjoint(mdgpu_ctx* c, const double* coords, const double box[3], const
         if (pme)
             HIP_TRY(hipMemcpyAsync(pot.data(), b.pot_rec, static_cast<size_t>(n) * sizeof(double),
                                    hipMemcpyDeviceToHost, c->stream), "read potential");
+        if (!mixed)
+            HIP_TRY(hipMemcpyAsync(ljd.data(), b.lj_d, ljd.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                    "read LJ sums");
         HIP_TRY(hipMemsetAsync(w.flags, 0, sizeof flags, c->stream), "clear flags");
         HIP_TRY(hipMemsetAsync(w.excl_rmax, 0, sizeof(float), c->stream), "clear flags");
         HIP_TRY(hipStreamSynchronize(c->stream), "adjoint");
@@ -937,8 +950,15 @@ int mdgpu_adjoint(mdgpu_ctx* c, const double* coords, const double box[3], const
         if (!dst) return;
         for (int k = 0; k < cnt; ++k) dst[k] = wgt * val(off + static_cast<size_t>(stride) * k, scale);
     };
-    fill(out->lj_a, b.lj_a, t.n_lj, 1, wt.vdw, mdg::kAdjScaleLJ);
-    fill(out->lj_b, b.lj_b, t.n_lj, 1, wt.vdw, mdg::kAdjScaleLJ);
+    if (mixed) {
+        fill(out->lj_a, b.lj_a, t.n_lj, 1, wt.vdw, mdg::kAdjScaleLJm);
+        fill(out->lj_b, b.lj_b, t.n_lj, 1, wt.vdw, mdg::kAdjScaleLJm);
+    } else {
+        for (int k = 0; k < t.n_lj; ++k) {
+            if (out->lj_a) out->lj_a[k] = wt.vdw * ljd[k];
+            if (out->lj_b) out->lj_b[k] = wt.vdw * ljd[t.n_lj + k];
+        }
+    }
     fill(out->lj14_a, b.lj14_a, t.n_lj, 1, wt.vdw14, mdg::kAdjScaleLJ);
     fill(out->lj14_b, b.lj14_b, t.n_lj, 1, wt.vdw14, mdg::kAdjScaleLJ);
     size_t o = b.bonded;

@@ -59,6 +59,11 @@ struct AdjointBuf {
     unsigned long long* acc = nullptr;
     double* pot_rec = nullptr;             // [N] reciprocal potential (FP64, plain stores)
     size_t cap_pot = 0;
+    // FP64 form: LJ derivative sums in FP64 (a pair at a fraction of an
+    // angstrom, possible for atoms without LJ, exceeds any fixed-point range
+    // fine enough for the FP64 bar); [2 n_lj]: dE/dA then dE/dB
+    double* lj_d = nullptr;
+    size_t cap_lj_d = 0;
     size_t q_dir = 0, q_excl = 0, q_14 = 0;          // [N] each
     size_t lj_a = 0, lj_b = 0, lj14_a = 0, lj14_b = 0;  // [n_lj] each
     size_t bonded = 0;                               // [2 x (bond, UB, angle, dihedral, improper types)]
@@ -67,7 +72,8 @@ struct AdjointBuf {
     size_t total = 0;
 };
 constexpr double kAdjScaleQ = 68719476736.0;        // 2^36: potentials
-constexpr double kAdjScaleLJ = 17592186044416.0;    // 2^44: LJ and CMAP coefficient sums
+constexpr double kAdjScaleLJ = 17592186044416.0;    // 2^44: 1-4 LJ and CMAP coefficient sums
+constexpr double kAdjScaleLJm = 4294967296.0;       // 2^32: LJ sums of the mixed form
 constexpr double kAdjScaleB = 1099511627776.0;      // 2^40: bonded constants and equilibria
 
 // Work arrays that depend on the atom count / box.
