@@ -253,6 +253,24 @@ def main():
     achieved = flops / (nb_ms * 1e-3) / 1e12
     traffic = read_traffic(os.path.join(ROOT, "profiles"))
     kernels_ms = {k: round(v[0] / max(v[1], 1), 5) for k, v in tim.items() if v[1] > 0}
+    # HBM-bound PME kernels: algorithmic bytes per launch (FP32 form) / HIP-event time
+    n_at, kg = WORKLOAD["atoms"], WORKLOAD["pme_grid"]
+    G, Gc = kg ** 3, kg * kg * (kg // 2 + 1)
+    pme_bytes = {
+        "spread_z_r2c": (16 * n_at + 8 * Gc, "16N (x, y, z, q) read + 8 Gc complex z-transformed grid written"),
+        "fft_y_y_inverse_z_c2r": (2 * 16 * Gc + 8 * Gc + 4 * G,
+                                  "y pass and inverse y pass 16 Gc each, z c2r 8 Gc read + 4 G written"),
+        "x_conv": (16 * Gc, "x pass + influence function + inverse x pass: 8 Gc read + 8 Gc written"),
+        "gather": (16 * n_at + 4 * G + 12 * n_at, "16N + 4G read, 12N written"),
+    }
+    group = {"spread_z_r2c": "spread", "fft_y_y_inverse_z_c2r": "fft", "x_conv": "conv", "gather": "gather"}
+    pme_roofline = {}
+    for k, (nbytes, what) in pme_bytes.items():
+        ms = kernels_ms.get(group[k])
+        if ms:
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            pme_roofline[k] = {"bound": "hbm", "bytes": nbytes, "ms": ms, "achieved": gbs, "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "algorithmic": what}
 
     batched = None if args.no_batched else bench_batched(torch, N, systems)
     adjoint = None if args.no_adjoint else bench_adjoint(torch, N, systems)
@@ -293,6 +311,7 @@ def main():
         "energy_rel_err_mixed_vs_fp64": term_err,
         "work": counts,
         "kernel_ms": kernels_ms,
+        "pme_roofline": pme_roofline,
         "roofline": {
             "bound": "mfma",
             "engine": "VALU FP32 (no MFMA: the pair kernel is not a dense contraction)",

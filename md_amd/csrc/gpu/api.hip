@@ -63,9 +63,10 @@ hipError_t ensure(T** p, size_t* cap, size_t n) {
 void free_work(mdgpu_ctx* c) {
     Work& w = c->work;
     void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.block_hist, w.block_off, w.col_start, w.col_cstart, w.col_tmp,
-                    w.slot_atom, w.slot_meta, w.slot_pos_d, w.slot_pos_f, w.cluster_col, w.cl_origin, w.cl_half,
+                    w.slot_atom, w.slot_meta, w.slot_pos_d, w.slot_pos_f, w.slot_pme_fl, w.slot_pme_wd,
+                    w.slot_pme_wf, w.cluster_col, w.cl_origin, w.cl_half,
                     w.list_count, w.list_j, w.list_mask, w.list_delta_d, w.list_delta_f, w.force_d, w.force_fx,
-                    w.partial, w.energies, w.force_out, w.flags, w.excl_rmax, w.grid_d, w.grid_fx, w.grid_f, w.grid_phi_d,
+                    w.partial, w.energies, w.force_out, w.flags, w.excl_rmax, w.grid_f, w.grid_phi_d,
                     w.cgrid, w.bsp_mod, w.bsp_mod_f, w.twiddle_d, w.twiddle_f};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -414,16 +415,12 @@ int ensure_pme(mdgpu_ctx* c, const double box[3]) {
     if (!same) {
         const size_t G = static_cast<size_t>(K[0]) * K[1] * K[2];
         const size_t Gc = static_cast<size_t>(K[0]) * K[1] * (K[2] / 2 + 1);
-        void* ptrs[] = {w.grid_d, w.grid_fx, w.grid_f, w.grid_phi_d, w.cgrid, w.bsp_mod, w.twiddle_d, w.twiddle_f};
+        void* ptrs[] = {w.grid_f, w.grid_phi_d, w.cgrid, w.bsp_mod, w.twiddle_d, w.twiddle_f};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.grid_d), G * sizeof(double)), "alloc grid");
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.grid_fx), G * sizeof(unsigned long long)), "alloc grid");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.grid_f), G * sizeof(float)), "alloc grid");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.grid_phi_d), G * sizeof(double)), "alloc grid");
         HIP_TRY(hipMalloc(&w.cgrid, Gc * sizeof(double2)), "alloc grid");
-        HIP_TRY(hipMemset(w.grid_d, 0, G * sizeof(double)), "clear grid");
-        HIP_TRY(hipMemset(w.grid_fx, 0, G * sizeof(unsigned long long)), "clear grid");
         const int nk = K[0] + K[1] + K[2];
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.bsp_mod), nk * sizeof(double)), "alloc moduli");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.twiddle_d), 2 * nk * sizeof(double)), "alloc twiddles");
@@ -521,14 +518,17 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     }
     const size_t ncl = c->n_clusters, nslots = ncl * mdg::kCluster;
     if (w.cap_slots < nslots) {
-        void* ptrs[] = {w.slot_atom, w.slot_meta, w.slot_pos_d, w.slot_pos_f, w.cluster_col, w.cl_origin, w.cl_half,
-                        w.list_count};
+        void* ptrs[] = {w.slot_atom, w.slot_meta, w.slot_pos_d, w.slot_pos_f, w.slot_pme_fl, w.slot_pme_wd,
+                        w.slot_pme_wf, w.cluster_col, w.cl_origin, w.cl_half, w.list_count};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_atom), nslots * sizeof(int)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_meta), nslots * sizeof(int2)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_pos_d), nslots * sizeof(double4)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_pos_f), nslots * sizeof(float4)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_pme_fl), nslots * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_pme_wd), nslots * sizeof(double4)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_pme_wf), nslots * sizeof(float4)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.cluster_col), ncl * sizeof(int)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.cl_origin), ncl * sizeof(double4)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.cl_half), ncl * sizeof(float4)), "alloc");

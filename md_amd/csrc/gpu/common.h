@@ -53,6 +53,21 @@ __device__ __forceinline__ void acc_add(const FixedAcc& a, int idx, double v) {
     if (q != 0) atomicAdd(a.f + idx, static_cast<unsigned long long>(q));
 }
 
+// The same conversion in FP32 arithmetic for a float contribution, scale
+// 2^s with s <= 32: v = t + fr with t = trunc(v) (an int32) and fr exact, so
+// round(v 2^s) ~ t 2^s + 2 trunc(fr 2^(s-1)) (error below 2^(1-s)).  half =
+// 2^(s-1) as float; |v| must stay below 2^31.
+__device__ __forceinline__ long long to_fixed_f32(float v, int s, float half, unsigned int* overflow) {
+    if (!(fabsf(v) < 2147483520.0f)) {  // also catches NaN
+        atomicOr(overflow, 1u);
+        return 0;
+    }
+    const float t = truncf(v);
+    const int hi = static_cast<int>(t);
+    const int lo = static_cast<int>((v - t) * half);
+    return static_cast<long long>(hi) * (1LL << s) + 2LL * static_cast<long long>(lo);
+}
+
 __device__ __forceinline__ void acc_add(double* f, int idx, double v) {
     if (v != 0.0) atomicAdd(f + idx, v);
 }

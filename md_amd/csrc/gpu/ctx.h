@@ -93,6 +93,11 @@ struct Work {
     int2* slot_meta = nullptr;       // [slots] (LJ type, original atom or -1)
     double4* slot_pos_d = nullptr;   // [slots] position relative to the cluster origin + charge
     float4* slot_pos_f = nullptr;
+    // PME per slot (AMBER with a grid): base grid index floor(K x / L) mod K
+    // per dimension packed 10 bits each, and the fractions w (x, y, z) + charge
+    int* slot_pme_fl = nullptr;
+    double4* slot_pme_wd = nullptr;
+    float4* slot_pme_wf = nullptr;
     int* cluster_col = nullptr;      // [clusters]
     double4* cl_origin = nullptr;    // [clusters] bbox centre (w unused)
     float4* cl_half = nullptr;       // [clusters] bbox half extents
@@ -118,8 +123,6 @@ struct Work {
     // PME
     int pme_k[3] = {0, 0, 0};
     size_t cap_grid = 0;
-    double* grid_d = nullptr;                   // [K^3] charge grid (FP64 form) / potential
-    unsigned long long* grid_fx = nullptr;      // [K^3] fixed-point charge grid (mixed form)
     float* grid_f = nullptr;                    // [K^3] potential (mixed form)
     double* grid_phi_d = nullptr;               // [K^3] potential (FP64 form)
     void* cgrid = nullptr;                      // [Kx*Ky*(Kz/2+1)] complex (double2 / float2)
@@ -163,7 +166,7 @@ struct mdgpu_ctx {
     int n_clusters = 0;
     int n_cols = 0;
     int split = 4;                  // waves per i-cluster list in the pair kernel
-    int fft_lines = 0;              // cap on FFT lines per block (power of two; 0 = 16)
+    int fft_lines = 4;              // cap on FFT lines per block (power of two; 0 = 16)
     const double* cur_x = nullptr;  // raw coordinates of the evaluation in progress (device)
     bool search_general = false;    // use the general (per-shift) cluster search
     std::vector<void*> topo_allocs;
@@ -214,6 +217,7 @@ int sort_blocks(int n);
 
 // fixed-point scale exponent of the mixed form for a semantics
 inline double fixed_scale(int semantics) { return semantics == MDGPU_SEM_REF ? 1048576.0 : 4294967296.0; }
+
 
 }  // namespace mdg
 

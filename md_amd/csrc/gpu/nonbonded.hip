@@ -53,9 +53,10 @@ struct NbArgs {
     real alpha, kc, krf_crf;           // krf_crf: 1/rc_ee (reaction field with eps = 1)
     double* force_d;
     FixedAcc fx;
+    int fx_shift;                      // log2 of the fixed-point scale
+    float fx_half;                     // 2^(fx_shift - 1)
     double* partial;
     int row0;
-    int count_pairs;
 };
 
 // erfc(x) for x >= 0 from e = exp(-x^2): Abramowitz & Stegun 7.1.26
@@ -91,15 +92,19 @@ struct Math<double> {
     }
 };
 
-template <typename real>
-__device__ __forceinline__ void add_force(const NbArgs<real>& a, int idx, real v);
-template <>
-__device__ __forceinline__ void add_force<float>(const NbArgs<float>& a, int idx, float v) {
-    acc_add(a.fx, idx, static_cast<double>(v));
-}
-template <>
-__device__ __forceinline__ void add_force<double>(const NbArgs<double>& a, int idx, double v) {
-    acc_add(a.force_d, idx, v);
+// Force contribution: FP64 atomics (FP64 form); 64-bit fixed point (mixed
+// form), converted in FP32 arithmetic for AMBER semantics (|F| < 2^31) and in
+// FP64 for REF semantics, whose bonded neighbours interact with full LJ.
+template <typename real, bool kAmber>
+__device__ __forceinline__ void add_force(const NbArgs<real>& a, int idx, real v) {
+    if constexpr (sizeof(real) == 8) {
+        acc_add(a.force_d, idx, v);
+    } else if constexpr (kAmber) {
+        const long long q = to_fixed_f32(v, a.fx_shift, a.fx_half, a.fx.overflow);
+        if (q != 0) atomicAdd(a.fx.f + idx, static_cast<unsigned long long>(q));
+    } else {
+        acc_add(a.fx, idx, static_cast<double>(v));
+    }
 }
 
 // Sum over the 8 consecutive lanes of a group with three DPP moves
@@ -146,10 +151,13 @@ __device__ __forceinline__ double bcast<double>(double v, int lane) {
 
 // Lane l handles i = l % 8 and j = l / 8 of the tile, so the per-tile j-force
 // sum runs over 8 consecutive lanes (DPP) and the i-force sum, done once per
-// wave, over lanes 8 apart.  List entries are fetched 64 at a time, one per
-// lane, and broadcast with readlane; the j-cluster data of the next tile is
-// loaded while the current tile is computed.
-template <typename real, bool kAmber, bool kSwitch>
+// work item, over lanes 8 apart.  List entries are fetched 64 at a time, one
+// per lane, and broadcast with readlane; the j-cluster data of the next tile
+// is loaded while the current tile is computed.
+//
+// One wave per work item (i-cluster, list part); blocks are ordered so that
+// consecutive clusters run on one XCD and share its L2.
+template <typename real, bool kAmber, bool kSwitch, bool kCount>
 __global__ __launch_bounds__(kWaves * kWave) void k_nonbonded(const NbArgs<real> a) {
     using R4 = typename V4<real>::T;
     using M = Math<real>;
@@ -165,134 +173,131 @@ __global__ __launch_bounds__(kWaves * kWave) void k_nonbonded(const NbArgs<real>
 
     const int lane = threadIdx.x & 63;
     const int ti = lane & 7, tj = lane >> 3;
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int gw = __builtin_amdgcn_readfirstlane(blk * kWaves + (threadIdx.x >> 6));
-    const int ci = gw / a.split, part = gw % a.split;
     const int ncl = min(*a.ncl_dev, a.max_clusters);
-    int k0 = 0, k1 = 0;
-    if (ci < ncl) {
-        const int cnt = a.list_count[ci];
-        k0 = static_cast<int>((static_cast<long long>(cnt) * part) / a.split);
-        k1 = static_cast<int>((static_cast<long long>(cnt) * (part + 1)) / a.split);
-    }
-    const int cis = ci < ncl ? ci : 0;
-    const R4 pi = a.pos[cis * kCluster + ti];
-    const int2 mi = a.meta[cis * kCluster + ti];
-    const int tyi = mi.x * a.ntp;
-    const real qi = pi.w * a.kc;
     const int mbit = 8 * ti + tj;  // mask bit of this lane's pair (i-major, as the search writes it)
 
-    real fix = 0, fiy = 0, fiz = 0;
     // energy accumulators: FP64 form in double; mixed form in float, flushed to
     // double every 32 tiles
     double e_vdw = 0, e_all = 0, e_list = 0, e_rf = 0, e_ew = 0, n_pairs = 0;
     real s_vdw = 0, s_all = 0, s_list = 0, s_rf = 0, s_ew = 0;
     const real tsp = static_cast<real>(1.1283791670955126) * a.alpha;  // 2 alpha / sqrt(pi)
     const real rc2 = a.rc2_lj > a.rc2_ee ? a.rc2_lj : a.rc2_ee;
-    const size_t base = static_cast<size_t>(cis) * a.cap;
     int flush = 0;
-    for (int kb = k0; kb < k1; kb += kWave) {
-        const int nb = min(kWave, k1 - kb);
-        // this lane's list entry of the batch
-        int e_j = 0;
-        unsigned long long e_m = 0;
-        R4 e_d = {0, 0, 0, 0};
-        if (lane < nb) {
-            e_j = a.list_j[base + kb + lane];
-            e_m = a.list_mask[base + kb + lane];
-            e_d = a.list_delta[base + kb + lane];
-        }
-        const int e_mlo = static_cast<int>(e_m & 0xffffffffull), e_mhi = static_cast<int>(e_m >> 32);
-        int cj = __builtin_amdgcn_readlane(e_j, 0);
-        R4 pj = a.pos[cj * kCluster + tj];
-        int2 mj = a.meta[cj * kCluster + tj];
-        for (int t = 0; t < nb; ++t) {
-            const R4 pcur = pj;
-            const int2 mcur = mj;
-            if (t + 1 < nb) {
-                cj = __builtin_amdgcn_readlane(e_j, t + 1);
-                pj = a.pos[cj * kCluster + tj];
-                mj = a.meta[cj * kCluster + tj];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int item = __builtin_amdgcn_readfirstlane(blk * kWaves + (threadIdx.x >> 6));
+    if (item < ncl * a.split) {
+        const int ci = item / a.split, part = item % a.split;
+        const int cnt = a.list_count[ci];
+        const int k0 = static_cast<int>((static_cast<long long>(cnt) * part) / a.split);
+        const int k1 = static_cast<int>((static_cast<long long>(cnt) * (part + 1)) / a.split);
+        const R4 pi = a.pos[ci * kCluster + ti];
+        const int2 mi = a.meta[ci * kCluster + ti];
+        const int tyi = mi.x * a.ntp;
+        const real qi = pi.w * a.kc;
+        real fix = 0, fiy = 0, fiz = 0;
+        const size_t base = static_cast<size_t>(ci) * a.cap;
+        for (int kb = k0; kb < k1; kb += kWave) {
+            const int nb = min(kWave, k1 - kb);
+            // this lane's list entry of the batch
+            int e_j = 0;
+            unsigned long long e_m = 0;
+            R4 e_d = {0, 0, 0, 0};
+            if (lane < nb) {
+                e_j = a.list_j[base + kb + lane];
+                e_m = a.list_mask[base + kb + lane];
+                e_d = a.list_delta[base + kb + lane];
             }
-            const unsigned long long mask =
-                (static_cast<unsigned long long>(static_cast<unsigned int>(__builtin_amdgcn_readlane(e_mhi, t))) << 32) |
-                static_cast<unsigned int>(__builtin_amdgcn_readlane(e_mlo, t));
-            const real dx = pi.x - pcur.x + bcast<real>(e_d.x, t);
-            const real dy = pi.y - pcur.y + bcast<real>(e_d.y, t);
-            const real dz = pi.z - pcur.z + bcast<real>(e_d.z, t);
-            const real r2 = dx * dx + dy * dy + dz * dz;
-            const bool act = ((mask >> mbit) & 1ull) && r2 <= rc2 && r2 >= static_cast<real>(1e-12);
-            real f = 0;
-            if (act) {
-                const real rinv = M::rsqrt(r2);
-                const real r = r2 * rinv;
-                const real r2inv = rinv * rinv;
-                const int tp = tyi + mcur.x;
-                const R4 lj = s_lj[tp];
-                // work count: real (non-padding) pairs inside the cutoff
-                if (a.count_pairs && mcur.y >= 0 && mi.y >= 0) n_pairs += 1.0;
-                if (r2 <= a.rc2_lj) {
-                    const real r6 = r2inv * r2inv * r2inv;
-                    const real ar6 = lj.x * r6;
-                    const real u = (ar6 - lj.y) * r6;  // A/r^12 - B/r^6
-                    const real g = (static_cast<real>(12) * ar6 - static_cast<real>(6) * lj.y) * r6 * r2inv;  // -U'/r
-                    real el, fl;
-                    if (kSwitch) {
-                        real xs = (r - a.rs) * a.inv_sw;
-                        xs = xs < 0 ? static_cast<real>(0) : xs;
-                        const real x2 = xs * xs;
-                        const real sw = 1 + x2 * xs * (static_cast<real>(-10) + xs * (static_cast<real>(15) - 6 * xs));
-                        const real dsw = x2 * (static_cast<real>(-30) + xs * (static_cast<real>(60) - 30 * xs)) * a.inv_sw;
-                        el = u * sw;
-                        fl = sw * g - u * dsw * rinv;
-                    } else {
-                        el = u - lj.z + (r - a.rc_lj) * (lj.w * a.rc_lj);
-                        fl = g - lj.w * a.rc_lj * rinv;
-                    }
-                    if (!kAmber && !s_listed[tp]) { el = 0; fl = 0; }
-                    s_vdw += el;
-                    f += fl;
+            const int e_mlo = static_cast<int>(e_m & 0xffffffffull), e_mhi = static_cast<int>(e_m >> 32);
+            int cj = __builtin_amdgcn_readlane(e_j, 0);
+            R4 pj = a.pos[cj * kCluster + tj];
+            int2 mj = a.meta[cj * kCluster + tj];
+            for (int t = 0; t < nb; ++t) {
+                const R4 pcur = pj;
+                const int2 mcur = mj;
+                if (t + 1 < nb) {
+                    cj = __builtin_amdgcn_readlane(e_j, t + 1);
+                    pj = a.pos[cj * kCluster + tj];
+                    mj = a.meta[cj * kCluster + tj];
                 }
-                if (r2 <= a.rc2_ee) {
-                    const real qq = qi * pcur.w;
-                    real ec, ex;
-                    M::erfc_exp(a.alpha * r, ec, ex);
-                    const real e_c = qq * rinv;
-                    if (kAmber) {
-                        s_ew += e_c * ec;
-                        f += qq * (ec * rinv + tsp * ex) * r2inv;
-                    } else {
-                        const real lst = s_listed[tp] ? static_cast<real>(1) : static_cast<real>(0);
-                        s_all += e_c;
-                        s_list += lst * e_c;
-                        s_rf += lst * qq * (rinv - a.krf_crf);
-                        s_ew += lst * e_c * ec;
-                        f += lst * e_c * r2inv;
+                const unsigned long long mask =
+                    (static_cast<unsigned long long>(static_cast<unsigned int>(__builtin_amdgcn_readlane(e_mhi, t))) << 32) |
+                    static_cast<unsigned int>(__builtin_amdgcn_readlane(e_mlo, t));
+                const real dx = pi.x - pcur.x + bcast<real>(e_d.x, t);
+                const real dy = pi.y - pcur.y + bcast<real>(e_d.y, t);
+                const real dz = pi.z - pcur.z + bcast<real>(e_d.z, t);
+                const real r2 = dx * dx + dy * dy + dz * dz;
+                const bool act = ((mask >> mbit) & 1ull) && r2 <= rc2 && r2 >= static_cast<real>(1e-12);
+                real f = 0;
+                if (act) {
+                    const real rinv = M::rsqrt(r2);
+                    const real r = r2 * rinv;
+                    const real r2inv = rinv * rinv;
+                    const int tp = tyi + mcur.x;
+                    const R4 lj = s_lj[tp];
+                    // work count: real (non-padding) pairs inside the cutoff
+                    if (kCount && mcur.y >= 0 && mi.y >= 0) n_pairs += 1.0;
+                    if (r2 <= a.rc2_lj) {
+                        const real r6 = r2inv * r2inv * r2inv;
+                        const real ar6 = lj.x * r6;
+                        const real u = (ar6 - lj.y) * r6;  // A/r^12 - B/r^6
+                        const real g = (static_cast<real>(12) * ar6 - static_cast<real>(6) * lj.y) * r6 * r2inv;  // -U'/r
+                        real el, fl;
+                        if (kSwitch) {
+                            real xs = (r - a.rs) * a.inv_sw;
+                            xs = xs < 0 ? static_cast<real>(0) : xs;
+                            const real x2 = xs * xs;
+                            const real sw = 1 + x2 * xs * (static_cast<real>(-10) + xs * (static_cast<real>(15) - 6 * xs));
+                            const real dsw = x2 * (static_cast<real>(-30) + xs * (static_cast<real>(60) - 30 * xs)) * a.inv_sw;
+                            el = u * sw;
+                            fl = sw * g - u * dsw * rinv;
+                        } else {
+                            el = u - lj.z + (r - a.rc_lj) * (lj.w * a.rc_lj);
+                            fl = g - lj.w * a.rc_lj * rinv;
+                        }
+                        if (!kAmber && !s_listed[tp]) { el = 0; fl = 0; }
+                        s_vdw += el;
+                        f += fl;
+                    }
+                    if (r2 <= a.rc2_ee) {
+                        const real qq = qi * pcur.w;
+                        real ec, ex;
+                        M::erfc_exp(a.alpha * r, ec, ex);
+                        const real e_c = qq * rinv;
+                        if (kAmber) {
+                            s_ew += e_c * ec;
+                            f += qq * (ec * rinv + tsp * ex) * r2inv;
+                        } else {
+                            const real lst = s_listed[tp] ? static_cast<real>(1) : static_cast<real>(0);
+                            s_all += e_c;
+                            s_list += lst * e_c;
+                            s_rf += lst * qq * (rinv - a.krf_crf);
+                            s_ew += lst * e_c * ec;
+                            f += lst * e_c * r2inv;
+                        }
                     }
                 }
-            }
-            const real fx = f * dx, fy = f * dy, fz = f * dz;
-            fix += fx;
-            fiy += fy;
-            fiz += fz;
-            if (__ballot(act) != 0ull) {
-                const real gx = sum8(fx), gy = sum8(fy), gz = sum8(fz);
-                if (ti < 3 && mcur.y >= 0) add_force<real>(a, 3 * mcur.y + ti, -(ti == 0 ? gx : (ti == 1 ? gy : gz)));
-            }
-            if (++flush == 32) {
-                flush = 0;
-                e_vdw += s_vdw; e_all += s_all; e_list += s_list; e_rf += s_rf; e_ew += s_ew;
-                s_vdw = s_all = s_list = s_rf = s_ew = 0;
+                const real fx = f * dx, fy = f * dy, fz = f * dz;
+                fix += fx;
+                fiy += fy;
+                fiz += fz;
+                if (__ballot(act) != 0ull) {
+                    const real gx = sum8(fx), gy = sum8(fy), gz = sum8(fz);
+                    if (ti < 3 && mcur.y >= 0) add_force<real, kAmber>(a, 3 * mcur.y + ti, -(ti == 0 ? gx : (ti == 1 ? gy : gz)));
+                }
+                if (++flush == 32) {
+                    flush = 0;
+                    e_vdw += s_vdw; e_all += s_all; e_list += s_list; e_rf += s_rf; e_ew += s_ew;
+                    s_vdw = s_all = s_list = s_rf = s_ew = 0;
+                }
             }
         }
+        // i-forces of this item: sum over the 8 lanes sharing i (8 apart)
+        fix += shfl_xor(fix, 8);  fiy += shfl_xor(fiy, 8);  fiz += shfl_xor(fiz, 8);
+        fix += shfl_xor(fix, 16); fiy += shfl_xor(fiy, 16); fiz += shfl_xor(fiz, 16);
+        fix += shfl_xor(fix, 32); fiy += shfl_xor(fiy, 32); fiz += shfl_xor(fiz, 32);
+        if (tj < 3 && k1 > k0 && mi.y >= 0) add_force<real, kAmber>(a, 3 * mi.y + tj, tj == 0 ? fix : (tj == 1 ? fiy : fiz));
     }
     e_vdw += s_vdw; e_all += s_all; e_list += s_list; e_rf += s_rf; e_ew += s_ew;
-    // i-forces: sum over the 8 lanes sharing i (8 apart)
-    fix += shfl_xor(fix, 8);  fiy += shfl_xor(fiy, 8);  fiz += shfl_xor(fiz, 8);
-    fix += shfl_xor(fix, 16); fiy += shfl_xor(fiy, 16); fiz += shfl_xor(fiz, 16);
-    fix += shfl_xor(fix, 32); fiy += shfl_xor(fiy, 32); fiz += shfl_xor(fiz, 32);
-    if (ci < ncl && tj < 3 && k1 > k0 && mi.y >= 0)
-        add_force<real>(a, 3 * mi.y + tj, tj == 0 ? fix : (tj == 1 ? fiy : fiz));
     double v[6] = {e_vdw, e_all, e_list, e_rf, e_ew, n_pairs};
     const int slots[6] = {E_VDW, E_EE_ALL, E_EE_LIST, E_EE_RF, E_EW_DIRECT, E_PAIRS};
     block_sum_store<6>(v, a.partial + static_cast<size_t>(a.row0 + blockIdx.x) * E_NSLOTS, slots);
@@ -336,19 +341,26 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0) {
     a.krf_crf = static_cast<real>(1.0 / p.ee_cutoff);
     a.force_d = w.force_d;
     a.fx = FixedAcc{w.force_fx, fixed_scale(p.semantics), w.flags};
+    a.fx_shift = p.semantics == MDGPU_SEM_REF ? 20 : 32;
+    a.fx_half = static_cast<float>(0.5 * fixed_scale(p.semantics));
     a.partial = w.partial;
     a.row0 = row0;
-    a.count_pairs = c->counting ? 1 : 0;
     const int nblk = nonbonded_rows(c);
     const size_t shm = (sizeof(typename V4<real>::T) + 1) * a.ntp * a.ntp;
     dim3 grid(nblk), block(kWaves * kWave);
+#define MDG_NB_LAUNCH(AM, SW)                                                                                   \
+    do {                                                                                                       \
+        if (c->counting) hipLaunchKernelGGL((k_nonbonded<real, AM, SW, true>), grid, block, shm, c->stream, a);  \
+        else hipLaunchKernelGGL((k_nonbonded<real, AM, SW, false>), grid, block, shm, c->stream, a);            \
+    } while (0)
     if (amber) {
-        if (sw) hipLaunchKernelGGL((k_nonbonded<real, true, true>), grid, block, shm, c->stream, a);
-        else hipLaunchKernelGGL((k_nonbonded<real, true, false>), grid, block, shm, c->stream, a);
+        if (sw) MDG_NB_LAUNCH(true, true);
+        else MDG_NB_LAUNCH(true, false);
     } else {
-        if (sw) hipLaunchKernelGGL((k_nonbonded<real, false, true>), grid, block, shm, c->stream, a);
-        else hipLaunchKernelGGL((k_nonbonded<real, false, false>), grid, block, shm, c->stream, a);
+        if (sw) MDG_NB_LAUNCH(false, true);
+        else MDG_NB_LAUNCH(false, false);
     }
+#undef MDG_NB_LAUNCH
     return hipGetLastError();
 }
 

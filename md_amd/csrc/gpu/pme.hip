@@ -14,9 +14,11 @@
 //   G(m) = kC exp(-pi^2 m^2 / a^2) / (pi V m^2) B(m),  m = (m_x/L_x, ...),
 //   F_i = -q_i sum_k grad_i theta(k) (IFFT(G FFT(Q)))(k).
 // Grid layout: real [Kx][Ky][Kz] (z fastest), complex [Kx][Ky][Kz/2+1].
-// Passes: z r2c (also clears the charge grid) -> y -> x (+ G, E, virial, x^-1)
-// -> y^-1 -> z c2r.
+// Passes: spread fused with z r2c -> y -> x (+ G, E, virial, x^-1) -> y^-1
+// -> z c2r.
 #include "ctx.h"
+
+#include <type_traits>
 
 namespace mdg {
 
@@ -51,55 +53,6 @@ struct PmeGeom {
     int kx, ky, kz, hz;
     double lx, ly, lz;
 };
-
-__device__ __forceinline__ void frac(const double* wrapped, int i, const PmeGeom& g, int* fl, double* w) {
-    const double u[3] = {wrapped[3 * i] * g.kx / g.lx, wrapped[3 * i + 1] * g.ky / g.ly,
-                         wrapped[3 * i + 2] * g.kz / g.lz};
-    for (int d = 0; d < 3; ++d) {
-        fl[d] = static_cast<int>(floor(u[d]));
-        w[d] = u[d] - fl[d];
-    }
-}
-
-__device__ __forceinline__ int wrapk(int k, int n) {
-    k %= n;
-    return k < 0 ? k + n : k;
-}
-
-// ---------------------------------------------------------------- spread --
-template <typename real>
-__global__ void k_spread(int n, const double* __restrict__ wrapped, const double* __restrict__ charge, PmeGeom g,
-                         double* __restrict__ grid_d, unsigned long long* __restrict__ grid_fx,
-                         unsigned int* overflow) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double q = charge[i];
-    if (q == 0.0) return;
-    int fl[3];
-    double w[3];
-    frac(wrapped, i, g, fl, w);
-    real th[3][4], dth[3][4];
-    for (int d = 0; d < 3; ++d) bspline4<real>(static_cast<real>(w[d]), th[d], dth[d]);
-    const real qr = static_cast<real>(q);
-    for (int a = 0; a < 4; ++a) {
-        const int ix = wrapk(fl[0] - a, g.kx);
-        for (int b = 0; b < 4; ++b) {
-            const int iy = wrapk(fl[1] - b, g.ky);
-            const real qab = qr * th[0][a] * th[1][b];
-            const size_t row = (static_cast<size_t>(ix) * g.ky + iy) * g.kz;
-            for (int c = 0; c < 4; ++c) {
-                const int iz = wrapk(fl[2] - c, g.kz);
-                const real v = qab * th[2][c];
-                if constexpr (sizeof(real) == 8) {
-                    atomicAdd(grid_d + row + iz, v);
-                } else {
-                    const long long fx = to_fixed(static_cast<double>(v), kGridScale, overflow);
-                    atomicAdd(grid_fx + row + iz, static_cast<unsigned long long>(fx));
-                }
-            }
-        }
-    }
-}
 
 // ------------------------------------------------------------ LDS FFT -----
 // Batched Stockham FFT of nl lines (nl a power of two) of length n held in
@@ -183,39 +136,123 @@ __device__ __forceinline__ const typename C2<real>::T* stage_twiddles(typename C
     return s_tw;
 }
 
-// --------------------------------------------------------------- z r2c ----
+// ------------------------------------------------- spread + z r2c --------
+// Owner-computes charge spreading fused with the z transform.  A block owns
+// bx x by z-lines (x in [X0, X0+bx), y in [Y0, Y0+by), all z) and builds them
+// in LDS.  It visits the slots of the sort columns that can hold atoms
+// reaching its lines (an atom with base index f contributes to points
+// f - 0..3, so these lines receive f_x in [X0, X0+bx+2], i.e. x in
+// [X0 hx, (X0+bx+3) hx)), reads each slot's spline record (base indices,
+// fractions, charge; written once per evaluation by the cluster set-up), and
+// adds q theta_x theta_y theta_z with 64-bit integer LDS atomics (2^40 fixed
+// point: the sum does not depend on their order).  It then converts the lines
+// and transforms them along z in place.  Every grid point is written by one
+// block, so the real grid never goes to memory and needs no clearing.
+struct SpreadCols {
+    int ncx, ncy;
+    double sx, sy;
+    const int* col_cstart;   // [ncol+1] cluster offsets
+    const int* fl;           // [slots] packed base indices
+    const void* w;           // [slots] fractions + charge (double4 / float4)
+};
+
+constexpr int kSpreadCols = 256;  // columns per pass of the candidate loop
+
 template <typename real>
-__global__ void k_fft_z_r2c(PmeGeom g, int lognl, double* grid_d, unsigned long long* grid_fx,
-                            typename C2<real>::T* __restrict__ cg, const typename C2<real>::T* __restrict__ tw_g) {
+__global__ __launch_bounds__(kThreads) void k_spread_fft_z(PmeGeom g, SpreadCols sc, int bx, int by, int lognl,
+                                                           typename C2<real>::T* __restrict__ cg,
+                                                           const typename C2<real>::T* __restrict__ tw_g,
+                                                           unsigned int* overflow) {
     using c2 = typename C2<real>::T;
+    using R4 = typename std::conditional<sizeof(real) == 8, double4, float4>::type;
     extern __shared__ unsigned char smem[];
-    const int nl = 1 << lognl;
+    __shared__ int s_first[kSpreadCols], s_pre[kSpreadCols + 1];
+    const int nl = 1 << lognl;  // == bx * by
     c2* A = reinterpret_cast<c2*>(smem);
     c2* B = A + g.kz * nl;
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(B);  // [kz][nl], aliases B
     const c2* tw = stage_twiddles<real>(B + g.kz * nl, tw_g, g.kz);
-    const int nlines = g.kx * g.ky;
-    const int l0 = blockIdx.x * nl;
-    const int nv = min(nl, nlines - l0);
-    const size_t base = static_cast<size_t>(l0) * g.kz;
-    for (int t = threadIdx.x; t < nl * g.kz; t += blockDim.x) {
-        const int l = t / g.kz, i = t % g.kz;
-        real v = 0;
-        if (l < nv) {
-            if constexpr (sizeof(real) == 8) {
-                v = grid_d[base + t];
-                grid_d[base + t] = 0.0;
-            } else {
-                v = static_cast<real>(static_cast<double>(static_cast<long long>(grid_fx[base + t])) / kGridScale);
-                grid_fx[base + t] = 0ull;
+    for (int t = threadIdx.x; t < nl * g.kz; t += blockDim.x) acc[t] = 0ull;
+    const int X0 = blockIdx.x * bx, Y0 = blockIdx.y * by;
+    const double hx = g.lx / g.kx, hy = g.ly / g.ky, tol = 1e-6;
+    // columns overlapping [X0 hx, (X0+bx+3) hx) x [Y0 hy, (Y0+by+3) hy), with
+    // 1e-6 A for the rounding of floor(u) against the column binning
+    int cx0 = static_cast<int>(floor((X0 * hx - tol) / sc.sx));
+    int ncxr = static_cast<int>(floor(((X0 + bx + 3) * hx + tol) / sc.sx)) - cx0 + 1;
+    int cy0 = static_cast<int>(floor((Y0 * hy - tol) / sc.sy));
+    int ncyr = static_cast<int>(floor(((Y0 + by + 3) * hy + tol) / sc.sy)) - cy0 + 1;
+    if (ncxr >= sc.ncx) { cx0 = 0; ncxr = sc.ncx; }
+    if (ncyr >= sc.ncy) { cy0 = 0; ncyr = sc.ncy; }
+    const int ncand = ncxr * ncyr;
+    const R4* wrec = static_cast<const R4*>(sc.w);
+    for (int cb = 0; cb < ncand; cb += kSpreadCols) {
+        const int nc = min(kSpreadCols, ncand - cb);
+        __syncthreads();  // the previous pass is done with s_first / s_pre
+        if (threadIdx.x < nc) {
+            const int k = cb + threadIdx.x;
+            const int cy = ((cy0 + k / ncxr) % sc.ncy + sc.ncy) % sc.ncy;
+            const int cx = ((cx0 + k % ncxr) % sc.ncx + sc.ncx) % sc.ncx;
+            const int col = cx + sc.ncx * cy;
+            s_first[threadIdx.x] = sc.col_cstart[col] * kCluster;
+            s_pre[threadIdx.x + 1] = (sc.col_cstart[col + 1] - sc.col_cstart[col]) * kCluster;
+        }
+        if (threadIdx.x == 0) s_pre[0] = 0;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int k = 0; k < nc; ++k) s_pre[k + 1] += s_pre[k];
+        __syncthreads();
+        const int total = s_pre[nc];
+        for (int t = threadIdx.x; t < total; t += blockDim.x) {
+            // column of candidate t: the last k with s_pre[k] <= t
+            int lo = 0, hi = nc - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_pre[mid] <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            const int s = s_first[lo] + (t - s_pre[lo]);
+            const int fl = sc.fl[s];
+            const R4 wq = wrec[s];
+            if (wq.w == 0) continue;  // padding slot or uncharged atom
+            const int flx = fl & 1023, fly = (fl >> 10) & 1023, flz = fl >> 20;
+            int dxp = flx - X0, dyp = fly - Y0;
+            dxp += dxp < 0 ? g.kx : 0;
+            dyp += dyp < 0 ? g.ky : 0;
+            if (dxp > bx + 2 || dyp > by + 2) continue;
+            real th[3][4], dth[3][4];
+            bspline4<real>(wq.x, th[0], dth[0]);
+            bspline4<real>(wq.y, th[1], dth[1]);
+            bspline4<real>(wq.z, th[2], dth[2]);
+            const double q = static_cast<double>(wq.w);
+            for (int a = 0; a < 4; ++a) {
+                const int li = dxp - a;
+                if (li < 0 || li >= bx || X0 + li >= g.kx) continue;
+                for (int b = 0; b < 4; ++b) {
+                    const int lj = dyp - b;
+                    if (lj < 0 || lj >= by || Y0 + lj >= g.ky) continue;
+                    const int l = li * by + lj;
+                    const double qab = q * static_cast<double>(th[0][a] * th[1][b]);
+                    for (int c = 0; c < 4; ++c) {
+                        int iz = flz - c;
+                        iz += iz < 0 ? g.kz : 0;
+                        const long long v = to_fixed(qab * static_cast<double>(th[2][c]), kGridScale, overflow);
+                        if (v != 0) atomicAdd(acc + iz * nl + l, static_cast<unsigned long long>(v));
+                    }
+                }
             }
         }
-        A[i * nl + l] = {v, static_cast<real>(0)};
     }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nl * g.kz; t += blockDim.x)
+        A[t] = {static_cast<real>(static_cast<double>(static_cast<long long>(acc[t])) * (1.0 / kGridScale)),
+                static_cast<real>(0)};
     __syncthreads();
     c2* R = lds_fft<real>(A, B, g.kz, lognl, tw, -1);
     for (int t = threadIdx.x; t < nl * g.hz; t += blockDim.x) {
-        const int l = t / g.hz, k = t % g.hz;
-        if (l < nv) cg[static_cast<size_t>(l0 + l) * g.hz + k] = R[k * nl + l];
+        const int l = t / g.hz, k = t - (t / g.hz) * g.hz;
+        const int li = l / by, lj = l - li * by;
+        if (X0 + li < g.kx && Y0 + lj < g.ky)
+            cg[(static_cast<size_t>(X0 + li) * g.ky + Y0 + lj) * g.hz + k] = R[k * nl + l];
     }
 }
 
@@ -345,32 +382,49 @@ __global__ void k_fft_z_c2r(PmeGeom g, int lognl, const typename C2<real>::T* __
 }
 
 // ---------------------------------------------------------------- gather --
+// One thread per slot, from the same spline records as the spreading.
 template <typename real>
-__global__ void k_gather(int n, const double* __restrict__ wrapped, const double* __restrict__ charge, PmeGeom g,
-                         const real* __restrict__ phi, double* force_d, FixedAcc fxa) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double q = charge[i];
-    if (q == 0.0) return;
-    int fl[3];
-    double w[3];
-    frac(wrapped, i, g, fl, w);
+__global__ void k_gather(const int* ncl_dev, int max_slots, const int* __restrict__ slot_atom,
+                         const int* __restrict__ fl_rec, const void* w_rec, PmeGeom g,
+                         const real* __restrict__ phi, double* force_d, FixedAcc fxa, double* pot) {
+    using R4 = typename std::conditional<sizeof(real) == 8, double4, float4>::type;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= max_slots || s >= *ncl_dev * kCluster) return;
+    const int i = slot_atom[s];
+    if (i < 0) return;  // padding slot
+    const R4 wq = static_cast<const R4*>(w_rec)[s];
+    if (wq.w == 0 && !pot) return;  // no force on an uncharged atom; its potential is still needed
+    const int fl = fl_rec[s];
+    const int flx = fl & 1023, fly = (fl >> 10) & 1023, flz = fl >> 20;
     real th[3][4], dth[3][4];
-    for (int d = 0; d < 3; ++d) bspline4<real>(static_cast<real>(w[d]), th[d], dth[d]);
-    real gx = 0, gy = 0, gz = 0;
+    bspline4<real>(wq.x, th[0], dth[0]);
+    bspline4<real>(wq.y, th[1], dth[1]);
+    bspline4<real>(wq.z, th[2], dth[2]);
+    real gx = 0, gy = 0, gz = 0, ph = 0;
     for (int a = 0; a < 4; ++a) {
-        const int ix = wrapk(fl[0] - a, g.kx);
+        int ix = flx - a;
+        ix += ix < 0 ? g.kx : 0;
         for (int b = 0; b < 4; ++b) {
-            const int iy = wrapk(fl[1] - b, g.ky);
+            int iy = fly - b;
+            iy += iy < 0 ? g.ky : 0;
             const real* row = phi + (static_cast<size_t>(ix) * g.ky + iy) * g.kz;
             for (int c = 0; c < 4; ++c) {
-                const real p = row[wrapk(fl[2] - c, g.kz)];
+                int iz = flz - c;
+                iz += iz < 0 ? g.kz : 0;
+                const real p = row[iz];
                 gx += dth[0][a] * th[1][b] * th[2][c] * p;
                 gy += th[0][a] * dth[1][b] * th[2][c] * p;
                 gz += th[0][a] * th[1][b] * dth[2][c] * p;
+                ph += th[0][a] * th[1][b] * th[2][c] * p;
             }
         }
     }
+    if (pot) {
+        // the adjoint of the spreading: dE_rec/dq_i = sum_k theta_i(k) phi(k)
+        pot[i] = static_cast<double>(ph);
+        return;
+    }
+    const double q = static_cast<double>(wq.w);
     const double fx = -q * static_cast<double>(gx) * g.kx / g.lx;
     const double fy = -q * static_cast<double>(gy) * g.ky / g.ly;
     const double fz = -q * static_cast<double>(gz) * g.kz / g.lz;
@@ -383,32 +437,6 @@ __global__ void k_gather(int n, const double* __restrict__ wrapped, const double
         acc_add(fxa, 3 * i + 1, fy);
         acc_add(fxa, 3 * i + 2, fz);
     }
-}
-
-// Reciprocal potential at each atom, dE_rec/dq_i = sum_k theta_i(k) phi(k)
-// (the adjoint of the spreading), one thread per atom, plain stores.
-template <typename real>
-__global__ void k_potential(int n, const double* __restrict__ wrapped, PmeGeom g, const real* __restrict__ phi,
-                            double* __restrict__ pot) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int fl[3];
-    double w[3];
-    frac(wrapped, i, g, fl, w);
-    real th[3][4], dth[3][4];
-    for (int d = 0; d < 3; ++d) bspline4<real>(static_cast<real>(w[d]), th[d], dth[d]);
-    double s = 0.0;
-    for (int a = 0; a < 4; ++a) {
-        const int ix = wrapk(fl[0] - a, g.kx);
-        for (int b = 0; b < 4; ++b) {
-            const int iy = wrapk(fl[1] - b, g.ky);
-            const real* row = phi + (static_cast<size_t>(ix) * g.ky + iy) * g.kz;
-            real sab = 0;
-            for (int c = 0; c < 4; ++c) sab += th[2][c] * row[wrapk(fl[2] - c, g.kz)];
-            s += static_cast<double>(th[0][a] * th[1][b] * sab);
-        }
-    }
-    pot[i] = s;
 }
 
 // Lines per FFT block: two LDS buffers of n x nl complex values plus n
@@ -442,18 +470,22 @@ hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used, double* pot) {
     auto mark = [&](int k, bool start) {
         if (T.mode == 1) (void)hipEventRecord(c->ev[2 * k + (start ? 0 : 1)], c->stream);
     };
-    const int nb = (n + kThreads - 1) / kThreads;
+    const int max_slots = c->n_clusters * kCluster;
+    const int ncol = c->grid.ncx * c->grid.ncy;
+    const void* wrec = dbl ? static_cast<const void*>(w.slot_pme_wd) : static_cast<const void*>(w.slot_pme_wf);
+    // spreading fused with the z transform (timed as the spread)
     mark(MDGPU_K_SPREAD, true);
-    hipLaunchKernelGGL(k_spread<real>, dim3(nb), dim3(kThreads), 0, c->stream, n, w.wrapped, c->topo.charge, g,
-                       w.grid_d, w.grid_fx, w.flags);
-    mark(MDGPU_K_SPREAD, false);
-    mark(MDGPU_K_FFT, true);
     {
         const int nl = lines_per_block(g.kz, sizeof(c2), c->fft_lines);
+        const int by = std::min(nl, 8), bx = nl / by;
         const size_t shm = (2 * static_cast<size_t>(g.kz) * nl + g.kz) * sizeof(c2);
-        hipLaunchKernelGGL(k_fft_z_r2c<real>, dim3((g.kx * g.ky + nl - 1) / nl), dim3(kThreads), shm, c->stream, g,
-                           ilog2(nl), w.grid_d, w.grid_fx, cg, twz);
+        const SpreadCols sc{c->grid.ncx, c->grid.ncy, c->grid.sx, c->grid.sy, w.col_cstart, w.slot_pme_fl, wrec};
+        const dim3 grid((g.kx + bx - 1) / bx, (g.ky + by - 1) / by);
+        hipLaunchKernelGGL(k_spread_fft_z<real>, grid, dim3(kThreads), shm, c->stream, g, sc, bx, by, ilog2(nl), cg,
+                           twz, w.flags);
     }
+    mark(MDGPU_K_SPREAD, false);
+    mark(MDGPU_K_FFT, true);
     const int nly = lines_per_block(g.ky, sizeof(c2), c->fft_lines);
     const size_t shy = (2 * static_cast<size_t>(g.ky) * nly + g.ky) * sizeof(c2);
     hipLaunchKernelGGL(k_fft_y<real>, dim3((g.hz + nly - 1) / nly, g.kx), dim3(kThreads), shy, c->stream, g,
@@ -479,11 +511,9 @@ hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used, double* pot) {
     }
     mark(MDGPU_K_COUNT, false);
     mark(MDGPU_K_GATHER, true);
-    if (pot)
-        hipLaunchKernelGGL(k_potential<real>, dim3(nb), dim3(kThreads), 0, c->stream, n, w.wrapped, g, phi, pot);
-    else
-        hipLaunchKernelGGL(k_gather<real>, dim3(nb), dim3(kThreads), 0, c->stream, n, w.wrapped, c->topo.charge, g,
-                           phi, w.force_d, FixedAcc{w.force_fx, fixed_scale(c->params.semantics), w.flags});
+    hipLaunchKernelGGL(k_gather<real>, dim3((max_slots + kThreads - 1) / kThreads), dim3(kThreads), 0, c->stream,
+                       w.col_cstart + ncol, max_slots, w.slot_atom, w.slot_pme_fl, wrec, g, phi, w.force_d,
+                       FixedAcc{w.force_fx, fixed_scale(c->params.semantics), w.flags}, pot);
     mark(MDGPU_K_GATHER, false);
     return hipGetLastError();
 }

@@ -161,12 +161,20 @@ struct Vec4<double> { using T = double4; };
 // Eight lanes per cluster: bounding box by shuffles, then the relative
 // coordinates.  Padding slots get the padding type, zero charge, and an
 // offset of kFarAway.
+// PME geometry for the per-slot spline records (k[0] == 0: no records).
+struct PmeSlots {
+    int k[3];
+    double l[3];
+    int* fl;
+    void* w;   // double4 / float4 per slot: fractions (x, y, z) and charge
+};
+
 template <typename real>
 __global__ void k_cluster_setup(const int* __restrict__ ncl_dev, int max_clusters, const int* __restrict__ slot_atom,
                                 const double* __restrict__ wrapped, const double* __restrict__ charge,
                                 const int* __restrict__ type, int pad_type, typename Vec4<real>::T* __restrict__ pos,
                                 int2* __restrict__ slot_meta, double4* __restrict__ origin,
-                                float4* __restrict__ half) {
+                                float4* __restrict__ half, const PmeSlots pme) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int cl = slot / kCluster;
     const int ncl = *ncl_dev;
@@ -203,6 +211,30 @@ __global__ void k_cluster_setup(const int* __restrict__ ncl_dev, int max_cluster
         slot_meta[slot] = make_int2(pad_type, -1);
     }
     pos[slot] = p;
+    if (pme.k[0] > 0) {
+        // u = K x / L in FP64; fl = floor(u) taken mod K (the stencil wraps
+        // anyway), w = u - floor(u)
+        int packed = 0;
+        typename Vec4<real>::T wq = {0, 0, 0, 0};
+        if (atom >= 0) {
+            const double xs[3] = {x, y, z};
+            real wv[3];
+            for (int d = 0; d < 3; ++d) {
+                const double u = xs[d] * pme.k[d] / pme.l[d];
+                const double f = floor(u);
+                int fl = static_cast<int>(f) % pme.k[d];
+                fl += fl < 0 ? pme.k[d] : 0;
+                packed |= fl << (10 * d);
+                wv[d] = static_cast<real>(u - f);
+            }
+            wq.x = wv[0];
+            wq.y = wv[1];
+            wq.z = wv[2];
+            wq.w = static_cast<real>(charge[atom]);
+        }
+        pme.fl[slot] = packed;
+        reinterpret_cast<typename Vec4<real>::T*>(pme.w)[slot] = wq;
+    }
     if ((slot & (kCluster - 1)) == 0) {
         origin[cl] = make_double4(ox, oy, oz, 0.0);
         // round the half extents up so the float box still contains every atom
@@ -535,14 +567,22 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed) {
     const int* ncl_dev = w.col_cstart + ncol;
     const int max_cl = c->n_clusters;
     const int nslots = max_cl * kCluster;
+    PmeSlots pme{};
+    const bool with_pme = c->params.semantics == MDGPU_SEM_AMBER && w.pme_k[0] > 0;
+    for (int d = 0; d < 3; ++d) {
+        pme.k[d] = with_pme ? w.pme_k[d] : 0;
+        pme.l[d] = g.box[d];
+    }
+    pme.fl = w.slot_pme_fl;
+    pme.w = mixed ? static_cast<void*>(w.slot_pme_wf) : static_cast<void*>(w.slot_pme_wd);
     if (mixed) {
         hipLaunchKernelGGL(k_cluster_setup<float>, dim3((nslots + 255) / 256), dim3(256), 0, c->stream, ncl_dev,
                            max_cl, w.slot_atom, w.wrapped, c->topo.charge, c->topo.type, c->topo.n_types,
-                           w.slot_pos_f, w.slot_meta, w.cl_origin, w.cl_half);
+                           w.slot_pos_f, w.slot_meta, w.cl_origin, w.cl_half, pme);
     } else {
         hipLaunchKernelGGL(k_cluster_setup<double>, dim3((nslots + 255) / 256), dim3(256), 0, c->stream, ncl_dev,
                            max_cl, w.slot_atom, w.wrapped, c->topo.charge, c->topo.type, c->topo.n_types,
-                           w.slot_pos_d, w.slot_meta, w.cl_origin, w.cl_half);
+                           w.slot_pos_d, w.slot_meta, w.cl_origin, w.cl_half, pme);
     }
     return hipGetLastError();
 }

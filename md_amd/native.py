@@ -456,6 +456,7 @@ class Batch:
         if st != 0:
             raise GpuError(st, f"mdgpu_batch_create(device={device}) failed")
         self.n_atoms = []
+        self._settings = {"streams": 4, "split": 4, "fft_lines": 4}  # mdgpu_batch_create's defaults
 
     def _check(self, st):
         if st != 0:
@@ -486,8 +487,14 @@ class Batch:
                                               f.ctypes.data_as(P_F64) if f is not None else None))
         return [y.as_dict() for y in e], (f.reshape(-1, 3) if f is not None else None)
 
-    def configure(self, mode=0, streams=4, split=4, fft_lines=0):
-        self._check(self.lib.mdgpu_batch_configure(self.b, mode, streams, split, fft_lines))
+    def configure(self, mode=0, streams=None, split=None, fft_lines=None):
+        """Launch settings; an argument left out keeps its current value."""
+        cur = self._settings
+        s = cur["streams"] if streams is None else streams
+        p = cur["split"] if split is None else split
+        f = cur["fft_lines"] if fft_lines is None else fft_lines
+        self._check(self.lib.mdgpu_batch_configure(self.b, mode, s, p, f))
+        self._settings = {"streams": s, "split": p, "fft_lines": f}
 
     def tune(self, coords_ptr: int, forces_ptr: int = 0, reps: int = 3) -> dict:
         s, p, fl = C.c_int(), C.c_int(), C.c_int()
@@ -495,6 +502,7 @@ class Batch:
         self._check(self.lib.mdgpu_batch_tune(self.b, C.c_void_p(coords_ptr),
                                               C.c_void_p(forces_ptr) if forces_ptr else None, reps,
                                               C.byref(s), C.byref(p), C.byref(fl), C.byref(ms)))
+        self._settings = {"streams": s.value, "split": p.value, "fft_lines": fl.value}
         return {"streams": s.value, "split": p.value, "fft_lines": fl.value, "ms_serial_pass": ms.value}
 
     def graph_state(self) -> int:

@@ -86,7 +86,7 @@ def test_graph_batch_matches_separate_contexts(prec):
         e = batch.evaluate_device(xd.data_ptr(), fd.data_ptr())
         _same(e, e_ref, fd.cpu().numpy(), f_ref, exact)
     # the serial mode (one synchronous evaluation per system)
-    batch.configure(mode=1, streams=4, split=4)
+    batch.configure(mode=1)
     fd.zero_()
     e = batch.evaluate_device(xd.data_ptr(), fd.data_ptr())
     _same(e, e_ref, fd.cpu().numpy(), f_ref, exact)
