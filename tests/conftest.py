@@ -36,3 +36,8 @@ def water_6k(sysdir):
 @pytest.fixture(scope="session")
 def polymer_29k(sysdir):
     return _system(sysdir, "polymer_29k")
+
+
+@pytest.fixture(scope="session")
+def scale_100k(sysdir):
+    return _system(sysdir, "scale_100k")
