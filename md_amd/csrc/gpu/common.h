@@ -73,6 +73,34 @@ __device__ __forceinline__ void acc_add(double* f, int idx, double v) {
 }
 
 // ---------------------------------------------------------------------------
+// Buffer atomics switched off per lane through the offset.  A lane whose byte
+// offset is kOffOOB lies beyond the resource's size and the hardware drops its
+// operation, so every lane issues the instruction and no branch surrounds it:
+// the compiler then counts the operation as always issued when it places the
+// waits for earlier loads.
+// ---------------------------------------------------------------------------
+using BufRsrc = __amdgpu_buffer_rsrc_t;
+constexpr unsigned int kOffOOB = 0x80000000u;
+// word 3 of a gfx9 buffer descriptor: 32-bit data format, no swizzle
+constexpr int kBufWord3 = 0x00020000;
+
+__device__ long long llvm_buf_atomic_add_i64(long long v, BufRsrc r, unsigned int off, int soff, int aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.atomic.add.i64");
+__device__ double llvm_buf_atomic_fadd_f64(double v, BufRsrc r, unsigned int off, int soff, int aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.atomic.fadd.f64");
+
+// Resource over `bytes` bytes from p; bytes must stay below kOffOOB.
+__device__ __forceinline__ BufRsrc buf_rsrc(void* p, unsigned int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(p, 0, static_cast<int>(bytes), kBufWord3);
+}
+__device__ __forceinline__ void buf_add(BufRsrc r, unsigned int off, long long q) {
+    llvm_buf_atomic_add_i64(q, r, off, 0, 0);
+}
+__device__ __forceinline__ void buf_add(BufRsrc r, unsigned int off, double v) {
+    llvm_buf_atomic_fadd_f64(v, r, off, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
 // wave-level helpers
 // ---------------------------------------------------------------------------
 template <typename T>

@@ -57,6 +57,7 @@ struct NbArgs {
     float fx_half;                     // 2^(fx_shift - 1)
     double* partial;
     int row0;
+    int n3;                            // 3 x atoms: entries of the force arrays
 };
 
 // erfc(x) for x >= 0 from e = exp(-x^2): Abramowitz & Stegun 7.1.26
@@ -92,18 +93,21 @@ struct Math<double> {
     }
 };
 
-// Force contribution: FP64 atomics (FP64 form); 64-bit fixed point (mixed
-// form), converted in FP32 arithmetic for AMBER semantics (|F| < 2^31) and in
-// FP64 for REF semantics, whose bonded neighbours interact with full LJ.
+// Force contribution of lanes with on set: FP64 atomics (FP64 form); 64-bit
+// fixed point (mixed form), converted in FP32 arithmetic for AMBER semantics
+// (|F| < 2^31) and in FP64 for REF semantics, whose bonded neighbours interact
+// with full LJ.  Every lane issues the buffer atomic; lanes that add nothing
+// point it past the end of the array.
 template <typename real, bool kAmber>
-__device__ __forceinline__ void add_force(const NbArgs<real>& a, int idx, real v) {
+__device__ __forceinline__ void add_force(const NbArgs<real>& a, BufRsrc r, bool on, int idx, real v) {
+    const unsigned int off = static_cast<unsigned int>(idx) * 8u;
     if constexpr (sizeof(real) == 8) {
-        acc_add(a.force_d, idx, v);
-    } else if constexpr (kAmber) {
-        const long long q = to_fixed_f32(v, a.fx_shift, a.fx_half, a.fx.overflow);
-        if (q != 0) atomicAdd(a.fx.f + idx, static_cast<unsigned long long>(q));
+        buf_add(r, on && v != 0.0 ? off : kOffOOB, v);
     } else {
-        acc_add(a.fx, idx, static_cast<double>(v));
+        long long q;
+        if constexpr (kAmber) q = to_fixed_f32(v, a.fx_shift, a.fx_half, a.fx.overflow);
+        else q = to_fixed(static_cast<double>(v), a.fx.scale, a.fx.overflow);
+        buf_add(r, on && q != 0 ? off : kOffOOB, q);
     }
 }
 
@@ -152,8 +156,10 @@ __device__ __forceinline__ double bcast<double>(double v, int lane) {
 // Lane l handles i = l % 8 and j = l / 8 of the tile, so the per-tile j-force
 // sum runs over 8 consecutive lanes (DPP) and the i-force sum, done once per
 // work item, over lanes 8 apart.  List entries are fetched 64 at a time, one
-// per lane, and broadcast with readlane; the j-cluster data of the next tile
-// is loaded while the current tile is computed.
+// per lane, and broadcast with readlane; the j-cluster data of a tile is
+// loaded two tiles ahead into one of two register sets.  The force atomics
+// are buffer atomics issued by every lane (see add_force), so the wait for a
+// prefetched set never has to cover the atomic of the tile just finished.
 //
 // One wave per work item (i-cluster, list part); blocks are ordered so that
 // consecutive clusters run on one XCD and share its L2.
@@ -183,6 +189,7 @@ __global__ __launch_bounds__(kWaves * kWave) void k_nonbonded(const NbArgs<real>
     const real tsp = static_cast<real>(1.1283791670955126) * a.alpha;  // 2 alpha / sqrt(pi)
     const real rc2 = a.rc2_lj > a.rc2_ee ? a.rc2_lj : a.rc2_ee;
     int flush = 0;
+    const BufRsrc rsrc = sizeof(real) == 8 ? buf_rsrc(a.force_d, 8u * a.n3) : buf_rsrc(a.fx.f, 8u * a.n3);
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int item = __builtin_amdgcn_readfirstlane(blk * kWaves + (threadIdx.x >> 6));
     if (item < ncl * a.split) {
@@ -208,20 +215,20 @@ __global__ __launch_bounds__(kWaves * kWave) void k_nonbonded(const NbArgs<real>
                 e_d = a.list_delta[base + kb + lane];
             }
             const int e_mlo = static_cast<int>(e_m & 0xffffffffull), e_mhi = static_cast<int>(e_m >> 32);
-            int cj = __builtin_amdgcn_readlane(e_j, 0);
-            R4 pj = a.pos[cj * kCluster + tj];
-            int2 mj = a.meta[cj * kCluster + tj];
-            for (int t = 0; t < nb; ++t) {
-                const R4 pcur = pj;
-                const int2 mcur = mj;
-                if (t + 1 < nb) {
-                    cj = __builtin_amdgcn_readlane(e_j, t + 1);
-                    pj = a.pos[cj * kCluster + tj];
-                    mj = a.meta[cj * kCluster + tj];
-                }
-                const unsigned long long mask =
+
+            // j-cluster data of tile t (clamped to the batch, so every fetch
+            // is issued and reads a valid cluster)
+            auto fetch = [&](int t, R4& p, int2& m) {
+                const int cj = __builtin_amdgcn_readlane(e_j, min(t, nb - 1));
+                p = a.pos[cj * kCluster + tj];
+                m = a.meta[cj * kCluster + tj];
+            };
+            // tile t of the batch; live = false runs it with an empty mask
+            auto tile = [&](const R4& pcur, const int2& mcur, int t, bool live) {
+                unsigned long long mask =
                     (static_cast<unsigned long long>(static_cast<unsigned int>(__builtin_amdgcn_readlane(e_mhi, t))) << 32) |
                     static_cast<unsigned int>(__builtin_amdgcn_readlane(e_mlo, t));
+                mask = live ? mask : 0ull;
                 const real dx = pi.x - pcur.x + bcast<real>(e_d.x, t);
                 const real dy = pi.y - pcur.y + bcast<real>(e_d.y, t);
                 const real dz = pi.z - pcur.z + bcast<real>(e_d.z, t);
@@ -280,22 +287,36 @@ __global__ __launch_bounds__(kWaves * kWave) void k_nonbonded(const NbArgs<real>
                 fix += fx;
                 fiy += fy;
                 fiz += fz;
-                if (__ballot(act) != 0ull) {
-                    const real gx = sum8(fx), gy = sum8(fy), gz = sum8(fz);
-                    if (ti < 3 && mcur.y >= 0) add_force<real, kAmber>(a, 3 * mcur.y + ti, -(ti == 0 ? gx : (ti == 1 ? gy : gz)));
-                }
+                // j-forces: every tile issues its atomic (an empty tile adds
+                // nothing), so the waits for the prefetched clusters stay short
+                const real gx = sum8(fx), gy = sum8(fy), gz = sum8(fz);
+                add_force<real, kAmber>(a, rsrc, ti < 3 && mcur.y >= 0, 3 * mcur.y + ti,
+                                        -(ti == 0 ? gx : (ti == 1 ? gy : gz)));
                 if (++flush == 32) {
                     flush = 0;
                     e_vdw += s_vdw; e_all += s_all; e_list += s_list; e_rf += s_rf; e_ew += s_ew;
                     s_vdw = s_all = s_list = s_rf = s_ew = 0;
                 }
+            };
+
+            // Two register sets, each fetched two tiles before its use: the
+            // wait for a set then covers operations issued a tile earlier.
+            R4 pA, pB;
+            int2 mA, mB;
+            fetch(0, pA, mA);
+            fetch(1, pB, mB);
+            for (int t = 0; t < nb; t += 2) {
+                tile(pA, mA, t, true);
+                fetch(t + 2, pA, mA);
+                tile(pB, mB, t + 1, t + 1 < nb);
+                fetch(t + 3, pB, mB);
             }
         }
         // i-forces of this item: sum over the 8 lanes sharing i (8 apart)
         fix += shfl_xor(fix, 8);  fiy += shfl_xor(fiy, 8);  fiz += shfl_xor(fiz, 8);
         fix += shfl_xor(fix, 16); fiy += shfl_xor(fiy, 16); fiz += shfl_xor(fiz, 16);
         fix += shfl_xor(fix, 32); fiy += shfl_xor(fiy, 32); fiz += shfl_xor(fiz, 32);
-        if (tj < 3 && k1 > k0 && mi.y >= 0) add_force<real, kAmber>(a, 3 * mi.y + tj, tj == 0 ? fix : (tj == 1 ? fiy : fiz));
+        add_force<real, kAmber>(a, rsrc, tj < 3 && k1 > k0 && mi.y >= 0, 3 * mi.y + tj, tj == 0 ? fix : (tj == 1 ? fiy : fiz));
     }
     e_vdw += s_vdw; e_all += s_all; e_list += s_list; e_rf += s_rf; e_ew += s_ew;
     double v[6] = {e_vdw, e_all, e_list, e_rf, e_ew, n_pairs};
@@ -345,6 +366,7 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0) {
     a.fx_half = static_cast<float>(0.5 * fixed_scale(p.semantics));
     a.partial = w.partial;
     a.row0 = row0;
+    a.n3 = 3 * t.n_atoms;
     const int nblk = nonbonded_rows(c);
     const size_t shm = (sizeof(typename V4<real>::T) + 1) * a.ntp * a.ntp;
     dim3 grid(nblk), block(kWaves * kWave);
