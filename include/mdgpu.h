@@ -163,7 +163,8 @@ enum {
     MDGPU_K_PAIRLIST = 1,   /* cluster-pair search                          */
     MDGPU_K_NONBONDED = 2,  /* LJ + Coulomb/Ewald pair kernel               */
     MDGPU_K_BONDED = 3,     /* fused bonded kernel                          */
-    MDGPU_K_CORRECT = 4,    /* FP64 excluded / 1-4 corrections              */
+    MDGPU_K_CORRECT = 4,    /* FP64 excluded / 1-4 corrections: run in the
+                               bonded launch and timed there (stays 0)      */
     MDGPU_K_SPREAD = 5,     /* PME charge spreading                         */
     MDGPU_K_FFT = 6,        /* PME r2c + c2c passes (forward + inverse)     */
     MDGPU_K_CONV = 7,       /* PME influence function + energy + virial     */

@@ -62,7 +62,7 @@ hipError_t ensure(T** p, size_t* cap, size_t n) {
 
 void free_work(mdgpu_ctx* c) {
     Work& w = c->work;
-    void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.block_hist, w.block_off, w.col_start, w.col_cstart, w.col_tmp,
+    void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_count, w.block_emax, w.block_off, w.col_start, w.col_cstart, w.col_key,
                     w.slot_atom, w.slot_meta, w.slot_pos_d, w.slot_pos_f, w.slot_pme_fl, w.slot_pme_wd,
                     w.slot_pme_wf, w.cluster_col, w.cl_origin, w.cl_half,
                     w.list_count, w.list_j, w.list_mask, w.list_delta_d, w.list_delta_f, w.force_d, w.force_fx,
@@ -131,6 +131,11 @@ int mdgpu_create(int device, mdgpu_ctx** out) {
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.flags), mdg::kFlags * sizeof(unsigned int));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.energies), (mdg::E_NSLOTS + mdg::kFlags) * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.excl_rmax), sizeof(float));
+    // energies and flags of every evaluation, written by the final reduction
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void**>(&c->h_energies), (mdg::E_NSLOTS + mdg::kFlags) * sizeof(double),
+                          hipHostMallocCoherent | hipHostMallocMapped);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_host_energies), c->h_energies, 0);
     // zero once; afterwards the final reduction clears them for the next evaluation
     if (e == hipSuccess) e = hipMemset(c->work.flags, 0, mdg::kFlags * sizeof(unsigned int));
     if (e == hipSuccess) e = hipMemset(c->work.excl_rmax, 0, sizeof(float));
@@ -153,6 +158,7 @@ int mdgpu_destroy(mdgpu_ctx* c) {
     if (c->adj.lj_d) (void)hipFree(c->adj.lj_d);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
+    if (c->h_energies) (void)hipHostFree(c->h_energies);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MDGPU_OK;
@@ -486,16 +492,17 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     size_t cap;
     cap = w.cap_atoms;
     if (w.cap_atoms < static_cast<size_t>(n)) {
-        void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_tmp, w.force_d, w.force_fx, w.force_out};
+        void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_key, w.force_d, w.force_fx, w.force_out};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         w.coords = w.wrapped = w.force_d = w.force_out = nullptr;
-        w.col_of = w.col_tmp = nullptr;
+        w.col_of = nullptr;
+        w.col_key = nullptr;
         w.force_fx = nullptr;
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.coords), 3 * n * sizeof(double)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.wrapped), 3 * n * sizeof(double)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_of), n * sizeof(int)), "alloc");
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_tmp), n * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_key), n * sizeof(unsigned long long)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_d), 3 * n * sizeof(double)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_fx), 3 * n * sizeof(unsigned long long)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_out), 3 * n * sizeof(double)), "alloc");
@@ -507,10 +514,13 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     (void)cap;
     const size_t nbc = static_cast<size_t>(mdg::sort_blocks(n)) * (ncol + 1);
     if (w.cap_cols < nbc) {
-        void* ptrs[] = {w.block_hist, w.block_off, w.col_start, w.col_cstart};
+        void* ptrs[] = {w.col_count, w.block_emax, w.block_off, w.col_start, w.col_cstart};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.block_hist), nbc * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_count), nbc * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.block_emax), nbc * sizeof(float)), "alloc");
+        // column totals start at zero; the column sort re-zeroes them every evaluation
+        HIP_TRY(hipMemset(w.col_count, 0, nbc * sizeof(int)), "clear");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.block_off), nbc * sizeof(int)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_start), nbc * sizeof(int)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_cstart), nbc * sizeof(int)), "alloc");
@@ -569,40 +579,55 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
 }
 
 // Final reduction.  Blocks [0, E_NSLOTS) each sum one energy slot over the
-// partial rows (fixed per-thread order and a fixed tree, so the result is
-// deterministic); block E_NSLOTS copies the flags to the output and clears
-// them; the remaining blocks convert the forces (fixed point -> FP64 in the
-// mixed form) into `out` and clear the accumulators for the next evaluation.
+// partial rows (fixed per-thread order, four independent partial sums per
+// thread, then a fixed wave and block tree, so the result is deterministic);
+// block E_NSLOTS copies the flags to the output and clears them; the
+// remaining blocks convert the forces (fixed point -> FP64 in the mixed form)
+// into `out` and clear the accumulators for the next evaluation.  Energies
+// and flags go to out_buf and, when out_host is set, also to that
+// host-mapped copy, which the host reads after the stream synchronisation.
 constexpr int kReduceThreads = 256;
 __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double* __restrict__ partial, int rows,
-                                                           double* __restrict__ out_buf, unsigned int* flags,
+                                                           double* __restrict__ out_buf, double* out_host,
+                                                           unsigned int* flags,
                                                            float* excl_rmax, int mixed, unsigned long long* fx,
                                                            double* fd, double inv_scale, int n3,
                                                            double* __restrict__ out,
                                                            const int* __restrict__ list_count,
                                                            const int* __restrict__ ncl_dev) {
-    __shared__ double red[kReduceThreads];
+    __shared__ double red[kReduceThreads / 64];
     const int b = blockIdx.x, t = threadIdx.x;
     if (b < mdg::E_NSLOTS) {
-        double s = 0.0;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
         if (b == mdg::E_TILES) {
             const int ncl = *ncl_dev;
-            for (int k = t; k < ncl; k += kReduceThreads) s += list_count[k];
+            for (int k = t; k < ncl; k += kReduceThreads) s0 += list_count[k];
         } else {
-            for (int r = t; r < rows; r += kReduceThreads) s += partial[static_cast<size_t>(r) * mdg::E_NSLOTS + b];
+            constexpr int kStep = 4 * kReduceThreads;
+            int r = t;
+            for (; r + 3 * kReduceThreads < rows; r += kStep) {
+                s0 += partial[static_cast<size_t>(r) * mdg::E_NSLOTS + b];
+                s1 += partial[static_cast<size_t>(r + kReduceThreads) * mdg::E_NSLOTS + b];
+                s2 += partial[static_cast<size_t>(r + 2 * kReduceThreads) * mdg::E_NSLOTS + b];
+                s3 += partial[static_cast<size_t>(r + 3 * kReduceThreads) * mdg::E_NSLOTS + b];
+            }
+            for (; r < rows; r += kReduceThreads) s0 += partial[static_cast<size_t>(r) * mdg::E_NSLOTS + b];
         }
-        red[t] = s;
+        double s = mdg::wave_sum((s0 + s1) + (s2 + s3));
+        if ((t & 63) == 0) red[t >> 6] = s;
         __syncthreads();
-        for (int m = kReduceThreads / 2; m > 0; m >>= 1) {
-            if (t < m) red[t] += red[t + m];
-            __syncthreads();
+        if (t == 0) {
+            s = (red[0] + red[1]) + (red[2] + red[3]);
+            out_buf[b] = s;
+            if (out_host) out_host[b] = s;
         }
-        if (t == 0) out_buf[b] = red[0];
         return;
     }
     if (b == mdg::E_NSLOTS) {
         if (t < mdg::kFlags) {
-            out_buf[mdg::E_NSLOTS + t] = static_cast<double>(flags[t]);
+            const double v = static_cast<double>(flags[t]);
+            out_buf[mdg::E_NSLOTS + t] = v;
+            if (out_host) out_host[mdg::E_NSLOTS + t] = v;
             flags[t] = 0u;
         }
         if (t == 0) *excl_rmax = 0.f;
@@ -636,8 +661,7 @@ int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out) {
         if (timed(k)) (void)hipEventRecord(c->ev[2 * k + (start ? 0 : 1)], c->stream);
     };
     mark(MDGPU_K_SORT, true);
-    if (amber && c->topo.has_excl) HIP_TRY(mdg::launch_excl_extent(c), "excl extent");
-    HIP_TRY(mdg::launch_sort(c, x, mixed), "sort");
+    HIP_TRY(mdg::launch_sort(c, x, mixed, amber && c->topo.has_excl), "sort");
     mark(MDGPU_K_SORT, false);
     mark(MDGPU_K_PAIRLIST, true);
     HIP_TRY(mdg::launch_pairsearch(c, mixed, amber), "pair search");
@@ -650,12 +674,8 @@ int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out) {
     mark(MDGPU_K_BONDED, true);
     HIP_TRY(mdg::launch_bonded(c, x, mixed, amber, row), "bonded");
     mark(MDGPU_K_BONDED, false);
-    row += mdg::bonded_rows(c);
+    row += mdg::bonded_rows(c) + (amber ? mdg::corrections_rows(c) : 0);
     if (amber) {
-        mark(MDGPU_K_CORRECT, true);
-        HIP_TRY(mdg::launch_corrections(c, mixed, row), "corrections");
-        mark(MDGPU_K_CORRECT, false);
-        row += mdg::corrections_rows(c);
         int used = 0;
         HIP_TRY(mdg::launch_pme(c, mixed, row, &used), "pme");
         row += used;
@@ -663,7 +683,7 @@ int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out) {
     mark(MDGPU_K_REDUCE, true);
     const int fblocks = std::max(1, std::min(512, (3 * n + kReduceThreads - 1) / kReduceThreads));
     hipLaunchKernelGGL(k_reduce, dim3(mdg::E_NSLOTS + 1 + fblocks), dim3(kReduceThreads), 0, c->stream, w.partial,
-                       row, w.energies, w.flags, w.excl_rmax, mixed ? 1 : 0, w.force_fx, w.force_d,
+                       row, w.energies, c->d_host_energies, w.flags, w.excl_rmax, mixed ? 1 : 0, w.force_fx, w.force_d,
                        1.0 / mdg::fixed_scale(p.semantics), 3 * n, d_out, w.list_count, w.col_cstart + c->n_cols);
     HIP_TRY(hipGetLastError(), "reduce");
     mark(MDGPU_K_REDUCE, false);
@@ -745,9 +765,10 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
         const double* x = coords_are_work ? w.coords : d_coords;
         st = enqueue_eval(c, x, d_out ? d_out : w.force_out);
         if (st != MDGPU_OK) return st;
-        double e[mdg::E_NSLOTS + mdg::kFlags];
-        HIP_TRY(hipMemcpyAsync(e, w.energies, sizeof e, hipMemcpyDeviceToHost, c->stream), "read energies");
+        // the reduction wrote energies and flags into host-mapped memory
         HIP_TRY(hipStreamSynchronize(c->stream), "evaluation");
+        double e[mdg::E_NSLOTS + mdg::kFlags];
+        std::memcpy(e, c->h_energies, sizeof e);
         st = check_flags(c, e);
         if (st == kRedo) continue;
         if (st != MDGPU_OK) return st;
@@ -760,7 +781,7 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
                     k == MDGPU_K_COUNT) {
                     if (!(amber && w.pme_k[0] > 0)) continue;
                 }
-                if (k == MDGPU_K_CORRECT && !amber) continue;
+                if (k == MDGPU_K_CORRECT) continue;  // timed within the bonded launch
                 float ms = 0.f;
                 if (hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]) == hipSuccess) {
                     const int slot = k == MDGPU_K_COUNT ? MDGPU_K_FFT : k;
@@ -896,8 +917,7 @@ int mdgpu_adjoint(mdgpu_ctx* c, const double* coords, const double box[3], const
         c->cur_x = w.coords;
         HIP_TRY(hipMemsetAsync(b.acc, 0, b.total * sizeof(unsigned long long), c->stream), "clear adjoint");
         HIP_TRY(hipMemsetAsync(b.lj_d, 0, b.cap_lj_d * sizeof(double), c->stream), "clear adjoint");
-        if (t.has_excl) HIP_TRY(mdg::launch_excl_extent(c), "excl extent");
-        HIP_TRY(mdg::launch_sort(c, w.coords, mixed), "sort");
+        HIP_TRY(mdg::launch_sort(c, w.coords, mixed, t.has_excl), "sort");
         HIP_TRY(mdg::launch_pairsearch(c, mixed, true), "pair search");
         HIP_TRY(mdg::launch_pair_adjoint(c, mixed), "pair adjoint");
         HIP_TRY(mdg::launch_corrections_adjoint(c), "corrections adjoint");

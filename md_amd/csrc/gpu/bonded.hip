@@ -9,7 +9,8 @@
 // taken in FP64 from the raw coordinates (the reference applies no minimum
 // image to bonded terms); the mixed form does the rest in FP32.
 //
-// k_corrections (AMBER semantics, FP64 in both forms) adds, with minimum
+// The corrections (AMBER semantics, FP64 in both forms; blocks after the
+// bonded ones in the same launch) add, with minimum
 // image and no cutoff, -k q_i q_j erf(a r)/r for every excluded pair (the
 // reciprocal sum counts it; cf. the exclusion loop of NonBondedEnergy.cpp:
 // 271-304) and, for every 1-4 pair, the 1-4 LJ (LENNARD_JONES_14 tables) plus
@@ -122,13 +123,13 @@ __device__ __forceinline__ real wrap_pi(real d) {
     return d;
 }
 
+// Term k of the concatenated lists; the block's energies go to row `row`.
 template <typename real>
-__global__ __launch_bounds__(kBlock) void k_bonded(const BArgs<real> a) {
+__device__ __forceinline__ void bonded_terms(const BArgs<real>& a, int k, int row) {
     const Topo& t = a.t;
     const double* x = a.x;
     const int nb = t.n_bonds, nu = nb + t.n_ubs, na = nu + t.n_angles, nd = na + t.n_dihedrals,
               ni = nd + t.n_impropers, nc = ni + t.n_cmaps;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double e[6] = {0, 0, 0, 0, 0, 0};  // bond, angle, dihedral, UB, improper, CMAP
     if (k < nu) {
         // harmonic bond / Urey-Bradley: E = k (r - r0)^2, F_a = -2k (r - r0) d / r
@@ -245,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_bonded(const BArgs<real> a) {
         }
     }
     const int slots[6] = {E_BOND, E_ANGLE, E_DIHEDRAL, E_UB, E_IMPROPER, E_CMAP};
-    block_sum_store<6>(e, a.partial + static_cast<size_t>(a.row0 + blockIdx.x) * E_NSLOTS, slots);
+    block_sum_store<6>(e, a.partial + static_cast<size_t>(row) * E_NSLOTS, slots);
 }
 
 struct CArgs {
@@ -281,8 +282,8 @@ __device__ __forceinline__ void addc(const CArgs& a, int atom, double fx, double
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_corrections(const CArgs a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+// Pair k of the excluded pairs followed by the 1-4 pairs.
+__device__ __forceinline__ void correction_pairs(const CArgs& a, int k, int row) {
     double e[3] = {0, 0, 0};  // ewald_excl, vdw14, ee14
     if (k < a.n_excl + a.n14) {
         const bool is14 = k >= a.n_excl;
@@ -316,7 +317,17 @@ __global__ __launch_bounds__(kBlock) void k_corrections(const CArgs a) {
         addc(a, p.y, -g * dx, -g * dy, -g * dz);
     }
     const int slots[3] = {E_EW_EXCL, E_VDW14, E_EE14};
-    block_sum_store<3>(e, a.partial + static_cast<size_t>(a.row0 + blockIdx.x) * E_NSLOTS, slots);
+    block_sum_store<3>(e, a.partial + static_cast<size_t>(row) * E_NSLOTS, slots);
+}
+
+// One launch for both: blocks [0, nbb) take the bonded terms, the rest the
+// corrections (both latency-bound, so they overlap instead of queueing).
+// Block b writes energy row row0 + b.
+template <typename real>
+__global__ __launch_bounds__(kBlock) void k_bonded(const BArgs<real> a, const CArgs ca, int nbb) {
+    const int b = static_cast<int>(blockIdx.x);
+    if (b < nbb) bonded_terms(a, b * kBlock + static_cast<int>(threadIdx.x), a.row0 + b);
+    else correction_pairs(ca, (b - nbb) * kBlock + static_cast<int>(threadIdx.x), a.row0 + b);
 }
 
 }  // namespace
@@ -332,48 +343,39 @@ int corrections_rows(const mdgpu_ctx* c) {
 }
 
 hipError_t launch_bonded(mdgpu_ctx* c, const double* coords, bool mixed, bool amber, int row0) {
-    const int nblk = bonded_rows(c);
-    if (nblk == 0) return hipSuccess;
-    if (mixed) {
-        BArgs<float> a{coords, c->topo, c->topo.cmap_coeff_f, c->work.force_d,
-                       FixedAcc{c->work.force_fx, fixed_scale(c->params.semantics), c->work.flags},
-                       c->work.partial, row0, amber ? 1 : 0};
-        hipLaunchKernelGGL(k_bonded<float>, dim3(nblk), dim3(kBlock), 0, c->stream, a);
-    } else {
-        BArgs<double> a{coords, c->topo, c->topo.cmap_coeff, c->work.force_d,
-                        FixedAcc{c->work.force_fx, fixed_scale(c->params.semantics), c->work.flags},
-                        c->work.partial, row0, amber ? 1 : 0};
-        hipLaunchKernelGGL(k_bonded<double>, dim3(nblk), dim3(kBlock), 0, c->stream, a);
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_corrections(mdgpu_ctx* c, bool mixed, int row0) {
-    const int nblk = corrections_rows(c);
-    if (nblk == 0) return hipSuccess;
+    const int nbb = bonded_rows(c);
+    const int ncb = amber ? corrections_rows(c) : 0;
+    if (nbb + ncb == 0) return hipSuccess;
     const Topo& t = c->topo;
-    CArgs a{};
-    a.x = c->cur_x;
-    a.lx = c->grid.box[0];
-    a.ly = c->grid.box[1];
-    a.lz = c->grid.box[2];
-    a.n_excl = t.n_excl_pairs;
-    a.excl = t.excl_pairs;
-    a.n14 = t.n_pairs14;
-    a.p14 = t.pairs14;
-    a.scee = t.scee14;
-    a.charge = t.charge;
-    a.type = t.type;
-    a.nb_index = t.nb_index;
-    a.lj14 = t.lj14;
-    a.ntypes = t.n_types;
-    a.alpha = c->params.ewald_alpha;
-    a.mixed = mixed ? 1 : 0;
-    a.force_d = c->work.force_d;
-    a.fx = FixedAcc{c->work.force_fx, fixed_scale(c->params.semantics), c->work.flags};
-    a.partial = c->work.partial;
-    a.row0 = row0;
-    hipLaunchKernelGGL(k_corrections, dim3(nblk), dim3(kBlock), 0, c->stream, a);
+    const FixedAcc fxa{c->work.force_fx, fixed_scale(c->params.semantics), c->work.flags};
+    CArgs ca{};
+    ca.x = coords;
+    ca.lx = c->grid.box[0];
+    ca.ly = c->grid.box[1];
+    ca.lz = c->grid.box[2];
+    ca.n_excl = t.n_excl_pairs;
+    ca.excl = t.excl_pairs;
+    ca.n14 = t.n_pairs14;
+    ca.p14 = t.pairs14;
+    ca.scee = t.scee14;
+    ca.charge = t.charge;
+    ca.type = t.type;
+    ca.nb_index = t.nb_index;
+    ca.lj14 = t.lj14;
+    ca.ntypes = t.n_types;
+    ca.alpha = c->params.ewald_alpha;
+    ca.mixed = mixed ? 1 : 0;
+    ca.force_d = c->work.force_d;
+    ca.fx = fxa;
+    ca.partial = c->work.partial;
+    ca.row0 = row0 + nbb;
+    if (mixed) {
+        BArgs<float> a{coords, t, t.cmap_coeff_f, c->work.force_d, fxa, c->work.partial, row0, amber ? 1 : 0};
+        hipLaunchKernelGGL(k_bonded<float>, dim3(nbb + ncb), dim3(kBlock), 0, c->stream, a, ca, nbb);
+    } else {
+        BArgs<double> a{coords, t, t.cmap_coeff, c->work.force_d, fxa, c->work.partial, row0, amber ? 1 : 0};
+        hipLaunchKernelGGL(k_bonded<double>, dim3(nbb + ncb), dim3(kBlock), 0, c->stream, a, ca, nbb);
+    }
     return hipGetLastError();
 }
 

@@ -82,11 +82,12 @@ struct Work {
     double* coords = nullptr;        // [3N] input (owned copy for host-pointer evals)
     double* wrapped = nullptr;       // [3N] wrapped into the box
     int* col_of = nullptr;           // [N] column of each atom
-    int* block_hist = nullptr;       // [sort blocks x ncol] column histogram per block
+    int* col_count = nullptr;        // [ncol] atoms per column (zero between evaluations)
     int* block_off = nullptr;        // [sort blocks x ncol] start of each block's run in a column
+    float* block_emax = nullptr;     // [sort blocks] largest excluded-pair distance per sort block
     int* col_start = nullptr;        // [ncol + 1] atom offsets
     int* col_cstart = nullptr;       // [ncol + 1] cluster offsets
-    int* col_tmp = nullptr;          // [N] atoms bucketed by column
+    unsigned long long* col_key = nullptr;  // [N] atoms bucketed by column as (z key << 32 | atom)
     size_t cap_cols = 0;
     size_t cap_slots = 0;            // clusters * 8
     int* slot_atom = nullptr;        // [slots] original atom or -1
@@ -176,17 +177,22 @@ struct mdgpu_ctx {
     std::vector<unsigned char> host_listed;
     // event pairs per kernel group; the extra pair times the inverse FFT passes
     hipEvent_t ev[2 * (MDGPU_K_COUNT + 1)] = {};
+    // energies and flags of the last evaluation in pinned, host-mapped memory
+    // (written by the final reduction kernel; d_ is its device address)
+    double* h_energies = nullptr;
+    double* d_host_energies = nullptr;
 };
 
 namespace mdg {
 
 // ---- launchers (each returns hipSuccess or the first HIP error) ---------
-hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed);
+// excl: also bound the excluded and 1-4 pair distances (AMBER with exclusions)
+hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl);
 hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber);
 hipError_t launch_nonbonded(mdgpu_ctx* c, bool mixed, bool amber, int partial_row0);
+// bonded terms, and with amber the FP64 excluded / 1-4 corrections in the
+// same launch (rows partial_row0 .. + bonded_rows + corrections_rows)
 hipError_t launch_bonded(mdgpu_ctx* c, const double* coords, bool mixed, bool amber, int partial_row0);
-hipError_t launch_corrections(mdgpu_ctx* c, bool mixed, int partial_row0);
-hipError_t launch_excl_extent(mdgpu_ctx* c);
 hipError_t launch_pme(mdgpu_ctx* c, bool mixed, int partial_row0, int* rows_used);
 hipError_t launch_reduce(mdgpu_ctx* c, bool mixed, int n_rows, double* out_forces);
 // adjoint kernels (AMBER semantics)

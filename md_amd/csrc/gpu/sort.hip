@@ -28,18 +28,89 @@ namespace {
 
 constexpr int kColMax = 2048;  // atoms per column the in-LDS sort handles
 
-// Column binning without global atomics: each block takes a contiguous chunk
-// of atoms, histograms their columns in LDS and stores its histogram row;
-// the scan derives every (block, column) offset from those rows, and the
-// scatter repeats the per-block LDS counting to place the atoms.
+// Column binning in three launches.
+//   k_bin:     each block wraps a contiguous chunk of atoms, histograms their
+//              columns in LDS and reserves its run in every column it touches
+//              with one global atomic per (block, column); the order of the
+//              runs inside a column is whatever the atomics give, and the
+//              column sort below makes the result independent of it.  The
+//              same blocks also measure the excluded and 1-4 pair distances.
+//   k_scatter: every block scans the column totals in LDS (block 0 also stores
+//              the atom and cluster offsets and the exclusion extent), then
+//              places its atoms.
+//   k_col_sort: one block per column ranks its atoms by (z, index), writes the
+//              slot map and sets up the column's clusters; it also clears the
+//              column total for the next evaluation.
 constexpr int kBinThreads = 1024;
 constexpr int kMaxCols = 8192;
+constexpr int kMaxSortBlocks = 64;
+
+// Excluded and 1-4 pairs whose minimum-image distances bound the tiles that
+// can hold an excluded pair (n1 = n2 = 0: not needed).
+struct ExclPairs {
+    const int2* p1;
+    int n1;
+    const int2* p2;
+    int n2;
+    int chunk;         // pairs per sort block
+    float* block_max;  // [sort blocks]
+    float* out;        // largest distance x 1.0001
+};
+
+// Exclusive scan of s[0..n) in LDS by the whole block (blockDim.x = 1024,
+// n <= 8 * 1024); returns the total.  Each thread scans 8 consecutive
+// entries in registers, the 1024 thread totals are scanned with wave
+// shuffles and one LDS round.
+__device__ int lds_exclusive_scan(int* s, int n) {
+    __shared__ int wsum[kBinThreads / kWave];
+    __shared__ int total;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int v[8];
+    int run = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int i = 8 * t + k;
+        v[k] = i < n ? s[i] : 0;
+        run += v[k];
+    }
+    // inclusive scan of the thread totals within the wave
+    int inc = run;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(inc, d, kWave);
+        if (lane >= d) inc += o;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int q = 0; q < kBinThreads / kWave; ++q) {
+            const int x = wsum[q];
+            wsum[q] = acc;
+            acc += x;
+        }
+        total = acc;
+    }
+    __syncthreads();
+    int base = wsum[w] + inc - run;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int i = 8 * t + k;
+        if (i < n) s[i] = base;
+        base += v[k];
+    }
+    const int tot = total;
+    __syncthreads();
+    return tot;
+}
 
 __global__ __launch_bounds__(kBinThreads) void k_bin(const double* __restrict__ x, int n, int chunk, double lx,
                                                      double ly, double lz, int ncx, int ncy, double sx, double sy,
                                                      double* __restrict__ wrapped, int* __restrict__ col_of,
-                                                     int* __restrict__ block_hist) {
+                                                     int* __restrict__ col_count, int* __restrict__ block_off,
+                                                     const ExclPairs ex) {
     __shared__ int hist[kMaxCols];
+    __shared__ float wmax[kBinThreads / kWave];
     const int ncol = ncx * ncy;
     for (int c = threadIdx.x; c < ncol; c += blockDim.x) hist[c] = 0;
     __syncthreads();
@@ -59,96 +130,81 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(const double* __restrict__ 
         col_of[i] = c;
         atomicAdd(hist + c, 1);
     }
+    // excluded / 1-4 pair distances of this block's share
+    float r = 0.f;
+    const int ne = ex.n1 + ex.n2;
+    const int e0 = blockIdx.x * ex.chunk, e1 = min(ne, e0 + ex.chunk);
+    for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        const int2 p = e < ex.n1 ? ex.p1[e] : ex.p2[e - ex.n1];
+        double dx = x[3 * p.x] - x[3 * p.y], dy = x[3 * p.x + 1] - x[3 * p.y + 1], dz = x[3 * p.x + 2] - x[3 * p.y + 2];
+        dx -= lx * rint(dx / lx);
+        dy -= ly * rint(dy / ly);
+        dz -= lz * rint(dz / lz);
+        r = fmaxf(r, static_cast<float>(sqrt(dx * dx + dy * dy + dz * dz)) * 1.0001f);
+    }
+    for (int m = 32; m >= 1; m >>= 1) r = fmaxf(r, __shfl_xor(r, m, kWave));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = r;
     __syncthreads();
-    for (int c = threadIdx.x; c < ncol; c += blockDim.x) block_hist[static_cast<size_t>(blockIdx.x) * ncol + c] = hist[c];
+    if (threadIdx.x == 0 && ne > 0) {
+        float m = 0.f;
+        for (int q = 0; q < kBinThreads / kWave; ++q) m = fmaxf(m, wmax[q]);
+        ex.block_max[blockIdx.x] = m;
+    }
+    // reserve this block's run in each column it touches
+    for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
+        const int h = hist[c];
+        block_off[static_cast<size_t>(blockIdx.x) * ncol + c] = h > 0 ? atomicAdd(col_count + c, h) : 0;
+    }
 }
 
-// One block: column totals, exclusive scans of atom and cluster counts, and
-// the start of every (block, column) run.
-__global__ __launch_bounds__(1024) void k_col_scan(const int* __restrict__ block_hist, int nblk, int ncol,
-                                                   int* __restrict__ col_start, int* __restrict__ col_cstart,
-                                                   int* __restrict__ block_off, unsigned int* flags) {
-    __shared__ int s_a[1024], s_c[1024];
-    __shared__ int carry_a, carry_c;
-    if (threadIdx.x == 0) { carry_a = 0; carry_c = 0; }
-    __syncthreads();
-    for (int base = 0; base < ncol; base += 1024) {
-        const int i = base + threadIdx.x;
-        int a = 0;
-        if (i < ncol)
-            for (int b = 0; b < nblk; ++b) a += block_hist[static_cast<size_t>(b) * ncol + i];
-        if (a > kColMax) atomicOr(flags + 2, 1u);
-        s_a[threadIdx.x] = a;
-        s_c[threadIdx.x] = (a + kCluster - 1) / kCluster;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            const int va = threadIdx.x >= off ? s_a[threadIdx.x - off] : 0;
-            const int vc = threadIdx.x >= off ? s_c[threadIdx.x - off] : 0;
-            __syncthreads();
-            s_a[threadIdx.x] += va;
-            s_c[threadIdx.x] += vc;
-            __syncthreads();
-        }
-        if (i < ncol) {
-            col_start[i + 1] = carry_a + s_a[threadIdx.x];
-            col_cstart[i + 1] = carry_c + s_c[threadIdx.x];
-            int run = carry_a + s_a[threadIdx.x] - a;  // exclusive start of column i
-            for (int b = 0; b < nblk; ++b) {
-                block_off[static_cast<size_t>(b) * ncol + i] = run;
-                run += block_hist[static_cast<size_t>(b) * ncol + i];
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x == 1023) {
-            carry_a += s_a[1023];
-            carry_c += s_c[1023];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        col_start[0] = 0;
-        col_cstart[0] = 0;
-    }
+// The sort key of an atom: its wrapped z as 32-bit fixed point of the box
+// height (zscale = 2^32 / L_z) above its index, so one unsigned comparison
+// orders atoms by z and breaks ties by index.
+__device__ __forceinline__ unsigned long long z_key(double z, double zscale, int atom) {
+    const double q = floor(z * zscale);
+    const unsigned long long zq = q <= 0.0 ? 0ull : (q >= 4294967295.0 ? 4294967295ull : static_cast<unsigned long long>(q));
+    return (zq << 32) | static_cast<unsigned int>(atom);
 }
 
 __global__ __launch_bounds__(kBinThreads) void k_scatter(int n, int chunk, int ncol, const int* __restrict__ col_of,
+                                                         const double* __restrict__ wrapped, double zscale,
+                                                         const int* __restrict__ col_count,
                                                          const int* __restrict__ block_off,
-                                                         int* __restrict__ col_tmp) {
+                                                         unsigned long long* __restrict__ col_key,
+                                                         int* __restrict__ col_start, int* __restrict__ col_cstart,
+                                                         unsigned int* flags, const ExclPairs ex) {
     __shared__ int off[kMaxCols];
-    for (int c = threadIdx.x; c < ncol; c += blockDim.x) off[c] = block_off[static_cast<size_t>(blockIdx.x) * ncol + c];
+    if (blockIdx.x == 0) {
+        // cluster offsets (8 atoms per cluster, the last one padded), the
+        // column-size check and the exclusion extent
+        for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
+            const int a = col_count[c];
+            if (a > kColMax) atomicOr(flags + 2, 1u);
+            off[c] = (a + kCluster - 1) / kCluster;
+        }
+        __syncthreads();
+        const int ncl = lds_exclusive_scan(off, ncol);
+        for (int c = threadIdx.x; c < ncol; c += blockDim.x) col_cstart[c] = off[c];
+        if (threadIdx.x == 0) col_cstart[ncol] = ncl;
+        if (threadIdx.x == 0 && ex.n1 + ex.n2 > 0) {
+            float m = 0.f;
+            for (int b = 0; b < gridDim.x; ++b) m = fmaxf(m, ex.block_max[b]);
+            *ex.out = m;
+        }
+        __syncthreads();
+    }
+    for (int c = threadIdx.x; c < ncol; c += blockDim.x) off[c] = col_count[c];
+    __syncthreads();
+    const int na = lds_exclusive_scan(off, ncol);
+    if (blockIdx.x == 0) {
+        for (int c = threadIdx.x; c < ncol; c += blockDim.x) col_start[c] = off[c];
+        if (threadIdx.x == 0) col_start[ncol] = na;
+    }
+    for (int c = threadIdx.x; c < ncol; c += blockDim.x) off[c] += block_off[static_cast<size_t>(blockIdx.x) * ncol + c];
     __syncthreads();
     const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
-    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) col_tmp[atomicAdd(off + col_of[i], 1)] = i;
-}
-
-// One block per column: rank the column's atoms by (z, index) and write the
-// slot -> atom map; the padding slots of the column's last cluster get -1.
-__global__ void k_col_sort(const int* __restrict__ col_start, const int* __restrict__ col_cstart,
-                           const int* __restrict__ col_tmp, const double* __restrict__ wrapped,
-                           int* __restrict__ slot_atom, int* __restrict__ cluster_col) {
-    __shared__ double s_z[kColMax];
-    __shared__ int s_i[kColMax];
-    const int c = blockIdx.x;
-    const int a0 = col_start[c], n = min(col_start[c + 1] - a0, kColMax);
-    const int cl0 = col_cstart[c], ncl = col_cstart[c + 1] - cl0;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        const int atom = col_tmp[a0 + k];
-        s_i[k] = atom;
-        s_z[k] = wrapped[3 * atom + 2];
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        const double z = s_z[k];
-        const int id = s_i[k];
-        int rank = 0;
-        for (int m = 0; m < n; ++m) {
-            const double zm = s_z[m];
-            rank += (zm < z) || (zm == z && s_i[m] < id);
-        }
-        slot_atom[cl0 * kCluster + rank] = id;
-    }
-    for (int k = n + threadIdx.x; k < ncl * kCluster; k += blockDim.x) slot_atom[cl0 * kCluster + k] = -1;
-    for (int k = threadIdx.x; k < ncl; k += blockDim.x) cluster_col[cl0 + k] = c;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x)
+        col_key[atomicAdd(off + col_of[i], 1)] = z_key(wrapped[3 * i + 2], zscale, i);
 }
 
 template <typename real>
@@ -158,9 +214,6 @@ struct Vec4<float> { using T = float4; };
 template <>
 struct Vec4<double> { using T = double4; };
 
-// Eight lanes per cluster: bounding box by shuffles, then the relative
-// coordinates.  Padding slots get the padding type, zero charge, and an
-// offset of kFarAway.
 // PME geometry for the per-slot spline records (k[0] == 0: no records).
 struct PmeSlots {
     int k[3];
@@ -169,78 +222,149 @@ struct PmeSlots {
     void* w;   // double4 / float4 per slot: fractions (x, y, z) and charge
 };
 
+// Per-cluster outputs of the column sort.
 template <typename real>
-__global__ void k_cluster_setup(const int* __restrict__ ncl_dev, int max_clusters, const int* __restrict__ slot_atom,
-                                const double* __restrict__ wrapped, const double* __restrict__ charge,
-                                const int* __restrict__ type, int pad_type, typename Vec4<real>::T* __restrict__ pos,
-                                int2* __restrict__ slot_meta, double4* __restrict__ origin,
-                                float4* __restrict__ half, const PmeSlots pme) {
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    const int cl = slot / kCluster;
-    const int ncl = *ncl_dev;
-    const bool live = cl < ncl && cl < max_clusters;
-    const int atom = live ? slot_atom[slot] : -1;
-    double x = 0, y = 0, z = 0;
-    if (atom >= 0) {
-        x = wrapped[3 * atom];
-        y = wrapped[3 * atom + 1];
-        z = wrapped[3 * atom + 2];
+struct ClusterOut {
+    int max_clusters;
+    const double* charge;
+    const int* type;
+    int pad_type;
+    int* slot_atom;
+    int* cluster_col;
+    typename Vec4<real>::T* pos;
+    int2* slot_meta;
+    double4* origin;
+    float4* half;
+    PmeSlots pme;
+};
+
+constexpr int kSortThreads = 256;
+
+// One block per column: rank the column's atoms by their sort key (z, then
+// index), then set up its clusters with eight lanes per cluster: bounding box
+// by shuffles, the coordinates relative to the box centre, type and charge,
+// and the PME spline record.  Padding slots get -1, the padding type, zero
+// charge and an offset of kFarAway.
+template <typename real>
+__global__ __launch_bounds__(kSortThreads) void k_col_sort(const int* __restrict__ col_start,
+                                                           const int* __restrict__ col_cstart,
+                                                           const unsigned long long* __restrict__ col_key,
+                                                           const double* __restrict__ wrapped,
+                                                           int* __restrict__ col_count, const ClusterOut<real> o) {
+    constexpr int kPer = kColMax / kSortThreads;  // atoms per thread at most
+    __shared__ unsigned long long s_key[kColMax];  // keys; after ranking, x by rank
+    __shared__ double s_y[kColMax], s_z[kColMax];
+    __shared__ int s_sorted[kColMax];
+    const int c = blockIdx.x;
+    const int a0 = col_start[c], n = min(col_start[c + 1] - a0, kColMax);
+    const int cl0 = col_cstart[c], ncl = col_cstart[c + 1] - cl0;
+    unsigned long long key[kPer];
+#pragma unroll
+    for (int p = 0; p < kPer; ++p) {
+        const int k = static_cast<int>(threadIdx.x) + p * kSortThreads;
+        key[p] = k < n ? col_key[a0 + k] : 0ull;
+        if (k < n) s_key[k] = key[p];
     }
-    double lo[3] = {atom >= 0 ? x : 1e300, atom >= 0 ? y : 1e300, atom >= 0 ? z : 1e300};
-    double hi[3] = {atom >= 0 ? x : -1e300, atom >= 0 ? y : -1e300, atom >= 0 ? z : -1e300};
+    __syncthreads();
+    if (threadIdx.x == 0) col_count[c] = 0;  // read by nobody after the scatter; zero for the next evaluation
+    int rank[kPer];
 #pragma unroll
-    for (int m = 1; m < kCluster; m <<= 1) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = fmin(lo[d], __shfl_xor(lo[d], m, kWave));
-            hi[d] = fmax(hi[d], __shfl_xor(hi[d], m, kWave));
+    for (int p = 0; p < kPer; ++p) {
+        const int k = static_cast<int>(threadIdx.x) + p * kSortThreads;
+        rank[p] = 0;
+        if (k < n) {
+            int r = 0;
+            for (int m = 0; m < n; ++m) r += s_key[m] < key[p] ? 1 : 0;  // keys are distinct (they hold the index)
+            rank[p] = r;
         }
     }
-    if (!live) return;
-    const double ox = 0.5 * (lo[0] + hi[0]), oy = 0.5 * (lo[1] + hi[1]), oz = 0.5 * (lo[2] + hi[2]);
-    typename Vec4<real>::T p;
-    if (atom >= 0) {
-        p.x = static_cast<real>(x - ox);
-        p.y = static_cast<real>(y - oy);
-        p.z = static_cast<real>(z - oz);
-        p.w = static_cast<real>(charge[atom]);
-        slot_meta[slot] = make_int2(type[atom], atom);
-    } else {
-        p.x = p.y = p.z = static_cast<real>(kFarAway);
-        p.w = 0;
-        slot_meta[slot] = make_int2(pad_type, -1);
+    __syncthreads();
+    double* s_x = reinterpret_cast<double*>(s_key);
+#pragma unroll
+    for (int p = 0; p < kPer; ++p) {
+        const int k = static_cast<int>(threadIdx.x) + p * kSortThreads;
+        if (k < n) {
+            const int id = static_cast<int>(key[p] & 0xffffffffull);
+            const int r = rank[p];
+            s_sorted[r] = id;
+            s_x[r] = wrapped[3 * id];
+            s_y[r] = wrapped[3 * id + 1];
+            s_z[r] = wrapped[3 * id + 2];
+        }
     }
-    pos[slot] = p;
-    if (pme.k[0] > 0) {
-        // u = K x / L in FP64; fl = floor(u) taken mod K (the stencil wraps
-        // anyway), w = u - floor(u)
-        int packed = 0;
-        typename Vec4<real>::T wq = {0, 0, 0, 0};
+    __syncthreads();
+    const int nslots = min(ncl * kCluster, kColMax);
+    // whole waves step together so the 8 lanes of a cluster stay in one wave
+    for (int k0 = 0; k0 < nslots; k0 += blockDim.x) {
+        const int k = k0 + static_cast<int>(threadIdx.x);
+        const int slot = cl0 * kCluster + k;
+        const int cl = slot / kCluster;
+        const bool live = k < nslots && cl < o.max_clusters;
+        const int atom = live && k < n ? s_sorted[k] : -1;
+        double x = 0, y = 0, z = 0;
         if (atom >= 0) {
-            const double xs[3] = {x, y, z};
-            real wv[3];
-            for (int d = 0; d < 3; ++d) {
-                const double u = xs[d] * pme.k[d] / pme.l[d];
-                const double f = floor(u);
-                int fl = static_cast<int>(f) % pme.k[d];
-                fl += fl < 0 ? pme.k[d] : 0;
-                packed |= fl << (10 * d);
-                wv[d] = static_cast<real>(u - f);
-            }
-            wq.x = wv[0];
-            wq.y = wv[1];
-            wq.z = wv[2];
-            wq.w = static_cast<real>(charge[atom]);
+            x = s_x[k];
+            y = s_y[k];
+            z = s_z[k];
         }
-        pme.fl[slot] = packed;
-        reinterpret_cast<typename Vec4<real>::T*>(pme.w)[slot] = wq;
-    }
-    if ((slot & (kCluster - 1)) == 0) {
-        origin[cl] = make_double4(ox, oy, oz, 0.0);
-        // round the half extents up so the float box still contains every atom
-        half[cl] = make_float4(static_cast<float>(0.5 * (hi[0] - lo[0])) * 1.0001f + 1e-4f,
-                               static_cast<float>(0.5 * (hi[1] - lo[1])) * 1.0001f + 1e-4f,
-                               static_cast<float>(0.5 * (hi[2] - lo[2])) * 1.0001f + 1e-4f, 0.f);
+        double lo[3] = {atom >= 0 ? x : 1e300, atom >= 0 ? y : 1e300, atom >= 0 ? z : 1e300};
+        double hi[3] = {atom >= 0 ? x : -1e300, atom >= 0 ? y : -1e300, atom >= 0 ? z : -1e300};
+#pragma unroll
+        for (int m = 1; m < kCluster; m <<= 1) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = fmin(lo[d], __shfl_xor(lo[d], m, kWave));
+                hi[d] = fmax(hi[d], __shfl_xor(hi[d], m, kWave));
+            }
+        }
+        if (!live) continue;
+        o.slot_atom[slot] = atom;
+        const double ox = 0.5 * (lo[0] + hi[0]), oy = 0.5 * (lo[1] + hi[1]), oz = 0.5 * (lo[2] + hi[2]);
+        typename Vec4<real>::T p;
+        if (atom >= 0) {
+            p.x = static_cast<real>(x - ox);
+            p.y = static_cast<real>(y - oy);
+            p.z = static_cast<real>(z - oz);
+            p.w = static_cast<real>(o.charge[atom]);
+            o.slot_meta[slot] = make_int2(o.type[atom], atom);
+        } else {
+            p.x = p.y = p.z = static_cast<real>(kFarAway);
+            p.w = 0;
+            o.slot_meta[slot] = make_int2(o.pad_type, -1);
+        }
+        o.pos[slot] = p;
+        if (o.pme.k[0] > 0) {
+            // u = K x / L in FP64; fl = floor(u) taken mod K (the stencil wraps
+            // anyway), w = u - floor(u)
+            int packed = 0;
+            typename Vec4<real>::T wq = {0, 0, 0, 0};
+            if (atom >= 0) {
+                const double xs[3] = {x, y, z};
+                real wv[3];
+                for (int d = 0; d < 3; ++d) {
+                    const double u = xs[d] * o.pme.k[d] / o.pme.l[d];
+                    const double f = floor(u);
+                    int fl = static_cast<int>(f) % o.pme.k[d];
+                    fl += fl < 0 ? o.pme.k[d] : 0;
+                    packed |= fl << (10 * d);
+                    wv[d] = static_cast<real>(u - f);
+                }
+                wq.x = wv[0];
+                wq.y = wv[1];
+                wq.z = wv[2];
+                wq.w = static_cast<real>(o.charge[atom]);
+            }
+            o.pme.fl[slot] = packed;
+            reinterpret_cast<typename Vec4<real>::T*>(o.pme.w)[slot] = wq;
+        }
+        if ((slot & (kCluster - 1)) == 0) {
+            o.cluster_col[cl] = c;
+            o.origin[cl] = make_double4(ox, oy, oz, 0.0);
+            // round the half extents up so the float box still contains every atom
+            o.half[cl] = make_float4(static_cast<float>(0.5 * (hi[0] - lo[0])) * 1.0001f + 1e-4f,
+                                     static_cast<float>(0.5 * (hi[1] - lo[1])) * 1.0001f + 1e-4f,
+                                     static_cast<float>(0.5 * (hi[2] - lo[2])) * 1.0001f + 1e-4f, 0.f);
+        }
     }
 }
 
@@ -516,57 +640,26 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     }
 }
 
-// Largest minimum-image distance over the excluded and 1-4 pairs (bounds the
-// tiles that can hold an excluded pair).
-__global__ void k_excl_extent(const double* __restrict__ x, int n1, const int2* __restrict__ p1, int n2,
-                              const int2* __restrict__ p2, double lx, double ly, double lz, float* out) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    float r = 0.f;
-    if (t < n1 + n2) {
-        const int2 p = t < n1 ? p1[t] : p2[t - n1];
-        double dx = x[3 * p.x] - x[3 * p.y], dy = x[3 * p.x + 1] - x[3 * p.y + 1], dz = x[3 * p.x + 2] - x[3 * p.y + 2];
-        dx -= lx * rint(dx / lx);
-        dy -= ly * rint(dy / ly);
-        dz -= lz * rint(dz / lz);
-        r = static_cast<float>(sqrt(dx * dx + dy * dy + dz * dz)) * 1.0001f;
-    }
-    for (int m = 32; m >= 1; m >>= 1) r = fmaxf(r, __shfl_xor(r, m, kWave));
-    if ((threadIdx.x & 63) == 0 && r > 0.f) atomicMax(reinterpret_cast<unsigned int*>(out), __float_as_uint(r));
-}
-
 }  // namespace
 
-int sort_blocks(int n) { return std::max(1, std::min(64, (n + 2047) / 2048)); }
+int sort_blocks(int n) { return std::max(1, std::min(kMaxSortBlocks, (n + 2047) / 2048)); }
 
-hipError_t launch_excl_extent(mdgpu_ctx* c) {
-    const Topo& t = c->topo;
-    const int n = t.n_excl_pairs + t.n_pairs14;
-    if (n == 0) return hipSuccess;
-    const double* x = c->cur_x;
-    hipLaunchKernelGGL(k_excl_extent, dim3((n + 255) / 256), dim3(256), 0, c->stream, x, t.n_excl_pairs,
-                       t.excl_pairs, t.n_pairs14, t.pairs14, c->grid.box[0], c->grid.box[1], c->grid.box[2],
-                       c->work.excl_rmax);
-    return hipGetLastError();
-}
-
-hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed) {
+hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl) {
     Work& w = c->work;
     const Grid& g = c->grid;
-    const int n = c->topo.n_atoms;
+    const Topo& t = c->topo;
+    const int n = t.n_atoms;
     const int ncol = g.ncx * g.ncy;
     if (ncol > kMaxCols) return hipErrorInvalidValue;
     const int nblk = sort_blocks(n), chunk = (n + nblk - 1) / nblk;
+    ExclPairs ex{t.excl_pairs, excl ? t.n_excl_pairs : 0, t.pairs14, excl ? t.n_pairs14 : 0, 0, w.block_emax,
+                 w.excl_rmax};
+    ex.chunk = (ex.n1 + ex.n2 + nblk - 1) / nblk;
     hipLaunchKernelGGL(k_bin, dim3(nblk), dim3(kBinThreads), 0, c->stream, coords, n, chunk, g.box[0], g.box[1],
-                       g.box[2], g.ncx, g.ncy, g.sx, g.sy, w.wrapped, w.col_of, w.block_hist);
-    hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, c->stream, w.block_hist, nblk, ncol, w.col_start,
-                       w.col_cstart, w.block_off, w.flags);
-    hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kBinThreads), 0, c->stream, n, chunk, ncol, w.col_of, w.block_off,
-                       w.col_tmp);
-    hipLaunchKernelGGL(k_col_sort, dim3(ncol), dim3(256), 0, c->stream, w.col_start, w.col_cstart, w.col_tmp,
-                       w.wrapped, w.slot_atom, w.cluster_col);
-    const int* ncl_dev = w.col_cstart + ncol;
-    const int max_cl = c->n_clusters;
-    const int nslots = max_cl * kCluster;
+                       g.box[2], g.ncx, g.ncy, g.sx, g.sy, w.wrapped, w.col_of, w.col_count, w.block_off, ex);
+    const double zscale = 4294967296.0 / g.box[2];
+    hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kBinThreads), 0, c->stream, n, chunk, ncol, w.col_of, w.wrapped,
+                       zscale, w.col_count, w.block_off, w.col_key, w.col_start, w.col_cstart, w.flags, ex);
     PmeSlots pme{};
     const bool with_pme = c->params.semantics == MDGPU_SEM_AMBER && w.pme_k[0] > 0;
     for (int d = 0; d < 3; ++d) {
@@ -574,15 +667,18 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed) {
         pme.l[d] = g.box[d];
     }
     pme.fl = w.slot_pme_fl;
-    pme.w = mixed ? static_cast<void*>(w.slot_pme_wf) : static_cast<void*>(w.slot_pme_wd);
     if (mixed) {
-        hipLaunchKernelGGL(k_cluster_setup<float>, dim3((nslots + 255) / 256), dim3(256), 0, c->stream, ncl_dev,
-                           max_cl, w.slot_atom, w.wrapped, c->topo.charge, c->topo.type, c->topo.n_types,
-                           w.slot_pos_f, w.slot_meta, w.cl_origin, w.cl_half, pme);
+        pme.w = w.slot_pme_wf;
+        const ClusterOut<float> o{c->n_clusters, t.charge, t.type, t.n_types, w.slot_atom, w.cluster_col,
+                                  w.slot_pos_f, w.slot_meta, w.cl_origin, w.cl_half, pme};
+        hipLaunchKernelGGL(k_col_sort<float>, dim3(ncol), dim3(kSortThreads), 0, c->stream, w.col_start, w.col_cstart,
+                           w.col_key, w.wrapped, w.col_count, o);
     } else {
-        hipLaunchKernelGGL(k_cluster_setup<double>, dim3((nslots + 255) / 256), dim3(256), 0, c->stream, ncl_dev,
-                           max_cl, w.slot_atom, w.wrapped, c->topo.charge, c->topo.type, c->topo.n_types,
-                           w.slot_pos_d, w.slot_meta, w.cl_origin, w.cl_half, pme);
+        pme.w = w.slot_pme_wd;
+        const ClusterOut<double> o{c->n_clusters, t.charge, t.type, t.n_types, w.slot_atom, w.cluster_col,
+                                   w.slot_pos_d, w.slot_meta, w.cl_origin, w.cl_half, pme};
+        hipLaunchKernelGGL(k_col_sort<double>, dim3(ncol), dim3(kSortThreads), 0, c->stream, w.col_start,
+                           w.col_cstart, w.col_key, w.wrapped, w.col_count, o);
     }
     return hipGetLastError();
 }
