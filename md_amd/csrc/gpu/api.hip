@@ -587,7 +587,7 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
 // and flags go to out_buf and, when out_host is set, also to that
 // host-mapped copy, which the host reads after the stream synchronisation.
 constexpr int kReduceThreads = 256;
-__global__ __launch_bounds__(kReduceThreads) void k_reduce(const double* __restrict__ partial, int rows,
+__global__ __launch_bounds__(kReduceThreads) void k_reduce(const double* __restrict__ partial, int ld, int rows,
                                                            double* __restrict__ out_buf, double* out_host,
                                                            unsigned int* flags,
                                                            float* excl_rmax, int mixed, unsigned long long* fx,
@@ -604,14 +604,15 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double* __restr
             for (int k = t; k < ncl; k += kReduceThreads) s0 += list_count[k];
         } else {
             constexpr int kStep = 4 * kReduceThreads;
+            const double* col = partial + static_cast<size_t>(b) * ld;  // this slot's rows, contiguous
             int r = t;
             for (; r + 3 * kReduceThreads < rows; r += kStep) {
-                s0 += partial[static_cast<size_t>(r) * mdg::E_NSLOTS + b];
-                s1 += partial[static_cast<size_t>(r + kReduceThreads) * mdg::E_NSLOTS + b];
-                s2 += partial[static_cast<size_t>(r + 2 * kReduceThreads) * mdg::E_NSLOTS + b];
-                s3 += partial[static_cast<size_t>(r + 3 * kReduceThreads) * mdg::E_NSLOTS + b];
+                s0 += col[r];
+                s1 += col[r + kReduceThreads];
+                s2 += col[r + 2 * kReduceThreads];
+                s3 += col[r + 3 * kReduceThreads];
             }
-            for (; r < rows; r += kReduceThreads) s0 += partial[static_cast<size_t>(r) * mdg::E_NSLOTS + b];
+            for (; r < rows; r += kReduceThreads) s0 += col[r];
         }
         double s = mdg::wave_sum((s0 + s1) + (s2 + s3));
         if ((t & 63) == 0) red[t >> 6] = s;
@@ -683,7 +684,7 @@ int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out) {
     mark(MDGPU_K_REDUCE, true);
     const int fblocks = std::max(1, std::min(512, (3 * n + kReduceThreads - 1) / kReduceThreads));
     hipLaunchKernelGGL(k_reduce, dim3(mdg::E_NSLOTS + 1 + fblocks), dim3(kReduceThreads), 0, c->stream, w.partial,
-                       row, w.energies, c->d_host_energies, w.flags, w.excl_rmax, mixed ? 1 : 0, w.force_fx, w.force_d,
+                       static_cast<int>(w.cap_partial), row, w.energies, c->d_host_energies, w.flags, w.excl_rmax, mixed ? 1 : 0, w.force_fx, w.force_d,
                        1.0 / mdg::fixed_scale(p.semantics), 3 * n, d_out, w.list_count, w.col_cstart + c->n_cols);
     HIP_TRY(hipGetLastError(), "reduce");
     mark(MDGPU_K_REDUCE, false);

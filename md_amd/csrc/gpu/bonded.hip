@@ -52,6 +52,7 @@ struct BArgs {
     double* force_d;
     FixedAcc fx;
     double* partial;
+    int ld;            // rows allocated in the slot-major partial buffer
     int row0;
     int amber;
 };
@@ -246,7 +247,7 @@ __device__ __forceinline__ void bonded_terms(const BArgs<real>& a, int k, int ro
         }
     }
     const int slots[6] = {E_BOND, E_ANGLE, E_DIHEDRAL, E_UB, E_IMPROPER, E_CMAP};
-    block_sum_store<6>(e, a.partial + static_cast<size_t>(row) * E_NSLOTS, slots);
+    block_sum_store<6>(e, a.partial, a.ld, row, slots);
 }
 
 struct CArgs {
@@ -267,6 +268,7 @@ struct CArgs {
     double* force_d;
     FixedAcc fx;
     double* partial;
+    int ld;
     int row0;
 };
 
@@ -317,7 +319,7 @@ __device__ __forceinline__ void correction_pairs(const CArgs& a, int k, int row)
         addc(a, p.y, -g * dx, -g * dy, -g * dz);
     }
     const int slots[3] = {E_EW_EXCL, E_VDW14, E_EE14};
-    block_sum_store<3>(e, a.partial + static_cast<size_t>(row) * E_NSLOTS, slots);
+    block_sum_store<3>(e, a.partial, a.ld, row, slots);
 }
 
 // One launch for both: blocks [0, nbb) take the bonded terms, the rest the
@@ -368,12 +370,15 @@ hipError_t launch_bonded(mdgpu_ctx* c, const double* coords, bool mixed, bool am
     ca.force_d = c->work.force_d;
     ca.fx = fxa;
     ca.partial = c->work.partial;
+    ca.ld = static_cast<int>(c->work.cap_partial);
     ca.row0 = row0 + nbb;
     if (mixed) {
-        BArgs<float> a{coords, t, t.cmap_coeff_f, c->work.force_d, fxa, c->work.partial, row0, amber ? 1 : 0};
+        BArgs<float> a{coords, t, t.cmap_coeff_f, c->work.force_d, fxa, c->work.partial,
+                       static_cast<int>(c->work.cap_partial), row0, amber ? 1 : 0};
         hipLaunchKernelGGL(k_bonded<float>, dim3(nbb + ncb), dim3(kBlock), 0, c->stream, a, ca, nbb);
     } else {
-        BArgs<double> a{coords, t, t.cmap_coeff, c->work.force_d, fxa, c->work.partial, row0, amber ? 1 : 0};
+        BArgs<double> a{coords, t, t.cmap_coeff, c->work.force_d, fxa, c->work.partial,
+                        static_cast<int>(c->work.cap_partial), row0, amber ? 1 : 0};
         hipLaunchKernelGGL(k_bonded<double>, dim3(nbb + ncb), dim3(kBlock), 0, c->stream, a, ca, nbb);
     }
     return hipGetLastError();

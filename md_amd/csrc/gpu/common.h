@@ -113,12 +113,14 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Block-wide sum of several doubles into one row of the partial-sum buffer:
-// slot slots[k] of the row receives the block total of v[k], every other
-// slot of the row receives 0 (so a row never keeps a value from an earlier
-// evaluation).  blockDim.x must be a multiple of 64, >= E_NSLOTS and <= 1024.
+// Block-wide sum of several doubles into row `row` of the partial-sum buffer,
+// stored slot-major (partial[slot * ld + row], ld = rows allocated, so the
+// final reduction reads each slot contiguously): slot slots[k] receives the
+// block total of v[k], every other slot of the row receives 0 (so a row never
+// keeps a value from an earlier evaluation).  blockDim.x must be a multiple of
+// 64, >= E_NSLOTS and <= 1024.
 template <int NV>
-__device__ __forceinline__ void block_sum_store(double (&v)[NV], double* partial_row, const int* slots) {
+__device__ __forceinline__ void block_sum_store(double (&v)[NV], double* partial, int ld, int row, const int* slots) {
     __shared__ double red[16][NV];
     __shared__ double tot[NV];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -140,7 +142,7 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* partial
 #pragma unroll
         for (int k = 0; k < NV; ++k)
             if (slots[k] == static_cast<int>(threadIdx.x)) s = tot[k];
-        partial_row[threadIdx.x] = s;
+        partial[static_cast<size_t>(threadIdx.x) * ld + row] = s;
     }
 }
 

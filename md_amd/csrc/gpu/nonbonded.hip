@@ -56,6 +56,7 @@ struct NbArgs {
     int fx_shift;                      // log2 of the fixed-point scale
     float fx_half;                     // 2^(fx_shift - 1)
     double* partial;
+    int ld;                            // rows allocated in the slot-major partial buffer
     int row0;
     int n3;                            // 3 x atoms: entries of the force arrays
 };
@@ -321,7 +322,9 @@ __global__ __launch_bounds__(kWaves * kWave) void k_nonbonded(const NbArgs<real>
     e_vdw += s_vdw; e_all += s_all; e_list += s_list; e_rf += s_rf; e_ew += s_ew;
     double v[6] = {e_vdw, e_all, e_list, e_rf, e_ew, n_pairs};
     const int slots[6] = {E_VDW, E_EE_ALL, E_EE_LIST, E_EE_RF, E_EW_DIRECT, E_PAIRS};
-    block_sum_store<6>(v, a.partial + static_cast<size_t>(a.row0 + blockIdx.x) * E_NSLOTS, slots);
+    // row by logical block: the 16 rows sharing a cache line of a slot come
+    // from blocks of one XCD
+    block_sum_store<6>(v, a.partial, a.ld, a.row0 + blk, slots);
 }
 
 template <typename real>
@@ -365,6 +368,7 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0) {
     a.fx_shift = p.semantics == MDGPU_SEM_REF ? 20 : 32;
     a.fx_half = static_cast<float>(0.5 * fixed_scale(p.semantics));
     a.partial = w.partial;
+    a.ld = static_cast<int>(w.cap_partial);
     a.row0 = row0;
     a.n3 = 3 * t.n_atoms;
     const int nblk = nonbonded_rows(c);

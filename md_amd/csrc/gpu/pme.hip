@@ -284,7 +284,7 @@ __global__ void k_fft_y(PmeGeom g, int lognl, typename C2<real>::T* __restrict__
 template <typename real>
 __global__ void k_fft_x_conv(PmeGeom g, int lognl, typename C2<real>::T* __restrict__ cg,
                              const typename C2<real>::T* __restrict__ tw_g, const double* __restrict__ bmod,
-                             double alpha, double* partial, int row0) {
+                             double alpha, double* partial, int ld, int row0) {
     using c2 = typename C2<real>::T;
     extern __shared__ unsigned char smem[];
     const int nl = 1 << lognl;
@@ -343,7 +343,7 @@ __global__ void k_fft_x_conv(PmeGeom g, int lognl, typename C2<real>::T* __restr
     }
     const int slots[7] = {E_EW_RECIP, E_VIR_XX, E_VIR_YY, E_VIR_ZZ, E_VIR_XY, E_VIR_XZ, E_VIR_YZ};
     const int row = row0 + blockIdx.y * gridDim.x + blockIdx.x;
-    block_sum_store<7>(e, partial + static_cast<size_t>(row) * E_NSLOTS, slots);
+    block_sum_store<7>(e, partial, ld, row, slots);
 }
 
 // --------------------------------------------------------------- z c2r ----
@@ -496,7 +496,7 @@ hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used, double* pot) {
     const size_t shx = (2 * static_cast<size_t>(g.kx) * nlx + g.kx) * sizeof(c2);
     const dim3 gx((g.hz + nlx - 1) / nlx, g.ky);
     hipLaunchKernelGGL(k_fft_x_conv<real>, gx, dim3(kThreads), shx, c->stream, g, ilog2(nlx), cg, twx, w.bsp_mod,
-                       c->params.ewald_alpha, w.partial, row0);
+                       c->params.ewald_alpha, w.partial, static_cast<int>(w.cap_partial), row0);
     mark(MDGPU_K_CONV, false);
     *rows_used = static_cast<int>(gx.x * gx.y);
     mark(MDGPU_K_COUNT, true);  // inverse passes: the extra event pair

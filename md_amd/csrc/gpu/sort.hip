@@ -43,7 +43,8 @@ constexpr int kColMax = 2048;  // atoms per column the in-LDS sort handles
 //              column total for the next evaluation.
 constexpr int kBinThreads = 1024;
 constexpr int kMaxCols = 8192;
-constexpr int kMaxSortBlocks = 64;
+constexpr int kMaxSortBlocks = 256;
+constexpr int kAtomsPerSortBlock = 1024;
 
 // Excluded and 1-4 pairs whose minimum-image distances bound the tiles that
 // can hold an excluded pair (n1 = n2 = 0: not needed).
@@ -642,7 +643,9 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
 
 }  // namespace
 
-int sort_blocks(int n) { return std::max(1, std::min(kMaxSortBlocks, (n + 2047) / 2048)); }
+int sort_blocks(int n) {
+    return std::max(1, std::min(kMaxSortBlocks, (n + kAtomsPerSortBlock - 1) / kAtomsPerSortBlock));
+}
 
 hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl) {
     Work& w = c->work;
