@@ -442,7 +442,8 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
             const int jcol = jx + a.ncx * jy;
             const int c0 = a.col_cstart[jcol], c1 = a.col_cstart[jcol + 1];
             const double shx = kx * a.lx, shy = ky * a.ly;
-            for (int b = c0; b < c1; b += kWave) {
+            // half list: only j-clusters >= ci (cluster indices run column by column)
+            for (int b = max(c0, ci); b < c1; b += kWave) {
                 const int cj = b + lane;
                 const bool valid = cj < c1;
                 double4 oj = make_double4(0, 0, 0, 0);
@@ -512,6 +513,65 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
     }
 }
 
+// Exclusion masks of the near tiles of one wave's i-cluster.  The sorted
+// exclusion lists of the cluster's eight atoms are staged in LDS (a list
+// longer than kExclCap is searched in memory instead).  Eight tiles are
+// handled per step: one load brings their j atoms (lane 8u + tj holds atom tj
+// of tile u), and lane (ti, tj) scans atom ti's list once, comparing every
+// entry with its j atom in all eight tiles.
+constexpr int kExclCap = 32;
+
+__device__ void near_tile_masks(const SearchArgs& a, int ci, int lane, int (*s_ex)[kExclCap], const int2* near,
+                                int nnear, size_t base_i) {
+    const unsigned long long self_mask = 0x0080C0E0F0F8FCFEull;  // bit 8*ti+tj set iff tj > ti
+    int e0 = 0, en = 0;
+    const int a_l = lane < kCluster ? a.slot_atom[ci * kCluster + lane] : -1;
+    if (a_l >= 0) {
+        e0 = a.excl_start[a_l];
+        en = a.excl_start[a_l + 1] - e0;
+    }
+#pragma unroll
+    for (int t = 0; t < kCluster; ++t) {
+        const int n_t = min(__shfl(en, t, kWave), kExclCap), s_t = __shfl(e0, t, kWave);
+        if (lane < n_t) s_ex[t][lane] = a.excl_list[s_t + lane];
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int ti = lane >> 3, tj = lane & 7;
+    const int ai = __shfl(a_l, ti, kWave), my_n = __shfl(en, ti, kWave);
+    const bool in_lds = my_n <= kExclCap;
+    // longest staged list of the cluster (uniform loop bound)
+    int nmax = in_lds ? my_n : 0;
+    for (int m = 32; m >= 1; m >>= 1) nmax = max(nmax, __shfl_xor(nmax, m, kWave));
+    for (int q0 = 0; q0 < nnear; q0 += kCluster) {
+        const int qq = q0 + (lane >> 3);
+        const int aj_l = qq < nnear ? a.slot_atom[near[qq].y * kCluster + tj] : -1;
+        int aj[kCluster];
+#pragma unroll
+        for (int u = 0; u < kCluster; ++u) aj[u] = __shfl(aj_l, kCluster * u + tj, kWave);
+        unsigned int exbits = 0;  // bit u: this lane's pair is excluded in tile q0 + u
+        if (ai >= 0) {
+            if (in_lds) {
+                for (int k = 0; k < nmax; ++k) {
+                    const int x = k < my_n ? s_ex[ti][k] : -2;
+#pragma unroll
+                    for (int u = 0; u < kCluster; ++u) exbits |= (x == aj[u] ? 1u : 0u) << u;
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < kCluster; ++u)
+                    if (aj[u] >= 0 && pair_excluded(a.excl_start, a.excl_list, ai, aj[u])) exbits |= 1u << u;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kCluster; ++u) {
+            if (q0 + u >= nnear) break;
+            const unsigned long long bal = __ballot((exbits >> u) & 1u);
+            const int2 en_q = near[q0 + u];
+            if (lane == 0) a.list_mask[base_i + en_q.x] = ((en_q.y == ci) ? self_mask : ~0ull) & ~bal;
+        }
+    }
+}
+
 // Fast search: every candidate j-cluster is taken at its minimum-image shift
 // (per lane, FP64 rint), so one pass over the clusters of each row of
 // columns in reach replaces the per-column, per-shift enumeration.  It is
@@ -519,11 +579,18 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
 // dimension (then no second image of a cluster can come within r_list);
 // a cluster that breaks the bound sets flags[4] and the host falls back to
 // k_search.
+//
+// The candidate clusters of the rows in reach form at most two contiguous
+// index ranges per row (the row's x window, split where it wraps).  Lane r
+// loads the bounds of range r, clipped to j >= i for the half list; the
+// ranges are then walked as one stream of candidates, 64 per step, with the
+// next step's cluster boxes loaded while the current step is tested.
 template <typename real>
 __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const SearchArgs a, double hlx, double hly,
                                                                      double hlz) {
     using R4 = typename Vec4<real>::T;
     __shared__ int2 s_near[kSearchWaves][kNearCap];
+    __shared__ int s_ex[kSearchWaves][kCluster][kExclCap];
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
     const int ci = __builtin_amdgcn_readfirstlane(blockIdx.x * kSearchWaves + wv);
     const int ncl = min(*a.ncl_dev, a.max_clusters);
@@ -532,7 +599,8 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     const int cx = col % a.ncx, cy = col / a.ncx;
     const double4 oi = a.origin[ci];
     const float4 hi = a.half[ci];
-    if (hi.x > hlx || hi.y > hly || hi.z > hlz) {
+    const int ny = min(2 * a.dcy + 1, a.ncy);
+    if (hi.x > hlx || hi.y > hly || hi.z > hlz || 2 * ny > kWave) {
         if (lane == 0) {
             atomicOr(a.flags + 4, 1u);
             a.list_count[ci] = 0;  // the pair kernel of this attempt reads an empty list
@@ -545,79 +613,122 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     double rex = 0.0;
     if (a.amber) rex = static_cast<double>(*a.excl_rmax) + 0.01;
     const double rex2 = rex * rex;
-    int count = 0, nnear = 0;
     const unsigned long long self_mask = 0x0080C0E0F0F8FCFEull;  // bit 8*ti+tj set iff tj > ti
-    const int ny = min(2 * a.dcy + 1, a.ncy);
-    const bool yall = ny == a.ncy;
-    const int y0 = yall ? 0 : cy - a.dcy;
-    for (int ry = 0; ry < ny; ++ry) {
-        int jy = (y0 + ry) % a.ncy;
-        jy += jy < 0 ? a.ncy : 0;
-        // columns of this row within reach: |dx| <= dxr, where the x gap
-        // (|dx| - 1) sx and the y gap (|dy| - 1) sy stay inside r_list
+
+    // ranges: lane r = 2 * row + segment
+    int r_b0 = 0, r_len = 0;
+    {
+        const bool yall = ny == a.ncy;
+        const int y0 = yall ? 0 : cy - a.dcy;
+        const int ry = lane >> 1, seg = lane & 1;
         int dxr = a.dcx;
-        if (!yall) {
+        bool use = lane < 2 * ny;
+        if (use && !yall) {
+            // columns of this row within reach: |dx| <= dxr, where the x gap
+            // (|dx| - 1) sx and the y gap (|dy| - 1) sy stay inside r_list
             const int dy = ry - a.dcy;
             const double gy = fmax(0.0, (abs(dy) - 1) * a.sy);
             const double rem = a.rl2 - gy * gy;
-            if (rem <= 0.0) continue;
-            dxr = min(a.dcx, static_cast<int>(sqrt(rem) / a.sx) + 1);
+            if (rem <= 0.0) use = false;
+            else dxr = min(a.dcx, static_cast<int>(sqrt(rem) / a.sx) + 1);
         }
-        const int nx = min(2 * dxr + 1, a.ncx);
-        const int x0 = nx == a.ncx ? 0 : cx - dxr;
-        // the x window of this row as at most two contiguous column segments
-        int xa = x0 % a.ncx;
-        xa += xa < 0 ? a.ncx : 0;
-        const int first = min(nx, a.ncx - xa);
-        for (int seg = 0; seg < 2; ++seg) {
+        if (use) {
+            int jy = (y0 + ry) % a.ncy;
+            jy += jy < 0 ? a.ncy : 0;
+            const int nx = min(2 * dxr + 1, a.ncx);
+            const int x0 = nx == a.ncx ? 0 : cx - dxr;
+            int xa = x0 % a.ncx;
+            xa += xa < 0 ? a.ncx : 0;
+            const int first = min(nx, a.ncx - xa);
             const int c_lo = seg == 0 ? xa : 0;
             const int c_n = seg == 0 ? first : nx - first;
-            if (c_n <= 0) continue;
-            const int rowc = a.ncx * jy;
-            const int b0 = a.col_cstart[rowc + c_lo], b1 = a.col_cstart[rowc + c_lo + c_n];
-            for (int b = b0; b < b1; b += kWave) {
-                const int cj = b + lane;
-                const bool valid = cj < b1 && cj >= ci;
-                double ddx = 0, ddy = 0, ddz = 0, d2 = 1e300;
-                if (valid) {
-                    const double4 oj = a.origin[cj];
-                    const float4 hj = a.half[cj];
-                    ddx = oj.x - oi.x;
-                    ddy = oj.y - oi.y;
-                    ddz = oj.z - oi.z;
-                    ddx -= a.lx * rint(ddx * ilx);
-                    ddy -= a.ly * rint(ddy * ily);
-                    ddz -= a.lz * rint(ddz * ilz);
-                    const double qx = fmax(0.0, fabs(ddx) - (double)(hi.x + hj.x));
-                    const double qy = fmax(0.0, fabs(ddy) - (double)(hi.y + hj.y));
-                    const double qz = fmax(0.0, fabs(ddz) - (double)(hi.z + hj.z));
-                    d2 = qx * qx + qy * qy + qz * qz;
+            if (c_n > 0) {
+                const int rowc = a.ncx * jy;
+                const int b0 = max(a.col_cstart[rowc + c_lo], ci), b1 = a.col_cstart[rowc + c_lo + c_n];
+                if (b1 > b0) {
+                    r_b0 = b0;
+                    r_len = b1 - b0;
                 }
-                const bool cand = valid && d2 < a.rl2;
-                const unsigned long long bal = __ballot(cand);
-                if (bal == 0) continue;
-                const int pos = count + __popcll(bal & ((1ull << lane) - 1ull));
-                const bool near = a.amber && cand && d2 < rex2;
-                if (cand && pos < a.cap) {
-                    const size_t e = base_i + pos;
-                    a.list_j[e] = cj;
-                    if (!near) a.list_mask[e] = (cj == ci) ? self_mask : ~0ull;
-                    R4 dl;
-                    dl.x = static_cast<real>(-ddx);
-                    dl.y = static_cast<real>(-ddy);
-                    dl.z = static_cast<real>(-ddz);
-                    dl.w = 0;
-                    delta[e] = dl;
-                }
-                const unsigned long long nbal = __ballot(near && pos < a.cap);
-                if (near && pos < a.cap) {
-                    const int nr = nnear + __popcll(nbal & ((1ull << lane) - 1ull));
-                    if (nr < kNearCap) s_near[wv][nr] = make_int2(pos, cj);
-                }
-                nnear += __popcll(nbal);
-                count += __popcll(bal);
             }
         }
+    }
+    int r_inc = r_len;
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int o = __shfl_up(r_inc, d, kWave);
+        if (lane >= d) r_inc += o;
+    }
+    const int r_pre = r_inc - r_len;
+    const int total = __shfl(r_inc, kWave - 1, kWave);
+    const int nr = 2 * ny;
+    // candidate q of the stream -> cluster index (-1 past the end).  Called
+    // with every lane active: the shuffles read the range lanes, which must
+    // not be masked off.
+    auto cand_of = [&](int q) {
+        int r = 0;
+        for (int k = 1; k < nr; ++k) {
+            const int pk = __builtin_amdgcn_readlane(r_pre, k), lk = __builtin_amdgcn_readlane(r_len, k);
+            r = (lk > 0 && pk <= q) ? k : r;
+        }
+        const int cj = __shfl(r_b0, r, kWave) + (q - __shfl(r_pre, r, kWave));
+        return q < total ? cj : -1;
+    };
+
+    int count = 0, nnear = 0;
+    int cj_n = cand_of(lane);
+    double4 oj_n = make_double4(0, 0, 0, 0);
+    float4 hj_n = make_float4(0, 0, 0, 0);
+    if (cj_n >= 0) {
+        oj_n = a.origin[cj_n];
+        hj_n = a.half[cj_n];
+    }
+    for (int b = 0; b < total; b += kWave) {
+        const int cj = cj_n;
+        const double4 oj = oj_n;
+        const float4 hj = hj_n;
+        if (b + kWave < total) {
+            cj_n = cand_of(b + kWave + lane);
+            if (cj_n >= 0) {
+                oj_n = a.origin[cj_n];
+                hj_n = a.half[cj_n];
+            }
+        }
+        const bool valid = cj >= 0;
+        double ddx = 0, ddy = 0, ddz = 0, d2 = 1e300;
+        if (valid) {
+            ddx = oj.x - oi.x;
+            ddy = oj.y - oi.y;
+            ddz = oj.z - oi.z;
+            ddx -= a.lx * rint(ddx * ilx);
+            ddy -= a.ly * rint(ddy * ily);
+            ddz -= a.lz * rint(ddz * ilz);
+            const double qx = fmax(0.0, fabs(ddx) - (double)(hi.x + hj.x));
+            const double qy = fmax(0.0, fabs(ddy) - (double)(hi.y + hj.y));
+            const double qz = fmax(0.0, fabs(ddz) - (double)(hi.z + hj.z));
+            d2 = qx * qx + qy * qy + qz * qz;
+        }
+        const bool cand = valid && d2 < a.rl2;
+        const unsigned long long bal = __ballot(cand);
+        if (bal == 0) continue;
+        const int pos = count + __popcll(bal & ((1ull << lane) - 1ull));
+        const bool near = a.amber && cand && d2 < rex2;
+        if (cand && pos < a.cap) {
+            const size_t e = base_i + pos;
+            a.list_j[e] = cj;
+            if (!near) a.list_mask[e] = (cj == ci) ? self_mask : ~0ull;
+            R4 dl;
+            dl.x = static_cast<real>(-ddx);
+            dl.y = static_cast<real>(-ddy);
+            dl.z = static_cast<real>(-ddz);
+            dl.w = 0;
+            delta[e] = dl;
+        }
+        const unsigned long long nbal = __ballot(near && pos < a.cap);
+        if (near && pos < a.cap) {
+            const int nrk = nnear + __popcll(nbal & ((1ull << lane) - 1ull));
+            if (nrk < kNearCap) s_near[wv][nrk] = make_int2(pos, cj);
+        }
+        nnear += __popcll(nbal);
+        count += __popcll(bal);
     }
     if (count > a.cap) {
         if (lane == 0) atomicOr(a.flags + 1, 1u);
@@ -629,16 +740,8 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
         if (lane == 0) atomicOr(a.flags + 3, 1u);
         nnear = kNearCap;
     }
-    const int ti = lane >> 3, tj = lane & 7;
-    const int ai = a.slot_atom[ci * kCluster + ti];
     __builtin_amdgcn_wave_barrier();
-    for (int q = 0; q < nnear; ++q) {
-        const int2 en = s_near[wv][q];
-        const int aj = a.slot_atom[en.y * kCluster + tj];
-        const bool ex = ai >= 0 && aj >= 0 && pair_excluded(a.excl_start, a.excl_list, ai, aj);
-        const unsigned long long bal = __ballot(ex);
-        if (lane == 0) a.list_mask[base_i + en.x] = ((en.y == ci) ? self_mask : ~0ull) & ~bal;
-    }
+    near_tile_masks(a, ci, lane, s_ex[wv], s_near[wv], nnear, base_i);
 }
 
 }  // namespace
