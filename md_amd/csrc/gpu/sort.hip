@@ -521,20 +521,37 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
 // entry with its j atom in all eight tiles.
 constexpr int kExclCap = 32;
 
-__device__ void near_tile_masks(const SearchArgs& a, int ci, int lane, int (*s_ex)[kExclCap], const int2* near,
-                                int nnear, size_t base_i) {
-    const unsigned long long self_mask = 0x0080C0E0F0F8FCFEull;  // bit 8*ti+tj set iff tj > ti
-    int e0 = 0, en = 0;
-    const int a_l = lane < kCluster ? a.slot_atom[ci * kCluster + lane] : -1;
-    if (a_l >= 0) {
-        e0 = a.excl_start[a_l];
-        en = a.excl_start[a_l + 1] - e0;
+// The i-cluster's exclusion lists as registers: issued at the start of the
+// search so that their dependent loads overlap the candidate scan.
+struct ExclRegs {
+    int a_l, e0, en;
+    int v[kCluster];  // entry `lane` of atom t's list
+};
+
+__device__ __forceinline__ ExclRegs excl_lists_load(const SearchArgs& a, int ci, int lane) {
+    ExclRegs r;
+    r.e0 = 0;
+    r.en = 0;
+    r.a_l = lane < kCluster ? a.slot_atom[ci * kCluster + lane] : -1;
+    if (r.a_l >= 0) {
+        r.e0 = a.excl_start[r.a_l];
+        r.en = a.excl_start[r.a_l + 1] - r.e0;
     }
 #pragma unroll
     for (int t = 0; t < kCluster; ++t) {
-        const int n_t = min(__shfl(en, t, kWave), kExclCap), s_t = __shfl(e0, t, kWave);
-        if (lane < n_t) s_ex[t][lane] = a.excl_list[s_t + lane];
+        const int n_t = min(__shfl(r.en, t, kWave), kExclCap), s_t = __shfl(r.e0, t, kWave);
+        r.v[t] = lane < n_t ? a.excl_list[s_t + lane] : -1;
     }
+    return r;
+}
+
+__device__ void near_tile_masks(const SearchArgs& a, int ci, int lane, const ExclRegs& xr, int (*s_ex)[kExclCap],
+                                const int2* near, int nnear, size_t base_i) {
+    const unsigned long long self_mask = 0x0080C0E0F0F8FCFEull;  // bit 8*ti+tj set iff tj > ti
+    const int a_l = xr.a_l, en = xr.en;
+#pragma unroll
+    for (int t = 0; t < kCluster; ++t)
+        if (lane < kExclCap) s_ex[t][lane] = xr.v[t];
     __builtin_amdgcn_wave_barrier();
     const int ti = lane >> 3, tj = lane & 7;
     const int ai = __shfl(a_l, ti, kWave), my_n = __shfl(en, ti, kWave);
@@ -614,6 +631,8 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     if (a.amber) rex = static_cast<double>(*a.excl_rmax) + 0.01;
     const double rex2 = rex * rex;
     const unsigned long long self_mask = 0x0080C0E0F0F8FCFEull;  // bit 8*ti+tj set iff tj > ti
+    ExclRegs xr{};
+    if (a.amber) xr = excl_lists_load(a, ci, lane);
 
     // ranges: lane r = 2 * row + segment
     int r_b0 = 0, r_len = 0;
@@ -741,7 +760,7 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
         nnear = kNearCap;
     }
     __builtin_amdgcn_wave_barrier();
-    near_tile_masks(a, ci, lane, s_ex[wv], s_near[wv], nnear, base_i);
+    near_tile_masks(a, ci, lane, xr, s_ex[wv], s_near[wv], nnear, base_i);
 }
 
 }  // namespace
