@@ -788,6 +788,8 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
                     const int slot = k == MDGPU_K_COUNT ? MDGPU_K_FFT : k;
                     T.total_ms[slot] += ms;
                     if (k != MDGPU_K_COUNT) T.launches[slot] += 1;
+                } else {
+                    (void)hipGetLastError();  // a timing failure must not surface in a later launch check
                 }
             }
         }
