@@ -159,6 +159,11 @@ int mdgpu_destroy(mdgpu_ctx* c) {
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (c->h_energies) (void)hipHostFree(c->h_energies);
+    for (auto& s : c->side)
+        if (s) (void)hipStreamDestroy(s);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    for (auto& ev : c->ev_join)
+        if (ev) (void)hipEventDestroy(ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MDGPU_OK;
@@ -646,10 +651,31 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double* __restr
     }
 }
 
+// Side streams for the PME chain and the bonded launch, created on first use.
+hipError_t ensure_side_streams(mdgpu_ctx* c) {
+    if (c->side[0]) return hipSuccess;
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming);
+    return e;
+}
+
+// Points c->stream at another stream for the launches of one scope.
+struct StreamSwap {
+    mdgpu_ctx* c;
+    hipStream_t saved;
+    StreamSwap(mdgpu_ctx* ctx, hipStream_t s) : c(ctx), saved(ctx->stream) { c->stream = s; }
+    ~StreamSwap() { c->stream = saved; }
+};
+
 // Launches one evaluation on c->stream after prepare(): no host
-// synchronisation and no allocation, so a caller can capture it into a graph.
-// Energies and flags land in w.energies, forces in d_out.
-int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out) {
+// synchronisation and no allocation, so a caller can capture it into a graph
+// (with fork = false).  With fork, the PME chain and the bonded launch run on
+// two side streams after the sort, concurrently with the pair search and the
+// pair kernel, and the final reduction waits for both.  Energies and flags
+// land in w.energies, forces in d_out.
+int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out, bool fork) {
     const mdgpu_params& p = c->params;
     const bool mixed = p.precision == MDGPU_PREC_MIXED;
     const bool amber = p.semantics == MDGPU_SEM_AMBER;
@@ -661,31 +687,60 @@ int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out) {
     auto mark = [&](int k, bool start) {
         if (timed(k)) (void)hipEventRecord(c->ev[2 * k + (start ? 0 : 1)], c->stream);
     };
+    if (fork) HIP_TRY(ensure_side_streams(c), "side streams");
     mark(MDGPU_K_SORT, true);
     HIP_TRY(mdg::launch_sort(c, x, mixed, amber && c->topo.has_excl), "sort");
     mark(MDGPU_K_SORT, false);
+    // energy rows: pair kernel, bonded + corrections, PME
+    const int row_nb = 0;
+    const int row_b = row_nb + mdg::nonbonded_rows(c);
+    const int row_pme = row_b + mdg::bonded_rows(c) + (amber ? mdg::corrections_rows(c) : 0);
+    int rows = row_pme;
+    const hipStream_t s_pme = fork ? c->side[0] : c->stream, s_bond = fork ? c->side[1] : c->stream;
+    if (fork) HIP_TRY(hipEventRecord(c->ev_fork, c->stream), "fork");
+    // the critical path first: the side streams' launches are issued while
+    // the search and the pair kernel already run
     mark(MDGPU_K_PAIRLIST, true);
     HIP_TRY(mdg::launch_pairsearch(c, mixed, amber), "pair search");
     mark(MDGPU_K_PAIRLIST, false);
-    int row = 0;
     mark(MDGPU_K_NONBONDED, true);
-    HIP_TRY(mdg::launch_nonbonded(c, mixed, amber, row), "nonbonded");
+    HIP_TRY(mdg::launch_nonbonded(c, mixed, amber, row_nb), "nonbonded");
     mark(MDGPU_K_NONBONDED, false);
-    row += mdg::nonbonded_rows(c);
-    mark(MDGPU_K_BONDED, true);
-    HIP_TRY(mdg::launch_bonded(c, x, mixed, amber, row), "bonded");
-    mark(MDGPU_K_BONDED, false);
-    row += mdg::bonded_rows(c) + (amber ? mdg::corrections_rows(c) : 0);
-    if (amber) {
-        int used = 0;
-        HIP_TRY(mdg::launch_pme(c, mixed, row, &used), "pme");
-        row += used;
+    if (fork) {
+        HIP_TRY(hipStreamWaitEvent(s_pme, c->ev_fork, 0), "fork");
+        HIP_TRY(hipStreamWaitEvent(s_bond, c->ev_fork, 0), "fork");
+        if (amber) {
+            StreamSwap sw(c, s_pme);
+            int used = 0;
+            HIP_TRY(mdg::launch_pme(c, mixed, row_pme, &used), "pme");
+            rows += used;
+        }
+        {
+            StreamSwap sw(c, s_bond);
+            mark(MDGPU_K_BONDED, true);
+            HIP_TRY(mdg::launch_bonded(c, x, mixed, amber, row_b), "bonded");
+            mark(MDGPU_K_BONDED, false);
+        }
+        HIP_TRY(hipEventRecord(c->ev_join[0], s_pme), "join");
+        HIP_TRY(hipEventRecord(c->ev_join[1], s_bond), "join");
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join[0], 0), "join");
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join[1], 0), "join");
+    } else {
+        mark(MDGPU_K_BONDED, true);
+        HIP_TRY(mdg::launch_bonded(c, x, mixed, amber, row_b), "bonded");
+        mark(MDGPU_K_BONDED, false);
+        if (amber) {
+            int used = 0;
+            HIP_TRY(mdg::launch_pme(c, mixed, row_pme, &used), "pme");
+            rows += used;
+        }
     }
     mark(MDGPU_K_REDUCE, true);
     const int fblocks = std::max(1, std::min(512, (3 * n + kReduceThreads - 1) / kReduceThreads));
     hipLaunchKernelGGL(k_reduce, dim3(mdg::E_NSLOTS + 1 + fblocks), dim3(kReduceThreads), 0, c->stream, w.partial,
-                       static_cast<int>(w.cap_partial), row, w.energies, c->d_host_energies, w.flags, w.excl_rmax, mixed ? 1 : 0, w.force_fx, w.force_d,
-                       1.0 / mdg::fixed_scale(p.semantics), 3 * n, d_out, w.list_count, w.col_cstart + c->n_cols);
+                       static_cast<int>(w.cap_partial), rows, w.energies, c->d_host_energies, w.flags, w.excl_rmax,
+                       mixed ? 1 : 0, w.force_fx, w.force_d, 1.0 / mdg::fixed_scale(p.semantics), 3 * n, d_out,
+                       w.list_count, w.col_cstart + c->n_cols);
     HIP_TRY(hipGetLastError(), "reduce");
     mark(MDGPU_K_REDUCE, false);
     return MDGPU_OK;
@@ -764,7 +819,7 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
         if (st != MDGPU_OK) return st;
         Work& w = c->work;
         const double* x = coords_are_work ? w.coords : d_coords;
-        st = enqueue_eval(c, x, d_out ? d_out : w.force_out);
+        st = enqueue_eval(c, x, d_out ? d_out : w.force_out, c->fork);
         if (st != MDGPU_OK) return st;
         // the reduction wrote energies and flags into host-mapped memory
         HIP_TRY(hipStreamSynchronize(c->stream), "evaluation");
@@ -804,7 +859,7 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
 namespace mdg {
 int ctx_prepare(mdgpu_ctx* c, const double box[3]) { return prepare(c, box); }
 int ctx_enqueue(mdgpu_ctx* c, const double* d_coords, double* d_forces) {
-    return enqueue_eval(c, d_coords, d_forces ? d_forces : c->work.force_out);
+    return enqueue_eval(c, d_coords, d_forces ? d_forces : c->work.force_out, false);
 }
 int ctx_check(mdgpu_ctx* c, const double* e) { return check_flags(c, e); }
 void ctx_outputs(mdgpu_ctx* c, const double* e, mdgpu_energies* out_e, double* out_virial) {

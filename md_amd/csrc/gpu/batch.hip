@@ -268,6 +268,9 @@ int mdgpu_batch_add(mdgpu_batch* b, const mdgpu_system_desc* sys, const mdgpu_pa
     mdgpu_ctx* c = nullptr;
     int st = mdgpu_create(b->device, &c);
     if (st != MDGPU_OK) return berr(b, st, "mdgpu_create failed");
+    // systems of a batch overlap each other on the batch's streams; they do
+    // not fork streams of their own
+    c->fork = false;
     st = mdgpu_upload_system(c, sys);
     if (st == MDGPU_OK) st = mdgpu_set_params(c, p);
     if (st == MDGPU_OK) {

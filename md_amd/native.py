@@ -399,6 +399,28 @@ class Evaluator:
                                                v.ctypes.data_as(P_F64)))
         return e.as_dict(), v.reshape(3, 3)
 
+    def set_box(self, box):
+        """Box for step_device (kept with its ctypes pointer between calls)."""
+        self._step_box = np.array(box, dtype=np.float64).reshape(3)
+        self._step_box_p = self._step_box.ctypes.data_as(P_F64)
+        self._step_e = Energies()
+        self._step_e_p = C.byref(self._step_e)
+        self._step_v = np.zeros(9, dtype=np.float64)
+        self._step_v_p = self._step_v.ctypes.data_as(P_F64)
+
+    def step_device(self, coords_ptr: int, forces_ptr: int):
+        """The device-buffer evaluation of evaluate_device in the box of set_box,
+        for time-step loops: it builds no Python objects; the energies and
+        virial of the call are read with last_energies()."""
+        st = self.lib.mdgpu_eval_device(self.ctx, coords_ptr, self._step_box_p, self._step_e_p, forces_ptr,
+                                        self._step_v_p)
+        if st != 0:
+            self._check(st)
+
+    def last_energies(self):
+        """(energies dict, virial) of the last step_device call."""
+        return self._step_e.as_dict(), self._step_v.copy().reshape(3, 3)
+
     def adjoint(self, coords, box, weights: dict, desc: SystemDesc) -> dict:
         """sum_t w_t dE_t/dtheta at one configuration (AMBER semantics).
 
