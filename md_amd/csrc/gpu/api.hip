@@ -651,13 +651,12 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double* __restr
     }
 }
 
-// Side streams for the PME chain and the bonded launch, created on first use.
+// Side stream for the PME chain, created on first use.
 hipError_t ensure_side_streams(mdgpu_ctx* c) {
     if (c->side[0]) return hipSuccess;
-    hipError_t e = hipSuccess;
-    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking);
+    hipError_t e = hipStreamCreateWithFlags(&c->side[0], hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
-    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join[0], hipEventDisableTiming);
     return e;
 }
 
@@ -671,9 +670,9 @@ struct StreamSwap {
 
 // Launches one evaluation on c->stream after prepare(): no host
 // synchronisation and no allocation, so a caller can capture it into a graph
-// (with fork = false).  With fork, the PME chain and the bonded launch run on
-// two side streams after the sort, concurrently with the pair search and the
-// pair kernel, and the final reduction waits for both.  Energies and flags
+// (with fork = false).  With fork, the PME chain runs on a side stream after
+// the sort, concurrently with the pair search, the pair kernel and the bonded
+// launch, and the final reduction waits for it.  Energies and flags
 // land in w.energies, forces in d_out.
 int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out, bool fork) {
     const mdgpu_params& p = c->params;
@@ -696,7 +695,7 @@ int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out, bool fork) {
     const int row_b = row_nb + mdg::nonbonded_rows(c);
     const int row_pme = row_b + mdg::bonded_rows(c) + (amber ? mdg::corrections_rows(c) : 0);
     int rows = row_pme;
-    const hipStream_t s_pme = fork ? c->side[0] : c->stream, s_bond = fork ? c->side[1] : c->stream;
+    const hipStream_t s_pme = fork ? c->side[0] : c->stream;
     if (fork) HIP_TRY(hipEventRecord(c->ev_fork, c->stream), "fork");
     // the critical path first: the side streams' launches are issued while
     // the search and the pair kernel already run
@@ -707,24 +706,20 @@ int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out, bool fork) {
     HIP_TRY(mdg::launch_nonbonded(c, mixed, amber, row_nb), "nonbonded");
     mark(MDGPU_K_NONBONDED, false);
     if (fork) {
-        HIP_TRY(hipStreamWaitEvent(s_pme, c->ev_fork, 0), "fork");
-        HIP_TRY(hipStreamWaitEvent(s_bond, c->ev_fork, 0), "fork");
         if (amber) {
+            HIP_TRY(hipStreamWaitEvent(s_pme, c->ev_fork, 0), "fork");
             StreamSwap sw(c, s_pme);
             int used = 0;
             HIP_TRY(mdg::launch_pme(c, mixed, row_pme, &used), "pme");
             rows += used;
+            HIP_TRY(hipEventRecord(c->ev_join[0], s_pme), "join");
         }
-        {
-            StreamSwap sw(c, s_bond);
-            mark(MDGPU_K_BONDED, true);
-            HIP_TRY(mdg::launch_bonded(c, x, mixed, amber, row_b), "bonded");
-            mark(MDGPU_K_BONDED, false);
-        }
-        HIP_TRY(hipEventRecord(c->ev_join[0], s_pme), "join");
-        HIP_TRY(hipEventRecord(c->ev_join[1], s_bond), "join");
-        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join[0], 0), "join");
-        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join[1], 0), "join");
+        // the bonded launch follows the pair kernel on the main stream while
+        // the PME chain finishes on the side stream
+        mark(MDGPU_K_BONDED, true);
+        HIP_TRY(mdg::launch_bonded(c, x, mixed, amber, row_b), "bonded");
+        mark(MDGPU_K_BONDED, false);
+        if (amber) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join[0], 0), "join");
     } else {
         mark(MDGPU_K_BONDED, true);
         HIP_TRY(mdg::launch_bonded(c, x, mixed, amber, row_b), "bonded");

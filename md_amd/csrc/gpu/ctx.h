@@ -181,11 +181,11 @@ struct mdgpu_ctx {
     // (written by the final reduction kernel; d_ is its device address)
     double* h_energies = nullptr;
     double* d_host_energies = nullptr;
-    // concurrent PME chain and bonded launch (single-context evaluations)
+    // concurrent PME chain (single-context evaluations)
     bool fork = true;
-    hipStream_t side[2] = {nullptr, nullptr};
+    hipStream_t side[1] = {nullptr};
     hipEvent_t ev_fork = nullptr;
-    hipEvent_t ev_join[2] = {nullptr, nullptr};
+    hipEvent_t ev_join[1] = {nullptr};
 };
 
 namespace mdg {
