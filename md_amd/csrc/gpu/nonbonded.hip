@@ -21,6 +21,8 @@
 // Ewald real-space energy and force.
 #include "ctx.h"
 
+#include <type_traits>
+
 namespace mdg {
 
 namespace {
@@ -102,18 +104,32 @@ struct Math<double> {
 // fixed point (mixed form), converted in FP32 arithmetic for AMBER semantics
 // (|F| < 2^31) and in FP64 for REF semantics, whose bonded neighbours interact
 // with full LJ.  Every lane issues the buffer atomic; lanes that add nothing
-// point it past the end of the array.
+// point it past the end of the array.  force_payload returns the value to add
+// and sets the byte offset (kOffOOB for nothing).
+template <typename real>
+using Payload = typename std::conditional<sizeof(real) == 8, double, long long>::type;
+
 template <typename real, bool kAmber>
-__device__ __forceinline__ void add_force(const NbArgs<real>& a, BufRsrc r, bool on, int idx, real v) {
-    const unsigned int off = static_cast<unsigned int>(idx) * 8u;
+__device__ __forceinline__ Payload<real> force_payload(const NbArgs<real>& a, bool on, int idx, real v,
+                                                       unsigned int& off) {
+    const unsigned int o = static_cast<unsigned int>(idx) * 8u;
     if constexpr (sizeof(real) == 8) {
-        buf_add(r, on && v != 0.0 ? off : kOffOOB, v);
+        off = on && v != 0.0 ? o : kOffOOB;
+        return v;
     } else {
         long long q;
         if constexpr (kAmber) q = to_fixed_f32(v, a.fx_shift, a.fx_half, a.fx.overflow);
         else q = to_fixed(static_cast<double>(v), a.fx.scale, a.fx.overflow);
-        buf_add(r, on && q != 0 ? off : kOffOOB, q);
+        off = on && q != 0 ? o : kOffOOB;
+        return q;
     }
+}
+
+template <typename real, bool kAmber>
+__device__ __forceinline__ void add_force(const NbArgs<real>& a, BufRsrc r, bool on, int idx, real v) {
+    unsigned int off;
+    const Payload<real> q = force_payload<real, kAmber>(a, on, idx, v, off);
+    buf_add(r, off, q);
 }
 
 // Sum over the 8 consecutive lanes of a group with three DPP moves
@@ -294,10 +310,16 @@ __global__ __launch_bounds__(kWavesOf<real> * kWave) void k_nonbonded(const NbAr
                 fiy += fy;
                 fiz += fz;
                 // j-forces: every tile issues its atomic (an empty tile adds
-                // nothing), so the waits for the prefetched clusters stay short
-                const real gx = sum8(fx), gy = sum8(fy), gz = sum8(fz);
-                add_force<real, kAmber>(a, rsrc, ti < 3 && mcur.y >= 0, 3 * mcur.y + ti,
-                                        -(ti == 0 ? gx : (ti == 1 ? gy : gz)));
+                // nothing and skips the sums and the conversion), so the waits
+                // for the prefetched clusters stay short
+                Payload<real> q = 0;
+                unsigned int off = kOffOOB;
+                if (__ballot(act) != 0ull) {
+                    const real gx = sum8(fx), gy = sum8(fy), gz = sum8(fz);
+                    q = force_payload<real, kAmber>(a, ti < 3 && mcur.y >= 0, 3 * mcur.y + ti,
+                                                    -(ti == 0 ? gx : (ti == 1 ? gy : gz)), off);
+                }
+                buf_add(rsrc, off, q);
                 if (++flush == 32) {
                     flush = 0;
                     e_vdw += s_vdw; e_all += s_all; e_list += s_list; e_rf += s_rf; e_ew += s_ew;
