@@ -406,6 +406,10 @@ struct SearchArgs {
     const int* excl_start;
     const int* excl_list;
     const float* excl_rmax;
+    // slot positions relative to the cluster origin (float4 / double4) for the
+    // atom-box test of the fast search
+    const void* pos;
+    double rp2;             // (r_list + margin)^2 for that test
 };
 
 constexpr int kNearCap = 256;
@@ -589,6 +593,16 @@ __device__ void near_tile_masks(const SearchArgs& a, int ci, int lane, const Exc
     }
 }
 
+__device__ __forceinline__ float bcast_lane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double bcast_lane(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
 // Fast search: every candidate j-cluster is taken at its minimum-image shift
 // (per lane, FP64 rint), so one pass over the clusters of each row of
 // columns in reach replaces the per-column, per-shift enumeration.  It is
@@ -692,6 +706,18 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
         return q < total ? cj : -1;
     };
 
+    // the i-cluster's atoms relative to its origin, in scalar registers
+    const R4* pos = static_cast<const R4*>(a.pos);
+    const R4 pi_l = pos[ci * kCluster + (lane & 7)];
+    real pix[kCluster], piy[kCluster], piz[kCluster];
+#pragma unroll
+    for (int k = 0; k < kCluster; ++k) {
+        pix[k] = bcast_lane(pi_l.x, k);
+        piy[k] = bcast_lane(pi_l.y, k);
+        piz[k] = bcast_lane(pi_l.z, k);
+    }
+    const real rp2 = static_cast<real>(a.rp2);
+
     int count = 0, nnear = 0;
     int cj_n = cand_of(lane);
     double4 oj_n = make_double4(0, 0, 0, 0);
@@ -725,8 +751,25 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
             const double qz = fmax(0.0, fabs(ddz) - (double)(hi.z + hj.z));
             d2 = qx * qx + qy * qy + qz * qz;
         }
-        const bool cand = valid && d2 < a.rl2;
-        const unsigned long long bal = __ballot(cand);
+        bool cand = valid && d2 < a.rl2;
+        unsigned long long bal = __ballot(cand);
+        if (bal == 0) continue;
+        // second test: a tile can only hold a pair within r_list if one of the
+        // i-atoms lies within r_list (plus a margin) of the j-cluster's box
+        if (cand) {
+            const real cx = static_cast<real>(ddx), cy = static_cast<real>(ddy), cz = static_cast<real>(ddz);
+            const real hx = static_cast<real>(hj.x), hy = static_cast<real>(hj.y), hz = static_cast<real>(hj.z);
+            bool any = false;
+#pragma unroll
+            for (int k = 0; k < kCluster; ++k) {
+                const real qx = fmax(static_cast<real>(0), fabs(cx - pix[k]) - hx);
+                const real qy = fmax(static_cast<real>(0), fabs(cy - piy[k]) - hy);
+                const real qz = fmax(static_cast<real>(0), fabs(cz - piz[k]) - hz);
+                any = any || qx * qx + qy * qy + qz * qz < rp2;
+            }
+            cand = any;
+        }
+        bal = __ballot(cand);
         if (bal == 0) continue;
         const int pos = count + __popcll(bal & ((1ull << lane) - 1ull));
         const bool near = a.amber && cand && d2 < rex2;
@@ -820,6 +863,7 @@ hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber) {
     a.origin = w.cl_origin;
     a.half = w.cl_half;
     a.slot_atom = w.slot_atom;
+    a.pos = mixed ? static_cast<const void*>(w.slot_pos_f) : static_cast<const void*>(w.slot_pos_d);
     a.ncx = g.ncx;
     a.ncy = g.ncy;
     a.dcx = g.dcx;
@@ -831,6 +875,7 @@ hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber) {
     a.lz = g.box[2];
     const double rl = std::max(c->params.lj_cutoff, c->params.ee_cutoff);
     a.rl2 = rl * rl;
+    a.rp2 = (rl + 1e-3) * (rl + 1e-3);
     a.cap = w.list_cap;
     a.list_count = w.list_count;
     a.list_j = w.list_j;
