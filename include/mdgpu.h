@@ -161,10 +161,10 @@ typedef struct {
 enum {
     MDGPU_K_SORT = 0,       /* spatial sort + cluster build                 */
     MDGPU_K_PAIRLIST = 1,   /* cluster-pair search                          */
-    MDGPU_K_NONBONDED = 2,  /* LJ + Coulomb/Ewald pair kernel               */
-    MDGPU_K_BONDED = 3,     /* fused bonded kernel                          */
-    MDGPU_K_CORRECT = 4,    /* FP64 excluded / 1-4 corrections: run in the
-                               bonded launch and timed there (stays 0)      */
+    MDGPU_K_NONBONDED = 2,  /* LJ + Coulomb/Ewald pair kernel (+ bonded)    */
+    MDGPU_K_BONDED = 3,     /* bonded terms: run as blocks of the pair
+                               launch and timed there (stays 0)             */
+    MDGPU_K_CORRECT = 4,    /* FP64 excluded / 1-4 corrections: likewise    */
     MDGPU_K_SPREAD = 5,     /* PME charge spreading                         */
     MDGPU_K_FFT = 6,        /* PME r2c + c2c passes (forward + inverse)     */
     MDGPU_K_CONV = 7,       /* PME influence function + energy + virial     */

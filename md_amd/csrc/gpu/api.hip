@@ -671,8 +671,9 @@ struct StreamSwap {
 // Launches one evaluation on c->stream after prepare(): no host
 // synchronisation and no allocation, so a caller can capture it into a graph
 // (with fork = false).  With fork, the PME chain runs on a side stream after
-// the sort, concurrently with the pair search, the pair kernel and the bonded
-// launch, and the final reduction waits for it.  Energies and flags
+// the sort, concurrently with the pair search and the pair launch (which also
+// carries the bonded terms and corrections), and the final reduction waits
+// for it.  Energies and flags
 // land in w.energies, forces in d_out.
 int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out, bool fork) {
     const mdgpu_params& p = c->params;
@@ -690,46 +691,29 @@ int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out, bool fork) {
     mark(MDGPU_K_SORT, true);
     HIP_TRY(mdg::launch_sort(c, x, mixed, amber && c->topo.has_excl), "sort");
     mark(MDGPU_K_SORT, false);
-    // energy rows: pair kernel, bonded + corrections, PME
+    // energy rows: pair kernel + bonded + corrections (one launch), PME
     const int row_nb = 0;
-    const int row_b = row_nb + mdg::nonbonded_rows(c);
-    const int row_pme = row_b + mdg::bonded_rows(c) + (amber ? mdg::corrections_rows(c) : 0);
+    const int row_pme = row_nb + mdg::nonbonded_rows(c) + mdg::bonded_rows(c) + (amber ? mdg::corrections_rows(c) : 0);
     int rows = row_pme;
     const hipStream_t s_pme = fork ? c->side[0] : c->stream;
     if (fork) HIP_TRY(hipEventRecord(c->ev_fork, c->stream), "fork");
-    // the critical path first: the side streams' launches are issued while
+    // the critical path first: the side stream's launches are issued while
     // the search and the pair kernel already run
     mark(MDGPU_K_PAIRLIST, true);
     HIP_TRY(mdg::launch_pairsearch(c, mixed, amber), "pair search");
     mark(MDGPU_K_PAIRLIST, false);
     mark(MDGPU_K_NONBONDED, true);
-    HIP_TRY(mdg::launch_nonbonded(c, mixed, amber, row_nb), "nonbonded");
+    HIP_TRY(mdg::launch_nonbonded(c, mixed, amber, row_nb, x), "nonbonded");
     mark(MDGPU_K_NONBONDED, false);
-    if (fork) {
-        if (amber) {
-            HIP_TRY(hipStreamWaitEvent(s_pme, c->ev_fork, 0), "fork");
-            StreamSwap sw(c, s_pme);
-            int used = 0;
-            HIP_TRY(mdg::launch_pme(c, mixed, row_pme, &used), "pme");
-            rows += used;
-            HIP_TRY(hipEventRecord(c->ev_join[0], s_pme), "join");
-        }
-        // the bonded launch follows the pair kernel on the main stream while
-        // the PME chain finishes on the side stream
-        mark(MDGPU_K_BONDED, true);
-        HIP_TRY(mdg::launch_bonded(c, x, mixed, amber, row_b), "bonded");
-        mark(MDGPU_K_BONDED, false);
-        if (amber) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join[0], 0), "join");
-    } else {
-        mark(MDGPU_K_BONDED, true);
-        HIP_TRY(mdg::launch_bonded(c, x, mixed, amber, row_b), "bonded");
-        mark(MDGPU_K_BONDED, false);
-        if (amber) {
-            int used = 0;
-            HIP_TRY(mdg::launch_pme(c, mixed, row_pme, &used), "pme");
-            rows += used;
-        }
+    if (amber) {
+        if (fork) HIP_TRY(hipStreamWaitEvent(s_pme, c->ev_fork, 0), "fork");
+        StreamSwap sw(c, s_pme);
+        int used = 0;
+        HIP_TRY(mdg::launch_pme(c, mixed, row_pme, &used), "pme");
+        rows += used;
+        if (fork) HIP_TRY(hipEventRecord(c->ev_join[0], s_pme), "join");
     }
+    if (fork && amber) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join[0], 0), "join");
     mark(MDGPU_K_REDUCE, true);
     const int fblocks = std::max(1, std::min(512, (3 * n + kReduceThreads - 1) / kReduceThreads));
     hipLaunchKernelGGL(k_reduce, dim3(mdg::E_NSLOTS + 1 + fblocks), dim3(kReduceThreads), 0, c->stream, w.partial,
@@ -832,7 +816,7 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
                     k == MDGPU_K_COUNT) {
                     if (!(amber && w.pme_k[0] > 0)) continue;
                 }
-                if (k == MDGPU_K_CORRECT) continue;  // timed within the bonded launch
+                if (k == MDGPU_K_CORRECT || k == MDGPU_K_BONDED) continue;  // timed within the pair launch
                 float ms = 0.f;
                 if (hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]) == hipSuccess) {
                     const int slot = k == MDGPU_K_COUNT ? MDGPU_K_FFT : k;

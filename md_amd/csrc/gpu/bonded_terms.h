@@ -1,0 +1,319 @@
+// Device code of the bonded terms and of the AMBER excluded-pair / 1-4
+// corrections, shared by the launch that runs them as extra blocks of the
+// pair kernel.  Reference routines: see bonded.hip.
+#ifndef MD_AMD_GPU_BONDED_TERMS_H
+#define MD_AMD_GPU_BONDED_TERMS_H
+
+#include "ctx.h"
+
+namespace mdg {
+
+namespace {
+
+template <typename real>
+struct R3 {
+    real x, y, z;
+};
+
+template <typename real>
+__device__ __forceinline__ R3<real> diff(const double* __restrict__ x, int a, int b) {
+    return {static_cast<real>(x[3 * a] - x[3 * b]), static_cast<real>(x[3 * a + 1] - x[3 * b + 1]),
+            static_cast<real>(x[3 * a + 2] - x[3 * b + 2])};
+}
+
+template <typename real>
+__device__ __forceinline__ R3<real> cross(const R3<real>& a, const R3<real>& b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+template <typename real>
+__device__ __forceinline__ real dot(const R3<real>& a, const R3<real>& b) {
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+
+template <typename real>
+struct BArgs {
+    const double* x;
+    Topo t;
+    const real* cmap_coeff;
+    double* force_d;
+    FixedAcc fx;
+    double* partial;
+    int ld;            // rows allocated in the slot-major partial buffer
+    int row0;
+    int amber;
+};
+
+template <typename real>
+__device__ __forceinline__ void addf(const BArgs<real>& a, int atom, real fx, real fy, real fz) {
+    if constexpr (sizeof(real) == 8) {
+        acc_add(a.force_d, 3 * atom, fx);
+        acc_add(a.force_d, 3 * atom + 1, fy);
+        acc_add(a.force_d, 3 * atom + 2, fz);
+    } else {
+        acc_add(a.fx, 3 * atom, static_cast<double>(fx));
+        acc_add(a.fx, 3 * atom + 1, static_cast<double>(fy));
+        acc_add(a.fx, 3 * atom + 2, static_cast<double>(fz));
+    }
+}
+
+// Metrics::dihedral (src/System/Metrics.cpp:42-82) from the three bond vectors
+// b1 = x2 - x1, b2 = x3 - x2, b3 = x4 - x3; NaN when degenerate.
+template <typename real>
+__device__ __forceinline__ real dihedral_angle(const R3<real>& b1, const R3<real>& b2, const R3<real>& b3) {
+    const R3<real> n1 = cross(b1, b2), n2 = cross(b2, b3);
+    const real m1 = dot(n1, n1), m2 = dot(n2, n2), l2 = sqrt(dot(b2, b2));
+    if (m1 < static_cast<real>(1e-28) || m2 < static_cast<real>(1e-28) || l2 < static_cast<real>(1e-14))
+        return static_cast<real>(NAN);
+    return atan2(dot(b1, n2) * l2, dot(n1, n2));
+}
+
+// Torque tau = -dE/dphi on (a, b, c, d); ba = x_a - x_b, cb = x_b - x_c,
+// dc = x_c - x_d.  REF semantics distributes it as the reference does
+// (BondedForces.cpp:195-239), which is not the gradient of the torsion
+// energy; AMBER semantics uses the gradient (Bekker's form):
+// F_a = -fa0, F_d = -fd0, F_b = (1 + pb) fa0 - pd fd0, F_c = (1 + pd) fd0 - pb fa0.
+// Returns false (no force) when a cross product or the middle bond is
+// degenerate.
+template <typename real>
+__device__ __forceinline__ bool torsion(const BArgs<real>& a, int ia, int ib, int ic, int id, const R3<real>& ba,
+                                        const R3<real>& cb, const R3<real>& dc, real tau, bool apply) {
+    const R3<real> m = cross(ba, cb), n = cross(cb, dc);
+    const real m2 = dot(m, m), n2 = dot(n, n), c2 = dot(cb, cb);
+    const real eps = static_cast<real>(1e-24);
+    if (m2 < eps || n2 < eps || c2 < eps) return false;
+    if (!apply) return true;
+    const real lc = sqrt(c2);
+    const real sa = tau * lc / m2, sd = -tau * lc / n2;
+    const R3<real> fa = {sa * m.x, sa * m.y, sa * m.z}, fd = {sd * n.x, sd * n.y, sd * n.z};
+    const real pb = dot(ba, cb) / c2, pd = dot(dc, cb) / c2;
+    if (a.amber) {
+        addf(a, ia, -fa.x, -fa.y, -fa.z);
+        addf(a, id, -fd.x, -fd.y, -fd.z);
+        addf(a, ib, (pb + 1) * fa.x - pd * fd.x, (pb + 1) * fa.y - pd * fd.y, (pb + 1) * fa.z - pd * fd.z);
+        addf(a, ic, (pd + 1) * fd.x - pb * fa.x, (pd + 1) * fd.y - pb * fa.y, (pd + 1) * fd.z - pb * fa.z);
+    } else {
+        addf(a, ia, fa.x, fa.y, fa.z);
+        addf(a, id, fd.x, fd.y, fd.z);
+        addf(a, ib, (pb - 1) * fa.x - pd * fd.x, (pb - 1) * fa.y - pd * fd.y, (pb - 1) * fa.z - pd * fd.z);
+        addf(a, ic, (pd - 1) * fd.x - pb * fa.x, (pd - 1) * fd.y - pb * fa.y, (pd - 1) * fd.z - pb * fa.z);
+    }
+    return true;
+}
+
+template <typename real>
+__device__ __forceinline__ real wrap_pi(real d) {
+    const real pi = static_cast<real>(kPi), two = static_cast<real>(2 * kPi);
+    // the reference's loops (CMapGroup.cpp:88-91); bounded so that a
+    // non-finite angle cannot keep a thread spinning
+    for (int it = 0; it < 8 && d <= -pi; ++it) d += two;
+    for (int it = 0; it < 8 && d > pi; ++it) d -= two;
+    return d;
+}
+
+// Term k of the concatenated lists; the block's energies go to row `row`.
+template <typename real>
+__device__ __forceinline__ void bonded_terms(const BArgs<real>& a, int k, int row) {
+    const Topo& t = a.t;
+    const double* x = a.x;
+    const int nb = t.n_bonds, nu = nb + t.n_ubs, na = nu + t.n_angles, nd = na + t.n_dihedrals,
+              ni = nd + t.n_impropers, nc = ni + t.n_cmaps;
+    double e[6] = {0, 0, 0, 0, 0, 0};  // bond, angle, dihedral, UB, improper, CMAP
+    if (k < nu) {
+        // harmonic bond / Urey-Bradley: E = k (r - r0)^2, F_a = -2k (r - r0) d / r
+        const bool ub = k >= nb;
+        const int q = ub ? k - nb : k;
+        const int2 at = ub ? t.ub_atoms[q] : t.bond_atoms[q];
+        const double2 par = ub ? t.ub_par[t.ub_type[q]] : t.bond_par[t.bond_type[q]];
+        const R3<real> d = diff<real>(x, at.x, at.y);
+        const real r = sqrt(dot(d, d));
+        const real dr = r - static_cast<real>(par.y);
+        e[ub ? 3 : 0] = static_cast<double>(static_cast<real>(par.x) * dr * dr);
+        if (r >= static_cast<real>(1e-12)) {
+            const real s = static_cast<real>(-2.0 * par.x) * dr / r;
+            addf(a, at.x, s * d.x, s * d.y, s * d.z);
+            addf(a, at.y, -s * d.x, -s * d.y, -s * d.z);
+        }
+    } else if (k < na) {
+        // harmonic angle (BondedEnergy.cpp:30-49, BondedForces.cpp:73-153)
+        const int q = k - nu;
+        const int4 at = t.angle_atoms[q];
+        const double2 par = t.angle_par[t.angle_type[q]];
+        const R3<real> ba = diff<real>(x, at.x, at.y), bc = diff<real>(x, at.z, at.y);
+        const real ab2 = dot(ba, ba), bc2 = dot(bc, bc), dt = dot(ba, bc);
+        real th;
+        if (ab2 < static_cast<real>(1e-14) || bc2 < static_cast<real>(1e-14)) {
+            th = static_cast<real>(NAN);
+        } else {
+            real cs = dt * (static_cast<real>(1) / sqrt(ab2 * bc2));
+            cs = cs < -1 ? static_cast<real>(-1) : (cs > 1 ? static_cast<real>(1) : cs);
+            th = acos(cs);
+        }
+        const real dth = th - static_cast<real>(par.y);
+        e[1] = static_cast<double>(static_cast<real>(par.x) * dth * dth);
+        const real ab = sqrt(ab2), cb = sqrt(bc2);
+        R3<real> pa = {dt * ba.x - ab2 * bc.x, dt * ba.y - ab2 * bc.y, dt * ba.z - ab2 * bc.z};
+        R3<real> pc = {dt * bc.x - bc2 * ba.x, dt * bc.y - bc2 * ba.y, dt * bc.z - bc2 * ba.z};
+        const real mpa = sqrt(dot(pa, pa)), mpc = sqrt(dot(pc, pc));
+        const real tiny = static_cast<real>(1e-12);
+        if (ab >= tiny && mpa >= tiny && cb >= tiny && mpc >= tiny) {
+            const real fm = static_cast<real>(-2.0 * par.x) * dth;
+            const real sa = fm / ab / mpa, sc = fm / cb / mpc;
+            const R3<real> fa = {sa * pa.x, sa * pa.y, sa * pa.z}, fc = {sc * pc.x, sc * pc.y, sc * pc.z};
+            addf(a, at.x, fa.x, fa.y, fa.z);
+            addf(a, at.z, fc.x, fc.y, fc.z);
+            addf(a, at.y, -fa.x - fc.x, -fa.y - fc.y, -fa.z - fc.z);
+        }
+    } else if (k < ni) {
+        // cosine dihedral (k < nd) or harmonic improper
+        const bool imp = k >= nd;
+        const int q = imp ? k - nd : k - na;
+        const int4 at = imp ? t.imp_atoms[q] : t.dih_atoms[q];
+        const R3<real> b1 = diff<real>(x, at.y, at.x), b2 = diff<real>(x, at.z, at.y), b3 = diff<real>(x, at.w, at.z);
+        const real phi = dihedral_angle(b1, b2, b3);
+        real tau;
+        if (!imp) {
+            const double4 par = t.dih_par[t.dih_type[q]];
+            const real arg = static_cast<real>(par.y) * phi - static_cast<real>(par.z);
+            e[2] = static_cast<double>(static_cast<real>(par.x) * (1 + cos(arg)));
+            tau = static_cast<real>(par.x * par.y) * sin(arg);
+        } else {
+            const double2 par = t.imp_par[t.imp_type[q]];
+            const real dl = phi - static_cast<real>(par.y);
+            const real dw = wrap_pi(dl);
+            e[4] = static_cast<double>(static_cast<real>(par.x) * (a.amber ? dw * dw : dl * dl));
+            tau = static_cast<real>(-2.0 * par.x) * dw;
+        }
+        const R3<real> ba = {-b1.x, -b1.y, -b1.z}, cb = {-b2.x, -b2.y, -b2.z}, dc = {-b3.x, -b3.y, -b3.z};
+        torsion(a, at.x, at.y, at.z, at.w, ba, cb, dc, tau, true);
+    } else if (k < nc) {
+        // CMAP (BondedEnergy.cpp:124-166, CMapGroup.cpp:82-237,
+        // BondedForces.cpp:334-479)
+        const int q = k - ni;
+        const int* at = t.cmap_atoms + 5 * q;
+        const R3<real> b1 = diff<real>(x, at[1], at[0]), b2 = diff<real>(x, at[2], at[1]),
+                       b3 = diff<real>(x, at[3], at[2]), b4 = diff<real>(x, at[4], at[3]);
+        const real phi = dihedral_angle(b1, b2, b3), psi = dihedral_angle(b2, b3, b4);
+        const int res = t.cmap_res;
+        const real sp = static_cast<real>(2 * kPi) / static_cast<real>(res);
+        const real pp = wrap_pi(phi) + static_cast<real>(kPi), ss = wrap_pi(psi) + static_cast<real>(kPi);
+        const real gp = pp / sp, gs = ss / sp;
+        int row = static_cast<int>(gp), col = static_cast<int>(gs);
+        const real u = gp - static_cast<real>(row), v = gs - static_cast<real>(col);
+        row %= res;
+        col %= res;
+        if (row < 0) row += res;
+        if (col < 0) col += res;
+        real en = 0, du = 0, dv = 0;
+        if (phi == phi && psi == psi) {
+            const real* c = a.cmap_coeff + (static_cast<size_t>(t.cmap_set[q] - 1) * res * res + row * res + col) * 16;
+            const real up[4] = {1, u, u * u, u * u * u}, vp[4] = {1, v, v * v, v * v * v};
+            const real dup[4] = {0, 1, 2 * u, 3 * u * u}, dvp[4] = {0, 1, 2 * v, 3 * v * v};
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const real cc = c[4 * i + j];
+                    en += cc * up[i] * vp[j];
+                    du += cc * dup[i] * vp[j];
+                    dv += cc * up[i] * dvp[j];
+                }
+        } else {
+            en = static_cast<real>(NAN);
+        }
+        e[5] = static_cast<double>(en);
+        const real dphi = du / sp, dpsi = dv / sp;
+        const R3<real> ba = {-b1.x, -b1.y, -b1.z}, cb = {-b2.x, -b2.y, -b2.z}, dc = {-b3.x, -b3.y, -b3.z},
+                       ed = {-b4.x, -b4.y, -b4.z};
+        // the reference skips both halves if any of the five products is degenerate
+        const bool ok = torsion(a, at[0], at[1], at[2], at[3], ba, cb, dc, -dphi, false) &&
+                        torsion(a, at[1], at[2], at[3], at[4], cb, dc, ed, -dpsi, false);
+        if (ok) {
+            torsion(a, at[0], at[1], at[2], at[3], ba, cb, dc, -dphi, true);
+            torsion(a, at[1], at[2], at[3], at[4], cb, dc, ed, -dpsi, true);
+        }
+    }
+    const int slots[6] = {E_BOND, E_ANGLE, E_DIHEDRAL, E_UB, E_IMPROPER, E_CMAP};
+    block_sum_store<6>(e, a.partial, a.ld, row, slots);
+}
+
+struct CArgs {
+    const double* x;
+    double lx, ly, lz;
+    int n_excl;
+    const int2* excl;
+    int n14;
+    const int2* p14;
+    const double* scee;
+    const double* charge;
+    const int* type;
+    const int* nb_index;      // [T*T] 1-based
+    const double2* lj14;      // [n_lj]
+    int ntypes;
+    double alpha;
+    int mixed;
+    double* force_d;
+    FixedAcc fx;
+    double* partial;
+    int ld;
+    int row0;
+};
+
+__device__ __forceinline__ void addc(const CArgs& a, int atom, double fx, double fy, double fz) {
+    if (a.mixed) {
+        acc_add(a.fx, 3 * atom, fx);
+        acc_add(a.fx, 3 * atom + 1, fy);
+        acc_add(a.fx, 3 * atom + 2, fz);
+    } else {
+        acc_add(a.force_d, 3 * atom, fx);
+        acc_add(a.force_d, 3 * atom + 1, fy);
+        acc_add(a.force_d, 3 * atom + 2, fz);
+    }
+}
+
+// Pair k of the excluded pairs followed by the 1-4 pairs.
+__device__ __forceinline__ void correction_pairs(const CArgs& a, int k, int row) {
+    double e[3] = {0, 0, 0};  // ewald_excl, vdw14, ee14
+    if (k < a.n_excl + a.n14) {
+        const bool is14 = k >= a.n_excl;
+        const int2 p = is14 ? a.p14[k - a.n_excl] : a.excl[k];
+        double dx = a.x[3 * p.x] - a.x[3 * p.y], dy = a.x[3 * p.x + 1] - a.x[3 * p.y + 1],
+               dz = a.x[3 * p.x + 2] - a.x[3 * p.y + 2];
+        dx -= a.lx * rint(dx / a.lx);
+        dy -= a.ly * rint(dy / a.ly);
+        dz -= a.lz * rint(dz / a.lz);
+        const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+        const double qq = kCoulomb * a.charge[p.x] * a.charge[p.y];
+        const double ef = erf(a.alpha * r);
+        const double g2 = 1.1283791670955126 * a.alpha * exp(-a.alpha * a.alpha * r2);
+        double g;
+        if (!is14) {
+            e[0] = -qq * ef / r;
+            g = qq * (g2 - ef / r) / r2;
+        } else {
+            const double is = 1.0 / a.scee[k - a.n_excl];
+            e[2] = qq * (is / r - ef / r);
+            g = qq * (is / r + g2 - ef / r) / r2;
+            const int nb = a.nb_index[a.type[p.x] * a.ntypes + a.type[p.y]];
+            if (nb > 0) {
+                const double2 ab = a.lj14[nb - 1];
+                const double i2 = 1.0 / r2, i6 = i2 * i2 * i2;
+                e[1] = (ab.x * i6 - ab.y) * i6;
+                g += (12.0 * ab.x * i6 - 6.0 * ab.y) * i6 * i2;
+            }
+        }
+        addc(a, p.x, g * dx, g * dy, g * dz);
+        addc(a, p.y, -g * dx, -g * dy, -g * dz);
+    }
+    const int slots[3] = {E_EW_EXCL, E_VDW14, E_EE14};
+    block_sum_store<3>(e, a.partial, a.ld, row, slots);
+}
+
+// One launch for both: blocks [0, nbb) take the bonded terms, the rest the
+// corrections (both latency-bound, so they overlap instead of queueing).
+// Block b writes energy row row0 + b.
+}  // namespace
+
+}  // namespace mdg
+
+#endif

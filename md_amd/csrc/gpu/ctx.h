@@ -194,10 +194,10 @@ namespace mdg {
 // excl: also bound the excluded and 1-4 pair distances (AMBER with exclusions)
 hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl);
 hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber);
-hipError_t launch_nonbonded(mdgpu_ctx* c, bool mixed, bool amber, int partial_row0);
-// bonded terms, and with amber the FP64 excluded / 1-4 corrections in the
-// same launch (rows partial_row0 .. + bonded_rows + corrections_rows)
-hipError_t launch_bonded(mdgpu_ctx* c, const double* coords, bool mixed, bool amber, int partial_row0);
+// pair kernel, with the bonded terms and (amber) the FP64 excluded / 1-4
+// corrections as further blocks of the same launch; energy rows partial_row0
+// .. + nonbonded_rows + bonded_rows + corrections_rows
+hipError_t launch_nonbonded(mdgpu_ctx* c, bool mixed, bool amber, int partial_row0, const double* coords);
 hipError_t launch_pme(mdgpu_ctx* c, bool mixed, int partial_row0, int* rows_used);
 hipError_t launch_reduce(mdgpu_ctx* c, bool mixed, int n_rows, double* out_forces);
 // adjoint kernels (AMBER semantics)
@@ -220,6 +220,7 @@ int ctx_eval_device(mdgpu_ctx* c, const double* d_coords, const double box[3], m
 int ctx_error(mdgpu_ctx* c, int status, const std::string& msg);
 
 // rows of the partial-energy buffer each kernel needs for a given launch
+int pair_block_threads(const mdgpu_ctx* c);  // threads per block of the pair launch
 int nonbonded_rows(const mdgpu_ctx* c);
 int bonded_rows(const mdgpu_ctx* c);
 int corrections_rows(const mdgpu_ctx* c);

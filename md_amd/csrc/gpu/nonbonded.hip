@@ -19,6 +19,7 @@
 // Coulomb forms, plain Coulomb force k qq d / r^3.
 // AMBER semantics: masked tiles (exclusions and 1-4 pairs removed), erfc
 // Ewald real-space energy and force.
+#include "bonded_terms.h"
 #include "ctx.h"
 
 #include <type_traits>
@@ -184,9 +185,21 @@ __device__ __forceinline__ double bcast<double>(double v, int lane) {
 //
 // One wave per work item (i-cluster, list part); blocks are ordered so that
 // consecutive clusters run on one XCD and share its L2.
+//
+// The launch also carries the bonded terms and the AMBER corrections as nbb +
+// ncb further blocks after the npb pair blocks: they are dispatched last and
+// fill the CUs that the pair blocks leave idle at the end of the launch.
 template <typename real, bool kAmber, bool kSwitch, bool kCount>
-__global__ __launch_bounds__(kWavesOf<real> * kWave) void k_nonbonded(const NbArgs<real> a) {
+__global__ __launch_bounds__(kWavesOf<real> * kWave) void k_nonbonded(const NbArgs<real> a, const BArgs<real> ba,
+                                                                      const CArgs ca, int npb, int nbb) {
     constexpr int kWaves = kWavesOf<real>;
+    if (static_cast<int>(blockIdx.x) >= npb) {
+        const int b = static_cast<int>(blockIdx.x) - npb;
+        const int k = b * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+        if (b < nbb) bonded_terms(ba, k, ba.row0 + b);
+        else correction_pairs(ca, k - nbb * static_cast<int>(blockDim.x), ca.row0 + (b - nbb));
+        return;
+    }
     using R4 = typename V4<real>::T;
     using M = Math<real>;
     extern __shared__ unsigned char smem[];
@@ -212,7 +225,7 @@ __global__ __launch_bounds__(kWavesOf<real> * kWave) void k_nonbonded(const NbAr
     const real rc2 = a.rc2_lj > a.rc2_ee ? a.rc2_lj : a.rc2_ee;
     int flush = 0;
     const BufRsrc rsrc = sizeof(real) == 8 ? buf_rsrc(a.force_d, 8u * a.n3) : buf_rsrc(a.fx.f, 8u * a.n3);
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int blk = xcd_block(blockIdx.x, npb);
     const int item = __builtin_amdgcn_readfirstlane(blk * kWaves + (threadIdx.x >> 6));
     if (item < ncl * a.split) {
         const int ci = item / a.split, part = item % a.split;
@@ -355,7 +368,7 @@ __global__ __launch_bounds__(kWavesOf<real> * kWave) void k_nonbonded(const NbAr
 }
 
 template <typename real>
-hipError_t run(mdgpu_ctx* c, bool amber, int row0) {
+hipError_t run(mdgpu_ctx* c, bool amber, int row0, const double* coords) {
     Work& w = c->work;
     const Topo& t = c->topo;
     const mdgpu_params& p = c->params;
@@ -398,13 +411,46 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0) {
     a.ld = static_cast<int>(w.cap_partial);
     a.row0 = row0;
     a.n3 = 3 * t.n_atoms;
-    const int nblk = nonbonded_rows(c);
+    const int npb = nonbonded_rows(c);
+    // bonded terms and corrections: rows after the pair blocks' rows
+    const int nbb = bonded_rows(c);
+    const int ncb = amber ? corrections_rows(c) : 0;
+    const FixedAcc fxa{w.force_fx, fixed_scale(p.semantics), w.flags};
+    const BArgs<real> ba{coords, t, sizeof(real) == 4 ? reinterpret_cast<const real*>(t.cmap_coeff_f)
+                                                      : reinterpret_cast<const real*>(t.cmap_coeff),
+                         w.force_d, fxa, w.partial, static_cast<int>(w.cap_partial), row0 + npb, amber ? 1 : 0};
+    CArgs ca{};
+    ca.x = coords;
+    ca.lx = c->grid.box[0];
+    ca.ly = c->grid.box[1];
+    ca.lz = c->grid.box[2];
+    ca.n_excl = t.n_excl_pairs;
+    ca.excl = t.excl_pairs;
+    ca.n14 = t.n_pairs14;
+    ca.p14 = t.pairs14;
+    ca.scee = t.scee14;
+    ca.charge = t.charge;
+    ca.type = t.type;
+    ca.nb_index = t.nb_index;
+    ca.lj14 = t.lj14;
+    ca.ntypes = t.n_types;
+    ca.alpha = p.ewald_alpha;
+    ca.mixed = sizeof(real) == 4 ? 1 : 0;
+    ca.force_d = w.force_d;
+    ca.fx = fxa;
+    ca.partial = w.partial;
+    ca.ld = static_cast<int>(w.cap_partial);
+    ca.row0 = row0 + npb + nbb;
     const size_t shm = (sizeof(typename V4<real>::T) + 1) * a.ntp * a.ntp;
-    dim3 grid(nblk), block(kWavesOf<real> * kWave);
-#define MDG_NB_LAUNCH(AM, SW)                                                                                   \
-    do {                                                                                                       \
-        if (c->counting) hipLaunchKernelGGL((k_nonbonded<real, AM, SW, true>), grid, block, shm, c->stream, a);  \
-        else hipLaunchKernelGGL((k_nonbonded<real, AM, SW, false>), grid, block, shm, c->stream, a);            \
+    dim3 grid(npb + nbb + ncb), block(kWavesOf<real> * kWave);
+#define MDG_NB_LAUNCH(AM, SW)                                                                                  \
+    do {                                                                                                      \
+        if (c->counting)                                                                                      \
+            hipLaunchKernelGGL((k_nonbonded<real, AM, SW, true>), grid, block, shm, c->stream, a, ba, ca, npb, \
+                               nbb);                                                                          \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_nonbonded<real, AM, SW, false>), grid, block, shm, c->stream, a, ba, ca, npb, \
+                               nbb);                                                                          \
     } while (0)
     if (amber) {
         if (sw) MDG_NB_LAUNCH(true, true);
@@ -419,13 +465,17 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0) {
 
 }  // namespace
 
+int pair_block_threads(const mdgpu_ctx* c) {
+    return kWave * (c->params.precision == MDGPU_PREC_MIXED ? kWavesOf<float> : kWavesOf<double>);
+}
+
 int nonbonded_rows(const mdgpu_ctx* c) {
-    const int w = c->params.precision == MDGPU_PREC_MIXED ? kWavesOf<float> : kWavesOf<double>;
+    const int w = pair_block_threads(c) / kWave;
     return (c->n_clusters * c->split + w - 1) / w;
 }
 
-hipError_t launch_nonbonded(mdgpu_ctx* c, bool mixed, bool amber, int row0) {
-    return mixed ? run<float>(c, amber, row0) : run<double>(c, amber, row0);
+hipError_t launch_nonbonded(mdgpu_ctx* c, bool mixed, bool amber, int row0, const double* coords) {
+    return mixed ? run<float>(c, amber, row0, coords) : run<double>(c, amber, row0, coords);
 }
 
 }  // namespace mdg
