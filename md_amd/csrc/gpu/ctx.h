@@ -167,7 +167,7 @@ struct mdgpu_ctx {
     int n_clusters = 0;
     int n_cols = 0;
     int split = 4;                  // waves per i-cluster list in the pair kernel
-    int fft_lines = 4;              // cap on FFT lines per block (power of two; 0 = 16)
+    int fft_lines = 8;              // cap on FFT lines per block (power of two; 0 = 16)
     const double* cur_x = nullptr;  // raw coordinates of the evaluation in progress (device)
     bool search_general = false;    // use the general (per-shift) cluster search
     std::vector<void*> topo_allocs;
