@@ -41,7 +41,7 @@ struct mdgpu_batch {
     std::vector<std::array<double, 3>> box;
     std::vector<int64_t> off;
     int64_t n_atoms = 0;
-    int mode = 0, streams = 4, split = 4, fft_lines = 4;
+    int mode = 0, streams = 4, split = 4, fft_lines = 8;  // split and fft_lines as a single context
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const double* cap_x = nullptr;

@@ -34,6 +34,17 @@ struct C2<double> { using T = double2; };
 constexpr int kThreads = 256;
 constexpr double kGridScale = 1099511627776.0;  // 2^40, fixed-point charge grid
 
+// v 2^40 as a 64-bit integer from FP32 arithmetic (|v| < 2^23): with
+// t = v 2^8 = h + f, h = floor(t), the high word is h and the low word
+// f 2^32 (f = t - h exact to FP32 rounding); the result is deterministic.
+__device__ __forceinline__ long long fixed40_f32(float v) {
+    const float t = v * 256.0f;
+    const float h = floorf(t);
+    const unsigned int lo = static_cast<unsigned int>(fminf((t - h) * 4294967296.0f, 4294967040.0f));
+    const unsigned long long hi = static_cast<unsigned long long>(static_cast<unsigned int>(static_cast<int>(h)));
+    return static_cast<long long>((hi << 32) | lo);
+}
+
 // theta_j = M4(w + j), dtheta_j = M4'(w + j), j = 0..3 (grid point fl - j)
 template <typename real>
 __device__ __forceinline__ void bspline4(real w, real* th, real* dth) {
@@ -224,20 +235,45 @@ __global__ __launch_bounds__(kThreads) void k_spread_fft_z(PmeGeom g, SpreadCols
             bspline4<real>(wq.x, th[0], dth[0]);
             bspline4<real>(wq.y, th[1], dth[1]);
             bspline4<real>(wq.z, th[2], dth[2]);
-            const double q = static_cast<double>(wq.w);
-            for (int a = 0; a < 4; ++a) {
-                const int li = dxp - a;
-                if (li < 0 || li >= bx || X0 + li >= g.kx) continue;
-                for (int b = 0; b < 4; ++b) {
-                    const int lj = dyp - b;
-                    if (lj < 0 || lj >= by || Y0 + lj >= g.ky) continue;
-                    const int l = li * by + lj;
-                    const double qab = q * static_cast<double>(th[0][a] * th[1][b]);
-                    for (int c = 0; c < 4; ++c) {
-                        int iz = flz - c;
-                        iz += iz < 0 ? g.kz : 0;
-                        const long long v = to_fixed(qab * static_cast<double>(th[2][c]), kGridScale, overflow);
-                        if (v != 0) atomicAdd(acc + iz * nl + l, static_cast<unsigned long long>(v));
+            if constexpr (sizeof(real) == 4) {
+                // FP32 products and conversion (charges far below 2^23 e;
+                // a non-finite record raises the overflow flag instead)
+                if (!(fabsf(wq.w) < 8.0e6f) || !(fabsf(wq.x) + fabsf(wq.y) + fabsf(wq.z) < 4.0f)) {
+                    atomicOr(overflow, 1u);
+                    continue;
+                }
+                for (int a = 0; a < 4; ++a) {
+                    const int li = dxp - a;
+                    if (li < 0 || li >= bx || X0 + li >= g.kx) continue;
+                    for (int b = 0; b < 4; ++b) {
+                        const int lj = dyp - b;
+                        if (lj < 0 || lj >= by || Y0 + lj >= g.ky) continue;
+                        const int l = li * by + lj;
+                        const float qab = wq.w * th[0][a] * th[1][b];
+                        for (int c = 0; c < 4; ++c) {
+                            int iz = flz - c;
+                            iz += iz < 0 ? g.kz : 0;
+                            const long long v = fixed40_f32(qab * th[2][c]);
+                            if (v != 0) atomicAdd(acc + iz * nl + l, static_cast<unsigned long long>(v));
+                        }
+                    }
+                }
+            } else {
+                const double q = static_cast<double>(wq.w);
+                for (int a = 0; a < 4; ++a) {
+                    const int li = dxp - a;
+                    if (li < 0 || li >= bx || X0 + li >= g.kx) continue;
+                    for (int b = 0; b < 4; ++b) {
+                        const int lj = dyp - b;
+                        if (lj < 0 || lj >= by || Y0 + lj >= g.ky) continue;
+                        const int l = li * by + lj;
+                        const double qab = q * static_cast<double>(th[0][a] * th[1][b]);
+                        for (int c = 0; c < 4; ++c) {
+                            int iz = flz - c;
+                            iz += iz < 0 ? g.kz : 0;
+                            const long long v = to_fixed(qab * static_cast<double>(th[2][c]), kGridScale, overflow);
+                            if (v != 0) atomicAdd(acc + iz * nl + l, static_cast<unsigned long long>(v));
+                        }
                     }
                 }
             }

@@ -478,7 +478,7 @@ class Batch:
         if st != 0:
             raise GpuError(st, f"mdgpu_batch_create(device={device}) failed")
         self.n_atoms = []
-        self._settings = {"streams": 4, "split": 4, "fft_lines": 4}  # mdgpu_batch_create's defaults
+        self._settings = {"streams": 4, "split": 4, "fft_lines": 8}  # mdgpu_batch_create's defaults
 
     def _check(self, st):
         if st != 0:
