@@ -53,19 +53,19 @@ __device__ __forceinline__ void acc_add(const FixedAcc& a, int idx, double v) {
     if (q != 0) atomicAdd(a.f + idx, static_cast<unsigned long long>(q));
 }
 
-// The same conversion in FP32 arithmetic for a float contribution, scale
-// 2^s with s <= 32: v = t + fr with t = trunc(v) (an int32) and fr exact, so
-// round(v 2^s) ~ t 2^s + 2 trunc(fr 2^(s-1)) (error below 2^(1-s)).  half =
-// 2^(s-1) as float; |v| must stay below 2^31.
-__device__ __forceinline__ long long to_fixed_f32(float v, int s, float half, unsigned int* overflow) {
+// The conversion at scale 2^32 in FP32 arithmetic for a float contribution
+// (|v| < 2^31): with h = floor(v) the high word of v 2^32 is h and the low
+// word (v - h) 2^32, so no 64-bit shifts are needed; the result is
+// deterministic and within 2^-24 |v| of v 2^32.
+__device__ __forceinline__ long long fixed32_f32(float v, unsigned int* overflow) {
     if (!(fabsf(v) < 2147483520.0f)) {  // also catches NaN
         atomicOr(overflow, 1u);
         return 0;
     }
-    const float t = truncf(v);
-    const int hi = static_cast<int>(t);
-    const int lo = static_cast<int>((v - t) * half);
-    return static_cast<long long>(hi) * (1LL << s) + 2LL * static_cast<long long>(lo);
+    const float h = floorf(v);
+    const unsigned int lo = static_cast<unsigned int>(fminf((v - h) * 4294967296.0f, 4294967040.0f));
+    const unsigned long long hi = static_cast<unsigned long long>(static_cast<unsigned int>(static_cast<int>(h)));
+    return static_cast<long long>((hi << 32) | lo);
 }
 
 __device__ __forceinline__ void acc_add(double* f, int idx, double v) {

@@ -60,8 +60,6 @@ struct NbArgs {
     real alpha, kc, krf_crf;           // krf_crf: 1/rc_ee (reaction field with eps = 1)
     double* force_d;
     FixedAcc fx;
-    int fx_shift;                      // log2 of the fixed-point scale
-    float fx_half;                     // 2^(fx_shift - 1)
     double* partial;
     int ld;                            // rows allocated in the slot-major partial buffer
     int row0;
@@ -119,7 +117,7 @@ __device__ __forceinline__ Payload<real> force_payload(const NbArgs<real>& a, bo
         return v;
     } else {
         long long q;
-        if constexpr (kAmber) q = to_fixed_f32(v, a.fx_shift, a.fx_half, a.fx.overflow);
+        if constexpr (kAmber) q = fixed32_f32(v, a.fx.overflow);  // AMBER scale 2^32
         else q = to_fixed(static_cast<double>(v), a.fx.scale, a.fx.overflow);
         off = on && q != 0 ? o : kOffOOB;
         return q;
@@ -405,8 +403,6 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0, const double* coords) {
     a.krf_crf = static_cast<real>(1.0 / p.ee_cutoff);
     a.force_d = w.force_d;
     a.fx = FixedAcc{w.force_fx, fixed_scale(p.semantics), w.flags};
-    a.fx_shift = p.semantics == MDGPU_SEM_REF ? 20 : 32;
-    a.fx_half = static_cast<float>(0.5 * fixed_scale(p.semantics));
     a.partial = w.partial;
     a.ld = static_cast<int>(w.cap_partial);
     a.row0 = row0;
