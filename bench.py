@@ -221,11 +221,16 @@ def main():
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    # the timed region: the pair kernel is bracketed by two HIP events on the
-    # evaluator's stream in every step, so its launch duration is measured live
-    ev32.set_timing(2)
+    # the timed region: in one step of every 16 the pair kernel is bracketed by
+    # two HIP events on the evaluator's stream, so its launch duration is
+    # measured live; the steps replay a captured graph of the evaluation (the
+    # sampled steps the instance with the two event nodes)
+    ev32.set_timing(3)
     t32 = timed(ev32, f32, args.steps, args.warmup)
     nb_total, nb_launches = ev32.timing()["nonbonded"]
+    graph_replay = ev32.graph_state() == 1
+    if nb_launches < (args.warmup + args.steps) // 16:
+        raise SystemExit(f"pair-kernel events timed {nb_launches} of {args.warmup + args.steps} steps")
     ev32.set_timing(0)
     nb_ms = nb_total / max(nb_launches, 1)
     steps64 = args.fp64_steps or max(10, args.steps // 5)
@@ -310,6 +315,7 @@ def main():
         "fp64_evals_per_s": rate64,
         "speedup_mixed_vs_fp64": rate32 / rate64,
         "host_buffer_evals_per_s": rate_host,
+        "eval_path": "captured-graph replay" if graph_replay else "direct launches",
         "force_rel_rms_mixed_vs_fp64": rms,
         "energy_rel_err_mixed_vs_fp64": term_err,
         "work": counts,
@@ -326,6 +332,7 @@ def main():
             "traffic": traffic,
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "avg_launch_ms": nb_ms,
+            "timed_launches": int(nb_launches),
             "algorithmic": f"{PAIR_FLOP} flop x {counts['pairs']} pairs within 12 A per launch",
         },
         "cpu_baseline": cpu,

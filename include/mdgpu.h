@@ -297,12 +297,27 @@ int mdgpu_batch_size(const mdgpu_batch* b, int* n_systems, int64_t* n_atoms);
  * reporting).  mdgpu_set_timing(ctx, mode) resets the totals; mode 1 makes
  * each later evaluation add the event-measured duration of every kernel
  * group, mode 2 only that of the pair kernel (two events per evaluation),
+ * mode 3 that of the pair kernel in one evaluation of every 16 (events in a
+ * replayed graph cost time, so the others replay a graph without them),
  * mode 0 turns timing off. */
 int mdgpu_set_timing(mdgpu_ctx* ctx, int mode);
 int mdgpu_get_timing(mdgpu_ctx* ctx, int kernel_id, double* total_ms, int64_t* launches);
 /* Work counts of the last evaluation: [0] atom pairs within cutoff (set when
  * counting is enabled), [1] cluster-pair tiles, [2] clusters, [3] atoms. */
 int mdgpu_get_counts(mdgpu_ctx* ctx, int64_t counts[4]);
+/* Replay of a captured evaluation (default on; no counterpart in the
+ * reference).  When two consecutive successful evaluations of a context have
+ * the same coordinate and force pointers, box, parameters and buffer sizes,
+ * the next one is captured into a hipGraph and replayed from then on, which
+ * removes the per-launch host work (timing events, if on, are captured as
+ * event nodes).  Any change of those or of the timing mode, or a redo (list
+ * over capacity), makes the context evaluate with direct launches again, for
+ * good.  Contexts of a batch never capture their own
+ * graph. */
+int mdgpu_set_graph(mdgpu_ctx* ctx, int enable);
+/* 0: no graph captured yet, 1: evaluations replay a captured graph,
+ * 2: direct launches for good (-1 for a null context). */
+int mdgpu_graph_state(const mdgpu_ctx* ctx);
 /* Enable (1) or disable (0) counting of pairs within the cutoff. */
 int mdgpu_set_counting(mdgpu_ctx* ctx, int enable);
 

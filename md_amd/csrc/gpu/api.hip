@@ -158,6 +158,9 @@ int mdgpu_destroy(mdgpu_ctx* c) {
     if (c->adj.lj_d) (void)hipFree(c->adj.lj_d);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph_exec_timed) (void)hipGraphExecDestroy(c->graph_exec_timed);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     if (c->h_energies) (void)hipHostFree(c->h_energies);
     for (auto& s : c->side)
         if (s) (void)hipStreamDestroy(s);
@@ -381,6 +384,7 @@ extern "C" int mdgpu_upload_system(mdgpu_ctx* c, const mdgpu_system_desc* s) {
         HIP_TRY(upload(c, &t.lj_pidx, pidx.data(), pidx.size()), "upload LJ index map");
     }
     c->have_system = true;
+    ++c->gen;  // a captured evaluation graph no longer applies
     if (c->have_params) return build_lj_table(c);
     return MDGPU_OK;
 }
@@ -409,6 +413,7 @@ extern "C" int mdgpu_set_params(mdgpu_ctx* c, const mdgpu_params* p) {
     HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
     c->params = *p;
     c->have_params = true;
+    ++c->gen;
     if (c->have_system) return build_lj_table(c);
     return MDGPU_OK;
 }
@@ -683,9 +688,9 @@ int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out, bool fork) {
     Work& w = c->work;
     c->cur_x = x;
     auto& T = c->timing;
-    auto timed = [&](int k) { return T.mode == 1 || (T.mode == 2 && k == MDGPU_K_NONBONDED); };
+    auto timed = [&](int k) { return T.mode == 1 || (T.mode >= 2 && k == MDGPU_K_NONBONDED); };
     auto mark = [&](int k, bool start) {
-        if (timed(k)) (void)hipEventRecord(c->ev[2 * k + (start ? 0 : 1)], c->stream);
+        if (timed(k)) (void)mdg::record_timing_event(c, c->ev[2 * k + (start ? 0 : 1)]);
     };
     if (fork) HIP_TRY(ensure_side_streams(c), "side streams");
     mark(MDGPU_K_SORT, true);
@@ -788,6 +793,124 @@ void fill_outputs(mdgpu_ctx* c, const double* e, mdgpu_energies* out_e, double* 
     }
 }
 
+// Everything a captured evaluation bakes into its kernel arguments.
+mdg::GraphKey graph_key(const mdgpu_ctx* c, const double* x, const double* out) {
+    const Work& w = c->work;
+    mdg::GraphKey k{};
+    k.x = x;
+    k.out = out;
+    for (int d = 0; d < 3; ++d) {
+        k.box[d] = c->grid.box[d];
+        k.pme_k[d] = w.pme_k[d];
+    }
+    k.gen = c->gen;
+    k.list_cap = w.list_cap;
+    k.search_general = c->search_general ? 1 : 0;
+    k.split = c->split;
+    k.fft_lines = c->fft_lines;
+    k.counting = c->counting ? 1 : 0;
+    k.timing = c->timing.mode;
+    k.n_clusters = c->n_clusters;
+    k.n_cols = c->n_cols;
+    k.caps[0] = w.cap_atoms;
+    k.caps[1] = w.cap_cols;
+    k.caps[2] = w.cap_slots;
+    k.caps[3] = w.cap_list;
+    k.caps[4] = w.cap_partial;
+    k.caps[5] = w.cap_grid;
+    return k;
+}
+
+bool same_key(const mdg::GraphKey& a, const mdg::GraphKey& b) {
+    bool eq = a.x == b.x && a.out == b.out && a.gen == b.gen && a.list_cap == b.list_cap &&
+              a.search_general == b.search_general && a.split == b.split && a.fft_lines == b.fft_lines &&
+              a.counting == b.counting && a.timing == b.timing && a.n_clusters == b.n_clusters &&
+              a.n_cols == b.n_cols;
+    for (int d = 0; d < 3; ++d) eq = eq && a.box[d] == b.box[d] && a.pme_k[d] == b.pme_k[d];
+    for (int k = 0; k < 6; ++k) eq = eq && a.caps[k] == b.caps[k];
+    return eq;
+}
+
+// Timing mode 2 in a captured evaluation: event-record nodes for the pair
+// kernel's two events, between the kernel node and its neighbours.
+hipError_t add_pair_event_nodes(mdgpu_ctx* c, hipGraph_t g) {
+    size_t n = 0;
+    hipError_t e = hipGraphGetNodes(g, nullptr, &n);
+    if (e != hipSuccess || n == 0) return e != hipSuccess ? e : hipErrorInvalidValue;
+    std::vector<hipGraphNode_t> nodes(n);
+    if ((e = hipGraphGetNodes(g, nodes.data(), &n)) != hipSuccess) return e;
+    const void* fn = mdg::pair_kernel_fn(c);
+    hipGraphNode_t pk = nullptr;
+    for (hipGraphNode_t nd : nodes) {
+        hipGraphNodeType ty;
+        if (hipGraphNodeGetType(nd, &ty) != hipSuccess || ty != hipGraphNodeTypeKernel) continue;
+        hipKernelNodeParams kp{};
+        if (hipGraphKernelNodeGetParams(nd, &kp) == hipSuccess && kp.func == fn) {
+            pk = nd;
+            break;
+        }
+    }
+    if (!pk) return hipErrorInvalidValue;
+    size_t nd_in = 0, nd_out = 0;
+    if ((e = hipGraphNodeGetDependencies(pk, nullptr, &nd_in)) != hipSuccess) return e;
+    if ((e = hipGraphNodeGetDependentNodes(pk, nullptr, &nd_out)) != hipSuccess) return e;
+    std::vector<hipGraphNode_t> in(nd_in), out(nd_out);
+    if (nd_in && (e = hipGraphNodeGetDependencies(pk, in.data(), &nd_in)) != hipSuccess) return e;
+    if (nd_out && (e = hipGraphNodeGetDependentNodes(pk, out.data(), &nd_out)) != hipSuccess) return e;
+    const int k = MDGPU_K_NONBONDED;
+    hipGraphNode_t e0 = nullptr, e1 = nullptr;
+    if ((e = hipGraphAddEventRecordNode(&e0, g, in.data(), nd_in, c->ev[2 * k])) != hipSuccess) return e;
+    for (hipGraphNode_t p : in)
+        if ((e = hipGraphRemoveDependencies(g, &p, &pk, 1)) != hipSuccess) return e;
+    if ((e = hipGraphAddDependencies(g, &e0, &pk, 1)) != hipSuccess) return e;
+    if ((e = hipGraphAddEventRecordNode(&e1, g, &pk, 1, c->ev[2 * k + 1])) != hipSuccess) return e;
+    for (hipGraphNode_t d : out) {
+        if ((e = hipGraphRemoveDependencies(g, &pk, &d, 1)) != hipSuccess) return e;
+        if ((e = hipGraphAddDependencies(g, &e1, &d, 1)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// Captures one evaluation on c->stream into c->graph_exec.  A failure leaves
+// the graph path switched off for this context.
+int capture_eval(mdgpu_ctx* c, const double* x, double* out) {
+    c->graph_state = 2;  // unless the capture below succeeds
+    HIP_TRY(ensure_side_streams(c), "side streams");
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return MDGPU_OK;
+    }
+    c->capturing = true;
+    const int st = enqueue_eval(c, x, out, true);
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (st != MDGPU_OK || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return MDGPU_OK;
+    }
+    hipGraphExec_t ge = nullptr, get = nullptr;
+    if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return MDGPU_OK;
+    }
+    // the timed instance: the same graph with the pair kernel's event nodes
+    if (c->timing.mode >= 2 &&
+        (add_pair_event_nodes(c, g) != hipSuccess || hipGraphInstantiate(&get, g, nullptr, nullptr, 0) != hipSuccess)) {
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return MDGPU_OK;
+    }
+    c->graph = g;
+    c->graph_exec = ge;
+    c->graph_exec_timed = get;
+    c->graph_state = 1;
+    return MDGPU_OK;
+}
+
 int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_energies* out_e, double* d_out,
               double* out_virial, bool coords_are_work) {
     if (!c->have_system || !c->have_params) return set_error(c, MDGPU_ERR_STATE, "upload a system and set params first");
@@ -798,18 +921,50 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
         if (st != MDGPU_OK) return st;
         Work& w = c->work;
         const double* x = coords_are_work ? w.coords : d_coords;
-        st = enqueue_eval(c, x, d_out ? d_out : w.force_out, c->fork);
-        if (st != MDGPU_OK) return st;
+        double* out = d_out ? d_out : w.force_out;
+        // Graph path: once an evaluation with the same key has succeeded
+        // (capacities settled), the next one is captured and replayed from
+        // then on; a different key or a redo switches the path off for good
+        // (the context is never captured twice).
+        bool graph = false;
+        const mdg::GraphKey key = graph_key(c, x, out);
+        auto& T = c->timing;
+        T.record = T.mode == 1 || T.mode == 2 || (T.mode == 3 && T.tick % mdg::kTimingSample == 0);
+        if (c->use_graph && c->fork && c->timing.mode != 1 && c->graph_state != 2) {
+            if (c->graph_state == 1) {
+                if (same_key(key, c->graph_key)) graph = true;
+                else c->graph_state = 2;
+            } else if (c->have_last_key && same_key(key, c->last_key)) {
+                st = capture_eval(c, x, out);
+                if (st != MDGPU_OK) return st;
+                if (c->graph_state == 1) {
+                    c->graph_key = key;
+                    graph = true;
+                }
+            }
+        }
+        if (graph) {
+            HIP_TRY(hipGraphLaunch(T.record ? c->graph_exec_timed : c->graph_exec, c->stream), "graph launch");
+        } else {
+            st = enqueue_eval(c, x, out, c->fork);
+            if (st != MDGPU_OK) return st;
+        }
         // the reduction wrote energies and flags into host-mapped memory
         HIP_TRY(hipStreamSynchronize(c->stream), "evaluation");
         double e[mdg::E_NSLOTS + mdg::kFlags];
         std::memcpy(e, c->h_energies, sizeof e);
         st = check_flags(c, e);
-        if (st == kRedo) continue;
+        c->have_last_key = false;
+        if (st == kRedo) {
+            if (graph) c->graph_state = 2;
+            continue;
+        }
         if (st != MDGPU_OK) return st;
-        auto& T = c->timing;
-        if (T.mode != 0) {
-            auto timed = [&](int k) { return T.mode == 1 || (T.mode == 2 && k == MDGPU_K_NONBONDED); };
+        c->last_key = key;
+        c->have_last_key = true;
+        ++T.tick;
+        if (T.record) {
+            auto timed = [&](int k) { return T.mode == 1 || (T.mode >= 2 && k == MDGPU_K_NONBONDED); };
             for (int k = 0; k <= MDGPU_K_COUNT; ++k) {
                 if (!timed(k == MDGPU_K_COUNT ? MDGPU_K_FFT : k)) continue;
                 if (k == MDGPU_K_SPREAD || k == MDGPU_K_FFT || k == MDGPU_K_CONV || k == MDGPU_K_GATHER ||
@@ -1042,7 +1197,7 @@ int mdgpu_adjoint(mdgpu_ctx* c, const double* coords, const double box[3], const
 
 int mdgpu_set_timing(mdgpu_ctx* c, int mode) {
     if (!c) return MDGPU_ERR_ARG;
-    if (mode < 0 || mode > 2) return set_error(c, MDGPU_ERR_ARG, "timing mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) return set_error(c, MDGPU_ERR_ARG, "timing mode must be 0, 1, 2 or 3");
     c->timing = mdg::Timing{};
     c->timing.mode = mode;
     return MDGPU_OK;
@@ -1060,6 +1215,14 @@ int mdgpu_get_counts(mdgpu_ctx* c, int64_t counts[4]) {
     for (int k = 0; k < 4; ++k) counts[k] = c->counts[k];
     return MDGPU_OK;
 }
+
+int mdgpu_set_graph(mdgpu_ctx* c, int enable) {
+    if (!c) return MDGPU_ERR_ARG;
+    c->use_graph = enable != 0;
+    return MDGPU_OK;
+}
+
+int mdgpu_graph_state(const mdgpu_ctx* c) { return c ? c->graph_state : -1; }
 
 int mdgpu_set_counting(mdgpu_ctx* c, int enable) {
     if (!c) return MDGPU_ERR_ARG;

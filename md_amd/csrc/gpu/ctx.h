@@ -141,12 +141,29 @@ struct Grid {
     int dcx = 0, dcy = 0;            // column search reach
 };
 
+constexpr int kTimingSample = 16;  // timing mode 3: one evaluation in 16 carries the events
+
 struct Timing {
-    int mode = 0;  // 0 off, 1 every kernel group, 2 the pair kernel only
+    int mode = 0;         // 0 off, 1 every kernel group, 2 the pair kernel, 3 the pair kernel sampled
+    bool record = false;  // this evaluation records its events
+    long long tick = 0;   // evaluations since the mode was set (mode 3 sampling)
     double total_ms[MDGPU_K_COUNT] = {};
     long long launches[MDGPU_K_COUNT] = {};
 };
 
+}  // namespace mdg
+
+namespace mdg {
+// What a captured single-context evaluation depends on besides buffer contents.
+struct GraphKey {
+    const double* x;
+    const double* out;
+    double box[3];
+    int pme_k[3];
+    unsigned long long gen;
+    int list_cap, search_general, split, fft_lines, counting, timing, n_clusters, n_cols;
+    size_t caps[6];
+};
 }  // namespace mdg
 
 struct mdgpu_ctx {
@@ -181,6 +198,16 @@ struct mdgpu_ctx {
     // (written by the final reduction kernel; d_ is its device address)
     double* h_energies = nullptr;
     double* d_host_energies = nullptr;
+    // captured evaluation (single contexts): 0 none yet, 1 captured, 2 off for good
+    bool use_graph = true;
+    int graph_state = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;        // without timing events
+    hipGraphExec_t graph_exec_timed = nullptr;  // with the pair kernel's events (timing modes 2, 3)
+    mdg::GraphKey graph_key{}, last_key{};
+    bool have_last_key = false;
+    bool capturing = false;         // an evaluation is being captured
+    unsigned long long gen = 0;     // bumped by system upload and set_params
     // concurrent PME chain (single-context evaluations)
     bool fork = true;
     hipStream_t side[1] = {nullptr};
@@ -189,6 +216,16 @@ struct mdgpu_ctx {
 };
 
 namespace mdg {
+
+// Timing events on c->stream, for evaluations that record them.  While an
+// evaluation is captured nothing is recorded: the capture adds the pair
+// kernel's two events as graph nodes itself (timing modes 2 and 3; mode 1 is
+// never captured).
+inline hipError_t record_timing_event(mdgpu_ctx* c, hipEvent_t ev) {
+    return c->capturing || !c->timing.record ? hipSuccess : hipEventRecord(ev, c->stream);
+}
+// the pair-kernel instance a launch with the context's settings uses
+const void* pair_kernel_fn(const mdgpu_ctx* c);
 
 // ---- launchers (each returns hipSuccess or the first HIP error) ---------
 // excl: also bound the excluded and 1-4 pair distances (AMBER with exclusions)

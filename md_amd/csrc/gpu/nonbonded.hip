@@ -461,6 +461,24 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0, const double* coords) {
 
 }  // namespace
 
+template <typename real, bool kAmber, bool kSwitch>
+const void* pair_fn(bool counting) {
+    return counting ? reinterpret_cast<const void*>(&k_nonbonded<real, kAmber, kSwitch, true>)
+                    : reinterpret_cast<const void*>(&k_nonbonded<real, kAmber, kSwitch, false>);
+}
+
+template <typename real>
+const void* pair_fn(bool amber, bool sw, bool counting) {
+    if (amber) return sw ? pair_fn<real, true, true>(counting) : pair_fn<real, true, false>(counting);
+    return sw ? pair_fn<real, false, true>(counting) : pair_fn<real, false, false>(counting);
+}
+
+const void* pair_kernel_fn(const mdgpu_ctx* c) {
+    const bool amber = c->params.semantics == MDGPU_SEM_AMBER, sw = c->params.lj_switch > 0.0;
+    return c->params.precision == MDGPU_PREC_MIXED ? pair_fn<float>(amber, sw, c->counting)
+                                                   : pair_fn<double>(amber, sw, c->counting);
+}
+
 int pair_block_threads(const mdgpu_ctx* c) {
     return kWave * (c->params.precision == MDGPU_PREC_MIXED ? kWavesOf<float> : kWavesOf<double>);
 }

@@ -505,7 +505,7 @@ hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used, double* pot) {
     const c2* twz = tw + g.kx + g.ky;
     auto& T = c->timing;
     auto mark = [&](int k, bool start) {
-        if (T.mode == 1) (void)hipEventRecord(c->ev[2 * k + (start ? 0 : 1)], c->stream);
+        if (T.mode == 1) (void)record_timing_event(c, c->ev[2 * k + (start ? 0 : 1)]);
     };
     const int max_slots = c->n_clusters * kCluster;
     const int ncol = c->grid.ncx * c->grid.ncy;

@@ -140,6 +140,8 @@ def gpu_lib() -> C.CDLL:
         lib.mdgpu_get_timing.argtypes = [C.c_void_p, C.c_int, P_F64, C.POINTER(C.c_int64)]
         lib.mdgpu_get_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         lib.mdgpu_set_counting.argtypes = [C.c_void_p, C.c_int]
+        lib.mdgpu_set_graph.argtypes = [C.c_void_p, C.c_int]
+        lib.mdgpu_graph_state.argtypes = [C.c_void_p]
         lib.mdgpu_adjoint.argtypes = [C.c_void_p, P_F64, P_F64, C.POINTER(Energies), C.POINTER(ParamGrads)]
         lib.mdgpu_batch_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
         lib.mdgpu_batch_destroy.argtypes = [C.c_void_p]
@@ -436,7 +438,8 @@ class Evaluator:
         return arrs
 
     def set_timing(self, mode):
-        """0/False off, 1/True every kernel group, 2 the pair kernel only."""
+        """0/False off, 1/True every kernel group, 2 the pair kernel only, 3 the pair
+        kernel in one evaluation of every 16."""
         self._check(self.lib.mdgpu_set_timing(self.ctx, int(mode)))
 
     def timing(self) -> dict:
@@ -447,6 +450,14 @@ class Evaluator:
             self._check(self.lib.mdgpu_get_timing(self.ctx, k, C.byref(ms), C.byref(n)))
             out[name] = (ms.value, n.value)
         return out
+
+    def set_graph(self, on: bool):
+        """Allow (default) or forbid the captured-graph replay of evaluations."""
+        self._check(self.lib.mdgpu_set_graph(self.ctx, 1 if on else 0))
+
+    def graph_state(self) -> int:
+        """0 no graph yet, 1 replaying a captured evaluation, 2 direct launches for good."""
+        return int(self.lib.mdgpu_graph_state(self.ctx))
 
     def set_counting(self, on: bool):
         self._check(self.lib.mdgpu_set_counting(self.ctx, 1 if on else 0))
