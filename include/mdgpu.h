@@ -321,6 +321,31 @@ int mdgpu_graph_state(const mdgpu_ctx* ctx);
 /* Enable (1) or disable (0) counting of pairs within the cutoff. */
 int mdgpu_set_counting(mdgpu_ctx* ctx, int enable);
 
+/* Verlet list with a GPU-resident rebuild criterion.  Replaces the
+ * reference's skin handling: System::set_lj_skin_ / set_ee_skin
+ * (src/System/NonBondedUtils.cpp:11-47), setup_neighbor_rebuild_threshold_lj
+ * (:65-75), copy_ref_coords_lj (:77-85) and ljpairs_needs_rebuild (:87-100),
+ * which the reference declares but never calls.
+ * skin > 0: the cluster-pair list is built with radius max(cutoffs) + skin
+ * and kept across evaluations; each evaluation first measures, on the GPU,
+ * every atom's minimum-image displacement since the build and rebuilds the
+ * list only when some atom has moved more than skin / 2 (the reference's
+ * test dr^2 > (skin/2)^2).  Kept clusters follow their atoms; energies and
+ * forces are those of a fresh list (every pair within the cutoff is in the
+ * list).  A new box, system, parameter set or skin, a redo and an adjoint
+ * call force the next evaluation to build.  The decision is made on the
+ * device, so captured evaluations replay it.  skin = 0 (default): the list
+ * is built at every evaluation.  Negative or non-finite: MDGPU_ERR_ARG. */
+int mdgpu_set_verlet(mdgpu_ctx* ctx, double skin);
+/* List builds and successful evaluations since the system upload. */
+int mdgpu_get_list_builds(const mdgpu_ctx* ctx, int64_t* builds, int64_t* evals);
+/* ljpairs_needs_rebuild (NonBondedUtils.cpp:87-100) on the GPU for device
+ * coordinates d_coords (3N doubles): *needs = 1 when some atom's squared
+ * minimum-image displacement from the positions of the last list build, in
+ * that build's box, exceeds (skin/2)^2; also 1 when no list is kept (skin 0,
+ * none built yet, or a build pending).  Changes nothing. */
+int mdgpu_pairs_need_rebuild(mdgpu_ctx* ctx, const double* d_coords, int* needs);
+
 #ifdef __cplusplus
 }
 #endif

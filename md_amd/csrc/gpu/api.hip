@@ -67,7 +67,7 @@ void free_work(mdgpu_ctx* c) {
                     w.slot_pme_wf, w.cluster_col, w.cl_origin, w.cl_half,
                     w.list_count, w.list_j, w.list_mask, w.list_delta_d, w.list_delta_f, w.force_d, w.force_fx,
                     w.partial, w.energies, w.force_out, w.flags, w.excl_rmax, w.grid_f, w.grid_phi_d,
-                    w.cgrid, w.bsp_mod, w.bsp_mod_f, w.twiddle_d, w.twiddle_f};
+                    w.cgrid, w.bsp_mod, w.bsp_mod_f, w.twiddle_d, w.twiddle_f, w.x_ref, w.verlet};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c->work = Work{};
@@ -131,6 +131,7 @@ int mdgpu_create(int device, mdgpu_ctx** out) {
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.flags), mdg::kFlags * sizeof(unsigned int));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.energies), (mdg::E_NSLOTS + mdg::kFlags) * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.excl_rmax), sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->work.verlet), sizeof(mdg::VerletState));
     // energies and flags of every evaluation, written by the final reduction
     if (e == hipSuccess)
         e = hipHostMalloc(reinterpret_cast<void**>(&c->h_energies), (mdg::E_NSLOTS + mdg::kFlags) * sizeof(double),
@@ -139,6 +140,7 @@ int mdgpu_create(int device, mdgpu_ctx** out) {
     // zero once; afterwards the final reduction clears them for the next evaluation
     if (e == hipSuccess) e = hipMemset(c->work.flags, 0, mdg::kFlags * sizeof(unsigned int));
     if (e == hipSuccess) e = hipMemset(c->work.excl_rmax, 0, sizeof(float));
+    if (e == hipSuccess) e = hipMemset(c->work.verlet, 0, sizeof(mdg::VerletState));
     if (e != hipSuccess) {
         mdgpu_destroy(c);
         return MDGPU_ERR_HIP;
@@ -385,6 +387,8 @@ extern "C" int mdgpu_upload_system(mdgpu_ctx* c, const mdgpu_system_desc* s) {
     }
     c->have_system = true;
     ++c->gen;  // a captured evaluation graph no longer applies
+    c->verlet_force = true;
+    c->list_builds = c->list_evals = 0;
     if (c->have_params) return build_lj_table(c);
     return MDGPU_OK;
 }
@@ -414,6 +418,7 @@ extern "C" int mdgpu_set_params(mdgpu_ctx* c, const mdgpu_params* p) {
     c->params = *p;
     c->have_params = true;
     ++c->gen;
+    c->verlet_force = true;
     if (c->have_system) return build_lj_table(c);
     return MDGPU_OK;
 }
@@ -483,7 +488,10 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     const double rc = std::max(p.lj_cutoff, p.ee_cutoff);
     if (std::min({box[0], box[1], box[2]}) < 2.0 * rc)
         return set_error(c, MDGPU_ERR_UNSUPPORTED, "every box length must be at least twice the cutoff");
+    // list radius: the cutoff, plus the skin of a Verlet list
+    const double rl = rc + (c->skin > 0 ? c->skin : 0.0);
     mdg::Grid& g = c->grid;
+    g.r_list = rl;
     g.box[0] = box[0];
     g.box[1] = box[1];
     g.box[2] = box[2];
@@ -493,8 +501,8 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     g.ncy = std::max(1, static_cast<int>(box[1] / s));
     g.sx = box[0] / g.ncx;
     g.sy = box[1] / g.ncy;
-    g.dcx = static_cast<int>(std::floor(rc / g.sx)) + 1;
-    g.dcy = static_cast<int>(std::floor(rc / g.sy)) + 1;
+    g.dcx = static_cast<int>(std::floor(rl / g.sx)) + 1;
+    g.dcy = static_cast<int>(std::floor(rl / g.sy)) + 1;
     const int ncol = g.ncx * g.ncy;
     c->n_cols = ncol;
     c->n_clusters = n / mdg::kCluster + ncol + 1;
@@ -502,10 +510,10 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     size_t cap;
     cap = w.cap_atoms;
     if (w.cap_atoms < static_cast<size_t>(n)) {
-        void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_key, w.force_d, w.force_fx, w.force_out};
+        void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_key, w.force_d, w.force_fx, w.force_out, w.x_ref};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
-        w.coords = w.wrapped = w.force_d = w.force_out = nullptr;
+        w.coords = w.wrapped = w.force_d = w.force_out = w.x_ref = nullptr;
         w.col_of = nullptr;
         w.col_key = nullptr;
         w.force_fx = nullptr;
@@ -516,6 +524,8 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_d), 3 * n * sizeof(double)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_fx), 3 * n * sizeof(unsigned long long)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_out), 3 * n * sizeof(double)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.x_ref), 3 * n * sizeof(double)), "alloc");
+        c->verlet_force = true;  // no list positions to compare with
         // the accumulators start at zero; every evaluation's reduction re-zeroes them
         HIP_TRY(hipMemset(w.force_d, 0, 3 * n * sizeof(double)), "clear");
         HIP_TRY(hipMemset(w.force_fx, 0, 3 * n * sizeof(unsigned long long)), "clear");
@@ -560,7 +570,7 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
         // half-list j-clusters per i-cluster: the clusters inside the cutoff
         // sphere grown by one cluster width, halved, with headroom
         const double width = std::cbrt(mdg::kCluster / std::max(dens, 1e-6));
-        const double r = rc + 2.0 * width;
+        const double r = rl + 2.0 * width;
         const double est = 0.5 * (4.0 / 3.0) * 3.14159265 * r * r * r * dens / mdg::kCluster;
         w.list_cap = std::max(64, static_cast<int>(1.5 * est) + 32);
     }
@@ -693,8 +703,9 @@ int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out, bool fork) {
         if (timed(k)) (void)mdg::record_timing_event(c, c->ev[2 * k + (start ? 0 : 1)]);
     };
     if (fork) HIP_TRY(ensure_side_streams(c), "side streams");
+    mdg::VerletState* vs = c->skin > 0 ? w.verlet : nullptr;
     mark(MDGPU_K_SORT, true);
-    HIP_TRY(mdg::launch_sort(c, x, mixed, amber && c->topo.has_excl), "sort");
+    HIP_TRY(mdg::launch_sort(c, x, mixed, amber && c->topo.has_excl, vs), "sort");
     mark(MDGPU_K_SORT, false);
     // energy rows: pair kernel + bonded + corrections (one launch), PME
     const int row_nb = 0;
@@ -705,7 +716,7 @@ int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out, bool fork) {
     // the critical path first: the side stream's launches are issued while
     // the search and the pair kernel already run
     mark(MDGPU_K_PAIRLIST, true);
-    HIP_TRY(mdg::launch_pairsearch(c, mixed, amber), "pair search");
+    HIP_TRY(mdg::launch_pairsearch(c, mixed, amber, vs), "pair search");
     mark(MDGPU_K_PAIRLIST, false);
     mark(MDGPU_K_NONBONDED, true);
     HIP_TRY(mdg::launch_nonbonded(c, mixed, amber, row_nb, x), "nonbonded");
@@ -916,12 +927,20 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
     if (!c->have_system || !c->have_params) return set_error(c, MDGPU_ERR_STATE, "upload a system and set params first");
     HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
     const bool amber = c->params.semantics == MDGPU_SEM_AMBER;
+    // a Verlet list holds shift vectors of the box it was built in
+    for (int d = 0; d < 3; ++d)
+        if (box[d] != c->grid.box[d]) c->verlet_force = true;
     for (int attempt = 0; attempt < 4; ++attempt) {
         int st = prepare(c, box);
         if (st != MDGPU_OK) return st;
         Work& w = c->work;
         const double* x = coords_are_work ? w.coords : d_coords;
         double* out = d_out ? d_out : w.force_out;
+        // a forced list build is requested through device memory, outside any
+        // capture, so that replays see it as well
+        if (c->skin > 0 && c->verlet_force)
+            HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(&w.verlet->force), 1, 1, c->stream),
+                    "verlet force");
         // Graph path: once an evaluation with the same key has succeeded
         // (capacities settled), the next one is captured and replayed from
         // then on; a different key or a redo switches the path off for good
@@ -957,9 +976,13 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
         c->have_last_key = false;
         if (st == kRedo) {
             if (graph) c->graph_state = 2;
+            c->verlet_force = true;
             continue;
         }
         if (st != MDGPU_OK) return st;
+        c->verlet_force = false;
+        ++c->list_evals;
+        if (c->skin <= 0 || e[mdg::E_NSLOTS + 5] != 0.0) ++c->list_builds;
         c->last_key = key;
         c->have_last_key = true;
         ++T.tick;
@@ -1109,8 +1132,11 @@ int mdgpu_adjoint(mdgpu_ctx* c, const double* coords, const double box[3], const
         c->cur_x = w.coords;
         HIP_TRY(hipMemsetAsync(b.acc, 0, b.total * sizeof(unsigned long long), c->stream), "clear adjoint");
         HIP_TRY(hipMemsetAsync(b.lj_d, 0, b.cap_lj_d * sizeof(double), c->stream), "clear adjoint");
-        HIP_TRY(mdg::launch_sort(c, w.coords, mixed, t.has_excl), "sort");
-        HIP_TRY(mdg::launch_pairsearch(c, mixed, true), "pair search");
+        // the adjoint builds its own list: a Verlet list of the evaluations
+        // is rebuilt at their next call
+        c->verlet_force = true;
+        HIP_TRY(mdg::launch_sort(c, w.coords, mixed, t.has_excl, nullptr), "sort");
+        HIP_TRY(mdg::launch_pairsearch(c, mixed, true, nullptr), "pair search");
         HIP_TRY(mdg::launch_pair_adjoint(c, mixed), "pair adjoint");
         HIP_TRY(mdg::launch_corrections_adjoint(c), "corrections adjoint");
         HIP_TRY(mdg::launch_bonded_adjoint(c, w.coords, mixed), "bonded adjoint");
@@ -1223,6 +1249,40 @@ int mdgpu_set_graph(mdgpu_ctx* c, int enable) {
 }
 
 int mdgpu_graph_state(const mdgpu_ctx* c) { return c ? c->graph_state : -1; }
+
+int mdgpu_set_verlet(mdgpu_ctx* c, double skin) {
+    if (!c) return MDGPU_ERR_ARG;
+    if (!(skin >= 0.0) || !std::isfinite(skin)) return set_error(c, MDGPU_ERR_ARG, "skin must be finite and >= 0");
+    c->skin = skin;
+    c->verlet_force = true;
+    ++c->gen;
+    return MDGPU_OK;
+}
+
+int mdgpu_get_list_builds(const mdgpu_ctx* c, int64_t* builds, int64_t* evals) {
+    if (!c || !builds || !evals) return MDGPU_ERR_ARG;
+    *builds = c->list_builds;
+    *evals = c->list_evals;
+    return MDGPU_OK;
+}
+
+int mdgpu_pairs_need_rebuild(mdgpu_ctx* c, const double* d_coords, int* needs) {
+    if (!c || !d_coords || !needs) return MDGPU_ERR_ARG;
+    if (!c->have_system || !c->have_params) return set_error(c, MDGPU_ERR_STATE, "upload a system and set params first");
+    // without a kept list (no skin, none built yet, or a build pending) the
+    // next evaluation builds one anyway
+    if (c->skin <= 0 || c->list_builds == 0 || c->verlet_force) {
+        *needs = 1;
+        return MDGPU_OK;
+    }
+    HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
+    HIP_TRY(mdg::launch_disp_query(c, d_coords), "displacement query");
+    unsigned int q = 0;
+    HIP_TRY(hipMemcpyAsync(&q, &c->work.verlet->query, sizeof q, hipMemcpyDeviceToHost, c->stream), "query copy");
+    HIP_TRY(hipStreamSynchronize(c->stream), "query");
+    *needs = q != 0u ? 1 : 0;
+    return MDGPU_OK;
+}
 
 int mdgpu_set_counting(mdgpu_ctx* c, int enable) {
     if (!c) return MDGPU_ERR_ARG;

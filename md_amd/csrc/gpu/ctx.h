@@ -76,6 +76,19 @@ constexpr double kAdjScaleLJ = 17592186044416.0;    // 2^44: 1-4 LJ and CMAP coe
 constexpr double kAdjScaleLJm = 4294967296.0;       // 2^32: LJ sums of the mixed form
 constexpr double kAdjScaleB = 1099511627776.0;      // 2^40: bonded constants and equilibria
 
+// Verlet-list state of a context on the device.  k_disp folds every atom's
+// squared minimum-image displacement since the last list build into max_d2
+// (the bits of a non-negative double order like the double); its last block
+// decides, publishes the decision in `build` for the sort and search kernels
+// of the same evaluation, and clears max_d2, ticket and force.
+struct VerletState {
+    unsigned long long max_d2;
+    unsigned int ticket;   // k_disp blocks finished
+    unsigned int build;    // 1: this evaluation builds the list
+    unsigned int force;    // set by the host: the next evaluation builds
+    unsigned int query;    // decision of the last query (mdgpu_pairs_need_rebuild)
+};
+
 // Work arrays that depend on the atom count / box.
 struct Work {
     size_t cap_atoms = 0;
@@ -119,7 +132,11 @@ struct Work {
     double* energies = nullptr;      // [E_NSLOTS + kFlags] final energies, then the flags as doubles
     double* force_out = nullptr;     // [3N] FP64 forces for host-buffer evaluations
     unsigned int* flags = nullptr;   // [kFlags]: 0 fixed overflow, 1 list overflow, 2 column overflow,
-                                     // 3 near-tile overflow, 4 cluster too wide for the fast search
+                                     // 3 near-tile overflow, 4 cluster too wide for the fast search,
+                                     // 5 the list was built (Verlet mode; not an error)
+    // Verlet mode: raw coordinates at the last list build, and the state
+    double* x_ref = nullptr;         // [3N]
+    VerletState* verlet = nullptr;
     float* excl_rmax = nullptr;      // [1] largest excluded-pair distance (as float)
     // PME
     int pme_k[3] = {0, 0, 0};
@@ -139,6 +156,7 @@ struct Grid {
     double sx = 0, sy = 0;
     double box[3] = {0, 0, 0};
     int dcx = 0, dcy = 0;            // column search reach
+    double r_list = 0;               // list radius: max cutoff + Verlet skin
 };
 
 constexpr int kTimingSample = 16;  // timing mode 3: one evaluation in 16 carries the events
@@ -208,6 +226,12 @@ struct mdgpu_ctx {
     bool have_last_key = false;
     bool capturing = false;         // an evaluation is being captured
     unsigned long long gen = 0;     // bumped by system upload and set_params
+    // Verlet list (skin > 0): kept across evaluations until an atom has moved
+    // more than skin / 2 since the build; verlet_force makes the next
+    // evaluation build it whatever the displacements
+    double skin = 0.0;
+    bool verlet_force = true;
+    long long list_builds = 0, list_evals = 0;
     // concurrent PME chain (single-context evaluations)
     bool fork = true;
     hipStream_t side[1] = {nullptr};
@@ -229,8 +253,14 @@ const void* pair_kernel_fn(const mdgpu_ctx* c);
 
 // ---- launchers (each returns hipSuccess or the first HIP error) ---------
 // excl: also bound the excluded and 1-4 pair distances (AMBER with exclusions)
-hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl);
-hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber);
+// verlet: the context's Verlet state (nullptr: build the list unconditionally).
+// With it, launch_sort first measures the displacements (k_disp); the sort and
+// search kernels then return at once unless the list is rebuilt, and a
+// refresh kernel moves the kept clusters' slot records to the new positions.
+hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl, VerletState* verlet);
+hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber, const VerletState* verlet);
+// displacement criterion alone: the decision goes to verlet->query
+hipError_t launch_disp_query(mdgpu_ctx* c, const double* coords);
 // pair kernel, with the bonded terms and (amber) the FP64 excluded / 1-4
 // corrections as further blocks of the same launch; energy rows partial_row0
 // .. + nonbonded_rows + bonded_rows + corrections_rows

@@ -165,6 +165,7 @@ struct SpreadCols {
     const int* col_cstart;   // [ncol+1] cluster offsets
     const int* fl;           // [slots] packed base indices
     const void* w;           // [slots] fractions + charge (double4 / float4)
+    double margin;           // how far an atom may sit outside its column (Verlet list: skin / 2)
 };
 
 constexpr int kSpreadCols = 256;  // columns per pass of the candidate loop
@@ -186,9 +187,10 @@ __global__ __launch_bounds__(kThreads) void k_spread_fft_z(PmeGeom g, SpreadCols
     const c2* tw = stage_twiddles<real>(B + g.kz * nl, tw_g, g.kz);
     for (int t = threadIdx.x; t < nl * g.kz; t += blockDim.x) acc[t] = 0ull;
     const int X0 = blockIdx.x * bx, Y0 = blockIdx.y * by;
-    const double hx = g.lx / g.kx, hy = g.ly / g.ky, tol = 1e-6;
+    const double hx = g.lx / g.kx, hy = g.ly / g.ky, tol = 1e-6 + sc.margin;
     // columns overlapping [X0 hx, (X0+bx+3) hx) x [Y0 hy, (Y0+by+3) hy), with
-    // 1e-6 A for the rounding of floor(u) against the column binning
+    // 1e-6 A for the rounding of floor(u) against the column binning, grown
+    // by how far an atom of a kept Verlet list may have left its column
     int cx0 = static_cast<int>(floor((X0 * hx - tol) / sc.sx));
     int ncxr = static_cast<int>(floor(((X0 + bx + 3) * hx + tol) / sc.sx)) - cx0 + 1;
     int cy0 = static_cast<int>(floor((Y0 * hy - tol) / sc.sy));
@@ -518,7 +520,8 @@ hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used, double* pot) {
         const int nl = lines_per_block(g.kz, sizeof(c2), kSpreadLines);
         const int by = std::min(nl, 8), bx = nl / by;
         const size_t shm = (2 * static_cast<size_t>(g.kz) * nl + g.kz) * sizeof(c2);
-        const SpreadCols sc{c->grid.ncx, c->grid.ncy, c->grid.sx, c->grid.sy, w.col_cstart, w.slot_pme_fl, wrec};
+        const SpreadCols sc{c->grid.ncx, c->grid.ncy, c->grid.sx, c->grid.sy, w.col_cstart, w.slot_pme_fl, wrec,
+                            c->skin > 0 ? 0.5 * c->skin : 0.0};
         const dim3 grid((g.kx + bx - 1) / bx, (g.ky + by - 1) / by);
         hipLaunchKernelGGL(k_spread_fft_z<real>, grid, dim3(kThreads), shm, c->stream, g, sc, bx, by, ilog2(nl), cg,
                            twz, w.flags);

@@ -109,9 +109,11 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(const double* __restrict__ 
                                                      double ly, double lz, int ncx, int ncy, double sx, double sy,
                                                      double* __restrict__ wrapped, int* __restrict__ col_of,
                                                      int* __restrict__ col_count, int* __restrict__ block_off,
-                                                     const ExclPairs ex) {
+                                                     const ExclPairs ex, double* __restrict__ xref,
+                                                     const VerletState* __restrict__ vs) {
     __shared__ int hist[kMaxCols];
     __shared__ float wmax[kBinThreads / kWave];
+    if (vs && !vs->build) return;  // Verlet list kept: nothing to sort
     const int ncol = ncx * ncy;
     for (int c = threadIdx.x; c < ncol; c += blockDim.x) hist[c] = 0;
     __syncthreads();
@@ -124,6 +126,12 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(const double* __restrict__ 
         wrapped[3 * i] = wx;
         wrapped[3 * i + 1] = wy;
         wrapped[3 * i + 2] = wz;
+        if (xref) {
+            // the positions the Verlet list is built from
+            xref[3 * i] = x[3 * i];
+            xref[3 * i + 1] = x[3 * i + 1];
+            xref[3 * i + 2] = x[3 * i + 2];
+        }
         int cx = static_cast<int>(wx / sx), cy = static_cast<int>(wy / sy);
         cx = cx < 0 ? 0 : (cx >= ncx ? ncx - 1 : cx);
         cy = cy < 0 ? 0 : (cy >= ncy ? ncy - 1 : cy);
@@ -173,8 +181,10 @@ __global__ __launch_bounds__(kBinThreads) void k_scatter(int n, int chunk, int n
                                                          const int* __restrict__ block_off,
                                                          unsigned long long* __restrict__ col_key,
                                                          int* __restrict__ col_start, int* __restrict__ col_cstart,
-                                                         unsigned int* flags, const ExclPairs ex) {
+                                                         unsigned int* flags, const ExclPairs ex,
+                                                         const VerletState* __restrict__ vs) {
     __shared__ int off[kMaxCols];
+    if (vs && !vs->build) return;
     if (blockIdx.x == 0) {
         // cluster offsets (8 atoms per cluster, the last one padded), the
         // column-size check and the exclusion extent
@@ -223,6 +233,24 @@ struct PmeSlots {
     void* w;   // double4 / float4 per slot: fractions (x, y, z) and charge
 };
 
+// PME spline record of a position: u = K x / L in FP64; the base index
+// floor(u) taken mod K (the stencil wraps anyway), packed 10 bits per
+// dimension, and the fractions u - floor(u).  The position may lie outside
+// the box (a Verlet-list atom that moved across a face since the build).
+template <typename real>
+__device__ __forceinline__ int pme_record(const double xs[3], const PmeSlots& pme, real wv[3]) {
+    int packed = 0;
+    for (int d = 0; d < 3; ++d) {
+        const double u = xs[d] * pme.k[d] / pme.l[d];
+        const double f = floor(u);
+        int fl = static_cast<int>(f) % pme.k[d];
+        fl += fl < 0 ? pme.k[d] : 0;
+        packed |= fl << (10 * d);
+        wv[d] = static_cast<real>(u - f);
+    }
+    return packed;
+}
+
 // Per-cluster outputs of the column sort.
 template <typename real>
 struct ClusterOut {
@@ -251,11 +279,13 @@ __global__ __launch_bounds__(kSortThreads) void k_col_sort(const int* __restrict
                                                            const int* __restrict__ col_cstart,
                                                            const unsigned long long* __restrict__ col_key,
                                                            const double* __restrict__ wrapped,
-                                                           int* __restrict__ col_count, const ClusterOut<real> o) {
+                                                           int* __restrict__ col_count, const ClusterOut<real> o,
+                                                           const VerletState* __restrict__ vs) {
     constexpr int kPer = kColMax / kSortThreads;  // atoms per thread at most
     __shared__ unsigned long long s_key[kColMax];  // keys; after ranking, x by rank
     __shared__ double s_y[kColMax], s_z[kColMax];
     __shared__ int s_sorted[kColMax];
+    if (vs && !vs->build) return;
     const int c = blockIdx.x;
     const int a0 = col_start[c], n = min(col_start[c + 1] - a0, kColMax);
     const int cl0 = col_cstart[c], ncl = col_cstart[c + 1] - cl0;
@@ -335,21 +365,12 @@ __global__ __launch_bounds__(kSortThreads) void k_col_sort(const int* __restrict
         }
         o.pos[slot] = p;
         if (o.pme.k[0] > 0) {
-            // u = K x / L in FP64; fl = floor(u) taken mod K (the stencil wraps
-            // anyway), w = u - floor(u)
             int packed = 0;
             typename Vec4<real>::T wq = {0, 0, 0, 0};
             if (atom >= 0) {
                 const double xs[3] = {x, y, z};
                 real wv[3];
-                for (int d = 0; d < 3; ++d) {
-                    const double u = xs[d] * o.pme.k[d] / o.pme.l[d];
-                    const double f = floor(u);
-                    int fl = static_cast<int>(f) % o.pme.k[d];
-                    fl += fl < 0 ? o.pme.k[d] : 0;
-                    packed |= fl << (10 * d);
-                    wv[d] = static_cast<real>(u - f);
-                }
+                packed = pme_record<real>(xs, o.pme, wv);
                 wq.x = wv[0];
                 wq.y = wv[1];
                 wq.z = wv[2];
@@ -410,6 +431,7 @@ struct SearchArgs {
     // atom-box test of the fast search
     const void* pos;
     double rp2;             // (r_list + margin)^2 for that test
+    const VerletState* vs;  // Verlet mode: search only when the list is rebuilt
 };
 
 constexpr int kNearCap = 256;
@@ -419,6 +441,7 @@ template <typename real>
 __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArgs a) {
     using R4 = typename Vec4<real>::T;
     __shared__ int2 s_near[kSearchWaves][kNearCap];  // (entry, j-cluster)
+    if (a.vs && !a.vs->build) return;
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
     const int ci = blockIdx.x * kSearchWaves + wv;
     const int ncl = min(*a.ncl_dev, a.max_clusters);
@@ -622,6 +645,7 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     using R4 = typename Vec4<real>::T;
     __shared__ int2 s_near[kSearchWaves][kNearCap];
     __shared__ int s_ex[kSearchWaves][kCluster][kExclCap];
+    if (a.vs && !a.vs->build) return;
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
     const int ci = __builtin_amdgcn_readfirstlane(blockIdx.x * kSearchWaves + wv);
     const int ncl = min(*a.ncl_dev, a.max_clusters);
@@ -806,13 +830,112 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     near_tile_masks(a, ci, lane, xr, s_ex[wv], s_near[wv], nnear, base_i);
 }
 
+// Verlet list criterion (System::ljpairs_needs_rebuild, reference
+// src/System/NonBondedUtils.cpp:87-100): the list is rebuilt when some atom's
+// squared minimum-image displacement since the build exceeds (skin/2)^2.
+// The displacement is formed as the reference forms it (d - L round(d / L),
+// then dx*dx + dy*dy + dz*dz without contraction), so the decision is the
+// same bit for bit; as in the reference a NaN never becomes the maximum.
+// Each block folds its maximum into vs->max_d2; the last block to finish
+// decides (also when the host has set vs->force), clears the state for the
+// next evaluation and publishes the decision: in vs->build and flags[5] for
+// an evaluation, in vs->query for a query.
+constexpr int kDispThreads = 256;
+constexpr int kDispMaxBlocks = 1024;
+__global__ __launch_bounds__(kDispThreads) void k_disp(const double* __restrict__ x, const double* __restrict__ xref,
+                                                       int n, double lx, double ly, double lz, double lim2,
+                                                       int query, VerletState* vs, unsigned int* flags) {
+#pragma clang fp contract(off)
+    __shared__ double wmax[kDispThreads / kWave];
+    double m = 0.0;
+    for (int i = blockIdx.x * kDispThreads + threadIdx.x; i < n; i += gridDim.x * kDispThreads) {
+        double dx = x[3 * i] - xref[3 * i];
+        double dy = x[3 * i + 1] - xref[3 * i + 1];
+        double dz = x[3 * i + 2] - xref[3 * i + 2];
+        dx = dx - lx * round(dx / lx);
+        dy = dy - ly * round(dy / ly);
+        dz = dz - lz * round(dz / lz);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 > m) m = d2;
+    }
+    for (int k = 32; k >= 1; k >>= 1) {
+        const double o = __shfl_xor(m, k, kWave);
+        if (o > m) m = o;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) wmax[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int q = 1; q < kDispThreads / kWave; ++q)
+        if (wmax[q] > m) m = wmax[q];
+    if (m > 0.0) atomicMax(&vs->max_d2, static_cast<unsigned long long>(__double_as_longlong(m)));
+    __threadfence();
+    if (atomicAdd(&vs->ticket, 1u) != gridDim.x - 1) return;
+    __threadfence();
+    const double mx = __longlong_as_double(static_cast<long long>(atomicAdd(&vs->max_d2, 0ull)));
+    const unsigned int build = (atomicAdd(&vs->force, 0u) != 0u || mx > lim2) ? 1u : 0u;
+    atomicExch(&vs->max_d2, 0ull);
+    atomicExch(&vs->ticket, 0u);
+    if (query) {
+        atomicExch(&vs->query, build);
+    } else {
+        atomicExch(&vs->build, build);
+        atomicExch(&vs->force, 0u);
+        if (build) atomicOr(flags + 5, 1u);
+    }
+}
+
+// Verlet list kept: every slot of the kept clusters follows its atom.  The
+// new position is the build's wrapped position plus the minimum-image
+// displacement since the build, so it stays within skin / 2 of the cluster
+// it was sorted into; the offset from the (unchanged) cluster origin and the
+// PME spline record are formed from it as the column sort forms them.
+constexpr int kRefreshThreads = 256;
+template <typename real>
+__global__ __launch_bounds__(kRefreshThreads) void k_refresh(const VerletState* __restrict__ vs,
+                                                             const int* __restrict__ ncl_dev,
+                                                             const double* __restrict__ x,
+                                                             const double* __restrict__ xref,
+                                                             const double* __restrict__ wrapped,
+                                                             const ClusterOut<real> o, double lx, double ly,
+                                                             double lz) {
+    if (vs->build) return;
+    const int s = blockIdx.x * kRefreshThreads + threadIdx.x;
+    if (s >= min(*ncl_dev, o.max_clusters) * kCluster) return;
+    const int atom = o.slot_atom[s];
+    if (atom < 0) return;
+    const double L[3] = {lx, ly, lz};
+    double xs[3];
+    for (int d = 0; d < 3; ++d) {
+        double dd = x[3 * atom + d] - xref[3 * atom + d];
+        dd = dd - L[d] * round(dd / L[d]);
+        xs[d] = wrapped[3 * atom + d] + dd;
+    }
+    const double4 org = o.origin[s / kCluster];
+    typename Vec4<real>::T p;
+    p.x = static_cast<real>(xs[0] - org.x);
+    p.y = static_cast<real>(xs[1] - org.y);
+    p.z = static_cast<real>(xs[2] - org.z);
+    p.w = static_cast<real>(o.charge[atom]);
+    o.pos[s] = p;
+    if (o.pme.k[0] > 0) {
+        real wv[3];
+        o.pme.fl[s] = pme_record<real>(xs, o.pme, wv);
+        typename Vec4<real>::T wq;
+        wq.x = wv[0];
+        wq.y = wv[1];
+        wq.z = wv[2];
+        wq.w = static_cast<real>(o.charge[atom]);
+        reinterpret_cast<typename Vec4<real>::T*>(o.pme.w)[s] = wq;
+    }
+}
+
 }  // namespace
 
 int sort_blocks(int n) {
     return std::max(1, std::min(kMaxSortBlocks, (n + kAtomsPerSortBlock - 1) / kAtomsPerSortBlock));
 }
 
-hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl) {
+hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl, VerletState* vs) {
     Work& w = c->work;
     const Grid& g = c->grid;
     const Topo& t = c->topo;
@@ -823,11 +946,17 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl
     ExclPairs ex{t.excl_pairs, excl ? t.n_excl_pairs : 0, t.pairs14, excl ? t.n_pairs14 : 0, 0, w.block_emax,
                  w.excl_rmax};
     ex.chunk = (ex.n1 + ex.n2 + nblk - 1) / nblk;
+    if (vs) {
+        const int db = std::max(1, std::min(kDispMaxBlocks, (n + kDispThreads - 1) / kDispThreads));
+        hipLaunchKernelGGL(k_disp, dim3(db), dim3(kDispThreads), 0, c->stream, coords, w.x_ref, n, g.box[0],
+                           g.box[1], g.box[2], (0.5 * c->skin) * (0.5 * c->skin), 0, vs, w.flags);
+    }
     hipLaunchKernelGGL(k_bin, dim3(nblk), dim3(kBinThreads), 0, c->stream, coords, n, chunk, g.box[0], g.box[1],
-                       g.box[2], g.ncx, g.ncy, g.sx, g.sy, w.wrapped, w.col_of, w.col_count, w.block_off, ex);
+                       g.box[2], g.ncx, g.ncy, g.sx, g.sy, w.wrapped, w.col_of, w.col_count, w.block_off, ex,
+                       vs ? w.x_ref : nullptr, vs);
     const double zscale = 4294967296.0 / g.box[2];
     hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kBinThreads), 0, c->stream, n, chunk, ncol, w.col_of, w.wrapped,
-                       zscale, w.col_count, w.block_off, w.col_key, w.col_start, w.col_cstart, w.flags, ex);
+                       zscale, w.col_count, w.block_off, w.col_key, w.col_start, w.col_cstart, w.flags, ex, vs);
     PmeSlots pme{};
     const bool with_pme = c->params.semantics == MDGPU_SEM_AMBER && w.pme_k[0] > 0;
     for (int d = 0; d < 3; ++d) {
@@ -835,23 +964,40 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl
         pme.l[d] = g.box[d];
     }
     pme.fl = w.slot_pme_fl;
+    const int rb = (c->n_clusters * kCluster + kRefreshThreads - 1) / kRefreshThreads;
     if (mixed) {
         pme.w = w.slot_pme_wf;
         const ClusterOut<float> o{c->n_clusters, t.charge, t.type, t.n_types, w.slot_atom, w.cluster_col,
                                   w.slot_pos_f, w.slot_meta, w.cl_origin, w.cl_half, pme};
         hipLaunchKernelGGL(k_col_sort<float>, dim3(ncol), dim3(kSortThreads), 0, c->stream, w.col_start, w.col_cstart,
-                           w.col_key, w.wrapped, w.col_count, o);
+                           w.col_key, w.wrapped, w.col_count, o, vs);
+        if (vs)
+            hipLaunchKernelGGL(k_refresh<float>, dim3(rb), dim3(kRefreshThreads), 0, c->stream, vs,
+                               w.col_cstart + ncol, coords, w.x_ref, w.wrapped, o, g.box[0], g.box[1], g.box[2]);
     } else {
         pme.w = w.slot_pme_wd;
         const ClusterOut<double> o{c->n_clusters, t.charge, t.type, t.n_types, w.slot_atom, w.cluster_col,
                                    w.slot_pos_d, w.slot_meta, w.cl_origin, w.cl_half, pme};
         hipLaunchKernelGGL(k_col_sort<double>, dim3(ncol), dim3(kSortThreads), 0, c->stream, w.col_start,
-                           w.col_cstart, w.col_key, w.wrapped, w.col_count, o);
+                           w.col_cstart, w.col_key, w.wrapped, w.col_count, o, vs);
+        if (vs)
+            hipLaunchKernelGGL(k_refresh<double>, dim3(rb), dim3(kRefreshThreads), 0, c->stream, vs,
+                               w.col_cstart + ncol, coords, w.x_ref, w.wrapped, o, g.box[0], g.box[1], g.box[2]);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber) {
+hipError_t launch_disp_query(mdgpu_ctx* c, const double* coords) {
+    Work& w = c->work;
+    const Grid& g = c->grid;
+    const int n = c->topo.n_atoms;
+    const int db = std::max(1, std::min(kDispMaxBlocks, (n + kDispThreads - 1) / kDispThreads));
+    hipLaunchKernelGGL(k_disp, dim3(db), dim3(kDispThreads), 0, c->stream, coords, w.x_ref, n, g.box[0], g.box[1],
+                       g.box[2], (0.5 * c->skin) * (0.5 * c->skin), 1, w.verlet, w.flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber, const VerletState* vs) {
     Work& w = c->work;
     const Grid& g = c->grid;
     const int ncol = g.ncx * g.ncy;
@@ -873,7 +1019,8 @@ hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber) {
     a.lx = g.box[0];
     a.ly = g.box[1];
     a.lz = g.box[2];
-    const double rl = std::max(c->params.lj_cutoff, c->params.ee_cutoff);
+    const double rl = g.r_list;
+    a.vs = vs;
     a.rl2 = rl * rl;
     a.rp2 = (rl + 1e-3) * (rl + 1e-3);
     a.cap = w.list_cap;

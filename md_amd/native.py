@@ -142,6 +142,9 @@ def gpu_lib() -> C.CDLL:
         lib.mdgpu_set_counting.argtypes = [C.c_void_p, C.c_int]
         lib.mdgpu_set_graph.argtypes = [C.c_void_p, C.c_int]
         lib.mdgpu_graph_state.argtypes = [C.c_void_p]
+        lib.mdgpu_set_verlet.argtypes = [C.c_void_p, C.c_double]
+        lib.mdgpu_get_list_builds.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        lib.mdgpu_pairs_need_rebuild.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
         lib.mdgpu_adjoint.argtypes = [C.c_void_p, P_F64, P_F64, C.POINTER(Energies), C.POINTER(ParamGrads)]
         lib.mdgpu_batch_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
         lib.mdgpu_batch_destroy.argtypes = [C.c_void_p]
@@ -458,6 +461,23 @@ class Evaluator:
     def graph_state(self) -> int:
         """0 no graph yet, 1 replaying a captured evaluation, 2 direct launches for good."""
         return int(self.lib.mdgpu_graph_state(self.ctx))
+
+    def set_verlet(self, skin: float):
+        """Keep the pair list across evaluations with this skin (A) until an atom
+        has moved more than skin / 2 since the build; 0 builds it every time."""
+        self._check(self.lib.mdgpu_set_verlet(self.ctx, float(skin)))
+
+    def list_builds(self) -> tuple:
+        """(list builds, successful evaluations) since the system upload."""
+        b, e = C.c_int64(), C.c_int64()
+        self._check(self.lib.mdgpu_get_list_builds(self.ctx, C.byref(b), C.byref(e)))
+        return b.value, e.value
+
+    def needs_rebuild(self, x_ptr: int) -> bool:
+        """The rebuild criterion for device coordinates at x_ptr (3N doubles)."""
+        out = C.c_int()
+        self._check(self.lib.mdgpu_pairs_need_rebuild(self.ctx, C.c_void_p(x_ptr), C.byref(out)))
+        return bool(out.value)
 
     def set_counting(self, on: bool):
         self._check(self.lib.mdgpu_set_counting(self.ctx, 1 if on else 0))

@@ -43,6 +43,8 @@ def lib() -> C.CDLL:
         L.oracle_param_grads.argtypes = [C.POINTER(SystemDesc), C.POINTER(Params), P_F64, P_F64,
                                          C.POINTER(Energies), C.POINTER(ParamGrads)]
         L.oracle_bspline.argtypes = [C.c_int, C.c_double, P_F64, P_F64]
+        L.oracle_pairs_need_rebuild.argtypes = [C.c_int, P_F64, P_F64, P_F64, C.c_double]
+        L.oracle_pairs_need_rebuild.restype = C.c_int
         L.oracle_coulomb_constant.restype = C.c_double
         _lib = L
     return _lib
@@ -131,3 +133,11 @@ def param_grads(desc: SystemDesc, params: Params, coords, box, weights: dict) ->
     if st != 0:
         raise RuntimeError("oracle_param_grads failed")
     return arrs
+
+
+def pairs_need_rebuild(coords, coords_ref, box, skin) -> bool:
+    """The Verlet rebuild criterion (NonBondedUtils.cpp:87-100)."""
+    x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+    r = np.ascontiguousarray(coords_ref, dtype=np.float64).reshape(-1)
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    return bool(lib().oracle_pairs_need_rebuild(x.size // 3, _p(x), _p(r), _p(b), float(skin)))

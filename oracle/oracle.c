@@ -16,6 +16,10 @@
  *                        FD of its forces; finite-strain derivative of virial
  *   B-splines            partition of unity, closed form of M_4, and
  *                        derivative = finite difference
+ *   rebuild criterion    single-atom displacements just below / at / above
+ *                        skin/2, across a periodic face (minimum image), a NaN
+ *                        coordinate, and agreement with an exact-rational
+ *                        brute force on random displacement sets
  */
 #include "oracle.h"
 
@@ -46,6 +50,25 @@ static void min_image(double* d, const double box[3], int pbc) {
     d[0] = min_image_1d(d[0], box[0]);
     d[1] = min_image_1d(d[1], box[1]);
     d[2] = min_image_1d(d[2], box[2]);
+}
+
+/* ------------------------------------------------------- Verlet list ---- */
+
+/* setup_neighbor_rebuild_threshold_lj (NonBondedUtils.cpp:65-69) sets the
+ * limit to (skin/2)^2; ljpairs_needs_rebuild (:87-100) keeps the running
+ * maximum of the squared minimum-image displacements and answers 1 as soon
+ * as it exceeds the limit. */
+int oracle_pairs_need_rebuild(int n, const double* x, const double* xref, const double box[3], double skin) {
+    const double half = 0.5 * skin, lim2 = half * half;
+    double max_dr2 = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double d[3] = {x[3 * i] - xref[3 * i], x[3 * i + 1] - xref[3 * i + 1], x[3 * i + 2] - xref[3 * i + 2]};
+        min_image(d, box, 1);
+        const double dr2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (dr2 > max_dr2) max_dr2 = dr2;
+        if (max_dr2 > lim2) return 1;
+    }
+    return 0;
 }
 
 /* ------------------------------------------------------------- metrics ---- */

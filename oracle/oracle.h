@@ -33,6 +33,11 @@ int oracle_eval(const mdgpu_system_desc* sys, const mdgpu_params* p, const doubl
 long oracle_ref_pairlist(const mdgpu_system_desc* sys, const double* x, const double box[3],
                          double list_cutoff, int* out, long cap);
 
+/* The Verlet rebuild criterion (NonBondedUtils.cpp:65-75, 87-100): 1 when
+ * some atom's squared minimum-image displacement from xref exceeds
+ * (skin/2)^2, else 0. */
+int oracle_pairs_need_rebuild(int n, const double* x, const double* xref, const double box[3], double skin);
+
 /* Structure-factor Ewald reciprocal energy, forces (added) and virial over
  * all integer vectors with |m_x|,|m_y|,|m_z| <= mmax, m != 0. */
 double oracle_ewald_recip_dense(int n, const double* q, const double* x, const double box[3],
