@@ -346,6 +346,27 @@ int mdgpu_get_list_builds(const mdgpu_ctx* ctx, int64_t* builds, int64_t* evals)
  * none built yet, or a build pending).  Changes nothing. */
 int mdgpu_pairs_need_rebuild(mdgpu_ctx* ctx, const double* d_coords, int* needs);
 
+/* Molecular dynamics: velocity Verlet, NVE, every step on the GPU.  No
+ * counterpart in the reference, whose README describes integrating Newton's
+ * equations but whose driver stops after one evaluation (apps/main.cpp:23-28).
+ * Units: A, A/ps, ps, amu, kcal/mol (a = 418.4 F/m A/ps^2).
+ * mdgpu_md_set_masses: masses [n_atoms] (host, amu, positive).
+ * mdgpu_md_run: nsteps steps from the device state (d_x, d_v, d_f: 3N
+ * doubles each; on entry d_f must hold the forces at d_x, e.g. from
+ * mdgpu_eval_device or a previous run; on return all three hold the final
+ * state).  The evaluation settings of the context apply (precision, Verlet
+ * list).  trace (host, [nsteps] or NULL) receives each step's potential
+ * energy (the evaluation's total) and kinetic energy after the step; last
+ * (or NULL) the final step's energies.  A step that needs a larger list
+ * restarts the run from the entry state. */
+typedef struct {
+    double potential;
+    double kinetic;
+} mdgpu_md_energy;
+int mdgpu_md_set_masses(mdgpu_ctx* ctx, const double* masses);
+int mdgpu_md_run(mdgpu_ctx* ctx, double* d_x, double* d_v, double* d_f, const double box[3], double dt,
+                 int32_t nsteps, mdgpu_md_energy* trace, mdgpu_energies* last);
+
 #ifdef __cplusplus
 }
 #endif

@@ -155,6 +155,7 @@ int mdgpu_destroy(mdgpu_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_topo(c);
     free_work(c);
+    mdg::md_free(c);
     if (c->adj.acc) (void)hipFree(c->adj.acc);
     if (c->adj.pot_rec) (void)hipFree(c->adj.pot_rec);
     if (c->adj.lj_d) (void)hipFree(c->adj.lj_d);
@@ -485,6 +486,9 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     const mdgpu_params& p = c->params;
     for (int d = 0; d < 3; ++d)
         if (!(box[d] > 0)) return set_error(c, MDGPU_ERR_ARG, "box lengths must be positive");
+    // a Verlet list holds shift vectors of the box it was built in
+    for (int d = 0; d < 3; ++d)
+        if (box[d] != c->grid.box[d]) c->verlet_force = true;
     const double rc = std::max(p.lj_cutoff, p.ee_cutoff);
     if (std::min({box[0], box[1], box[2]}) < 2.0 * rc)
         return set_error(c, MDGPU_ERR_UNSUPPORTED, "every box length must be at least twice the cutoff");
@@ -743,6 +747,15 @@ int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out, bool fork) {
 
 constexpr int kRedo = mdg::kEvalRedo;
 
+// A forced Verlet-list build is requested through device memory, outside
+// any capture, so that captured evaluations see it as well.
+int verlet_prologue(mdgpu_ctx* c) {
+    if (c->skin > 0 && c->verlet_force)
+        HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(&c->work.verlet->force), 1, 1, c->stream),
+                "verlet force");
+    return MDGPU_OK;
+}
+
 // Acts on the flags of a finished evaluation (e: host copy of w.energies).
 // Returns MDGPU_OK, kRedo after adjusting the context so that a repeat
 // succeeds (larger list, general search), or an error status.
@@ -927,20 +940,14 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
     if (!c->have_system || !c->have_params) return set_error(c, MDGPU_ERR_STATE, "upload a system and set params first");
     HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
     const bool amber = c->params.semantics == MDGPU_SEM_AMBER;
-    // a Verlet list holds shift vectors of the box it was built in
-    for (int d = 0; d < 3; ++d)
-        if (box[d] != c->grid.box[d]) c->verlet_force = true;
     for (int attempt = 0; attempt < 4; ++attempt) {
         int st = prepare(c, box);
         if (st != MDGPU_OK) return st;
         Work& w = c->work;
         const double* x = coords_are_work ? w.coords : d_coords;
         double* out = d_out ? d_out : w.force_out;
-        // a forced list build is requested through device memory, outside any
-        // capture, so that replays see it as well
-        if (c->skin > 0 && c->verlet_force)
-            HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(&w.verlet->force), 1, 1, c->stream),
-                    "verlet force");
+        st = verlet_prologue(c);
+        if (st != MDGPU_OK) return st;
         // Graph path: once an evaluation with the same key has succeeded
         // (capacities settled), the next one is captured and replayed from
         // then on; a different key or a redo switches the path off for good
@@ -1027,6 +1034,16 @@ int ctx_eval_device(mdgpu_ctx* c, const double* d_coords, const double box[3], m
     return eval_impl(c, d_coords, box, out_e, d_forces, nullptr, false);
 }
 int ctx_error(mdgpu_ctx* c, int status, const std::string& msg) { return set_error(c, status, msg.c_str()); }
+int ctx_verlet_prologue(mdgpu_ctx* c) { return verlet_prologue(c); }
+int ctx_enqueue_step(mdgpu_ctx* c, const double* d_coords, double* d_forces) {
+    return enqueue_eval(c, d_coords, d_forces, c->fork);
+}
+void ctx_steps_done(mdgpu_ctx* c, const double* rows, int stride, int nsteps) {
+    c->verlet_force = false;
+    c->list_evals += nsteps;
+    for (int k = 0; k < nsteps; ++k)
+        if (c->skin <= 0 || rows[static_cast<size_t>(k) * stride + mdg::E_NSLOTS + 5] != 0.0) ++c->list_builds;
+}
 }  // namespace mdg
 
 extern "C" {

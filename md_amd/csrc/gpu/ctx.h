@@ -232,6 +232,7 @@ struct mdgpu_ctx {
     double skin = 0.0;
     bool verlet_force = true;
     long long list_builds = 0, list_evals = 0;
+    void* md = nullptr;             // molecular-dynamics buffers (md.hip), after mdgpu_md_set_masses
     // concurrent PME chain (single-context evaluations)
     bool fork = true;
     hipStream_t side[1] = {nullptr};
@@ -285,6 +286,14 @@ void ctx_outputs(mdgpu_ctx* c, const double* energies_and_flags, mdgpu_energies*
 int ctx_eval_device(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_energies* out_e,
                     double* d_forces);
 int ctx_error(mdgpu_ctx* c, int status, const std::string& msg);
+// several evaluations in a row without synchronisation (molecular dynamics):
+// request a forced Verlet-list build if one is pending (before the first
+// launch); enqueue one evaluation with the context's stream layout; account
+// for nsteps finished evaluations whose energy rows are rows[k * stride]
+int ctx_verlet_prologue(mdgpu_ctx* c);
+int ctx_enqueue_step(mdgpu_ctx* c, const double* d_coords, double* d_forces);
+void ctx_steps_done(mdgpu_ctx* c, const double* rows, int stride, int nsteps);
+void md_free(mdgpu_ctx* c);
 
 // rows of the partial-energy buffer each kernel needs for a given launch
 int pair_block_threads(const mdgpu_ctx* c);  // threads per block of the pair launch

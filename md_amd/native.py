@@ -143,6 +143,9 @@ def gpu_lib() -> C.CDLL:
         lib.mdgpu_set_graph.argtypes = [C.c_void_p, C.c_int]
         lib.mdgpu_graph_state.argtypes = [C.c_void_p]
         lib.mdgpu_set_verlet.argtypes = [C.c_void_p, C.c_double]
+        lib.mdgpu_md_set_masses.argtypes = [C.c_void_p, P_F64]
+        lib.mdgpu_md_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, P_F64, C.c_double, C.c_int32,
+                                     C.c_void_p, C.POINTER(Energies)]
         lib.mdgpu_get_list_builds.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         lib.mdgpu_pairs_need_rebuild.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
         lib.mdgpu_adjoint.argtypes = [C.c_void_p, P_F64, P_F64, C.POINTER(Energies), C.POINTER(ParamGrads)]
@@ -478,6 +481,22 @@ class Evaluator:
         out = C.c_int()
         self._check(self.lib.mdgpu_pairs_need_rebuild(self.ctx, C.c_void_p(x_ptr), C.byref(out)))
         return bool(out.value)
+
+    def md_set_masses(self, masses):
+        """Atom masses (amu) for molecular dynamics."""
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        self._check(self.lib.mdgpu_md_set_masses(self.ctx, m.ctypes.data_as(P_F64)))
+
+    def md_run(self, x_ptr: int, v_ptr: int, f_ptr: int, box, dt: float, nsteps: int):
+        """Velocity Verlet on the device state (3N doubles each; f holds the forces
+        at x on entry).  Returns (trace (nsteps, 2): potential, kinetic; last energies)."""
+        b = np.ascontiguousarray(box, dtype=np.float64).reshape(-1)[:3]
+        tr = np.zeros((max(int(nsteps), 1), 2))
+        e = Energies()
+        self._check(self.lib.mdgpu_md_run(self.ctx, C.c_void_p(x_ptr), C.c_void_p(v_ptr), C.c_void_p(f_ptr),
+                                          b.ctypes.data_as(P_F64), float(dt), int(nsteps),
+                                          tr.ctypes.data_as(C.c_void_p), C.byref(e)))
+        return tr[:int(nsteps)], e.as_dict()
 
     def set_counting(self, on: bool):
         self._check(self.lib.mdgpu_set_counting(self.ctx, 1 if on else 0))

@@ -45,6 +45,8 @@ def lib() -> C.CDLL:
         L.oracle_bspline.argtypes = [C.c_int, C.c_double, P_F64, P_F64]
         L.oracle_pairs_need_rebuild.argtypes = [C.c_int, P_F64, P_F64, P_F64, C.c_double]
         L.oracle_pairs_need_rebuild.restype = C.c_int
+        L.oracle_md_run.argtypes = [C.POINTER(SystemDesc), C.POINTER(Params), P_F64, P_F64, P_F64, P_F64, P_F64,
+                                    C.c_double, C.c_int, P_F64]
         L.oracle_coulomb_constant.restype = C.c_double
         _lib = L
     return _lib
@@ -141,3 +143,18 @@ def pairs_need_rebuild(coords, coords_ref, box, skin) -> bool:
     r = np.ascontiguousarray(coords_ref, dtype=np.float64).reshape(-1)
     b = np.ascontiguousarray(box, dtype=np.float64)
     return bool(lib().oracle_pairs_need_rebuild(x.size // 3, _p(x), _p(r), _p(b), float(skin)))
+
+
+def md_run(desc: SystemDesc, params: Params, masses, x, v, f, box, dt, nsteps):
+    """Velocity Verlet; returns (x, v, f, trace (nsteps, 2): potential, kinetic)."""
+    m = np.ascontiguousarray(masses, dtype=np.float64)
+    x = np.array(x, dtype=np.float64).reshape(-1)
+    v = np.array(v, dtype=np.float64).reshape(-1)
+    f = np.array(f, dtype=np.float64).reshape(-1)
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    tr = np.zeros(2 * max(nsteps, 1))
+    st = lib().oracle_md_run(C.byref(desc), C.byref(params), _p(m), _p(x), _p(v), _p(f), _p(b), float(dt),
+                             int(nsteps), _p(tr))
+    if st != 0:
+        raise RuntimeError("oracle_md_run failed")
+    return x.reshape(-1, 3), v.reshape(-1, 3), f.reshape(-1, 3), tr[:2 * nsteps].reshape(-1, 2)

@@ -16,6 +16,10 @@
  *                        FD of its forces; finite-strain derivative of virial
  *   B-splines            partition of unity, closed form of M_4, and
  *                        derivative = finite difference
+ *   velocity Verlet      the closed-form discrete solution of a harmonic bond
+ *                        (x_n = A cos(n theta), cos theta = 1 - (w dt)^2 / 2),
+ *                        time reversal, and the dt^2 scaling of the energy
+ *                        fluctuation
  *   rebuild criterion    single-atom displacements just below / at / above
  *                        skin/2, across a periodic face (minimum image), a NaN
  *                        coordinate, and agreement with an exact-rational
@@ -1239,6 +1243,36 @@ int oracle_param_grads(const mdgpu_system_desc* s, const mdgpu_params* p, const 
             pme_core(n, s->charge, x, box, alpha, p->pme_grid, p->pme_order, NULL, NULL, pot);
             for (int i = 0; i < n; ++i) g->charge[i] += w.ewald_recip * pot[i];
             free(pot);
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ MD ---- */
+
+/* The reference has no integrator (apps/main.cpp:23-28 stops after one
+ * evaluation).  Textbook velocity Verlet: half kick, drift, new forces, half
+ * kick; kinetic energy sum m v^2 / 2 converted to kcal/mol. */
+int oracle_md_run(const mdgpu_system_desc* s, const mdgpu_params* p, const double* masses, double* x, double* v,
+                  double* f, const double box[3], double dt, int nsteps, double* trace) {
+    const double acc = 418.4, hdt = 0.5 * dt;
+    const int n = s->n_atoms;
+    oracle_opts o = {2.0, 1};
+    mdgpu_energies e;
+    for (int k = 0; k < nsteps; ++k) {
+        for (int i = 0; i < 3 * n; ++i) {
+            v[i] = v[i] + hdt * (acc * f[i] * (1.0 / masses[i / 3]));
+            x[i] = x[i] + dt * v[i];
+        }
+        if (oracle_eval(s, p, x, box, &o, &e, f, NULL) != 0) return 1;
+        double ke = 0.0;
+        for (int i = 0; i < 3 * n; ++i) {
+            v[i] = v[i] + hdt * (acc * f[i] * (1.0 / masses[i / 3]));
+            ke += masses[i / 3] * v[i] * v[i];
+        }
+        if (trace) {
+            trace[2 * k] = e.total;
+            trace[2 * k + 1] = ke * (0.5 / acc);
         }
     }
     return 0;

@@ -38,6 +38,14 @@ long oracle_ref_pairlist(const mdgpu_system_desc* sys, const double* x, const do
  * (skin/2)^2, else 0. */
 int oracle_pairs_need_rebuild(int n, const double* x, const double* xref, const double box[3], double skin);
 
+/* Velocity Verlet (NVE) for nsteps steps of length dt (ps) from (x, v, f),
+ * f being the forces at x; masses in amu, a = 418.4 F / m A/ps^2.  Updates
+ * x, v, f in place; trace[2k], trace[2k+1] (or NULL) receive step k's
+ * potential energy (the evaluation's total) and kinetic energy.  Forces come
+ * from oracle_eval (without the O(N^2) REF "EE" pass).  Returns 0 on success. */
+int oracle_md_run(const mdgpu_system_desc* sys, const mdgpu_params* p, const double* masses, double* x, double* v,
+                  double* f, const double box[3], double dt, int nsteps, double* trace);
+
 /* Structure-factor Ewald reciprocal energy, forces (added) and virial over
  * all integer vectors with |m_x|,|m_y|,|m_z| <= mmax, m != 0. */
 double oracle_ewald_recip_dense(int n, const double* q, const double* x, const double box[3],
