@@ -364,6 +364,32 @@ typedef struct {
     double kinetic;
 } mdgpu_md_energy;
 int mdgpu_md_set_masses(mdgpu_ctx* ctx, const double* masses);
+
+/* Energy minimisation by L-BFGS on the GPU.  The reference's counterpart is
+ * the sys.minimize() left commented out in apps/main.cpp:28, whose solver its
+ * comments name as L-BFGS (include/System/System.h:84,113); it has no body.
+ * d_x (device, 3N doubles) is minimised in place with the context's
+ * evaluation settings.  Iterations stop when the RMS force per coordinate is
+ * <= rms_force_tol (kcal/mol/A), after max_iter iterations, or when the
+ * backtracking line search finds no decrease.  No atom moves more than
+ * max_step A per iteration; history = L-BFGS pairs kept (1..16).  opts NULL:
+ * {500, 1e-2, 0.2, 8}. */
+typedef struct {
+    int32_t max_iter;
+    double rms_force_tol;
+    double max_step;
+    int32_t history;
+} mdgpu_min_opts;
+typedef struct {
+    int32_t iterations;
+    int32_t evaluations;
+    int32_t converged;     /* 1: the RMS force reached the tolerance        */
+    double energy;         /* total energy at the returned coordinates     */
+    double rms_force;      /* RMS force per coordinate there (kcal/mol/A)  */
+    double max_force;      /* largest atom force there                     */
+} mdgpu_min_result;
+int mdgpu_minimize(mdgpu_ctx* ctx, double* d_x, const double box[3], const mdgpu_min_opts* opts,
+                   mdgpu_min_result* result);
 int mdgpu_md_run(mdgpu_ctx* ctx, double* d_x, double* d_v, double* d_f, const double box[3], double dt,
                  int32_t nsteps, mdgpu_md_energy* trace, mdgpu_energies* last);
 

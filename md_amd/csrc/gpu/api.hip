@@ -156,6 +156,7 @@ int mdgpu_destroy(mdgpu_ctx* c) {
     free_topo(c);
     free_work(c);
     mdg::md_free(c);
+    mdg::min_free(c);
     if (c->adj.acc) (void)hipFree(c->adj.acc);
     if (c->adj.pot_rec) (void)hipFree(c->adj.pot_rec);
     if (c->adj.lj_d) (void)hipFree(c->adj.lj_d);

@@ -233,6 +233,7 @@ struct mdgpu_ctx {
     bool verlet_force = true;
     long long list_builds = 0, list_evals = 0;
     void* md = nullptr;             // molecular-dynamics buffers (md.hip), after mdgpu_md_set_masses
+    void* minbuf = nullptr;         // minimiser buffers (minimize.hip)
     // concurrent PME chain (single-context evaluations)
     bool fork = true;
     hipStream_t side[1] = {nullptr};
@@ -294,6 +295,7 @@ int ctx_verlet_prologue(mdgpu_ctx* c);
 int ctx_enqueue_step(mdgpu_ctx* c, const double* d_coords, double* d_forces);
 void ctx_steps_done(mdgpu_ctx* c, const double* rows, int stride, int nsteps);
 void md_free(mdgpu_ctx* c);
+void min_free(mdgpu_ctx* c);
 
 // rows of the partial-energy buffer each kernel needs for a given launch
 int pair_block_threads(const mdgpu_ctx* c);  // threads per block of the pair launch

@@ -144,6 +144,7 @@ def gpu_lib() -> C.CDLL:
         lib.mdgpu_graph_state.argtypes = [C.c_void_p]
         lib.mdgpu_set_verlet.argtypes = [C.c_void_p, C.c_double]
         lib.mdgpu_md_set_masses.argtypes = [C.c_void_p, P_F64]
+        lib.mdgpu_minimize.argtypes = [C.c_void_p, C.c_void_p, P_F64, C.POINTER(MinOpts), C.POINTER(MinResult)]
         lib.mdgpu_md_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, P_F64, C.c_double, C.c_int32,
                                      C.c_void_p, C.POINTER(Energies)]
         lib.mdgpu_get_list_builds.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
@@ -358,6 +359,16 @@ def make_params(semantics=SEM_REF, precision=PREC_FP64, lj_cutoff=10.0, ee_cutof
     return p
 
 
+class MinOpts(C.Structure):
+    _fields_ = [("max_iter", C.c_int32), ("rms_force_tol", C.c_double), ("max_step", C.c_double),
+                ("history", C.c_int32)]
+
+
+class MinResult(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("evaluations", C.c_int32), ("converged", C.c_int32),
+                ("energy", C.c_double), ("rms_force", C.c_double), ("max_force", C.c_double)]
+
+
 class GpuError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{ERRORS.get(status, status)}: {msg}")
@@ -481,6 +492,16 @@ class Evaluator:
         out = C.c_int()
         self._check(self.lib.mdgpu_pairs_need_rebuild(self.ctx, C.c_void_p(x_ptr), C.byref(out)))
         return bool(out.value)
+
+    def minimize(self, x_ptr: int, box, max_iter=500, rms_force_tol=1e-2, max_step=0.2, history=8) -> dict:
+        """L-BFGS on device coordinates at x_ptr (3N doubles, minimised in place)."""
+        b = np.ascontiguousarray(box, dtype=np.float64).reshape(-1)[:3]
+        o = MinOpts(int(max_iter), float(rms_force_tol), float(max_step), int(history))
+        r = MinResult()
+        self._check(self.lib.mdgpu_minimize(self.ctx, C.c_void_p(x_ptr), b.ctypes.data_as(P_F64), C.byref(o),
+                                            C.byref(r)))
+        return {"iterations": r.iterations, "evaluations": r.evaluations, "converged": bool(r.converged),
+                "energy": r.energy, "rms_force": r.rms_force, "max_force": r.max_force}
 
     def md_set_masses(self, masses):
         """Atom masses (amu) for molecular dynamics."""

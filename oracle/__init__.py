@@ -45,6 +45,8 @@ def lib() -> C.CDLL:
         L.oracle_bspline.argtypes = [C.c_int, C.c_double, P_F64, P_F64]
         L.oracle_pairs_need_rebuild.argtypes = [C.c_int, P_F64, P_F64, P_F64, C.c_double]
         L.oracle_pairs_need_rebuild.restype = C.c_int
+        L.oracle_minimize.argtypes = [C.POINTER(SystemDesc), C.POINTER(Params), P_F64, P_F64, C.c_int, C.c_double,
+                                      C.c_double, C.c_int, P_F64, P_F64]
         L.oracle_md_run.argtypes = [C.POINTER(SystemDesc), C.POINTER(Params), P_F64, P_F64, P_F64, P_F64, P_F64,
                                     C.c_double, C.c_int, P_F64]
         L.oracle_coulomb_constant.restype = C.c_double
@@ -158,3 +160,19 @@ def md_run(desc: SystemDesc, params: Params, masses, x, v, f, box, dt, nsteps):
     if st != 0:
         raise RuntimeError("oracle_md_run failed")
     return x.reshape(-1, 3), v.reshape(-1, 3), f.reshape(-1, 3), tr[:2 * nsteps].reshape(-1, 2)
+
+
+def minimize(desc: SystemDesc, params: Params, x, box, max_iter=500, rms_tol=1e-2, max_step=0.2, history=8):
+    """L-BFGS; returns (x, energies per accepted iteration, result dict)."""
+    x = np.array(x, dtype=np.float64).reshape(-1)
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    en = np.zeros(max_iter + 2)
+    out = np.zeros(5)
+    st = lib().oracle_minimize(C.byref(desc), C.byref(params), _p(x), _p(b), int(max_iter), float(rms_tol),
+                               float(max_step), int(history), _p(en), _p(out))
+    if st != 0:
+        raise RuntimeError("oracle_minimize failed")
+    it = int(out[0])
+    res = {"iterations": it, "evaluations": int(out[1]), "converged": bool(out[2]), "rms_force": out[3],
+           "max_force": out[4], "energy": en[it]}
+    return x.reshape(-1, 3), en[:it + 1], res

@@ -16,6 +16,11 @@
  *                        FD of its forces; finite-strain derivative of virial
  *   B-splines            partition of unity, closed form of M_4, and
  *                        derivative = finite difference
+ *   L-BFGS minimiser     a harmonic bond and a bonded chain reach their known
+ *                        minima (E = 0, r = r0) with the RMS force at the
+ *                        tolerance, the energy never rises, and a single
+ *                        quadratic step lands on the minimum once a pair is
+ *                        kept (H0 = s.y / y.y is exact for one coordinate)
  *   velocity Verlet      the closed-form discrete solution of a harmonic bond
  *                        (x_n = A cos(n theta), cos theta = 1 - (w dt)^2 / 2),
  *                        time reversal, and the dt^2 scaling of the energy
@@ -1275,5 +1280,111 @@ int oracle_md_run(const mdgpu_system_desc* s, const mdgpu_params* p, const doubl
             trace[2 * k + 1] = ke * (0.5 / acc);
         }
     }
+    return 0;
+}
+
+/* ------------------------------------------------------------ minimise ---- */
+
+/* The reference leaves sys.minimize() commented out (apps/main.cpp:28) and
+ * names L-BFGS in its comments (System.h:84,113).  Nocedal & Wright,
+ * Numerical Optimization, Algorithms 7.4 and 7.5, with backtracking Armijo. */
+static double dotv(const double* a, const double* b, int n) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+static double max_atom2(const double* a, int n_atoms) {
+    double m = 0.0;
+    for (int i = 0; i < n_atoms; ++i) {
+        double t = a[3 * i] * a[3 * i] + a[3 * i + 1] * a[3 * i + 1] + a[3 * i + 2] * a[3 * i + 2];
+        if (t > m) m = t;
+    }
+    return m;
+}
+
+int oracle_minimize(const mdgpu_system_desc* s, const mdgpu_params* p, double* x, const double box[3], int max_iter,
+                    double rms_tol, double max_step, int m, double* energies, double* out) {
+    const int n = s->n_atoms, n3 = 3 * n;
+    oracle_opts o = {2.0, 1};
+    mdgpu_energies e;
+    double* f = (double*)malloc(sizeof(double) * (size_t)n3 * (6 + 2 * (size_t)m));
+    if (!f) return 1;
+    double *ft = f + n3, *xt = ft + n3, *pv = xt + n3, *q = pv + n3, *sv = q + n3, *yv = sv + (size_t)m * n3;
+    double rho[16], alpha[16];
+    int kept = 0, newest = -1, it = 0, evals = 1, conv = 0;
+    if (oracle_eval(s, p, x, box, &o, &e, f, NULL) != 0) { free(f); return 1; }
+    double energy = e.total, rms = 0.0, fmax = 0.0;
+    if (energies) energies[0] = energy;
+    for (it = 0;; ++it) {
+        double f2 = dotv(f, f, n3), fm2 = max_atom2(f, n);
+        rms = sqrt(f2 / n3);
+        fmax = sqrt(fm2);
+        if (rms <= rms_tol) { conv = 1; break; }
+        if (it >= max_iter) break;
+        /* q and r carry F = -g; the alphas and betas come out negated */
+        memcpy(q, f, sizeof(double) * n3);
+        for (int k = 0; k < kept; ++k) {
+            int i = (newest - k + m) % m;
+            alpha[i] = dotv(sv + (size_t)i * n3, q, n3) * rho[i];
+            for (int j = 0; j < n3; ++j) q[j] = q[j] + (-alpha[i]) * yv[(size_t)i * n3 + j];
+        }
+        if (kept > 0) {
+            double yy = dotv(yv + (size_t)newest * n3, yv + (size_t)newest * n3, n3);
+            double g0 = 1.0 / (rho[newest] * yy);
+            for (int j = 0; j < n3; ++j) pv[j] = g0 * q[j];
+            for (int k = kept - 1; k >= 0; --k) {
+                int i = (newest - k + m) % m;
+                double beta = dotv(yv + (size_t)i * n3, pv, n3) * rho[i];
+                double c = alpha[i] - beta;
+                for (int j = 0; j < n3; ++j) pv[j] = pv[j] + c * sv[(size_t)i * n3 + j];
+            }
+        } else {
+            memcpy(pv, q, sizeof(double) * n3);
+        }
+        double gp = -dotv(f, pv, n3), pmax = sqrt(max_atom2(pv, n));
+        if (!(gp < 0.0)) {
+            kept = 0;
+            memcpy(pv, f, sizeof(double) * n3);
+            gp = -f2;
+            pmax = fmax;
+        }
+        double a = 1.0, et = 0.0;
+        if (a * pmax > max_step) a = max_step / pmax;
+        int accepted = 0;
+        for (int ls = 0; ls < 30; ++ls) {
+            for (int j = 0; j < n3; ++j) xt[j] = x[j] + a * pv[j];
+            if (oracle_eval(s, p, xt, box, &o, &e, ft, NULL) != 0) { free(f); return 1; }
+            ++evals;
+            et = e.total;
+            if (et <= energy + 1e-4 * a * gp) { accepted = 1; break; }
+            a *= 0.5;
+        }
+        if (!accepted) break;
+        int slot = (newest + 1) % m;
+        double* ss = sv + (size_t)slot * n3;
+        double* yy = yv + (size_t)slot * n3;
+        for (int j = 0; j < n3; ++j) {
+            ss[j] = a * pv[j];
+            yy[j] = f[j] + (-1.0) * ft[j];
+        }
+        double ys = dotv(yy, ss, n3);
+        memcpy(x, xt, sizeof(double) * n3);
+        memcpy(f, ft, sizeof(double) * n3);
+        energy = et;
+        if (energies) energies[it + 1] = energy;
+        if (ys > 0.0) {
+            rho[slot] = 1.0 / ys;
+            newest = slot;
+            kept = kept + 1 < m ? kept + 1 : m;
+        }
+    }
+    if (out) {
+        out[0] = it;
+        out[1] = evals;
+        out[2] = conv;
+        out[3] = rms;
+        out[4] = fmax;
+    }
+    free(f);
     return 0;
 }

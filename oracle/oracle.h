@@ -46,6 +46,16 @@ int oracle_pairs_need_rebuild(int n, const double* x, const double* xref, const 
 int oracle_md_run(const mdgpu_system_desc* sys, const mdgpu_params* p, const double* masses, double* x, double* v,
                   double* f, const double box[3], double dt, int nsteps, double* trace);
 
+/* L-BFGS minimisation (two-loop recursion, H0 = (s.y/y.y) I, pairs kept
+ * only when s.y > 0, backtracking Armijo line search with c1 = 1e-4 and
+ * halving, steps capped at max_step A per atom, steepest descent when the
+ * direction is not a descent one), the algorithm mdgpu_minimize runs.
+ * Minimises x in place; energies[k] (or NULL, up to max_iter + 1 entries)
+ * receives the energy after k accepted iterations.  out[0..4] = iterations,
+ * evaluations, converged, rms force, max atom force. */
+int oracle_minimize(const mdgpu_system_desc* sys, const mdgpu_params* p, double* x, const double box[3],
+                    int max_iter, double rms_tol, double max_step, int history, double* energies, double* out);
+
 /* Structure-factor Ewald reciprocal energy, forces (added) and virial over
  * all integer vectors with |m_x|,|m_y|,|m_z| <= mmax, m != 0. */
 double oracle_ewald_recip_dense(int n, const double* q, const double* x, const double box[3],

@@ -57,3 +57,31 @@ def thermal_velocities(masses, seed, temperature=300.0):
     sig = np.sqrt(kT * 418.4 / np.asarray(masses, dtype=np.float64))
     rng = np.random.default_rng(seed)
     return rng.normal(size=(sig.size, 3)) * sig[:, None]
+
+
+def bonded_chain(n=12, seed=2, box=40.0):
+    """An uncharged chain without LJ: bonds k 300, r0 1.5 and angles k 50,
+    theta0 1.9 rad, from a perturbed zig-zag.  Its minimum energy is 0 (every
+    bond and angle at rest).  Returns (desc, keep, coords, box)."""
+    rng = np.random.default_rng(seed)
+    arr = _empty()
+    bonds = np.array([[i, i + 1] for i in range(n - 1)], np.int32).reshape(-1)
+    angles = np.array([[i, i + 1, i + 2] for i in range(n - 2)], np.int32).reshape(-1)
+    arr.update({
+        "charge": np.zeros(n), "type": np.zeros(n, np.int32), "n_types": 1, "nb_index": np.array([1], np.int32),
+        "lj_a": np.zeros(1), "lj_b": np.zeros(1), "lj14_a": np.zeros(1), "lj14_b": np.zeros(1),
+        "bond_atoms": bonds, "bond_type": np.zeros(n - 1, np.int32), "bond_k": np.array([300.0]),
+        "bond_r0": np.array([1.5]),
+        "angle_atoms": angles, "angle_type": np.zeros(n - 2, np.int32), "angle_k": np.array([50.0]),
+        "angle_theta0": np.array([1.9]),
+        "excl_start": np.zeros(n + 1, np.int32), "excl_list": np.zeros(0, np.int32),
+        "excl_pairs": np.zeros(0, np.int32),
+    })
+    keep = []
+    desc = N.desc_from_arrays(arr, keep)
+    keep.append(arr)
+    x = np.zeros((n, 3))
+    for i in range(n):
+        x[i] = [0.5 * box - 8.0 + 1.25 * i, 0.5 * box + (0.8 if i % 2 else 0.0), 0.5 * box]
+    x += rng.normal(0.0, 0.15, x.shape)
+    return desc, keep, x, np.array([box, box, box])
