@@ -8,7 +8,8 @@
 // kernels on the GPU.
 //
 // Additions: set_semantics/set_precision/set_pme_grid/set_pme_order/
-// set_device, evaluate(), energies(), forces(), virial().
+// set_device, evaluate(), energies(), forces(), virial(), and minimize() (the
+// reference's commented-out next step, apps/main.cpp:28).
 #ifndef MD_AMD_SYSTEM_H
 #define MD_AMD_SYSTEM_H
 
@@ -72,6 +73,14 @@ public:
     void set_coordinates(const std::vector<double>& xyz);
     // One energy + force evaluation on the GPU with the current settings.
     void evaluate();
+    // L-BFGS energy minimisation of the coordinates on the GPU (the
+    // reference's commented-out sys.minimize(), apps/main.cpp:28), then one
+    // evaluation at the minimised coordinates.  With a skin set
+    // (set_lj_skin_ / set_ee_skin) the pair list is a Verlet list kept across
+    // the minimiser's evaluations.
+    void minimize();
+    void set_minimize_options(const mdgpu_min_opts& o) { min_opts_ = o; }
+    [[nodiscard]] const mdgpu_min_result& minimize_result() const { return min_result_; }
     [[nodiscard]] const mdgpu_energies& energies() const { return energies_; }
     [[nodiscard]] const std::vector<double>& forces() const { return forces_; }
     [[nodiscard]] const std::array<double, 9>& virial() const { return virial_; }
@@ -90,12 +99,15 @@ private:
     static double min_image_1d(double d, double L);
     double calculate_center_of_mass_along_direction(std::string_view direction);
     void ensure_device_system();
+    void apply_settings();
     [[noreturn]] void fail(int status, const char* what) const;
 
     AmberTopology::AmberTopology topology_;
     std::vector<double> forces_;
     mdgpu_energies energies_{};
     std::array<double, 9> virial_{};
+    mdgpu_min_opts min_opts_{500, 1e-2, 0.2, 8};
+    mdgpu_min_result min_result_{};
 
     Semantics semantics_;
     Precision precision_ = Precision::FP64;

@@ -390,6 +390,9 @@ typedef struct {
 } mdgpu_min_result;
 int mdgpu_minimize(mdgpu_ctx* ctx, double* d_x, const double box[3], const mdgpu_min_opts* opts,
                    mdgpu_min_result* result);
+/* The same for host coordinates x (3N doubles, updated in place). */
+int mdgpu_minimize_host(mdgpu_ctx* ctx, double* x, const double box[3], const mdgpu_min_opts* opts,
+                        mdgpu_min_result* result);
 int mdgpu_md_run(mdgpu_ctx* ctx, double* d_x, double* d_v, double* d_f, const double box[3], double dt,
                  int32_t nsteps, mdgpu_md_energy* trace, mdgpu_energies* last);
 

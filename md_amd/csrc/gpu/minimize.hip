@@ -139,12 +139,13 @@ struct MinBuf {
     double* sc = nullptr;    // [kNumScalars]
     double* part = nullptr;  // [kMinBlocks]
     unsigned int* ticket = nullptr;
+    double* x_own = nullptr; // coordinates of host-buffer minimisations
 };
 
 void min_free(mdgpu_ctx* c) {
     MinBuf* b = static_cast<MinBuf*>(c->minbuf);
     if (!b) return;
-    void* ptrs[] = {b->x_t, b->f, b->f_t, b->p, b->q, b->s, b->y, b->sc, b->part, b->ticket};
+    void* ptrs[] = {b->x_t, b->f, b->f_t, b->p, b->q, b->s, b->y, b->sc, b->part, b->ticket, b->x_own};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -181,6 +182,7 @@ int ensure_min(mdgpu_ctx* c, int m) {
     MIN_TRY(hipMalloc(reinterpret_cast<void**>(&b->sc), mdg::kNumScalars * sizeof(double)), "alloc");
     MIN_TRY(hipMalloc(reinterpret_cast<void**>(&b->part), mdg::kMinBlocks * sizeof(double)), "alloc");
     MIN_TRY(hipMalloc(reinterpret_cast<void**>(&b->ticket), sizeof(unsigned int)), "alloc");
+    MIN_TRY(hipMalloc(reinterpret_cast<void**>(&b->x_own), v), "alloc");
     MIN_TRY(hipMemset(b->ticket, 0, sizeof(unsigned int)), "clear");
     MIN_TRY(hipMemset(b->sc, 0, mdg::kNumScalars * sizeof(double)), "clear");
     return MDGPU_OK;
@@ -283,8 +285,14 @@ int mdgpu_minimize(mdgpu_ctx* c, double* d_x, const double box[3], const mdgpu_m
         for (int ls = 0; ls < 30; ++ls) {
             hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(kMinThreads), 0, s, b->x_t, d_x, b->p, a, n3);
             st = ctx_eval_device(c, b->x_t, box, &e, b->f_t);
-            if (st != MDGPU_OK) return st;
             ++r.evaluations;
+            if (st == MDGPU_ERR_OVERFLOW) {
+                // a force beyond the fixed-point range at the trial point: an
+                // energy too high to accept, so the step is halved
+                a *= 0.5;
+                continue;
+            }
+            if (st != MDGPU_OK) return st;
             e_t = e.total;
             if (e_t <= energy + 1e-4 * a * gp) {
                 accepted = true;
@@ -313,6 +321,24 @@ int mdgpu_minimize(mdgpu_ctx* c, double* d_x, const double box[3], const mdgpu_m
     }
     MIN_TRY(hipStreamSynchronize(s), "minimize");
     if (res) *res = r;
+    return MDGPU_OK;
+}
+
+int mdgpu_minimize_host(mdgpu_ctx* c, double* x, const double box[3], const mdgpu_min_opts* opts,
+                        mdgpu_min_result* res) {
+    using namespace mdg;
+    if (!c || !x || !box) return MDGPU_ERR_ARG;
+    if (!c->have_system) return ctx_error(c, MDGPU_ERR_STATE, "upload a system first");
+    MIN_TRY(hipSetDevice(c->device), "hipSetDevice");
+    const int m = opts ? opts->history : 8;
+    int st = ensure_min(c, m >= 1 && m <= kMaxHistory ? m : 8);
+    if (st != MDGPU_OK) return st;
+    auto* b = static_cast<MinBuf*>(c->minbuf);
+    const size_t bytes = static_cast<size_t>(b->n3) * sizeof(double);
+    MIN_TRY(hipMemcpy(b->x_own, x, bytes, hipMemcpyHostToDevice), "coordinates in");
+    st = mdgpu_minimize(c, b->x_own, box, opts, res);
+    if (st != MDGPU_OK) return st;
+    MIN_TRY(hipMemcpy(x, b->x_own, bytes, hipMemcpyDeviceToHost), "coordinates out");
     return MDGPU_OK;
 }
 

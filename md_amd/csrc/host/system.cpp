@@ -24,6 +24,12 @@ bool is_2a3b(int n) {
     while (n % 3 == 0) n /= 3;
     return n == 1;
 }
+// the smallest 2^a 3^b >= n (the lengths the GPU FFT handles)
+int next_2a3b(int n) {
+    int k = std::max(n, 2);
+    while (!is_2a3b(k)) ++k;
+    return k;
+}
 }  // namespace
 
 System::System(AmberTopology::AmberTopology topology)
@@ -195,7 +201,10 @@ void System::ensure_device_system() {
     }
 }
 
-void System::evaluate() {
+// Parameters of the current settings to the device context; the largest
+// skin makes the pair list a Verlet list (built at cutoff + skin, rebuilt on
+// the GPU once an atom has moved more than skin / 2).
+void System::apply_settings() {
     if (!pbc_enabled_) throw std::runtime_error("evaluate: the GPU path needs a periodic box");
     calculate_ewald_alpha();
     ensure_device_system();
@@ -207,7 +216,12 @@ void System::evaluate() {
     p.lj_switch = lj_use_switch_ ? lj_switch_ : 0.0;
     p.ewald_alpha = ewald_alpha_;
     if (semantics_ == Semantics::AMBER) {
-        if (pme_grid_[0] == 0) calculate_ewald_nnodes();
+        if (pme_grid_[0] == 0) {
+            // the reference's formula, each length rounded up to a length the
+            // GPU FFT handles (2^a 3^b): a finer grid, never a coarser one
+            calculate_ewald_nnodes();
+            for (int& k : pme_grid_) k = next_2a3b(k);
+        }
         init_pme_fft();
         p.pme_grid[0] = pme_grid_[0];
         p.pme_grid[1] = pme_grid_[1];
@@ -216,9 +230,25 @@ void System::evaluate() {
     p.pme_order = pme_order_;
     int st = mdgpu_set_params(ctx_, &p);
     if (st != MDGPU_OK) fail(st, "mdgpu_set_params");
+    st = mdgpu_set_verlet(ctx_, std::max({0.0, lj_skin_, ee_skin_}));
+    if (st != MDGPU_OK) fail(st, "mdgpu_set_verlet");
+}
+
+void System::evaluate() {
+    apply_settings();
     const double box[3] = {boxLx_, boxLy_, boxLz_};
-    st = mdgpu_eval(ctx_, topology_.get_coordinates().data(), box, &energies_, forces_.data(), virial_.data());
+    const int st = mdgpu_eval(ctx_, topology_.get_coordinates().data(), box, &energies_, forces_.data(), virial_.data());
     if (st != MDGPU_OK) fail(st, "mdgpu_eval");
+}
+
+void System::minimize() {
+    apply_settings();
+    const double box[3] = {boxLx_, boxLy_, boxLz_};
+    std::vector<double> x = topology_.get_coordinates();
+    const int st = mdgpu_minimize_host(ctx_, x.data(), box, &min_opts_, &min_result_);
+    if (st != MDGPU_OK) fail(st, "mdgpu_minimize_host");
+    topology_.set_coordinates(x);
+    evaluate();
 }
 
 // The reference's System::init() sequence and report (src/System/init.cpp:

@@ -133,6 +133,34 @@ def _lj_pair_tables(sigma, eps):
     return a, b
 
 
+def with_hydrogen_lj(arrays: dict, h_type: int, sigma: float = 0.4, eps: float = 0.046) -> dict:
+    """A copy of system arrays (md_amd.native.Topology.arrays layout) whose LJ
+    type ``h_type`` (0-based; a type without LJ, such as the TIP3P hydrogens)
+    gets ``sigma`` / ``eps``, combined with every type by Lorentz-Berthelot.
+    The defaults are the hydrogen parameters of the CHARMM variant of TIP3P.
+    Without them a hydrogen and an unlike charge have no repulsion, so the
+    energy has no lower bound and a minimiser can follow a hydrogen into
+    another atom; energies and forces are unchanged in form.  The other
+    types' sigma and eps are recovered from the diagonal table entries
+    (A = 4 eps sigma^12, B = 4 eps sigma^6)."""
+    out = dict(arrays)
+    t = int(arrays["n_types"])
+    nb = np.asarray(arrays["nb_index"]).reshape(t, t)
+    lj_a, lj_b = np.array(arrays["lj_a"], dtype=np.float64), np.array(arrays["lj_b"], dtype=np.float64)
+    sig, ep = np.zeros(t), np.zeros(t)
+    for k in range(t):
+        a, b = lj_a[nb[k, k] - 1], lj_b[nb[k, k] - 1]
+        if a > 0 and b > 0:
+            sig[k], ep[k] = (a / b) ** (1.0 / 6.0), b * b / (4.0 * a)
+    sig[h_type], ep[h_type] = sigma, eps
+    for k in range(t):
+        s, e = 0.5 * (sig[h_type] + sig[k]), math.sqrt(ep[h_type] * ep[k])
+        lj_a[nb[h_type, k] - 1] = 4.0 * e * s ** 12
+        lj_b[nb[h_type, k] - 1] = 4.0 * e * s ** 6
+    out["lj_a"], out["lj_b"] = lj_a, lj_b
+    return out
+
+
 # --------------------------------------------------------------------------
 # water
 # --------------------------------------------------------------------------

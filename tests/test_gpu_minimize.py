@@ -146,3 +146,39 @@ def test_minimised_water_conserves_energy(water_6k):
     ratio = spread[0] / spread[1]
     print(f"  spread ratio {ratio:.2f}")
     assert 2.5 <= ratio <= 5.5
+
+
+def test_cpp_driver_minimize(polymer_2k):
+    """md_main --minimize: the reference's driver with its commented-out
+    sys.minimize() line enabled, through the C++ System API (AMBER semantics,
+    rc 10 A, switch 9 A, skin 2 A, Ewald tolerance 1e-6, the reference's PME
+    grid formula rounded up to 2^a 3^b).  The same minimisation through the C
+    ABI from Python gives the same iterations and energy."""
+    import math
+    import os
+    import re
+    import subprocess
+    from md_amd import systems
+    print("case md_main --minimize 20 polymer_2k", flush=True)
+    prm_path, rst_path = polymer_2k
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "md_amd", "md_main")
+    out = subprocess.run([exe, prm_path, rst_path, "--minimize", "20"], capture_output=True, text=True, timeout=120)
+    print(out.stdout[-500:], out.stderr[-300:])
+    assert out.returncode == 0
+    e_cpp = float(re.search(r"Minimized:.*\nEnergy: (\S+)", out.stdout).group(1))
+    it_cpp = int(re.search(r"Minimized: iterations (\d+)", out.stdout).group(1))
+    top = N.Topology(prm_path, rst_path, N.SEM_AMBER)
+    tol, rc = 1e-6, 10.0
+    alpha = math.sqrt(-math.log(2.0 * tol)) / rc
+    grid = tuple(systems.smallest_2a3b(int(2.0 * alpha * L / (3.0 * tol ** 0.2))) for L in top.box)
+    prm = N.make_params(N.SEM_AMBER, N.PREC_FP64, lj_cutoff=rc, lj_switch=9.0, ewald_alpha=alpha, pme_grid=grid)
+    ev = _ev(top.desc, prm, 2.0)
+    try:
+        xd = _dev(top.coords)
+        res = ev.minimize(xd.data_ptr(), top.box, max_iter=20, rms_force_tol=1e-2, max_step=0.2)
+    finally:
+        ev.close()
+    print(f"  C++ driver {e_cpp:.6f} in {it_cpp} iterations; C ABI {res}")
+    assert it_cpp == res["iterations"] == 20
+    # the driver prints six significant digits
+    assert abs(e_cpp - res["energy"]) <= 1e-5 * abs(res["energy"])

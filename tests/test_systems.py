@@ -49,3 +49,23 @@ def _factors(n):
     if n > 1:
         out.append(n)
     return out
+
+
+def test_with_hydrogen_lj(sysdir):
+    """The hydrogen type gets Lorentz-Berthelot entries with every type; the
+    other entries are untouched."""
+    from md_amd import native as N
+    prm, rst = systems.cached_system("water_box_6k", sysdir)
+    top = N.Topology(prm, rst, N.SEM_AMBER)
+    a = top.arrays
+    h = int(a["type"][1])
+    b = systems.with_hydrogen_lj(a, h, sigma=0.4, eps=0.046)
+    t = int(a["n_types"])
+    nb = np.asarray(a["nb_index"]).reshape(t, t)
+    o = int(a["type"][0])
+    so, eo = systems.O_SIGMA, systems.O_EPS
+    s, e = 0.5 * (0.4 + so), math.sqrt(0.046 * eo)
+    assert math.isclose(b["lj_a"][nb[h, o] - 1], 4 * e * s ** 12, rel_tol=1e-12)
+    assert math.isclose(b["lj_b"][nb[h, o] - 1], 4 * e * s ** 6, rel_tol=1e-12)
+    assert math.isclose(b["lj_a"][nb[h, h] - 1], 4 * 0.046 * 0.4 ** 12, rel_tol=1e-12)
+    assert b["lj_a"][nb[o, o] - 1] == a["lj_a"][nb[o, o] - 1]
