@@ -13,8 +13,9 @@
  *   - every function returns an int status (MDGPU_OK == 0); no exception or
  *     abort crosses the boundary; mdgpu_last_error() gives the message;
  *   - host buffers belong to the caller, device buffers to the context;
- *   - one context = one device + one HIP stream; calls are synchronous on
- *     return unless stated; a context is not thread-safe (use one per thread);
+ *   - one context = one device and its own HIP streams (a main stream, and a
+ *     side stream for the PME chain); calls are synchronous on return unless
+ *     stated; a context is not thread-safe (use one per thread);
  *   - atom indices are 0-based; coordinates and forces are 3N interleaved
  *     x,y,z doubles (the reference's layout, AmberTopology.h:252, System.h:175);
  *   - energies in kcal/mol, forces in kcal/mol/A, lengths in A.
