@@ -176,6 +176,27 @@ def bench_md(torch, N, systems, top, p32, box, steps=200, warmup=20, dt=0.0005):
             "dynamics": legs}
 
 
+def bench_io(N, top, reps=5):
+    """Amber restart write and read (coordinates, velocities and box of the
+    28,800-atom workload, 6F12.7) through the host library, in MB/s of file."""
+    import tempfile
+    v = np.random.default_rng(8).normal(0.0, 5.0, top.coords.shape)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "w.rst7")
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            N.write_rst7(p, top.coords, box=top.box, velocities=v, title="bench", time=1.0)
+        tw = (time.perf_counter() - t0) / reps
+        size = os.path.getsize(p)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = N.read_rst7(p)
+        tr = (time.perf_counter() - t0) / reps
+    ok = bool(np.abs(r["coords"] - top.coords).max() <= 5.01e-8)
+    return {"file_bytes": size, "write_MB_per_s": size / tw / 1e6, "read_MB_per_s": size / tr / 1e6,
+            "round_trip_within_F12.7": ok, "workload": "polymer_29k coordinates + velocities + box"}
+
+
 def bench_adjoint(torch, N, systems, n_frames=64):
     """dLoss/dtheta, loss = sum_f (E_f - E_target,f)^2 over 64 frames of the
     6,450-atom water box (0.1 A Gaussian perturbations, seeds 5000+f), targets
@@ -346,6 +367,7 @@ def main():
                                "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "algorithmic": what}
 
     md = None if args.no_md else bench_md(torch, N, systems, top, p32, box)
+    io = bench_io(N, top)
     batched = None if args.no_batched else bench_batched(torch, N, systems)
     adjoint = None if args.no_adjoint else bench_adjoint(torch, N, systems)
 
@@ -403,6 +425,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "md": md,
+        "io": io,
         "batched": batched,
         "adjoint": adjoint,
     }

@@ -47,6 +47,8 @@ public:
 
     const std::vector<double>& get_coordinates() { return coordinates_; }
     [[nodiscard]] const std::vector<double>& get_coordinates() const { return coordinates_; }
+    // addition: velocities (A/ps) of an AMBER-semantics rst7 that holds them, else empty
+    [[nodiscard]] const std::vector<double>& get_velocities() const { return velocities_; }
     void set_coordinates(const std::vector<double>& c) { coordinates_ = c; }
     [[nodiscard]] std::vector<Bonded::HarmonicBond>& get_harmonic_bonds() { return bonds_; }
     [[nodiscard]] std::vector<Bonded::HarmonicAngle>& get_harmonic_angles() { return angles_; }
@@ -137,6 +139,7 @@ private:
 
     double box_x_ = 0.0, box_y_ = 0.0, box_z_ = 0.0;
     double box_alpha_ = 90.0, box_beta_ = 90.0, box_gamma_ = 90.0;
+    std::vector<double> velocities_;  // 3N A/ps when an AMBER-semantics rst7 holds them, else empty
 };
 
 // CMAP patch coefficients of n_sets grids of res x res values (16 per cell,

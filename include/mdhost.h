@@ -37,6 +37,29 @@ void mdhost_free(mdhost_system* h);
  * output).  No counterpart in the reference's API. */
 int mdhost_cmap_coefficients(int res, int n_sets, const double* grid, double* coeff);
 int mdhost_cmap_grid_grad(int res, int n_sets, const double* d_coeff, double* d_grid);
+/* Amber restart files (no counterpart in the reference's API: it reads an
+ * rst7 inside AmberTopology, src/AmberTopology/AmberTopology.cpp:388-438,
+ * taking any values after the 3N coordinates as the box, and writes none).
+ * mdhost_rst7_read: title, atom count and time, 3N coordinates, optional 3N
+ * velocities (returned in A/ps) and an optional box (3 lengths, or lengths
+ * and angles in degrees), read from 12-column fields when every field of a
+ * line parses, else from blank-separated tokens.  MDGPU_ERR_ARG (message in
+ * mdhost_last_error) when the file is unreadable or its value count fits no
+ * layout; the time is NaN when the file has none.  mdhost_rst7_write: the Amber ASCII layout (title, I6 count and
+ * E15.7 time, 6F12.7 coordinates, velocities and box); velocities (A/ps,
+ * or NULL) and box6 (or NULL) optional. */
+typedef struct mdhost_rst7 mdhost_rst7;
+int mdhost_rst7_read(const char* path, mdhost_rst7** out);
+int mdhost_rst7_info(const mdhost_rst7* h, int32_t* n_atoms, int32_t* has_velocities, int32_t* has_box,
+                     double* time);
+const double* mdhost_rst7_coords(const mdhost_rst7* h);
+const double* mdhost_rst7_velocities(const mdhost_rst7* h);  /* NULL without velocities */
+int mdhost_rst7_box(const mdhost_rst7* h, double box[6]);
+const char* mdhost_rst7_title(const mdhost_rst7* h);
+void mdhost_rst7_free(mdhost_rst7* h);
+int mdhost_rst7_write(const char* path, const char* title, int32_t n_atoms, const double* coords,
+                      const double* velocities, const double* box6, double time, int32_t has_time);
+
 /* Message of the last failure on this thread. */
 const char* mdhost_last_error(void);
 

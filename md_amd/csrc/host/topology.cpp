@@ -7,6 +7,7 @@
 // reference's term-list construction (src/AmberTopology/CreateBondedFromParsed.
 // cpp, CreateNonBondedFromParsed.cpp); Semantics::AMBER builds complete lists.
 #include "AmberTopology/AmberTopology.h"
+#include "IO/Restart.h"
 
 #include <algorithm>
 #include <cmath>
@@ -36,6 +37,51 @@ std::string trim_right(std::string s) {
 
 bool blank(const std::string& s) {
     return std::all_of(s.begin(), s.end(), [](unsigned char c) { return std::isspace(c) != 0; });
+}
+
+// Field width of a "%FORMAT(...)" line: (10I8) -> 8, (5E16.8) -> 16,
+// (20a4) -> 4, (8(F9.5)) -> 9; 0 when none can be read.
+int format_width(const std::string& line) {
+    size_t i = line.find('(');
+    if (i == std::string::npos) return 0;
+    ++i;
+    while (i < line.size() && std::isdigit(static_cast<unsigned char>(line[i]))) ++i;
+    if (i < line.size() && line[i] == '(') ++i;
+    if (i >= line.size() || !std::isalpha(static_cast<unsigned char>(line[i]))) return 0;
+    ++i;
+    int w = 0;
+    while (i < line.size() && std::isdigit(static_cast<unsigned char>(line[i]))) w = 10 * w + (line[i++] - '0');
+    return w;
+}
+
+// The values of one numeric line of a section with field width w.  When
+// every blank-separated run of characters ends on a field boundary (the
+// right-justified layout the FORMAT describes), the line is cut into fields
+// of w columns, which also separates fields that touch ("-1234567-1234567");
+// otherwise (a FORMAT that does not describe the line) the runs are the values.
+void numeric_fields(const std::string& line, int w, std::vector<std::string>& tok) {
+    bool aligned = w > 0;
+    for (size_t i = 0; aligned && i < line.size();) {
+        if (std::isspace(static_cast<unsigned char>(line[i]))) {
+            ++i;
+            continue;
+        }
+        size_t j = i;
+        while (j < line.size() && !std::isspace(static_cast<unsigned char>(line[j]))) ++j;
+        aligned = j % static_cast<size_t>(w) == 0;
+        i = j;
+    }
+    if (aligned) {
+        for (size_t i = 0; i < line.size(); i += static_cast<size_t>(w)) {
+            std::string f = line.substr(i, static_cast<size_t>(w));
+            f.erase(std::remove_if(f.begin(), f.end(), [](unsigned char c) { return std::isspace(c); }), f.end());
+            if (!f.empty()) tok.push_back(f);
+        }
+        return;
+    }
+    std::istringstream ss(line);
+    std::string t;
+    while (ss >> t) tok.push_back(t);
 }
 
 uint64_t pair_key(int a, int b) {
@@ -106,6 +152,7 @@ void AmberTopology::parse_prmtop(const std::string& path) {
     if (!std::getline(in, line) || line.rfind("%VERSION", 0) != 0)
         throw std::runtime_error("Invalid File Format. File does not start with %VERSION.");
     std::string current;
+    int width = 0;  // field width of the current section's %FORMAT
     while (std::getline(in, line)) {
         if (!line.empty() && line.back() == '\r') line.pop_back();
         if (blank(line)) continue;
@@ -115,6 +162,9 @@ void AmberTopology::parse_prmtop(const std::string& path) {
                 current = sp == std::string::npos ? std::string() : trim_right(line.substr(sp + 1));
                 while (!current.empty() && current.front() == ' ') current.erase(current.begin());
                 sections_[current];
+                width = 0;
+            } else if (line.rfind("%FORMAT", 0) == 0) {
+                width = format_width(line);
             }
             continue;
         }
@@ -127,7 +177,11 @@ void AmberTopology::parse_prmtop(const std::string& path) {
                         f.end());
                 tok.push_back(f);
             }
+        } else if (semantics_ == Semantics::AMBER) {
+            // fixed columns per the section's FORMAT where they describe the line
+            numeric_fields(line, width, tok);
         } else {
+            // blank-separated tokens, as the reference splits (IO.cpp:109-136)
             std::istringstream ss(line);
             std::string t;
             while (ss >> t) tok.push_back(t);
@@ -441,6 +495,23 @@ double AmberTopology::get_scee_scale_for_pair(int i, int j) const {
 }
 
 void AmberTopology::read_coords(const std::string& path) {
+    if (semantics_ == Semantics::AMBER) {
+        // fixed-column numbers, velocities and a three- or six-value box
+        const io::Restart r = io::read_rst7(path);
+        if (static_cast<size_t>(r.n_atoms) != atom_list_.size())
+            throw std::runtime_error("rst7 holds " + std::to_string(r.n_atoms) + " atoms, topology has " +
+                                     std::to_string(atom_list_.size()));
+        coordinates_ = r.coords;
+        velocities_ = r.velocities;
+        if (!r.has_box) {
+            box_x_ = box_y_ = box_z_ = 0.0;
+            box_alpha_ = box_beta_ = box_gamma_ = 90.0;
+            return;
+        }
+        box_x_ = r.box[0]; box_y_ = r.box[1]; box_z_ = r.box[2];
+        box_alpha_ = r.box[3]; box_beta_ = r.box[4]; box_gamma_ = r.box[5];
+        return;
+    }
     std::ifstream in(path);
     if (!in) throw std::runtime_error("Error opening file: " + path);
     std::string line;

@@ -185,6 +185,19 @@ def host_lib() -> C.CDLL:
         lib.mdhost_last_error.restype = C.c_char_p
         lib.mdhost_cmap_coefficients.argtypes = [C.c_int, C.c_int, P_F64, P_F64]
         lib.mdhost_cmap_grid_grad.argtypes = [C.c_int, C.c_int, P_F64, P_F64]
+        lib.mdhost_rst7_read.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        lib.mdhost_rst7_info.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int32), P_F64]
+        lib.mdhost_rst7_coords.argtypes = [C.c_void_p]
+        lib.mdhost_rst7_coords.restype = P_F64
+        lib.mdhost_rst7_velocities.argtypes = [C.c_void_p]
+        lib.mdhost_rst7_velocities.restype = P_F64
+        lib.mdhost_rst7_box.argtypes = [C.c_void_p, P_F64]
+        lib.mdhost_rst7_title.argtypes = [C.c_void_p]
+        lib.mdhost_rst7_title.restype = C.c_char_p
+        lib.mdhost_rst7_free.argtypes = [C.c_void_p]
+        lib.mdhost_rst7_write.argtypes = [C.c_char_p, C.c_char_p, C.c_int32, P_F64, P_F64, P_F64, C.c_double,
+                                          C.c_int32]
         _host = lib
     return _host
 
@@ -209,6 +222,47 @@ def cmap_grid_grad(d_coeff, res: int, n_sets: int) -> np.ndarray:
         if st != 0:
             raise ValueError("mdhost_cmap_grid_grad: bad arguments")
     return out
+
+
+def read_rst7(path: str) -> dict:
+    """An Amber restart: title, n_atoms, time (or None), coords (N, 3), velocities
+    (N, 3) in A/ps or None, box (6,) lengths and angles or None."""
+    lib = host_lib()
+    h = C.c_void_p()
+    if lib.mdhost_rst7_read(path.encode(), C.byref(h)) != 0:
+        raise ValueError(f"mdhost_rst7_read: {lib.mdhost_last_error().decode()}")
+    try:
+        n, hv, hb = C.c_int32(), C.c_int32(), C.c_int32()
+        t = C.c_double()
+        lib.mdhost_rst7_info(h, C.byref(n), C.byref(hv), C.byref(hb), C.byref(t))
+        box = np.zeros(6)
+        lib.mdhost_rst7_box(h, box.ctypes.data_as(P_F64))
+        out = {"title": lib.mdhost_rst7_title(h).decode(), "n_atoms": n.value,
+               "coords": _arr(lib.mdhost_rst7_coords(h), 3 * n.value, np.float64).reshape(-1, 3),
+               "velocities": (_arr(lib.mdhost_rst7_velocities(h), 3 * n.value, np.float64).reshape(-1, 3)
+                              if hv.value else None),
+               "box": box if hb.value else None}
+        out["time"] = None if np.isnan(t.value) else t.value
+        return out
+    finally:
+        lib.mdhost_rst7_free(h)
+
+
+def write_rst7(path: str, coords, box=None, velocities=None, title: str = "", time=None):
+    """Write an Amber restart (velocities in A/ps; box: 3 lengths or lengths and angles)."""
+    x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+    v = None if velocities is None else np.ascontiguousarray(velocities, dtype=np.float64).reshape(-1)
+    b = None
+    if box is not None:
+        b = np.asarray(box, dtype=np.float64).reshape(-1)
+        b = np.ascontiguousarray(np.concatenate([b, [90.0, 90.0, 90.0]])[:6] if b.size == 3 else b[:6])
+    lib = host_lib()
+    st = lib.mdhost_rst7_write(path.encode(), title.encode(), x.size // 3, x.ctypes.data_as(P_F64),
+                               v.ctypes.data_as(P_F64) if v is not None else None,
+                               b.ctypes.data_as(P_F64) if b is not None else None,
+                               float(time or 0.0), 1 if time is not None else 0)
+    if st != 0:
+        raise ValueError(f"mdhost_rst7_write: {lib.mdhost_last_error().decode()}")
 
 
 def _arr(ptr, n, dtype):
