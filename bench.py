@@ -116,7 +116,8 @@ def bench_md(torch, N, systems, top, p32, box, steps=200, warmup=20, dt=0.0005):
     """Energy minimisation, then NVE molecular dynamics of the same geometry
     (mixed form, velocity Verlet, dt 0.5 fs, 300 K Maxwell-Boltzmann start,
     seed 31), with the pair list rebuilt every step and as Verlet lists of
-    skin 0.5 and 1.0 A.  The water hydrogens get the CHARMM TIP3P LJ radius
+    skin 0.5 and 1.0 A, and with the full virial (pressure) on a skin of 0.5 A.
+    The water hydrogens get the CHARMM TIP3P LJ radius
     (sigma 0.4 A, eps 0.046 kcal/mol): without it a hydrogen and an unlike
     charge have no repulsion and a minimiser can follow a hydrogen into
     another atom.  Each MD leg starts from the minimised coordinates;
@@ -144,8 +145,9 @@ def bench_md(torch, N, systems, top, p32, box, steps=200, warmup=20, dt=0.0005):
     v0 = torch.tensor((np.random.default_rng(31).normal(size=(m.size, 3)) *
                        np.sqrt(kT * 418.4 / m)[:, None]).reshape(-1), dtype=torch.float64, device=dev)
     legs = {}
-    for skin in (0.0, 0.5, 1.0):
+    for skin, vir in ((0.0, False), (0.5, False), (1.0, False), (0.5, True)):
         ev.set_verlet(skin)
+        ev.set_virial(vir)
         x.copy_(xm)
         v = v0.clone()
         ev.evaluate_device(x.data_ptr(), box, f.data_ptr())
@@ -157,13 +159,15 @@ def bench_md(torch, N, systems, top, p32, box, steps=200, warmup=20, dt=0.0005):
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         b1 = ev.list_builds()
-        et = tr.sum(1)
-        legs[f"skin_{skin:.1f}"] = {
+        et = tr[:, 0] + tr[:, 1]
+        legs[f"skin_{skin:.1f}" + ("_virial" if vir else "")] = {
             "steps_per_s": steps / t, "ns_per_day": steps / t * dt * 86400.0 / 1000.0,
             "list_builds_per_step": (b1[0] - b0[0]) / steps,
             "total_energy_spread_over_mean_kinetic": float((et.max() - et.min()) / tr[:, 1].mean()),
             "temperature_K": float(2.0 * tr[-1, 1] / (3 * m.size * 0.0019872041)),
         }
+        if vir:
+            legs[f"skin_{skin:.1f}_virial"]["mean_pressure_bar"] = float(np.mean(tr[:, 2]))
     ev.close()
     return {"workload": "polymer_29k with CHARMM TIP3P hydrogen LJ, mixed form, AMBER, rc 12 A, PME 72^3; "
                         "L-BFGS (max 300 iterations, "

@@ -322,6 +322,18 @@ int mdgpu_graph_state(const mdgpu_ctx* ctx);
 /* Enable (1) or disable (0) counting of pairs within the cutoff. */
 int mdgpu_set_counting(mdgpu_ctx* ctx, int enable);
 
+/* Full virial and pressure (no counterpart in the reference, which forms no
+ * virial; survey rank 4 of the next steps).  With mdgpu_set_virial(ctx, 1)
+ * each evaluation also forms the virial of the pair, bonded and correction
+ * forces, W_ab = sum over interactions of (relative position)_a (force)_b,
+ * and mdgpu_get_virial returns it plus the reciprocal-space virial of the
+ * PME pass: W = -dE/d(strain), so the pressure tensor is (sum m v v^T + W) / V.
+ * The evaluation then runs the pair kernel's statistics instance (which also
+ * counts pairs).  mdgpu_get_virial: MDGPU_ERR_STATE when the last evaluation
+ * did not form it. */
+int mdgpu_set_virial(mdgpu_ctx* ctx, int enable);
+int mdgpu_get_virial(const mdgpu_ctx* ctx, double virial[9]);
+
 /* Verlet list with a GPU-resident rebuild criterion.  Replaces the
  * reference's skin handling: System::set_lj_skin_ / set_ee_skin
  * (src/System/NonBondedUtils.cpp:11-47), setup_neighbor_rebuild_threshold_lj
@@ -356,13 +368,15 @@ int mdgpu_pairs_need_rebuild(mdgpu_ctx* ctx, const double* d_coords, int* needs)
  * doubles each; on entry d_f must hold the forces at d_x, e.g. from
  * mdgpu_eval_device or a previous run; on return all three hold the final
  * state).  The evaluation settings of the context apply (precision, Verlet
- * list).  trace (host, [nsteps] or NULL) receives each step's potential
- * energy (the evaluation's total) and kinetic energy after the step; last
+ * list, virial).  trace (host, [nsteps] or NULL) receives each step's
+ * potential energy (the evaluation's total), kinetic energy after the step
+ * and, with the virial on, the pressure; last
  * (or NULL) the final step's energies.  A step that needs a larger list
  * restarts the run from the entry state. */
 typedef struct {
     double potential;
     double kinetic;
+    double pressure;   /* bar, (2 KE + trace W) / 3V; NaN unless mdgpu_set_virial is on */
 } mdgpu_md_energy;
 int mdgpu_md_set_masses(mdgpu_ctx* ctx, const double* masses);
 

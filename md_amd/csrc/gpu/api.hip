@@ -810,6 +810,8 @@ void fill_outputs(mdgpu_ctx* c, const double* e, mdgpu_energies* out_e, double* 
                               o.ee14
                         : bonded + o.vdw + o.ee_pairlist;
     }
+    c->have_virial = c->counting || c->virial;
+    for (int k = 0; k < 6; ++k) c->last_virial[k] = e[mdg::E_VIR_XX + k] + e[mdg::E_VSR_XX + k];
     if (out_virial) {
         const double xx = e[mdg::E_VIR_XX], yy = e[mdg::E_VIR_YY], zz = e[mdg::E_VIR_ZZ];
         const double xy = e[mdg::E_VIR_XY], xz = e[mdg::E_VIR_XZ], yz = e[mdg::E_VIR_YZ];
@@ -833,7 +835,7 @@ mdg::GraphKey graph_key(const mdgpu_ctx* c, const double* x, const double* out) 
     k.search_general = c->search_general ? 1 : 0;
     k.split = c->split;
     k.fft_lines = c->fft_lines;
-    k.counting = c->counting ? 1 : 0;
+    k.counting = (c->counting ? 1 : 0) | (c->virial ? 2 : 0);
     k.timing = c->timing.mode;
     k.n_clusters = c->n_clusters;
     k.n_cols = c->n_cols;
@@ -1267,6 +1269,21 @@ int mdgpu_set_graph(mdgpu_ctx* c, int enable) {
 }
 
 int mdgpu_graph_state(const mdgpu_ctx* c) { return c ? c->graph_state : -1; }
+
+int mdgpu_set_virial(mdgpu_ctx* c, int enable) {
+    if (!c) return MDGPU_ERR_ARG;
+    c->virial = enable != 0;
+    return MDGPU_OK;
+}
+
+int mdgpu_get_virial(const mdgpu_ctx* c, double w[9]) {
+    if (!c || !w) return MDGPU_ERR_ARG;
+    if (!c->have_virial) return MDGPU_ERR_STATE;
+    const double* v = c->last_virial;  // xx, yy, zz, xy, xz, yz
+    const double t[9] = {v[0], v[3], v[4], v[3], v[1], v[5], v[4], v[5], v[2]};
+    for (int k = 0; k < 9; ++k) w[k] = t[k];
+    return MDGPU_OK;
+}
 
 int mdgpu_set_verlet(mdgpu_ctx* c, double skin) {
     if (!c) return MDGPU_ERR_ARG;

@@ -42,7 +42,21 @@ struct BArgs {
     int ld;            // rows allocated in the slot-major partial buffer
     int row0;
     int amber;
+    int virial;        // also form the virial of the bonded forces
 };
+
+// Virial of forces f_k on the atoms of one term at positions r_k relative to
+// one atom of the term (sum_k r_k f_k^T, symmetric for these terms): xx, yy,
+// zz, xy, xz, yz.
+template <typename real>
+__device__ __forceinline__ void vir_add(double* w, const R3<real>& r, const R3<real>& f) {
+    w[0] += static_cast<double>(r.x * f.x);
+    w[1] += static_cast<double>(r.y * f.y);
+    w[2] += static_cast<double>(r.z * f.z);
+    w[3] += static_cast<double>(r.x * f.y);
+    w[4] += static_cast<double>(r.x * f.z);
+    w[5] += static_cast<double>(r.y * f.z);
+}
 
 template <typename real>
 __device__ __forceinline__ void addf(const BArgs<real>& a, int atom, real fx, real fy, real fz) {
@@ -77,7 +91,8 @@ __device__ __forceinline__ real dihedral_angle(const R3<real>& b1, const R3<real
 // degenerate.
 template <typename real>
 __device__ __forceinline__ bool torsion(const BArgs<real>& a, int ia, int ib, int ic, int id, const R3<real>& ba,
-                                        const R3<real>& cb, const R3<real>& dc, real tau, bool apply) {
+                                        const R3<real>& cb, const R3<real>& dc, real tau, bool apply,
+                                        double* w) {
     const R3<real> m = cross(ba, cb), n = cross(cb, dc);
     const real m2 = dot(m, m), n2 = dot(n, n), c2 = dot(cb, cb);
     const real eps = static_cast<real>(1e-24);
@@ -87,16 +102,20 @@ __device__ __forceinline__ bool torsion(const BArgs<real>& a, int ia, int ib, in
     const real sa = tau * lc / m2, sd = -tau * lc / n2;
     const R3<real> fa = {sa * m.x, sa * m.y, sa * m.z}, fd = {sd * n.x, sd * n.y, sd * n.z};
     const real pb = dot(ba, cb) / c2, pd = dot(dc, cb) / c2;
-    if (a.amber) {
-        addf(a, ia, -fa.x, -fa.y, -fa.z);
-        addf(a, id, -fd.x, -fd.y, -fd.z);
-        addf(a, ib, (pb + 1) * fa.x - pd * fd.x, (pb + 1) * fa.y - pd * fd.y, (pb + 1) * fa.z - pd * fd.z);
-        addf(a, ic, (pd + 1) * fd.x - pb * fa.x, (pd + 1) * fd.y - pb * fa.y, (pd + 1) * fd.z - pb * fa.z);
-    } else {
-        addf(a, ia, fa.x, fa.y, fa.z);
-        addf(a, id, fd.x, fd.y, fd.z);
-        addf(a, ib, (pb - 1) * fa.x - pd * fd.x, (pb - 1) * fa.y - pd * fd.y, (pb - 1) * fa.z - pd * fd.z);
-        addf(a, ic, (pd - 1) * fd.x - pb * fa.x, (pd - 1) * fd.y - pb * fa.y, (pd - 1) * fd.z - pb * fa.z);
+    const real sg = a.amber ? static_cast<real>(-1) : static_cast<real>(1);
+    const real cbf = a.amber ? pb + 1 : pb - 1, cdf = a.amber ? pd + 1 : pd - 1;
+    const R3<real> Fa = {sg * fa.x, sg * fa.y, sg * fa.z}, Fd = {sg * fd.x, sg * fd.y, sg * fd.z};
+    const R3<real> Fb = {cbf * fa.x - pd * fd.x, cbf * fa.y - pd * fd.y, cbf * fa.z - pd * fd.z};
+    const R3<real> Fc = {cdf * fd.x - pb * fa.x, cdf * fd.y - pb * fa.y, cdf * fd.z - pb * fa.z};
+    addf(a, ia, Fa.x, Fa.y, Fa.z);
+    addf(a, id, Fd.x, Fd.y, Fd.z);
+    addf(a, ib, Fb.x, Fb.y, Fb.z);
+    addf(a, ic, Fc.x, Fc.y, Fc.z);
+    if (a.virial) {
+        // positions relative to atom b: x_a - x_b = ba, x_c - x_b = -cb, x_d - x_b = -(cb + dc)
+        vir_add(w, ba, Fa);
+        vir_add(w, R3<real>{-cb.x, -cb.y, -cb.z}, Fc);
+        vir_add(w, R3<real>{-cb.x - dc.x, -cb.y - dc.y, -cb.z - dc.z}, Fd);
     }
     return true;
 }
@@ -118,7 +137,8 @@ __device__ __forceinline__ void bonded_terms(const BArgs<real>& a, int k, int ro
     const double* x = a.x;
     const int nb = t.n_bonds, nu = nb + t.n_ubs, na = nu + t.n_angles, nd = na + t.n_dihedrals,
               ni = nd + t.n_impropers, nc = ni + t.n_cmaps;
-    double e[6] = {0, 0, 0, 0, 0, 0};  // bond, angle, dihedral, UB, improper, CMAP
+    double e[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // bond, angle, dihedral, UB, improper, CMAP, virial
+    double* w = e + 6;
     if (k < nu) {
         // harmonic bond / Urey-Bradley: E = k (r - r0)^2, F_a = -2k (r - r0) d / r
         const bool ub = k >= nb;
@@ -133,6 +153,7 @@ __device__ __forceinline__ void bonded_terms(const BArgs<real>& a, int k, int ro
             const real s = static_cast<real>(-2.0 * par.x) * dr / r;
             addf(a, at.x, s * d.x, s * d.y, s * d.z);
             addf(a, at.y, -s * d.x, -s * d.y, -s * d.z);
+            if (a.virial) vir_add(w, d, R3<real>{s * d.x, s * d.y, s * d.z});
         }
     } else if (k < na) {
         // harmonic angle (BondedEnergy.cpp:30-49, BondedForces.cpp:73-153)
@@ -163,6 +184,10 @@ __device__ __forceinline__ void bonded_terms(const BArgs<real>& a, int k, int ro
             addf(a, at.x, fa.x, fa.y, fa.z);
             addf(a, at.z, fc.x, fc.y, fc.z);
             addf(a, at.y, -fa.x - fc.x, -fa.y - fc.y, -fa.z - fc.z);
+            if (a.virial) {
+                vir_add(w, ba, fa);
+                vir_add(w, bc, fc);
+            }
         }
     } else if (k < ni) {
         // cosine dihedral (k < nd) or harmonic improper
@@ -185,7 +210,7 @@ __device__ __forceinline__ void bonded_terms(const BArgs<real>& a, int k, int ro
             tau = static_cast<real>(-2.0 * par.x) * dw;
         }
         const R3<real> ba = {-b1.x, -b1.y, -b1.z}, cb = {-b2.x, -b2.y, -b2.z}, dc = {-b3.x, -b3.y, -b3.z};
-        torsion(a, at.x, at.y, at.z, at.w, ba, cb, dc, tau, true);
+        torsion(a, at.x, at.y, at.z, at.w, ba, cb, dc, tau, true, w);
     } else if (k < nc) {
         // CMAP (BondedEnergy.cpp:124-166, CMapGroup.cpp:82-237,
         // BondedForces.cpp:334-479)
@@ -226,15 +251,16 @@ __device__ __forceinline__ void bonded_terms(const BArgs<real>& a, int k, int ro
         const R3<real> ba = {-b1.x, -b1.y, -b1.z}, cb = {-b2.x, -b2.y, -b2.z}, dc = {-b3.x, -b3.y, -b3.z},
                        ed = {-b4.x, -b4.y, -b4.z};
         // the reference skips both halves if any of the five products is degenerate
-        const bool ok = torsion(a, at[0], at[1], at[2], at[3], ba, cb, dc, -dphi, false) &&
-                        torsion(a, at[1], at[2], at[3], at[4], cb, dc, ed, -dpsi, false);
+        const bool ok = torsion(a, at[0], at[1], at[2], at[3], ba, cb, dc, -dphi, false, w) &&
+                        torsion(a, at[1], at[2], at[3], at[4], cb, dc, ed, -dpsi, false, w);
         if (ok) {
-            torsion(a, at[0], at[1], at[2], at[3], ba, cb, dc, -dphi, true);
-            torsion(a, at[1], at[2], at[3], at[4], cb, dc, ed, -dpsi, true);
+            torsion(a, at[0], at[1], at[2], at[3], ba, cb, dc, -dphi, true, w);
+            torsion(a, at[1], at[2], at[3], at[4], cb, dc, ed, -dpsi, true, w);
         }
     }
-    const int slots[6] = {E_BOND, E_ANGLE, E_DIHEDRAL, E_UB, E_IMPROPER, E_CMAP};
-    block_sum_store<6>(e, a.partial, a.ld, row, slots);
+    const int slots[12] = {E_BOND, E_ANGLE, E_DIHEDRAL, E_UB, E_IMPROPER, E_CMAP,
+                           E_VSR_XX, E_VSR_YY, E_VSR_ZZ, E_VSR_XY, E_VSR_XZ, E_VSR_YZ};
+    block_sum_store<12>(e, a.partial, a.ld, row, slots);
 }
 
 struct CArgs {
@@ -257,6 +283,7 @@ struct CArgs {
     double* partial;
     int ld;
     int row0;
+    int virial;               // also form the virial of the correction forces
 };
 
 __device__ __forceinline__ void addc(const CArgs& a, int atom, double fx, double fy, double fz) {
@@ -273,7 +300,7 @@ __device__ __forceinline__ void addc(const CArgs& a, int atom, double fx, double
 
 // Pair k of the excluded pairs followed by the 1-4 pairs.
 __device__ __forceinline__ void correction_pairs(const CArgs& a, int k, int row) {
-    double e[3] = {0, 0, 0};  // ewald_excl, vdw14, ee14
+    double e[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // ewald_excl, vdw14, ee14, virial
     if (k < a.n_excl + a.n14) {
         const bool is14 = k >= a.n_excl;
         const int2 p = is14 ? a.p14[k - a.n_excl] : a.excl[k];
@@ -304,9 +331,17 @@ __device__ __forceinline__ void correction_pairs(const CArgs& a, int k, int row)
         }
         addc(a, p.x, g * dx, g * dy, g * dz);
         addc(a, p.y, -g * dx, -g * dy, -g * dz);
+        if (a.virial) {
+            e[3] = g * dx * dx;
+            e[4] = g * dy * dy;
+            e[5] = g * dz * dz;
+            e[6] = g * dx * dy;
+            e[7] = g * dx * dz;
+            e[8] = g * dy * dz;
+        }
     }
-    const int slots[3] = {E_EW_EXCL, E_VDW14, E_EE14};
-    block_sum_store<3>(e, a.partial, a.ld, row, slots);
+    const int slots[9] = {E_EW_EXCL, E_VDW14, E_EE14, E_VSR_XX, E_VSR_YY, E_VSR_ZZ, E_VSR_XY, E_VSR_XZ, E_VSR_YZ};
+    block_sum_store<9>(e, a.partial, a.ld, row, slots);
 }
 
 // One launch for both: blocks [0, nbb) take the bonded terms, the rest the

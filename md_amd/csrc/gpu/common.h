@@ -21,7 +21,10 @@ enum EnergySlot {
     E_BOND = 0, E_ANGLE, E_DIHEDRAL, E_UB, E_IMPROPER, E_CMAP,
     E_VDW, E_EE_ALL, E_EE_LIST, E_EE_RF, E_EW_DIRECT,
     E_EW_RECIP, E_EW_EXCL, E_VDW14, E_EE14,
-    E_VIR_XX, E_VIR_YY, E_VIR_ZZ, E_VIR_XY, E_VIR_XZ, E_VIR_YZ,
+    E_VIR_XX, E_VIR_YY, E_VIR_ZZ, E_VIR_XY, E_VIR_XZ, E_VIR_YZ,   // reciprocal-space virial
+    // virial of the pair, bonded and correction forces (statistics runs only):
+    // W_ab = sum over interactions of (relative position)_a (force)_b
+    E_VSR_XX, E_VSR_YY, E_VSR_ZZ, E_VSR_XY, E_VSR_XZ, E_VSR_YZ,
     E_PAIRS,   // number of atom pairs inside the cutoff (work count)
     E_TILES,   // number of cluster-pair tiles (work count)
     E_NSLOTS

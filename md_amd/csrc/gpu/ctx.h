@@ -198,6 +198,9 @@ struct mdgpu_ctx {
     mdg::AdjointBuf adj;
     std::vector<double> host_charge;
     bool counting = false;
+    bool virial = false;            // statistics runs also form the full virial
+    bool have_virial = false;       // the last evaluation formed it
+    double last_virial[6] = {};     // its xx, yy, zz, xy, xz, yz (recip + pair + bonded + corrections)
     long long counts[4] = {0, 0, 0, 0};
     int n_clusters = 0;
     int n_cols = 0;

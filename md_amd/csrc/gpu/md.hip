@@ -237,6 +237,10 @@ int mdgpu_md_run(mdgpu_ctx* c, double* d_x, double* d_v, double* d_f, const doub
             continue;
         }
         mdg::ctx_steps_done(c, tr.data(), mdg::kTraceRow, nsteps);
+        // pressure (bar) from the virial when the evaluations formed it:
+        // P = (2 KE + trace W) / (3 V), 1 kcal/mol/A^3 = 69476.95 bar
+        const bool stats = c->counting || c->virial;
+        const double vol = box[0] * box[1] * box[2];
         for (int k = 0; k < nsteps; ++k) {
             const double* row = &tr[static_cast<size_t>(k) * mdg::kTraceRow];
             mdgpu_energies e{};
@@ -244,6 +248,10 @@ int mdgpu_md_run(mdgpu_ctx* c, double* d_x, double* d_v, double* d_f, const doub
             if (trace_out) {
                 trace_out[k].potential = e.total;
                 trace_out[k].kinetic = row[mdg::kRow];
+                double w = 0.0;
+                for (int d = 0; d < 3; ++d) w += row[mdg::E_VIR_XX + d] + row[mdg::E_VSR_XX + d];
+                trace_out[k].pressure =
+                    stats ? (2.0 * row[mdg::kRow] + w) / (3.0 * vol) * 69476.95 : std::nan("");
             }
             if (last && k == nsteps - 1) *last = e;
         }

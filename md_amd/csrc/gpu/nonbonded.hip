@@ -219,6 +219,9 @@ __global__ __launch_bounds__(kWavesOf<real> * kWave) void k_nonbonded(const NbAr
     // double every 32 tiles
     double e_vdw = 0, e_all = 0, e_list = 0, e_rf = 0, e_ew = 0, n_pairs = 0;
     real s_vdw = 0, s_all = 0, s_list = 0, s_rf = 0, s_ew = 0;
+    // statistics instance: virial of the pair forces, sum f d_a d_b
+    double w_d[6] = {0, 0, 0, 0, 0, 0};
+    real w_s[6] = {0, 0, 0, 0, 0, 0};
     const real tsp = static_cast<real>(1.1283791670955126) * a.alpha;  // 2 alpha / sqrt(pi)
     const real rc2 = a.rc2_lj > a.rc2_ee ? a.rc2_lj : a.rc2_ee;
     int flush = 0;
@@ -317,6 +320,14 @@ __global__ __launch_bounds__(kWavesOf<real> * kWave) void k_nonbonded(const NbAr
                     }
                 }
                 const real fx = f * dx, fy = f * dy, fz = f * dz;
+                if constexpr (kCount) {
+                    w_s[0] += fx * dx;
+                    w_s[1] += fy * dy;
+                    w_s[2] += fz * dz;
+                    w_s[3] += fx * dy;
+                    w_s[4] += fx * dz;
+                    w_s[5] += fy * dz;
+                }
                 fix += fx;
                 fiy += fy;
                 fiz += fz;
@@ -335,6 +346,12 @@ __global__ __launch_bounds__(kWavesOf<real> * kWave) void k_nonbonded(const NbAr
                     flush = 0;
                     e_vdw += s_vdw; e_all += s_all; e_list += s_list; e_rf += s_rf; e_ew += s_ew;
                     s_vdw = s_all = s_list = s_rf = s_ew = 0;
+                    if constexpr (kCount) {
+                        for (int q = 0; q < 6; ++q) {
+                            w_d[q] += static_cast<double>(w_s[q]);
+                            w_s[q] = 0;
+                        }
+                    }
                 }
             };
 
@@ -358,11 +375,19 @@ __global__ __launch_bounds__(kWavesOf<real> * kWave) void k_nonbonded(const NbAr
         add_force<real, kAmber>(a, rsrc, tj < 3 && k1 > k0 && mi.y >= 0, 3 * mi.y + tj, tj == 0 ? fix : (tj == 1 ? fiy : fiz));
     }
     e_vdw += s_vdw; e_all += s_all; e_list += s_list; e_rf += s_rf; e_ew += s_ew;
-    double v[6] = {e_vdw, e_all, e_list, e_rf, e_ew, n_pairs};
-    const int slots[6] = {E_VDW, E_EE_ALL, E_EE_LIST, E_EE_RF, E_EW_DIRECT, E_PAIRS};
     // row by logical block: the 16 rows sharing a cache line of a slot come
     // from blocks of one XCD
-    block_sum_store<6>(v, a.partial, a.ld, a.row0 + blk, slots);
+    if constexpr (kCount) {
+        double v[12] = {e_vdw, e_all, e_list, e_rf, e_ew, n_pairs};
+        for (int q = 0; q < 6; ++q) v[6 + q] = w_d[q] + static_cast<double>(w_s[q]);
+        const int slots[12] = {E_VDW, E_EE_ALL, E_EE_LIST, E_EE_RF, E_EW_DIRECT, E_PAIRS,
+                               E_VSR_XX, E_VSR_YY, E_VSR_ZZ, E_VSR_XY, E_VSR_XZ, E_VSR_YZ};
+        block_sum_store<12>(v, a.partial, a.ld, a.row0 + blk, slots);
+    } else {
+        double v[6] = {e_vdw, e_all, e_list, e_rf, e_ew, n_pairs};
+        const int slots[6] = {E_VDW, E_EE_ALL, E_EE_LIST, E_EE_RF, E_EW_DIRECT, E_PAIRS};
+        block_sum_store<6>(v, a.partial, a.ld, a.row0 + blk, slots);
+    }
 }
 
 template <typename real>
@@ -412,9 +437,12 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0, const double* coords) {
     const int nbb = bonded_rows(c);
     const int ncb = amber ? corrections_rows(c) : 0;
     const FixedAcc fxa{w.force_fx, fixed_scale(p.semantics), w.flags};
+    // statistics runs (pair counting, virial) use the counting instance of the pair kernel
+    const bool stats = c->counting || c->virial;
     const BArgs<real> ba{coords, t, sizeof(real) == 4 ? reinterpret_cast<const real*>(t.cmap_coeff_f)
                                                       : reinterpret_cast<const real*>(t.cmap_coeff),
-                         w.force_d, fxa, w.partial, static_cast<int>(w.cap_partial), row0 + npb, amber ? 1 : 0};
+                         w.force_d, fxa, w.partial, static_cast<int>(w.cap_partial), row0 + npb, amber ? 1 : 0,
+                         stats ? 1 : 0};
     CArgs ca{};
     ca.x = coords;
     ca.lx = c->grid.box[0];
@@ -437,11 +465,12 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0, const double* coords) {
     ca.partial = w.partial;
     ca.ld = static_cast<int>(w.cap_partial);
     ca.row0 = row0 + npb + nbb;
+    ca.virial = stats ? 1 : 0;
     const size_t shm = (sizeof(typename V4<real>::T) + 1) * a.ntp * a.ntp;
     dim3 grid(npb + nbb + ncb), block(kWavesOf<real> * kWave);
 #define MDG_NB_LAUNCH(AM, SW)                                                                                  \
     do {                                                                                                      \
-        if (c->counting)                                                                                      \
+        if (stats)                                                                                            \
             hipLaunchKernelGGL((k_nonbonded<real, AM, SW, true>), grid, block, shm, c->stream, a, ba, ca, npb, \
                                nbb);                                                                          \
         else                                                                                                  \
@@ -475,8 +504,9 @@ const void* pair_fn(bool amber, bool sw, bool counting) {
 
 const void* pair_kernel_fn(const mdgpu_ctx* c) {
     const bool amber = c->params.semantics == MDGPU_SEM_AMBER, sw = c->params.lj_switch > 0.0;
-    return c->params.precision == MDGPU_PREC_MIXED ? pair_fn<float>(amber, sw, c->counting)
-                                                   : pair_fn<double>(amber, sw, c->counting);
+    const bool stats = c->counting || c->virial;
+    return c->params.precision == MDGPU_PREC_MIXED ? pair_fn<float>(amber, sw, stats)
+                                                   : pair_fn<double>(amber, sw, stats);
 }
 
 int pair_block_threads(const mdgpu_ctx* c) {

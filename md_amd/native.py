@@ -143,6 +143,8 @@ def gpu_lib() -> C.CDLL:
         lib.mdgpu_set_graph.argtypes = [C.c_void_p, C.c_int]
         lib.mdgpu_graph_state.argtypes = [C.c_void_p]
         lib.mdgpu_set_verlet.argtypes = [C.c_void_p, C.c_double]
+        lib.mdgpu_set_virial.argtypes = [C.c_void_p, C.c_int]
+        lib.mdgpu_get_virial.argtypes = [C.c_void_p, P_F64]
         lib.mdgpu_md_set_masses.argtypes = [C.c_void_p, P_F64]
         lib.mdgpu_minimize.argtypes = [C.c_void_p, C.c_void_p, P_F64, C.POINTER(MinOpts), C.POINTER(MinResult)]
         lib.mdgpu_md_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, P_F64, C.c_double, C.c_int32,
@@ -564,9 +566,10 @@ class Evaluator:
 
     def md_run(self, x_ptr: int, v_ptr: int, f_ptr: int, box, dt: float, nsteps: int):
         """Velocity Verlet on the device state (3N doubles each; f holds the forces
-        at x on entry).  Returns (trace (nsteps, 2): potential, kinetic; last energies)."""
+        at x on entry).  Returns (trace (nsteps, 3): potential, kinetic, pressure (bar,
+        NaN unless the virial is on); last energies)."""
         b = np.ascontiguousarray(box, dtype=np.float64).reshape(-1)[:3]
-        tr = np.zeros((max(int(nsteps), 1), 2))
+        tr = np.zeros((max(int(nsteps), 1), 3))
         e = Energies()
         self._check(self.lib.mdgpu_md_run(self.ctx, C.c_void_p(x_ptr), C.c_void_p(v_ptr), C.c_void_p(f_ptr),
                                           b.ctypes.data_as(P_F64), float(dt), int(nsteps),
@@ -575,6 +578,16 @@ class Evaluator:
 
     def set_counting(self, on: bool):
         self._check(self.lib.mdgpu_set_counting(self.ctx, 1 if on else 0))
+
+    def set_virial(self, on: bool):
+        """Form the full virial (pair, bonded, correction and reciprocal) in each evaluation."""
+        self._check(self.lib.mdgpu_set_virial(self.ctx, 1 if on else 0))
+
+    def virial(self) -> np.ndarray:
+        """Full virial W = -dE/d(strain) of the last evaluation (3, 3), kcal/mol."""
+        w = np.zeros(9)
+        self._check(self.lib.mdgpu_get_virial(self.ctx, w.ctypes.data_as(P_F64)))
+        return w.reshape(3, 3)
 
     def counts(self) -> dict:
         c = (C.c_int64 * 4)()

@@ -17,7 +17,7 @@ _lib = None
 
 
 class Opts(C.Structure):
-    _fields_ = [("list_skin", C.c_double), ("skip_all_pairs", C.c_int)]
+    _fields_ = [("list_skin", C.c_double), ("skip_all_pairs", C.c_int), ("full_virial", C.c_int)]
 
 
 def lib() -> C.CDLL:
@@ -59,14 +59,16 @@ def _p(a):
 
 
 def evaluate(desc: SystemDesc, params: Params, coords, box, list_skin=2.0, skip_all_pairs=False,
-             want_forces=True):
+             want_forces=True, full_virial=False):
     """Returns (energies dict, forces (N,3) or None, virial (3,3))."""
     x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
     b = np.ascontiguousarray(box, dtype=np.float64)
     f = np.zeros(x.size) if want_forces else None
     v = np.zeros(9)
     e = Energies()
-    o = Opts(list_skin, 1 if skip_all_pairs else 0)
+    o = Opts(list_skin, 1 if skip_all_pairs else 0, 1 if full_virial else 0)
+    if full_virial and f is None:
+        f = np.zeros(x.size)
     st = lib().oracle_eval(C.byref(desc), C.byref(params), _p(x), _p(b), C.byref(o), C.byref(e),
                            _p(f) if f is not None else None, _p(v))
     if st != 0:

@@ -126,6 +126,19 @@ static void addf(double* f, int i, double x, double y, double z) {
     f[3 * i + 2] += z;
 }
 
+/* Pair virial sum g d_a d_b of the pair forces (full-virial evaluations):
+ * the pair routines add into it when it is set. */
+static double* g_wpair = NULL;
+
+/* Force g d on a and -g d on b (d = x_a - x_b at its minimum image). */
+static void pair_force(double* f, int a, int b, double g, const double* d) {
+    addf(f, a, g * d[0], g * d[1], g * d[2]);
+    addf(f, b, -g * d[0], -g * d[1], -g * d[2]);
+    if (g_wpair)
+        for (int u = 0; u < 3; ++u)
+            for (int v = 0; v < 3; ++v) g_wpair[3 * u + v] += g * d[u] * d[v];
+}
+
 /* BondedEnergy.cpp:10-28 and BondedForces.cpp:10-39 (bonds); the same form
  * for Urey-Bradley, BondedEnergy.cpp:74-95 / BondedForces.cpp:41-71. */
 static double harmonic_pairs(int n, const int32_t* at, const int32_t* ty, const double* k, const double* r0,
@@ -563,8 +576,7 @@ static void ref_nonbonded(const mdgpu_system_desc* s, const mdgpu_params* p, con
                 g -= gcut * (rc_lj / r);
             }
             if (f && r >= 1e-12) {
-                addf(f, a, g * d[0], g * d[1], g * d[2]);
-                addf(f, b, -g * d[0], -g * d[1], -g * d[2]);
+                pair_force(f, a, b, g, d);
             }
         }
         /* calculate_EE_energy_pairlist (:176-204), _with_cutoff (:206-234),
@@ -585,8 +597,7 @@ static void ref_nonbonded(const mdgpu_system_desc* s, const mdgpu_params* p, con
             }
             if (f && r >= 1e-12) {
                 double g = (1 / scale) * (double)(((qa * qb) / (r * r * r * 1.0)) * SQRT_KC * SQRT_KC);
-                addf(f, a, g * d[0], g * d[1], g * d[2]);
-                addf(f, b, -g * d[0], -g * d[1], -g * d[2]);
+                pair_force(f, a, b, g, d);
             }
         }
     }
@@ -720,8 +731,7 @@ static void amber_nonbonded(const mdgpu_system_desc* s, const mdgpu_params* p, c
                     g += qq * (ec / r + tsp * exp(-a2 * r2)) / r2;
                 }
                 if (f) {
-                    addf(f, i, g * d[0], g * d[1], g * d[2]);
-                    addf(f, j, -g * d[0], -g * d[1], -g * d[2]);
+                    pair_force(f, i, j, g, d);
                 }
             }
         }
@@ -742,8 +752,7 @@ static void amber_nonbonded(const mdgpu_system_desc* s, const mdgpu_params* p, c
         eex -= qq * ef / r;
         if (f) {
             double g = qq * (tsp * exp(-a2 * r2) - ef / r) / r2;
-            addf(f, i, g * d[0], g * d[1], g * d[2]);
-            addf(f, j, -g * d[0], -g * d[1], -g * d[2]);
+            pair_force(f, i, j, g, d);
         }
     }
     e->ewald_excl = eex;
@@ -767,8 +776,7 @@ static void amber_nonbonded(const mdgpu_system_desc* s, const mdgpu_params* p, c
         ee14 += qq * (is / r - ef / r);
         g += qq * (is / r + tsp * exp(-a2 * r2) - ef / r) / r2;
         if (f) {
-            addf(f, i, g * d[0], g * d[1], g * d[2]);
-            addf(f, j, -g * d[0], -g * d[1], -g * d[2]);
+            pair_force(f, i, j, g, d);
         }
     }
     e->vdw14 = ev14;
@@ -1038,7 +1046,7 @@ double oracle_ewald_recip_dense(int n, const double* q, const double* x, const d
 
 int oracle_eval(const mdgpu_system_desc* s, const mdgpu_params* p, const double* x, const double box[3],
                 const oracle_opts* opt, mdgpu_energies* e, double* f, double* vir) {
-    oracle_opts dflt = {2.0, 0};
+    oracle_opts dflt = {2.0, 0, 0};
     if (!opt) opt = &dflt;
     if (!s || !p || !x || !e) return 1;
     memset(e, 0, sizeof *e);
@@ -1047,13 +1055,28 @@ int oracle_eval(const mdgpu_system_desc* s, const mdgpu_params* p, const double*
     if (vir) memset(vir, 0, sizeof(double) * 9);
     int amber = p->semantics == MDGPU_SEM_AMBER;
     g_amber_torsion = amber;
+    /* full virial: bonded forces into their own array (each term's forces sum
+     * to zero, so sum_i x_i f_i is its virial with the coordinates as given),
+     * pair forces through pair_force, the reciprocal part from the PME pass */
+    const int full = opt->full_virial && vir && f;
+    double* fb = full ? (double*)calloc(3 * (size_t)n, sizeof(double)) : f;
+    double wpair[9] = {0};
     /* calculate_energies / calculate_forces (System.cpp:9-39) */
-    e->bond = harmonic_pairs(s->n_bonds, s->bond_atoms, s->bond_type, s->bond_k, s->bond_r0, x, f);
-    e->angle = angles(s, x, f);
-    e->dihedral = dihedrals(s, x, f);
-    e->urey_bradley = harmonic_pairs(s->n_ubs, s->ub_atoms, s->ub_type, s->ub_k, s->ub_r0, x, f);
-    e->improper = impropers(s, x, f, amber);
-    e->cmap = cmaps(s, x, f);
+    e->bond = harmonic_pairs(s->n_bonds, s->bond_atoms, s->bond_type, s->bond_k, s->bond_r0, x, fb);
+    e->angle = angles(s, x, fb);
+    e->dihedral = dihedrals(s, x, fb);
+    e->urey_bradley = harmonic_pairs(s->n_ubs, s->ub_atoms, s->ub_type, s->ub_k, s->ub_r0, x, fb);
+    e->improper = impropers(s, x, fb, amber);
+    e->cmap = cmaps(s, x, fb);
+    if (full) {
+        for (int i = 0; i < n; ++i)
+            for (int u = 0; u < 3; ++u) {
+                f[3 * i + u] += fb[3 * i + u];
+                for (int v = 0; v < 3; ++v) vir[3 * u + v] += x[3 * i + u] * fb[3 * i + v];
+            }
+        free(fb);
+        g_wpair = wpair;
+    }
     /* calculate_EE_ewald_self_term (NonBondedEnergy.cpp:307-316) */
     {
         const double kc = (double)(SQRT_KC * SQRT_KC);
@@ -1065,13 +1088,22 @@ int oracle_eval(const mdgpu_system_desc* s, const mdgpu_params* p, const double*
     if (!amber) {
         ref_nonbonded(s, p, x, box, opt, e, f);
         e->total = bonded + e->vdw + e->ee_pairlist;
-        return 0;
+    } else {
+        amber_nonbonded(s, p, x, box, e, f);
+        if (p->pme_grid[0] > 0) {
+            double wr[9];
+            e->ewald_recip = oracle_pme_recip(n, s->charge, x, box, p->ewald_alpha, p->pme_grid, p->pme_order, f,
+                                              vir ? wr : NULL);
+            if (vir)
+                for (int k = 0; k < 9; ++k) vir[k] += wr[k];
+        }
+        e->total = bonded + e->vdw + e->vdw14 + e->ewald_direct + e->ewald_recip + e->ewald_self + e->ewald_excl +
+                   e->ee14;
     }
-    amber_nonbonded(s, p, x, box, e, f);
-    if (p->pme_grid[0] > 0) {
-        e->ewald_recip = oracle_pme_recip(n, s->charge, x, box, p->ewald_alpha, p->pme_grid, p->pme_order, f, vir);
+    if (full) {
+        g_wpair = NULL;
+        for (int k = 0; k < 9; ++k) vir[k] += wpair[k];
     }
-    e->total = bonded + e->vdw + e->vdw14 + e->ewald_direct + e->ewald_recip + e->ewald_self + e->ewald_excl + e->ee14;
     return 0;
 }
 

@@ -21,6 +21,7 @@ extern "C" {
 typedef struct {
     double list_skin;      /* REF pair-list skin (reference default 2 A)       */
     int skip_all_pairs;    /* 1: skip the O(N^2) "EE" pass (ee_all = NaN)      */
+    int full_virial;       /* 1: virial = bonded + pair + correction + recip   */
 } oracle_opts;
 
 /* One evaluation.  Returns 0 on success. */

@@ -157,7 +157,7 @@ def test_mixed_reproducible_and_reversible(polymer_2k):
         finally:
             md.close()
     assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
-    assert np.array_equal(runs[0][2], runs[1][2])
+    assert np.array_equal(runs[0][2], runs[1][2], equal_nan=True)
 
 
 def test_overflow_restart(water_6k):
