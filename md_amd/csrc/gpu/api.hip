@@ -66,6 +66,7 @@ void free_work(mdgpu_ctx* c) {
                     w.slot_atom, w.slot_meta, w.slot_pos_d, w.slot_pos_f, w.slot_pme_fl, w.slot_pme_wd,
                     w.slot_pme_wf, w.cluster_col, w.cl_origin, w.cl_half,
                     w.list_count, w.list_j, w.list_mask, w.list_delta_d, w.list_delta_f, w.force_d, w.force_fx,
+                    w.force_d_s, w.force_fx_s, w.atom_slot,
                     w.partial, w.energies, w.force_out, w.flags, w.excl_rmax, w.grid_f, w.grid_phi_d,
                     w.cgrid, w.bsp_mod, w.bsp_mod_f, w.twiddle_d, w.twiddle_f, w.x_ref, w.verlet};
     for (void* p : ptrs)
@@ -515,10 +516,12 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     size_t cap;
     cap = w.cap_atoms;
     if (w.cap_atoms < static_cast<size_t>(n)) {
-        void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_key, w.force_d, w.force_fx, w.force_out, w.x_ref};
+        void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_key, w.force_d, w.force_fx, w.force_out, w.x_ref,
+                        w.atom_slot};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         w.coords = w.wrapped = w.force_d = w.force_out = w.x_ref = nullptr;
+        w.atom_slot = nullptr;
         w.col_of = nullptr;
         w.col_key = nullptr;
         w.force_fx = nullptr;
@@ -530,6 +533,8 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_fx), 3 * n * sizeof(unsigned long long)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_out), 3 * n * sizeof(double)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.x_ref), 3 * n * sizeof(double)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.atom_slot), n * sizeof(int)), "alloc");
+        HIP_TRY(hipMemset(w.atom_slot, 0xff, n * sizeof(int)), "clear");  // -1: no slot yet
         c->verlet_force = true;  // no list positions to compare with
         // the accumulators start at zero; every evaluation's reduction re-zeroes them
         HIP_TRY(hipMemset(w.force_d, 0, 3 * n * sizeof(double)), "clear");
@@ -554,7 +559,8 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     const size_t ncl = c->n_clusters, nslots = ncl * mdg::kCluster;
     if (w.cap_slots < nslots) {
         void* ptrs[] = {w.slot_atom, w.slot_meta, w.slot_pos_d, w.slot_pos_f, w.slot_pme_fl, w.slot_pme_wd,
-                        w.slot_pme_wf, w.cluster_col, w.cl_origin, w.cl_half, w.list_count};
+                        w.slot_pme_wf, w.cluster_col, w.cl_origin, w.cl_half, w.list_count, w.force_d_s,
+                        w.force_fx_s};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.slot_atom), nslots * sizeof(int)), "alloc");
@@ -568,6 +574,12 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.cl_origin), ncl * sizeof(double4)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.cl_half), ncl * sizeof(float4)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.list_count), ncl * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_d_s), 3 * nslots * sizeof(double)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_fx_s), 3 * nslots * sizeof(unsigned long long)), "alloc");
+        // start at zero; every evaluation's reduction re-zeroes what it read
+        HIP_TRY(hipMemset(w.force_d_s, 0, 3 * nslots * sizeof(double)), "clear");
+        HIP_TRY(hipMemset(w.force_fx_s, 0, 3 * nslots * sizeof(unsigned long long)), "clear");
+        c->verlet_force = true;  // new slot arrays: the slot map must be rebuilt
         w.cap_slots = nslots;
         w.cap_list = 0;  // list arrays are sized by clusters x capacity
     }
@@ -619,7 +631,9 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double* __restr
                                                            double* fd, double inv_scale, int n3,
                                                            double* __restrict__ out,
                                                            const int* __restrict__ list_count,
-                                                           const int* __restrict__ ncl_dev) {
+                                                           const int* __restrict__ ncl_dev,
+                                                           unsigned long long* fx_s, double* fd_s,
+                                                           const int* __restrict__ atom_slot, int max_slots) {
     __shared__ double red[kReduceThreads / 64];
     const int b = blockIdx.x, t = threadIdx.x;
     if (b < mdg::E_NSLOTS) {
@@ -661,12 +675,20 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(const double* __restr
     }
     const int nb = gridDim.x - mdg::E_NSLOTS - 1;
     for (int i = (b - mdg::E_NSLOTS - 1) * kReduceThreads + t; i < n3; i += nb * kReduceThreads) {
+        // the slot-ordered part of this coordinate (pair kernel and PME gather)
+        const int atom = i / 3, slot = atom_slot[atom];
+        const bool has = slot >= 0 && slot < max_slots;
+        const int j = 3 * slot + (i - 3 * atom);
         if (mixed) {
-            out[i] = static_cast<double>(static_cast<long long>(fx[i])) * inv_scale;
+            // integer sum first: the result does not depend on the split
+            const unsigned long long v = fx[i] + (has ? fx_s[j] : 0ull);
+            out[i] = static_cast<double>(static_cast<long long>(v)) * inv_scale;
             fx[i] = 0ull;
+            if (has) fx_s[j] = 0ull;
         } else {
-            out[i] = fd[i];
+            out[i] = fd[i] + (has ? fd_s[j] : 0.0);
             fd[i] = 0.0;
+            if (has) fd_s[j] = 0.0;
         }
     }
 }
@@ -740,7 +762,8 @@ int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out, bool fork) {
     hipLaunchKernelGGL(k_reduce, dim3(mdg::E_NSLOTS + 1 + fblocks), dim3(kReduceThreads), 0, c->stream, w.partial,
                        static_cast<int>(w.cap_partial), rows, w.energies, c->d_host_energies, w.flags, w.excl_rmax,
                        mixed ? 1 : 0, w.force_fx, w.force_d, 1.0 / mdg::fixed_scale(p.semantics), 3 * n, d_out,
-                       w.list_count, w.col_cstart + c->n_cols);
+                       w.list_count, w.col_cstart + c->n_cols, w.force_fx_s, w.force_d_s, w.atom_slot,
+                       static_cast<int>(w.cap_slots));
     HIP_TRY(hipGetLastError(), "reduce");
     mark(MDGPU_K_REDUCE, false);
     return MDGPU_OK;

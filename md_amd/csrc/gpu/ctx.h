@@ -126,6 +126,12 @@ struct Work {
     // forces
     double* force_d = nullptr;                 // [3N]
     unsigned long long* force_fx = nullptr;    // [3N] fixed point
+    // pair-kernel and PME-gather forces in slot order (3 per slot), so the
+    // atomics of one tile's cluster fall on consecutive addresses; the final
+    // reduction adds them to the atom-ordered accumulators through atom_slot
+    double* force_d_s = nullptr;               // [3 slots]
+    unsigned long long* force_fx_s = nullptr;  // [3 slots] fixed point
+    int* atom_slot = nullptr;                  // [N] slot of each atom (the last list build), -1 before one
     // energies: partial rows, one per block of each producing kernel
     size_t cap_partial = 0;
     double* partial = nullptr;       // [rows * E_NSLOTS]

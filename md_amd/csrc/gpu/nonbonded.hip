@@ -58,12 +58,12 @@ struct NbArgs {
     int ntp;                           // T + 1
     real rc2_lj, rc2_ee, rc_lj, rc_ee, rs, inv_sw;
     real alpha, kc, krf_crf;           // krf_crf: 1/rc_ee (reaction field with eps = 1)
-    double* force_d;
+    double* force_d;                   // slot-ordered force accumulators (3 per slot)
     FixedAcc fx;
     double* partial;
     int ld;                            // rows allocated in the slot-major partial buffer
     int row0;
-    int n3;                            // 3 x atoms: entries of the force arrays
+    int n3;                            // 3 x slots: entries of the force arrays
 };
 
 // erfc(x) for x >= 0 from e = exp(-x^2): Abramowitz & Stegun 7.1.26
@@ -225,6 +225,8 @@ __global__ __launch_bounds__(kWavesOf<real> * kWave) void k_nonbonded(const NbAr
     const real tsp = static_cast<real>(1.1283791670955126) * a.alpha;  // 2 alpha / sqrt(pi)
     const real rc2 = a.rc2_lj > a.rc2_ee ? a.rc2_lj : a.rc2_ee;
     int flush = 0;
+    // forces go to slot-ordered arrays: the 24 adds of a tile's j-cluster
+    // fall on 192 consecutive bytes
     const BufRsrc rsrc = sizeof(real) == 8 ? buf_rsrc(a.force_d, 8u * a.n3) : buf_rsrc(a.fx.f, 8u * a.n3);
     const int blk = xcd_block(blockIdx.x, npb);
     const int item = __builtin_amdgcn_readfirstlane(blk * kWaves + (threadIdx.x >> 6));
@@ -338,7 +340,8 @@ __global__ __launch_bounds__(kWavesOf<real> * kWave) void k_nonbonded(const NbAr
                 unsigned int off = kOffOOB;
                 if (__ballot(act) != 0ull) {
                     const real gx = sum8(fx), gy = sum8(fy), gz = sum8(fz);
-                    q = force_payload<real, kAmber>(a, ti < 3 && mcur.y >= 0, 3 * mcur.y + ti,
+                    const int cjt = __builtin_amdgcn_readlane(e_j, t);
+                    q = force_payload<real, kAmber>(a, ti < 3 && mcur.y >= 0, 3 * (cjt * kCluster + tj) + ti,
                                                     -(ti == 0 ? gx : (ti == 1 ? gy : gz)), off);
                 }
                 buf_add(rsrc, off, q);
@@ -372,7 +375,8 @@ __global__ __launch_bounds__(kWavesOf<real> * kWave) void k_nonbonded(const NbAr
         fix += shfl_xor(fix, 8);  fiy += shfl_xor(fiy, 8);  fiz += shfl_xor(fiz, 8);
         fix += shfl_xor(fix, 16); fiy += shfl_xor(fiy, 16); fiz += shfl_xor(fiz, 16);
         fix += shfl_xor(fix, 32); fiy += shfl_xor(fiy, 32); fiz += shfl_xor(fiz, 32);
-        add_force<real, kAmber>(a, rsrc, tj < 3 && k1 > k0 && mi.y >= 0, 3 * mi.y + tj, tj == 0 ? fix : (tj == 1 ? fiy : fiz));
+        add_force<real, kAmber>(a, rsrc, tj < 3 && k1 > k0 && mi.y >= 0, 3 * (ci * kCluster + ti) + tj,
+                                tj == 0 ? fix : (tj == 1 ? fiy : fiz));
     }
     e_vdw += s_vdw; e_all += s_all; e_list += s_list; e_rf += s_rf; e_ew += s_ew;
     // row by logical block: the 16 rows sharing a cache line of a slot come
@@ -426,12 +430,12 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0, const double* coords) {
     a.alpha = static_cast<real>(p.ewald_alpha);
     a.kc = static_cast<real>(kCoulomb);
     a.krf_crf = static_cast<real>(1.0 / p.ee_cutoff);
-    a.force_d = w.force_d;
-    a.fx = FixedAcc{w.force_fx, fixed_scale(p.semantics), w.flags};
+    a.force_d = w.force_d_s;
+    a.fx = FixedAcc{w.force_fx_s, fixed_scale(p.semantics), w.flags};
     a.partial = w.partial;
     a.ld = static_cast<int>(w.cap_partial);
     a.row0 = row0;
-    a.n3 = 3 * t.n_atoms;
+    a.n3 = 3 * c->n_clusters * kCluster;
     const int npb = nonbonded_rows(c);
     // bonded terms and corrections: rows after the pair blocks' rows
     const int nbb = bonded_rows(c);

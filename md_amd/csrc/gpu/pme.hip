@@ -421,7 +421,8 @@ __global__ void k_fft_z_c2r(PmeGeom g, int lognl, const typename C2<real>::T* __
 }
 
 // ---------------------------------------------------------------- gather --
-// One thread per slot, from the same spline records as the spreading.
+// One thread per slot, from the same spline records as the spreading; the
+// forces go to the slot-ordered accumulators (3 per slot).
 template <typename real>
 __global__ void k_gather(const int* ncl_dev, int max_slots, const int* __restrict__ slot_atom,
                          const int* __restrict__ fl_rec, const void* w_rec, PmeGeom g,
@@ -468,13 +469,13 @@ __global__ void k_gather(const int* ncl_dev, int max_slots, const int* __restric
     const double fy = -q * static_cast<double>(gy) * g.ky / g.ly;
     const double fz = -q * static_cast<double>(gz) * g.kz / g.lz;
     if constexpr (sizeof(real) == 8) {
-        acc_add(force_d, 3 * i, fx);
-        acc_add(force_d, 3 * i + 1, fy);
-        acc_add(force_d, 3 * i + 2, fz);
+        acc_add(force_d, 3 * s, fx);
+        acc_add(force_d, 3 * s + 1, fy);
+        acc_add(force_d, 3 * s + 2, fz);
     } else {
-        acc_add(fxa, 3 * i, fx);
-        acc_add(fxa, 3 * i + 1, fy);
-        acc_add(fxa, 3 * i + 2, fz);
+        acc_add(fxa, 3 * s, fx);
+        acc_add(fxa, 3 * s + 1, fy);
+        acc_add(fxa, 3 * s + 2, fz);
     }
 }
 
@@ -554,8 +555,8 @@ hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used, double* pot) {
     mark(MDGPU_K_COUNT, false);
     mark(MDGPU_K_GATHER, true);
     hipLaunchKernelGGL(k_gather<real>, dim3((max_slots + kThreads - 1) / kThreads), dim3(kThreads), 0, c->stream,
-                       w.col_cstart + ncol, max_slots, w.slot_atom, w.slot_pme_fl, wrec, g, phi, w.force_d,
-                       FixedAcc{w.force_fx, fixed_scale(c->params.semantics), w.flags}, pot);
+                       w.col_cstart + ncol, max_slots, w.slot_atom, w.slot_pme_fl, wrec, g, phi, w.force_d_s,
+                       FixedAcc{w.force_fx_s, fixed_scale(c->params.semantics), w.flags}, pot);
     mark(MDGPU_K_GATHER, false);
     return hipGetLastError();
 }

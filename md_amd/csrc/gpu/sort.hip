@@ -259,6 +259,7 @@ struct ClusterOut {
     const int* type;
     int pad_type;
     int* slot_atom;
+    int* atom_slot;          // inverse map, atom -> slot
     int* cluster_col;
     typename Vec4<real>::T* pos;
     int2* slot_meta;
@@ -353,6 +354,7 @@ __global__ __launch_bounds__(kSortThreads) void k_col_sort(const int* __restrict
         const double ox = 0.5 * (lo[0] + hi[0]), oy = 0.5 * (lo[1] + hi[1]), oz = 0.5 * (lo[2] + hi[2]);
         typename Vec4<real>::T p;
         if (atom >= 0) {
+            o.atom_slot[atom] = slot;
             p.x = static_cast<real>(x - ox);
             p.y = static_cast<real>(y - oy);
             p.z = static_cast<real>(z - oz);
@@ -967,7 +969,7 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl
     const int rb = (c->n_clusters * kCluster + kRefreshThreads - 1) / kRefreshThreads;
     if (mixed) {
         pme.w = w.slot_pme_wf;
-        const ClusterOut<float> o{c->n_clusters, t.charge, t.type, t.n_types, w.slot_atom, w.cluster_col,
+        const ClusterOut<float> o{c->n_clusters, t.charge, t.type, t.n_types, w.slot_atom, w.atom_slot, w.cluster_col,
                                   w.slot_pos_f, w.slot_meta, w.cl_origin, w.cl_half, pme};
         hipLaunchKernelGGL(k_col_sort<float>, dim3(ncol), dim3(kSortThreads), 0, c->stream, w.col_start, w.col_cstart,
                            w.col_key, w.wrapped, w.col_count, o, vs);
@@ -976,7 +978,7 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl
                                w.col_cstart + ncol, coords, w.x_ref, w.wrapped, o, g.box[0], g.box[1], g.box[2]);
     } else {
         pme.w = w.slot_pme_wd;
-        const ClusterOut<double> o{c->n_clusters, t.charge, t.type, t.n_types, w.slot_atom, w.cluster_col,
+        const ClusterOut<double> o{c->n_clusters, t.charge, t.type, t.n_types, w.slot_atom, w.atom_slot, w.cluster_col,
                                    w.slot_pos_d, w.slot_meta, w.cl_origin, w.cl_half, pme};
         hipLaunchKernelGGL(k_col_sort<double>, dim3(ncol), dim3(kSortThreads), 0, c->stream, w.col_start,
                            w.col_cstart, w.col_key, w.wrapped, w.col_count, o, vs);
