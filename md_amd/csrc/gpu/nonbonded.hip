@@ -187,8 +187,13 @@ __device__ __forceinline__ double bcast<double>(double v, int lane) {
 // The launch also carries the bonded terms and the AMBER corrections as nbb +
 // ncb further blocks after the npb pair blocks: they are dispatched last and
 // fill the CUs that the pair blocks leave idle at the end of the launch.
+// The FP32 AMBER instances without statistics are held to 6 waves per SIMD
+// (at most 80 VGPRs, no spills; 24 waves per CU instead of 16): more waves
+// hide the load and atomic latency of the tile loop.  The FP64, REF and
+// statistics instances need more registers and keep the compiler's choice.
 template <typename real, bool kAmber, bool kSwitch, bool kCount>
-__global__ __launch_bounds__(kWavesOf<real> * kWave) void k_nonbonded(const NbArgs<real> a, const BArgs<real> ba,
+__global__ __launch_bounds__(kWavesOf<real> * kWave)
+__attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 && kAmber && !kCount ? 6 : 1))) void k_nonbonded(const NbArgs<real> a, const BArgs<real> ba,
                                                                       const CArgs ca, int npb, int nbb) {
     constexpr int kWaves = kWavesOf<real>;
     if (static_cast<int>(blockIdx.x) >= npb) {
