@@ -372,7 +372,8 @@ int mdgpu_pairs_need_rebuild(mdgpu_ctx* ctx, const double* d_coords, int* needs)
  * potential energy (the evaluation's total), kinetic energy after the step
  * and, with the virial on, the pressure; last
  * (or NULL) the final step's energies.  A step that needs a larger list
- * restarts the run from the entry state. */
+ * restarts the run from the entry state; a step that fails (e.g. a force
+ * beyond the fixed-point range) leaves the entry state and returns the error. */
 typedef struct {
     double potential;
     double kinetic;

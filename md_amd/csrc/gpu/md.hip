@@ -222,18 +222,21 @@ int mdgpu_md_run(mdgpu_ctx* c, double* d_x, double* d_v, double* d_f, const doub
         MD_TRY(hipMemcpyAsync(tr.data(), m->trace, tr.size() * sizeof(double), hipMemcpyDeviceToHost, s), "trace");
         MD_TRY(hipStreamSynchronize(s), "md run");
         bool redo = false;
-        for (int k = 0; k < nsteps && !redo; ++k) {
+        int err = MDGPU_OK;
+        for (int k = 0; k < nsteps && !redo && err == MDGPU_OK; ++k) {
             st = mdg::ctx_check(c, &tr[static_cast<size_t>(k) * mdg::kTraceRow]);
             if (st == mdg::kEvalRedo) redo = true;
-            else if (st != MDGPU_OK) return st;
+            else if (st != MDGPU_OK) err = st;
         }
-        if (redo) {
-            // back to the saved state; the adjusted context builds a new list
+        if (redo || err != MDGPU_OK) {
+            // back to the saved state: a redo runs again with the adjusted
+            // context (which builds a new list); an error leaves the entry state
             MD_TRY(hipMemcpyAsync(d_x, m->save, bytes, hipMemcpyDeviceToDevice, s), "restore");
             MD_TRY(hipMemcpyAsync(d_v, m->save + n3, bytes, hipMemcpyDeviceToDevice, s), "restore");
             MD_TRY(hipMemcpyAsync(d_f, m->save + 2 * n3, bytes, hipMemcpyDeviceToDevice, s), "restore");
             MD_TRY(hipStreamSynchronize(s), "restore");
             c->verlet_force = true;
+            if (err != MDGPU_OK) return err;
             continue;
         }
         mdg::ctx_steps_done(c, tr.data(), mdg::kTraceRow, nsteps);
