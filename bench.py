@@ -28,7 +28,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# algorithmic work per unit (see DESIGN.md for the derivations)
+# algorithmic work per unit
 PAIR_FLOP = 77            # FP32 flops per atom pair inside the cutoff
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
@@ -257,7 +257,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X; "
-              "multi-GPU runs are not part of this build (see DESIGN.md on sharding).", file=sys.stderr)
+              "a multi-GPU run is not measured here.", file=sys.stderr)
         sys.exit(2)
 
     import torch

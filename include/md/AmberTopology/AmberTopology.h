@@ -91,6 +91,8 @@ public:
 
     // Additions (not in the reference).
     [[nodiscard]] Semantics get_semantics() const { return semantics_; }
+    // cell angles alpha, beta, gamma (degrees) from the coordinate file's box line
+    [[nodiscard]] std::array<double, 3> get_box_angles() const noexcept { return {box_alpha_, box_beta_, box_gamma_}; }
     [[nodiscard]] int get_nTypes() const { return static_cast<int>(nTypes_); }
     // per-type parameter tables and the per-term type index of each list
     struct TypeTables {

@@ -323,7 +323,7 @@ int mdgpu_graph_state(const mdgpu_ctx* ctx);
 int mdgpu_set_counting(mdgpu_ctx* ctx, int enable);
 
 /* Full virial and pressure (no counterpart in the reference, which forms no
- * virial; survey rank 4 of the next steps).  With mdgpu_set_virial(ctx, 1)
+ * virial).  With mdgpu_set_virial(ctx, 1)
  * each evaluation also forms the virial of the pair, bonded and correction
  * forces, W_ab = sum over interactions of (relative position)_a (force)_b,
  * and mdgpu_get_virial returns it plus the reciprocal-space virial of the
@@ -350,6 +350,21 @@ int mdgpu_get_virial(const mdgpu_ctx* ctx, double virial[9]);
  * device, so captured evaluations replay it.  skin = 0 (default): the list
  * is built at every evaluation.  Negative or non-finite: MDGPU_ERR_ARG. */
 int mdgpu_set_verlet(mdgpu_ctx* ctx, double skin);
+/* Triclinic cells (no counterpart in the reference, whose periodic boundary
+ * is orthorhombic only, PBC.cpp:72-81).  angles = alpha, beta, gamma in
+ * degrees, as on the box line of an rst7 file; they hold for every later
+ * evaluation, minimisation and dynamics run of the context, whose box
+ * arguments then give the edge lengths |a|, |b|, |c|.  The cell is
+ * a = (|a|, 0, 0), b = |b| (cos gamma, sin gamma, 0) and c in the half-space
+ * z > 0 with the given angles; coordinates are taken in that frame.  All
+ * three angles exactly 90 (the default) is the orthorhombic cell.  A
+ * triclinic cell needs AMBER semantics, every distance between opposite faces
+ * at least twice the cutoff and at least the cutoff plus the Verlet skin
+ * (checked at each evaluation: MDGPU_ERR_UNSUPPORTED), and is not available
+ * to mdgpu_adjoint.  Angles outside (0, 180) or that form no cell:
+ * MDGPU_ERR_ARG.  A change of angles forces the next list build. */
+int mdgpu_set_box_angles(mdgpu_ctx* ctx, const double angles[3]);
+int mdgpu_get_box_angles(const mdgpu_ctx* ctx, double angles[3]);
 /* List builds and successful evaluations since the system upload. */
 int mdgpu_get_list_builds(const mdgpu_ctx* ctx, int64_t* builds, int64_t* evals);
 /* ljpairs_needs_rebuild (NonBondedUtils.cpp:87-100) on the GPU for device

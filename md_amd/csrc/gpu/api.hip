@@ -483,6 +483,44 @@ int ensure_pme(mdgpu_ctx* c, const double box[3]) {
     return MDGPU_OK;
 }
 
+// The cell of edge lengths box and angles (degrees): a along x, b in the
+// x-y plane, c completing a right-handed cell (the rst7 convention).  An
+// angle of exactly 90 degrees has cosine 0, so all-90 angles give the
+// orthorhombic cell.  face: the distances between opposite faces.  False
+// when the angles do not form a cell.
+bool make_cell(const double box[3], const double ang[3], mdg::Cell& cell, double face[3]) {
+    cell = mdg::Cell{};
+    for (int d = 0; d < 3; ++d) cell.l[d] = box[d];
+    cell.tri = !(ang[0] == 90.0 && ang[1] == 90.0 && ang[2] == 90.0);
+    if (!cell.tri) {
+        for (int d = 0; d < 3; ++d) face[d] = box[d];
+        return true;
+    }
+    const double rad = mdg::kPi / 180.0;
+    auto cosd = [&](double a) { return a == 90.0 ? 0.0 : std::cos(a * rad); };
+    const double ca = cosd(ang[0]), cb = cosd(ang[1]), cg = cosd(ang[2]);
+    const double sg = ang[2] == 90.0 ? 1.0 : std::sin(ang[2] * rad);
+    if (!(sg > 1e-6)) return false;
+    cell.ax = box[0];
+    cell.bx = box[1] * cg;
+    cell.by = box[1] * sg;
+    cell.cx = box[2] * cb;
+    cell.cy = box[2] * (ca - cb * cg) / sg;
+    const double cz2 = box[2] * box[2] - cell.cx * cell.cx - cell.cy * cell.cy;
+    if (!(cz2 > 1e-12 * box[2] * box[2])) return false;
+    cell.cz = std::sqrt(cz2);
+    cell.r00 = 1.0 / cell.ax;
+    cell.r11 = 1.0 / cell.by;
+    cell.r22 = 1.0 / cell.cz;
+    cell.r01 = -cell.bx / (cell.ax * cell.by);
+    cell.r12 = -cell.cy / (cell.by * cell.cz);
+    cell.r02 = (cell.bx * cell.cy - cell.by * cell.cx) / (cell.ax * cell.by * cell.cz);
+    face[0] = 1.0 / std::sqrt(cell.r00 * cell.r00 + cell.r01 * cell.r01 + cell.r02 * cell.r02);
+    face[1] = 1.0 / std::sqrt(cell.r11 * cell.r11 + cell.r12 * cell.r12);
+    face[2] = cell.cz;
+    return true;
+}
+
 int prepare(mdgpu_ctx* c, const double box[3]) {
     const int n = c->topo.n_atoms;
     const mdgpu_params& p = c->params;
@@ -491,24 +529,38 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     // a Verlet list holds shift vectors of the box it was built in
     for (int d = 0; d < 3; ++d)
         if (box[d] != c->grid.box[d]) c->verlet_force = true;
+    mdg::Cell cell{};
+    double face[3];
+    if (!make_cell(box, c->angles, cell, face))
+        return set_error(c, MDGPU_ERR_ARG, "the cell angles do not form a cell");
+    if (cell.tri && p.semantics != MDGPU_SEM_AMBER)
+        return set_error(c, MDGPU_ERR_UNSUPPORTED, "triclinic cells are evaluated in AMBER semantics only");
     const double rc = std::max(p.lj_cutoff, p.ee_cutoff);
-    if (std::min({box[0], box[1], box[2]}) < 2.0 * rc)
-        return set_error(c, MDGPU_ERR_UNSUPPORTED, "every box length must be at least twice the cutoff");
     // list radius: the cutoff, plus the skin of a Verlet list
     const double rl = rc + (c->skin > 0 ? c->skin : 0.0);
+    if (!cell.tri && std::min({box[0], box[1], box[2]}) < 2.0 * rc)
+        return set_error(c, MDGPU_ERR_UNSUPPORTED, "every box length must be at least twice the cutoff");
+    if (cell.tri && (std::min({face[0], face[1], face[2]}) < 2.0 * rc || std::min({face[0], face[1], face[2]}) < rl))
+        return set_error(c, MDGPU_ERR_UNSUPPORTED,
+                         "every distance between opposite cell faces must be at least twice the cutoff and at least "
+                         "the cutoff plus the skin");
     mdg::Grid& g = c->grid;
     g.r_list = rl;
     g.box[0] = box[0];
     g.box[1] = box[1];
     g.box[2] = box[2];
-    const double dens = n / (box[0] * box[1] * box[2]);
+    g.cell = cell;
+    const double dens = n / cell_volume(cell);
     const double s = std::cbrt(mdg::kCluster / std::max(dens, 1e-6));
-    g.ncx = std::max(1, static_cast<int>(box[0] / s));
-    g.ncy = std::max(1, static_cast<int>(box[1] / s));
+    g.ncx = std::max(1, static_cast<int>(face[0] / s));
+    g.ncy = std::max(1, static_cast<int>(face[1] / s));
     g.sx = box[0] / g.ncx;
     g.sy = box[1] / g.ncy;
-    g.dcx = static_cast<int>(std::floor(rl / g.sx)) + 1;
-    g.dcy = static_cast<int>(std::floor(rl / g.sy)) + 1;
+    g.wx = cell.tri ? face[0] / g.ncx : g.sx;
+    g.wy = cell.tri ? face[1] / g.ncy : g.sy;
+    g.frac_scale = cell.tri ? std::max(box[0] / face[0], box[1] / face[1]) : 1.0;
+    g.dcx = static_cast<int>(std::floor(rl / g.wx)) + 1;
+    g.dcy = static_cast<int>(std::floor(rl / g.wy)) + 1;
     const int ncol = g.ncx * g.ncy;
     c->n_cols = ncol;
     c->n_clusters = n / mdg::kCluster + ncol + 1;
@@ -851,6 +903,7 @@ mdg::GraphKey graph_key(const mdgpu_ctx* c, const double* x, const double* out) 
     k.out = out;
     for (int d = 0; d < 3; ++d) {
         k.box[d] = c->grid.box[d];
+        k.angles[d] = c->angles[d];
         k.pme_k[d] = w.pme_k[d];
     }
     k.gen = c->gen;
@@ -876,7 +929,8 @@ bool same_key(const mdg::GraphKey& a, const mdg::GraphKey& b) {
               a.search_general == b.search_general && a.split == b.split && a.fft_lines == b.fft_lines &&
               a.counting == b.counting && a.timing == b.timing && a.n_clusters == b.n_clusters &&
               a.n_cols == b.n_cols;
-    for (int d = 0; d < 3; ++d) eq = eq && a.box[d] == b.box[d] && a.pme_k[d] == b.pme_k[d];
+    for (int d = 0; d < 3; ++d)
+        eq = eq && a.box[d] == b.box[d] && a.angles[d] == b.angles[d] && a.pme_k[d] == b.pme_k[d];
     for (int k = 0; k < 6; ++k) eq = eq && a.caps[k] == b.caps[k];
     return eq;
 }
@@ -1154,6 +1208,8 @@ int mdgpu_adjoint(mdgpu_ctx* c, const double* coords, const double box[3], const
     const mdgpu_params& p = c->params;
     if (p.semantics != MDGPU_SEM_AMBER)
         return set_error(c, MDGPU_ERR_UNSUPPORTED, "the adjoint is defined for AMBER semantics only");
+    if (!(c->angles[0] == 90.0 && c->angles[1] == 90.0 && c->angles[2] == 90.0))
+        return set_error(c, MDGPU_ERR_UNSUPPORTED, "the adjoint is defined for orthorhombic cells only");
     HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
     const bool mixed = p.precision == MDGPU_PREC_MIXED;
     const Topo& t = c->topo;
@@ -1314,6 +1370,29 @@ int mdgpu_set_verlet(mdgpu_ctx* c, double skin) {
     c->skin = skin;
     c->verlet_force = true;
     ++c->gen;
+    return MDGPU_OK;
+}
+
+int mdgpu_set_box_angles(mdgpu_ctx* c, const double angles[3]) {
+    if (!c || !angles) return MDGPU_ERR_ARG;
+    for (int d = 0; d < 3; ++d)
+        if (!(angles[d] > 0.0 && angles[d] < 180.0))
+            return set_error(c, MDGPU_ERR_ARG, "cell angles must lie strictly between 0 and 180 degrees");
+    const double unit[3] = {1.0, 1.0, 1.0};
+    mdg::Cell cell;
+    double face[3];
+    if (!make_cell(unit, angles, cell, face)) return set_error(c, MDGPU_ERR_ARG, "the cell angles do not form a cell");
+    bool same = true;
+    for (int d = 0; d < 3; ++d) same = same && c->angles[d] == angles[d];
+    if (same) return MDGPU_OK;
+    for (int d = 0; d < 3; ++d) c->angles[d] = angles[d];
+    c->verlet_force = true;  // the kept list holds shifts of the old cell
+    return MDGPU_OK;
+}
+
+int mdgpu_get_box_angles(const mdgpu_ctx* c, double angles[3]) {
+    if (!c || !angles) return MDGPU_ERR_ARG;
+    for (int d = 0; d < 3; ++d) angles[d] = c->angles[d];
     return MDGPU_OK;
 }
 

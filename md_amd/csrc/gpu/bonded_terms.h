@@ -266,6 +266,7 @@ __device__ __forceinline__ void bonded_terms(const BArgs<real>& a, int k, int ro
 struct CArgs {
     const double* x;
     double lx, ly, lz;
+    Cell cell;
     int n_excl;
     const int2* excl;
     int n14;
@@ -306,9 +307,13 @@ __device__ __forceinline__ void correction_pairs(const CArgs& a, int k, int row)
         const int2 p = is14 ? a.p14[k - a.n_excl] : a.excl[k];
         double dx = a.x[3 * p.x] - a.x[3 * p.y], dy = a.x[3 * p.x + 1] - a.x[3 * p.y + 1],
                dz = a.x[3 * p.x + 2] - a.x[3 * p.y + 2];
-        dx -= a.lx * rint(dx / a.lx);
-        dy -= a.ly * rint(dy / a.ly);
-        dz -= a.lz * rint(dz / a.lz);
+        if (!a.cell.tri) {
+            dx -= a.lx * rint(dx / a.lx);
+            dy -= a.ly * rint(dy / a.ly);
+            dz -= a.lz * rint(dz / a.lz);
+        } else {
+            cell_min_image_tri(a.cell, dx, dy, dz);
+        }
         const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
         const double qq = kCoulomb * a.charge[p.x] * a.charge[p.y];
         const double ef = erf(a.alpha * r);

@@ -16,6 +16,44 @@ constexpr double kPi = 3.14159265358979323846;
 // cutoff from it, and it has zero charge and LJ parameters.
 constexpr double kFarAway = 1.0e5;
 
+// Periodic cell.  Orthorhombic (tri == 0): edge lengths l.  Triclinic: edge
+// vectors a = (ax, 0, 0), b = (bx, by, 0), c = (cx, cy, cz) (the columns of
+// H) and the upper-triangular R = H^-1, so that s = R x are the fractional
+// coordinates; l holds |a|, |b|, |c|.
+struct Cell {
+    double l[3];
+    int tri;
+    double ax, bx, by, cx, cy, cz;
+    double r00, r01, r02, r11, r12, r22;
+};
+
+__host__ __device__ __forceinline__ double cell_volume(const Cell& b) {
+    return b.tri ? b.ax * b.by * b.cz : b.l[0] * b.l[1] * b.l[2];
+}
+
+// Fractional coordinates s = R x of a triclinic cell.
+// Written without contraction into fused multiply-adds, so that host and
+// device code round alike.
+__host__ __device__ __forceinline__ void cell_frac(const Cell& b, double x, double y, double z, double s[3]) {
+#pragma clang fp contract(off)
+    s[2] = b.r22 * z;
+    s[1] = b.r11 * y + b.r12 * z;
+    s[0] = b.r00 * x + b.r01 * y + b.r02 * z;
+}
+
+// Minimum image of a separation d in a triclinic cell: d - H round(R d).
+// When the cutoff is below half the smallest distance between opposite
+// faces, this is the only image that can lie within it.
+__host__ __device__ __forceinline__ void cell_min_image_tri(const Cell& b, double& dx, double& dy, double& dz) {
+#pragma clang fp contract(off)
+    double s[3];
+    cell_frac(b, dx, dy, dz, s);
+    const double n0 = rint(s[0]), n1 = rint(s[1]), n2 = rint(s[2]);
+    dx = dx - (b.ax * n0 + b.bx * n1 + b.cx * n2);
+    dy = dy - (b.by * n1 + b.cy * n2);
+    dz = dz - b.cz * n2;
+}
+
 // Energy accumulator slots of the partial-sum buffers.
 enum EnergySlot {
     E_BOND = 0, E_ANGLE, E_DIHEDRAL, E_UB, E_IMPROPER, E_CMAP,

@@ -163,6 +163,15 @@ struct Grid {
     double box[3] = {0, 0, 0};
     int dcx = 0, dcy = 0;            // column search reach
     double r_list = 0;               // list radius: max cutoff + Verlet skin
+    // Triclinic cells bin in fractional coordinates: column (cx, cy) holds
+    // s_x in [cx, cx + 1) / ncx and s_y in [cy, cy + 1) / ncy, and sx, sy are
+    // the column widths in units of the edge lengths (|a| / ncx, |b| / ncy).
+    // wx, wy: the distance between a column's opposite faces (sx, sy when
+    // orthorhombic); frac_scale: the largest edge length over face distance
+    // ratio, which converts a distance into a bound on s_x |a| and s_y |b|.
+    Cell cell{};
+    double wx = 0, wy = 0;
+    double frac_scale = 1.0;
 };
 
 constexpr int kTimingSample = 16;  // timing mode 3: one evaluation in 16 carries the events
@@ -183,6 +192,7 @@ struct GraphKey {
     const double* x;
     const double* out;
     double box[3];
+    double angles[3];
     int pme_k[3];
     unsigned long long gen;
     int list_cap, search_general, split, fft_lines, counting, timing, n_clusters, n_cols;
@@ -239,6 +249,8 @@ struct mdgpu_ctx {
     // more than skin / 2 since the build; verlet_force makes the next
     // evaluation build it whatever the displacements
     double skin = 0.0;
+    // cell angles alpha, beta, gamma in degrees (all 90: orthorhombic)
+    double angles[3] = {90.0, 90.0, 90.0};
     bool verlet_force = true;
     long long list_builds = 0, list_evals = 0;
     void* md = nullptr;             // molecular-dynamics buffers (md.hip), after mdgpu_md_set_masses

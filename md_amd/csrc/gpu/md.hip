@@ -243,7 +243,7 @@ int mdgpu_md_run(mdgpu_ctx* c, double* d_x, double* d_v, double* d_f, const doub
         // pressure (bar) from the virial when the evaluations formed it:
         // P = (2 KE + trace W) / (3 V), 1 kcal/mol/A^3 = 69476.95 bar
         const bool stats = c->counting || c->virial;
-        const double vol = box[0] * box[1] * box[2];
+        const double vol = mdg::cell_volume(c->grid.cell);
         for (int k = 0; k < nsteps; ++k) {
             const double* row = &tr[static_cast<size_t>(k) * mdg::kTraceRow];
             mdgpu_energies e{};

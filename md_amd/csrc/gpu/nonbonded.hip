@@ -457,6 +457,7 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0, const double* coords) {
     ca.lx = c->grid.box[0];
     ca.ly = c->grid.box[1];
     ca.lz = c->grid.box[2];
+    ca.cell = c->grid.cell;
     ca.n_excl = t.n_excl_pairs;
     ca.excl = t.excl_pairs;
     ca.n14 = t.n_pairs14;

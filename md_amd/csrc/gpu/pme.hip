@@ -63,6 +63,7 @@ __device__ __forceinline__ void bspline4(real w, real* th, real* dth) {
 struct PmeGeom {
     int kx, ky, kz, hz;
     double lx, ly, lz;
+    Cell cell;  // triclinic cells: m = R^T (mx, my, mz), forces through R^T
 };
 
 // ------------------------------------------------------------ LDS FFT -----
@@ -165,7 +166,8 @@ struct SpreadCols {
     const int* col_cstart;   // [ncol+1] cluster offsets
     const int* fl;           // [slots] packed base indices
     const void* w;           // [slots] fractions + charge (double4 / float4)
-    double margin;           // how far an atom may sit outside its column (Verlet list: skin / 2)
+    double margin;           // how far an atom may sit outside its column (Verlet list: skin / 2, in
+                             // units of s_x |a| and s_y |b| for a triclinic cell)
 };
 
 constexpr int kSpreadCols = 256;  // columns per pass of the candidate loop
@@ -340,7 +342,7 @@ __global__ void k_fft_x_conv(PmeGeom g, int lognl, typename C2<real>::T* __restr
     c2* S = lds_fft<real>(A, B, g.kx, lognl, tw, -1);
     c2* other = S == A ? B : A;
     // G(m) |S|^2, virial, and S <- G S
-    const double V = g.lx * g.ly * g.lz;
+    const double V = g.cell.tri ? cell_volume(g.cell) : g.lx * g.ly * g.lz;
     const double pa2 = kPi * kPi / (alpha * alpha);
     const int my = y <= g.ky / 2 ? y : y - g.ky;
     const double* bx = bmod;
@@ -356,7 +358,18 @@ __global__ void k_fft_x_conv(PmeGeom g, int lognl, typename C2<real>::T* __restr
             if (mx == 0 && my == 0 && kz == 0) {
                 s = {0, 0};
             } else {
-                const double fx = mx / g.lx, fy = my / g.ly, fz = kz / g.lz;
+                double fx, fy, fz;
+                if (!g.cell.tri) {
+                    fx = mx / g.lx;
+                    fy = my / g.ly;
+                    fz = kz / g.lz;
+                } else {
+                    // the conjugate half (weight 2) is taken at -m, also on
+                    // the Nyquist planes
+                    fx = mx * g.cell.r00;
+                    fy = mx * g.cell.r01 + my * g.cell.r11;
+                    fz = mx * g.cell.r02 + my * g.cell.r12 + kz * g.cell.r22;
+                }
                 const double m2 = fx * fx + fy * fy + fz * fz;
                 const double gm = kCoulomb * exp(-pa2 * m2) / (kPi * V * m2) * bx[x] * by[y] * bz[kz];
                 const double wgt = (kz == 0 || 2 * kz == g.kz) ? 1.0 : 2.0;
@@ -465,9 +478,19 @@ __global__ void k_gather(const int* ncl_dev, int max_slots, const int* __restric
         return;
     }
     const double q = static_cast<double>(wq.w);
-    const double fx = -q * static_cast<double>(gx) * g.kx / g.lx;
-    const double fy = -q * static_cast<double>(gy) * g.ky / g.ly;
-    const double fz = -q * static_cast<double>(gz) * g.kz / g.lz;
+    double fx, fy, fz;
+    if (!g.cell.tri) {
+        fx = -q * static_cast<double>(gx) * g.kx / g.lx;
+        fy = -q * static_cast<double>(gy) * g.ky / g.ly;
+        fz = -q * static_cast<double>(gz) * g.kz / g.lz;
+    } else {
+        // dE/dx = R^T dE/ds, dE/ds_d = K_d dE/du_d
+        const double ux = static_cast<double>(gx) * g.kx, uy = static_cast<double>(gy) * g.ky;
+        const double uz = static_cast<double>(gz) * g.kz;
+        fx = -q * (ux * g.cell.r00);
+        fy = -q * (ux * g.cell.r01 + uy * g.cell.r11);
+        fz = -q * (ux * g.cell.r02 + uy * g.cell.r12 + uz * g.cell.r22);
+    }
     if constexpr (sizeof(real) == 8) {
         acc_add(force_d, 3 * s, fx);
         acc_add(force_d, 3 * s + 1, fy);
@@ -499,7 +522,7 @@ hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used, double* pot) {
     Work& w = c->work;
     const int n = c->topo.n_atoms;
     PmeGeom g{w.pme_k[0], w.pme_k[1], w.pme_k[2], w.pme_k[2] / 2 + 1, c->grid.box[0], c->grid.box[1],
-              c->grid.box[2]};
+              c->grid.box[2], c->grid.cell};
     const bool dbl = sizeof(real) == 8;
     c2* cg = static_cast<c2*>(w.cgrid);
     const c2* tw = dbl ? reinterpret_cast<const c2*>(w.twiddle_d) : reinterpret_cast<const c2*>(w.twiddle_f);
@@ -522,7 +545,7 @@ hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used, double* pot) {
         const int by = std::min(nl, 8), bx = nl / by;
         const size_t shm = (2 * static_cast<size_t>(g.kz) * nl + g.kz) * sizeof(c2);
         const SpreadCols sc{c->grid.ncx, c->grid.ncy, c->grid.sx, c->grid.sy, w.col_cstart, w.slot_pme_fl, wrec,
-                            c->skin > 0 ? 0.5 * c->skin : 0.0};
+                            c->skin > 0 ? 0.5 * c->skin * c->grid.frac_scale : 0.0};
         const dim3 grid((g.kx + bx - 1) / bx, (g.ky + by - 1) / by);
         hipLaunchKernelGGL(k_spread_fft_z<real>, grid, dim3(kThreads), shm, c->stream, g, sc, bx, by, ilog2(nl), cg,
                            twz, w.flags);

@@ -105,8 +105,8 @@ __device__ int lds_exclusive_scan(int* s, int n) {
     return tot;
 }
 
-__global__ __launch_bounds__(kBinThreads) void k_bin(const double* __restrict__ x, int n, int chunk, double lx,
-                                                     double ly, double lz, int ncx, int ncy, double sx, double sy,
+__global__ __launch_bounds__(kBinThreads) void k_bin(const double* __restrict__ x, int n, int chunk, const Cell cell,
+                                                     int ncx, int ncy, double sx, double sy,
                                                      double* __restrict__ wrapped, int* __restrict__ col_of,
                                                      int* __restrict__ col_count, int* __restrict__ block_off,
                                                      const ExclPairs ex, double* __restrict__ xref,
@@ -117,12 +117,30 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(const double* __restrict__ 
     const int ncol = ncx * ncy;
     for (int c = threadIdx.x; c < ncol; c += blockDim.x) hist[c] = 0;
     __syncthreads();
+    const double lx = cell.l[0], ly = cell.l[1], lz = cell.l[2];
     const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
     for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        // fractional wrapping in FP64: x - L floor(x / L)
-        const double wx = x[3 * i] - lx * floor(x[3 * i] / lx);
-        const double wy = x[3 * i + 1] - ly * floor(x[3 * i + 1] / ly);
-        const double wz = x[3 * i + 2] - lz * floor(x[3 * i + 2] / lz);
+        double wx, wy, wz;
+        int cx, cy;
+        if (!cell.tri) {
+            // fractional wrapping in FP64: x - L floor(x / L)
+            wx = x[3 * i] - lx * floor(x[3 * i] / lx);
+            wy = x[3 * i + 1] - ly * floor(x[3 * i + 1] / ly);
+            wz = x[3 * i + 2] - lz * floor(x[3 * i + 2] / lz);
+            cx = static_cast<int>(wx / sx);
+            cy = static_cast<int>(wy / sy);
+        } else {
+            // triclinic: wrap the fractional coordinates into [0, 1) and bin
+            // by them; the wrapped position is H s
+            double f[3];
+            cell_frac(cell, x[3 * i], x[3 * i + 1], x[3 * i + 2], f);
+            for (int d = 0; d < 3; ++d) f[d] -= floor(f[d]);
+            wx = cell.ax * f[0] + cell.bx * f[1] + cell.cx * f[2];
+            wy = cell.by * f[1] + cell.cy * f[2];
+            wz = cell.cz * f[2];
+            cx = static_cast<int>(f[0] * ncx);
+            cy = static_cast<int>(f[1] * ncy);
+        }
         wrapped[3 * i] = wx;
         wrapped[3 * i + 1] = wy;
         wrapped[3 * i + 2] = wz;
@@ -132,7 +150,6 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(const double* __restrict__ 
             xref[3 * i + 1] = x[3 * i + 1];
             xref[3 * i + 2] = x[3 * i + 2];
         }
-        int cx = static_cast<int>(wx / sx), cy = static_cast<int>(wy / sy);
         cx = cx < 0 ? 0 : (cx >= ncx ? ncx - 1 : cx);
         cy = cy < 0 ? 0 : (cy >= ncy ? ncy - 1 : cy);
         const int c = cx + ncx * cy;
@@ -146,9 +163,13 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(const double* __restrict__ 
     for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
         const int2 p = e < ex.n1 ? ex.p1[e] : ex.p2[e - ex.n1];
         double dx = x[3 * p.x] - x[3 * p.y], dy = x[3 * p.x + 1] - x[3 * p.y + 1], dz = x[3 * p.x + 2] - x[3 * p.y + 2];
-        dx -= lx * rint(dx / lx);
-        dy -= ly * rint(dy / ly);
-        dz -= lz * rint(dz / lz);
+        if (!cell.tri) {
+            dx -= lx * rint(dx / lx);
+            dy -= ly * rint(dy / ly);
+            dz -= lz * rint(dz / lz);
+        } else {
+            cell_min_image_tri(cell, dx, dy, dz);
+        }
         r = fmaxf(r, static_cast<float>(sqrt(dx * dx + dy * dy + dz * dz)) * 1.0001f);
     }
     for (int m = 32; m >= 1; m >>= 1) r = fmaxf(r, __shfl_xor(r, m, kWave));
@@ -228,20 +249,23 @@ struct Vec4<double> { using T = double4; };
 // PME geometry for the per-slot spline records (k[0] == 0: no records).
 struct PmeSlots {
     int k[3];
-    double l[3];
+    Cell cell;
     int* fl;
     void* w;   // double4 / float4 per slot: fractions (x, y, z) and charge
 };
 
-// PME spline record of a position: u = K x / L in FP64; the base index
+// PME spline record of a position: u = K x / L in FP64 (u = K s, s = R x,
+// in a triclinic cell); the base index
 // floor(u) taken mod K (the stencil wraps anyway), packed 10 bits per
 // dimension, and the fractions u - floor(u).  The position may lie outside
 // the box (a Verlet-list atom that moved across a face since the build).
 template <typename real>
 __device__ __forceinline__ int pme_record(const double xs[3], const PmeSlots& pme, real wv[3]) {
     int packed = 0;
+    double fr[3] = {0.0, 0.0, 0.0};
+    if (pme.cell.tri) cell_frac(pme.cell, xs[0], xs[1], xs[2], fr);
     for (int d = 0; d < 3; ++d) {
-        const double u = xs[d] * pme.k[d] / pme.l[d];
+        const double u = pme.cell.tri ? fr[d] * pme.k[d] : xs[d] * pme.k[d] / pme.cell.l[d];
         const double f = floor(u);
         int fl = static_cast<int>(f) % pme.k[d];
         fl += fl < 0 ? pme.k[d] : 0;
@@ -416,7 +440,9 @@ struct SearchArgs {
     const float4* half;
     const int* slot_atom;
     int ncx, ncy, dcx, dcy;
-    double sx, sy, lx, ly, lz;
+    double sx, sy;          // distance between a column's opposite faces
+    double lx, ly, lz;      // edge lengths (lz: the height cz of a triclinic cell)
+    Cell cell;
     double rl2;             // list cutoff squared
     int cap;
     int* list_count;
@@ -467,10 +493,13 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
         for (int dx = -a.dcx; dx <= a.dcx; ++dx) {
             const int jxr = cx + dx, kx = floordiv(jxr, a.ncx), jx = jxr - kx * a.ncx;
             const double gx = fmax(0.0, (abs(dx) - 1) * a.sx);
-            if (gx * gx + gy * gy >= a.rl2) continue;
+            // the column gaps bound the distance along the face normals; in a
+            // triclinic cell the two normals are not orthogonal
+            if ((a.cell.tri ? fmax(gx * gx, gy * gy) : gx * gx + gy * gy) >= a.rl2) continue;
             const int jcol = jx + a.ncx * jy;
             const int c0 = a.col_cstart[jcol], c1 = a.col_cstart[jcol + 1];
-            const double shx = kx * a.lx, shy = ky * a.ly;
+            const double shx = a.cell.tri ? kx * a.cell.ax + ky * a.cell.bx : kx * a.lx;
+            const double shy = a.cell.tri ? ky * a.cell.by : ky * a.ly;
             // half list: only j-clusters >= ci (cluster indices run column by column)
             for (int b = max(c0, ci); b < c1; b += kWave) {
                 const int cj = b + lane;
@@ -482,13 +511,19 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
                     hj = a.half[cj];
                 }
                 for (int kz = -1; kz <= 1; ++kz) {
-                    const double ddx = oj.x + shx - oi.x, ddy = oj.y + shy - oi.y, ddz = oj.z + kz * a.lz - oi.z;
+                    const double tx = a.cell.tri ? shx + kz * a.cell.cx : shx;
+                    const double ty = a.cell.tri ? shy + kz * a.cell.cy : shy;
+                    const double ddx = oj.x + tx - oi.x, ddy = oj.y + ty - oi.y, ddz = oj.z + kz * a.lz - oi.z;
                     const double qx = fmax(0.0, fabs(ddx) - (double)(hi.x + hj.x));
                     const double qy = fmax(0.0, fabs(ddy) - (double)(hi.y + hj.y));
                     const double qz = fmax(0.0, fabs(ddz) - (double)(hi.z + hj.z));
                     const double d2 = qx * qx + qy * qy + qz * qz;
                     const bool noshift = kx == 0 && ky == 0 && kz == 0;
-                    const bool cand = valid && (cj > ci || (cj == ci && noshift)) && d2 < a.rl2;
+                    // a cluster with its own image: one of each pair of opposite
+                    // shifts (the other holds the same atom pairs), all 64 pairs
+                    const bool self_up = kz > 0 || (kz == 0 && (ky > 0 || (ky == 0 && kx > 0)));
+                    const bool cand = valid && (cj > ci || (cj == ci && (noshift || self_up))) && d2 < a.rl2;
+                    const bool tri_mask = cj == ci && noshift;
                     const unsigned long long bal = __ballot(cand);
                     if (bal == 0) continue;
                     const int rank = __popcll(bal & ((1ull << lane) - 1ull));
@@ -498,7 +533,7 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
                         const size_t e = base_i + pos;
                         a.list_j[e] = cj;
                         // masks of tiles that may hold excluded pairs are written below
-                        if (!near) a.list_mask[e] = (cj == ci) ? self_mask : ~0ull;
+                        if (!near) a.list_mask[e] = tri_mask ? self_mask : ~0ull;
                         R4 dl;
                         dl.x = static_cast<real>(-ddx);
                         dl.y = static_cast<real>(-ddy);
@@ -509,7 +544,8 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
                     const unsigned long long nbal = __ballot(near && pos < a.cap);
                     if (near && pos < a.cap) {
                         const int nr = nnear + __popcll(nbal & ((1ull << lane) - 1ull));
-                        if (nr < kNearCap) s_near[wv][nr] = make_int2(pos, cj);
+                        // bit 30 of the entry: a shifted image of the cluster itself
+                        if (nr < kNearCap) s_near[wv][nr] = make_int2(pos | (cj == ci && !noshift ? 1 << 30 : 0), cj);
                     }
                     nnear += __popcll(nbal);
                     count += __popcll(bal);
@@ -538,7 +574,9 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
         const int aj = a.slot_atom[en.y * kCluster + tj];
         const bool ex = ai >= 0 && aj >= 0 && pair_excluded(a.excl_start, a.excl_list, ai, aj);
         const unsigned long long bal = __ballot(ex);
-        if (lane == 0) a.list_mask[base_i + en.x] = ((en.y == ci) ? self_mask : ~0ull) & ~bal;
+        const bool shifted = (en.x >> 30) & 1;
+        const int pos = en.x & ((1 << 30) - 1);
+        if (lane == 0) a.list_mask[base_i + pos] = ((en.y == ci && !shifted) ? self_mask : ~0ull) & ~bal;
     }
 }
 
@@ -845,18 +883,23 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
 constexpr int kDispThreads = 256;
 constexpr int kDispMaxBlocks = 1024;
 __global__ __launch_bounds__(kDispThreads) void k_disp(const double* __restrict__ x, const double* __restrict__ xref,
-                                                       int n, double lx, double ly, double lz, double lim2,
-                                                       int query, VerletState* vs, unsigned int* flags) {
+                                                       int n, const Cell cell, double lim2, int query, VerletState* vs,
+                                                       unsigned int* flags) {
 #pragma clang fp contract(off)
     __shared__ double wmax[kDispThreads / kWave];
+    const double lx = cell.l[0], ly = cell.l[1], lz = cell.l[2];
     double m = 0.0;
     for (int i = blockIdx.x * kDispThreads + threadIdx.x; i < n; i += gridDim.x * kDispThreads) {
         double dx = x[3 * i] - xref[3 * i];
         double dy = x[3 * i + 1] - xref[3 * i + 1];
         double dz = x[3 * i + 2] - xref[3 * i + 2];
-        dx = dx - lx * round(dx / lx);
-        dy = dy - ly * round(dy / ly);
-        dz = dz - lz * round(dz / lz);
+        if (!cell.tri) {
+            dx = dx - lx * round(dx / lx);
+            dy = dy - ly * round(dy / ly);
+            dz = dz - lz * round(dz / lz);
+        } else {
+            cell_min_image_tri(cell, dx, dy, dz);
+        }
         const double d2 = dx * dx + dy * dy + dz * dz;
         if (d2 > m) m = d2;
     }
@@ -898,20 +941,21 @@ __global__ __launch_bounds__(kRefreshThreads) void k_refresh(const VerletState* 
                                                              const double* __restrict__ x,
                                                              const double* __restrict__ xref,
                                                              const double* __restrict__ wrapped,
-                                                             const ClusterOut<real> o, double lx, double ly,
-                                                             double lz) {
+                                                             const ClusterOut<real> o, const Cell cell) {
     if (vs->build) return;
     const int s = blockIdx.x * kRefreshThreads + threadIdx.x;
     if (s >= min(*ncl_dev, o.max_clusters) * kCluster) return;
     const int atom = o.slot_atom[s];
     if (atom < 0) return;
-    const double L[3] = {lx, ly, lz};
-    double xs[3];
-    for (int d = 0; d < 3; ++d) {
-        double dd = x[3 * atom + d] - xref[3 * atom + d];
-        dd = dd - L[d] * round(dd / L[d]);
-        xs[d] = wrapped[3 * atom + d] + dd;
+    double dd[3];
+    for (int d = 0; d < 3; ++d) dd[d] = x[3 * atom + d] - xref[3 * atom + d];
+    if (!cell.tri) {
+        for (int d = 0; d < 3; ++d) dd[d] = dd[d] - cell.l[d] * round(dd[d] / cell.l[d]);
+    } else {
+        cell_min_image_tri(cell, dd[0], dd[1], dd[2]);
     }
+    double xs[3];
+    for (int d = 0; d < 3; ++d) xs[d] = wrapped[3 * atom + d] + dd[d];
     const double4 org = o.origin[s / kCluster];
     typename Vec4<real>::T p;
     p.x = static_cast<real>(xs[0] - org.x);
@@ -950,21 +994,19 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl
     ex.chunk = (ex.n1 + ex.n2 + nblk - 1) / nblk;
     if (vs) {
         const int db = std::max(1, std::min(kDispMaxBlocks, (n + kDispThreads - 1) / kDispThreads));
-        hipLaunchKernelGGL(k_disp, dim3(db), dim3(kDispThreads), 0, c->stream, coords, w.x_ref, n, g.box[0],
-                           g.box[1], g.box[2], (0.5 * c->skin) * (0.5 * c->skin), 0, vs, w.flags);
+        hipLaunchKernelGGL(k_disp, dim3(db), dim3(kDispThreads), 0, c->stream, coords, w.x_ref, n, g.cell,
+                           (0.5 * c->skin) * (0.5 * c->skin), 0, vs, w.flags);
     }
-    hipLaunchKernelGGL(k_bin, dim3(nblk), dim3(kBinThreads), 0, c->stream, coords, n, chunk, g.box[0], g.box[1],
-                       g.box[2], g.ncx, g.ncy, g.sx, g.sy, w.wrapped, w.col_of, w.col_count, w.block_off, ex,
-                       vs ? w.x_ref : nullptr, vs);
-    const double zscale = 4294967296.0 / g.box[2];
+    hipLaunchKernelGGL(k_bin, dim3(nblk), dim3(kBinThreads), 0, c->stream, coords, n, chunk, g.cell, g.ncx, g.ncy,
+                       g.sx, g.sy, w.wrapped, w.col_of, w.col_count, w.block_off, ex, vs ? w.x_ref : nullptr, vs);
+    // the sort key is the wrapped z over the cell height
+    const double zscale = 4294967296.0 / (g.cell.tri ? g.cell.cz : g.box[2]);
     hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kBinThreads), 0, c->stream, n, chunk, ncol, w.col_of, w.wrapped,
                        zscale, w.col_count, w.block_off, w.col_key, w.col_start, w.col_cstart, w.flags, ex, vs);
     PmeSlots pme{};
     const bool with_pme = c->params.semantics == MDGPU_SEM_AMBER && w.pme_k[0] > 0;
-    for (int d = 0; d < 3; ++d) {
-        pme.k[d] = with_pme ? w.pme_k[d] : 0;
-        pme.l[d] = g.box[d];
-    }
+    for (int d = 0; d < 3; ++d) pme.k[d] = with_pme ? w.pme_k[d] : 0;
+    pme.cell = g.cell;
     pme.fl = w.slot_pme_fl;
     const int rb = (c->n_clusters * kCluster + kRefreshThreads - 1) / kRefreshThreads;
     if (mixed) {
@@ -975,7 +1017,7 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl
                            w.col_key, w.wrapped, w.col_count, o, vs);
         if (vs)
             hipLaunchKernelGGL(k_refresh<float>, dim3(rb), dim3(kRefreshThreads), 0, c->stream, vs,
-                               w.col_cstart + ncol, coords, w.x_ref, w.wrapped, o, g.box[0], g.box[1], g.box[2]);
+                               w.col_cstart + ncol, coords, w.x_ref, w.wrapped, o, g.cell);
     } else {
         pme.w = w.slot_pme_wd;
         const ClusterOut<double> o{c->n_clusters, t.charge, t.type, t.n_types, w.slot_atom, w.atom_slot, w.cluster_col,
@@ -984,7 +1026,7 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl
                            w.col_cstart, w.col_key, w.wrapped, w.col_count, o, vs);
         if (vs)
             hipLaunchKernelGGL(k_refresh<double>, dim3(rb), dim3(kRefreshThreads), 0, c->stream, vs,
-                               w.col_cstart + ncol, coords, w.x_ref, w.wrapped, o, g.box[0], g.box[1], g.box[2]);
+                               w.col_cstart + ncol, coords, w.x_ref, w.wrapped, o, g.cell);
     }
     return hipGetLastError();
 }
@@ -994,8 +1036,8 @@ hipError_t launch_disp_query(mdgpu_ctx* c, const double* coords) {
     const Grid& g = c->grid;
     const int n = c->topo.n_atoms;
     const int db = std::max(1, std::min(kDispMaxBlocks, (n + kDispThreads - 1) / kDispThreads));
-    hipLaunchKernelGGL(k_disp, dim3(db), dim3(kDispThreads), 0, c->stream, coords, w.x_ref, n, g.box[0], g.box[1],
-                       g.box[2], (0.5 * c->skin) * (0.5 * c->skin), 1, w.verlet, w.flags);
+    hipLaunchKernelGGL(k_disp, dim3(db), dim3(kDispThreads), 0, c->stream, coords, w.x_ref, n, g.cell,
+                       (0.5 * c->skin) * (0.5 * c->skin), 1, w.verlet, w.flags);
     return hipGetLastError();
 }
 
@@ -1016,11 +1058,12 @@ hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber, const VerletS
     a.ncy = g.ncy;
     a.dcx = g.dcx;
     a.dcy = g.dcy;
-    a.sx = g.sx;
-    a.sy = g.sy;
+    a.sx = g.wx;
+    a.sy = g.wy;
     a.lx = g.box[0];
     a.ly = g.box[1];
-    a.lz = g.box[2];
+    a.lz = g.cell.tri ? g.cell.cz : g.box[2];
+    a.cell = g.cell;
     const double rl = g.r_list;
     a.vs = vs;
     a.rl2 = rl * rl;
@@ -1039,7 +1082,8 @@ hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber, const VerletS
     // half-extent bound of the minimum-image search in each dimension
     auto hlim = [&](double L) { return 0.5 * (0.5 * L - rl); };
     const double hx = hlim(g.box[0]), hy = hlim(g.box[1]), hz = hlim(g.box[2]);
-    const bool fast = !c->search_general && hx > 1.0 && hy > 1.0 && hz > 1.0;
+    // the fast search takes minimum images along the axes: orthorhombic cells only
+    const bool fast = !c->search_general && !g.cell.tri && hx > 1.0 && hy > 1.0 && hz > 1.0;
     if (fast) {
         if (mixed)
             hipLaunchKernelGGL(k_search_mi<float>, dim3(nb), dim3(kSearchWaves * kWave), 0, c->stream, a, hx, hy, hz);

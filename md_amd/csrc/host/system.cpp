@@ -232,6 +232,12 @@ void System::apply_settings() {
     if (st != MDGPU_OK) fail(st, "mdgpu_set_params");
     st = mdgpu_set_verlet(ctx_, std::max({0.0, lj_skin_, ee_skin_}));
     if (st != MDGPU_OK) fail(st, "mdgpu_set_verlet");
+    // AMBER semantics take the cell angles of the coordinate file; REF
+    // semantics keep the reference's orthorhombic cell (PBC.cpp:72-81)
+    const std::array<double, 3> right = {90.0, 90.0, 90.0};
+    const std::array<double, 3> ang = semantics_ == Semantics::AMBER ? topology_.get_box_angles() : right;
+    st = mdgpu_set_box_angles(ctx_, ang.data());
+    if (st != MDGPU_OK) fail(st, "mdgpu_set_box_angles");
 }
 
 void System::evaluate() {

@@ -143,6 +143,8 @@ def gpu_lib() -> C.CDLL:
         lib.mdgpu_set_graph.argtypes = [C.c_void_p, C.c_int]
         lib.mdgpu_graph_state.argtypes = [C.c_void_p]
         lib.mdgpu_set_verlet.argtypes = [C.c_void_p, C.c_double]
+        lib.mdgpu_set_box_angles.argtypes = [C.c_void_p, P_F64]
+        lib.mdgpu_get_box_angles.argtypes = [C.c_void_p, P_F64]
         lib.mdgpu_set_virial.argtypes = [C.c_void_p, C.c_int]
         lib.mdgpu_get_virial.argtypes = [C.c_void_p, P_F64]
         lib.mdgpu_md_set_masses.argtypes = [C.c_void_p, P_F64]
@@ -536,6 +538,17 @@ class Evaluator:
         """Keep the pair list across evaluations with this skin (A) until an atom
         has moved more than skin / 2 since the build; 0 builds it every time."""
         self._check(self.lib.mdgpu_set_verlet(self.ctx, float(skin)))
+
+    def set_box_angles(self, angles):
+        """Cell angles alpha, beta, gamma in degrees for every later call (the
+        box arguments then give the edge lengths); 90, 90, 90 is orthorhombic."""
+        a = np.ascontiguousarray(angles, dtype=np.float64).reshape(3)
+        self._check(self.lib.mdgpu_set_box_angles(self.ctx, a.ctypes.data_as(P_F64)))
+
+    def box_angles(self) -> np.ndarray:
+        a = np.zeros(3)
+        self._check(self.lib.mdgpu_get_box_angles(self.ctx, a.ctypes.data_as(P_F64)))
+        return a
 
     def list_builds(self) -> tuple:
         """(list builds, successful evaluations) since the system upload."""

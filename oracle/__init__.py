@@ -5,6 +5,7 @@ cpu_baseline leg of bench.py as the checker.  The product never imports it.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 
@@ -50,6 +51,10 @@ def lib() -> C.CDLL:
         L.oracle_md_run.argtypes = [C.POINTER(SystemDesc), C.POINTER(Params), P_F64, P_F64, P_F64, P_F64, P_F64,
                                     C.c_double, C.c_int, P_F64]
         L.oracle_coulomb_constant.restype = C.c_double
+        L.oracle_set_box_angles.argtypes = [P_F64]
+        L.oracle_set_box_angles.restype = C.c_int
+        L.oracle_cell_matrix.argtypes = [P_F64, P_F64, P_F64]
+        L.oracle_cell_matrix.restype = C.c_int
         _lib = L
     return _lib
 
@@ -58,22 +63,48 @@ def _p(a):
     return a.ctypes.data_as(P_F64)
 
 
+@contextlib.contextmanager
+def _cell(box):
+    """Edge lengths of box (3 values, or 6 with the angles alpha, beta, gamma
+    in degrees, as on an rst7 box line); the angles hold for the duration."""
+    b = np.ascontiguousarray(box, dtype=np.float64).reshape(-1)
+    ang = b[3:6] if b.size >= 6 else np.array([90.0, 90.0, 90.0])
+    ang = np.ascontiguousarray(ang, dtype=np.float64)
+    if lib().oracle_set_box_angles(_p(ang)) != 0:
+        raise ValueError(f"bad cell angles {ang}")
+    try:
+        yield np.ascontiguousarray(b[:3])
+    finally:
+        lib().oracle_set_box_angles(_p(np.array([90.0, 90.0, 90.0])))
+
+
 def evaluate(desc: SystemDesc, params: Params, coords, box, list_skin=2.0, skip_all_pairs=False,
              want_forces=True, full_virial=False):
     """Returns (energies dict, forces (N,3) or None, virial (3,3))."""
     x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
-    b = np.ascontiguousarray(box, dtype=np.float64)
     f = np.zeros(x.size) if want_forces else None
     v = np.zeros(9)
     e = Energies()
     o = Opts(list_skin, 1 if skip_all_pairs else 0, 1 if full_virial else 0)
     if full_virial and f is None:
         f = np.zeros(x.size)
-    st = lib().oracle_eval(C.byref(desc), C.byref(params), _p(x), _p(b), C.byref(o), C.byref(e),
-                           _p(f) if f is not None else None, _p(v))
+    with _cell(box) as b:
+        st = lib().oracle_eval(C.byref(desc), C.byref(params), _p(x), _p(b), C.byref(o), C.byref(e),
+                               _p(f) if f is not None else None, _p(v))
     if st != 0:
         raise RuntimeError("oracle_eval failed")
     return e.as_dict(), (f.reshape(-1, 3) if f is not None else None), v.reshape(3, 3)
+
+
+def cell_matrix(box):
+    """(H with the edge vectors a, b, c as columns, face distances) of a box
+    of 3 lengths or 6 values with the angles."""
+    h = np.zeros(9)
+    face = np.zeros(3)
+    with _cell(box) as b:
+        if lib().oracle_cell_matrix(_p(b), _p(h), _p(face)) != 0:
+            raise ValueError("the box forms no cell")
+    return h.reshape(3, 3), face
 
 
 def ref_pairlist(desc: SystemDesc, coords, box, list_cutoff):
@@ -89,21 +120,21 @@ def ref_pairlist(desc: SystemDesc, coords, box, list_cutoff):
 def ewald_recip_dense(q, coords, box, alpha, mmax):
     q = np.ascontiguousarray(q, dtype=np.float64)
     x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
-    b = np.ascontiguousarray(box, dtype=np.float64)
     f = np.zeros(x.size)
     v = np.zeros(9)
-    e = lib().oracle_ewald_recip_dense(q.size, _p(q), _p(x), _p(b), alpha, mmax, _p(f), _p(v))
+    with _cell(box) as b:
+        e = lib().oracle_ewald_recip_dense(q.size, _p(q), _p(x), _p(b), alpha, mmax, _p(f), _p(v))
     return e, f.reshape(-1, 3), v.reshape(3, 3)
 
 
 def pme_recip(q, coords, box, alpha, grid, order=4):
     q = np.ascontiguousarray(q, dtype=np.float64)
     x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
-    b = np.ascontiguousarray(box, dtype=np.float64)
     g = (C.c_int * 3)(*grid)
     f = np.zeros(x.size)
     v = np.zeros(9)
-    e = lib().oracle_pme_recip(q.size, _p(q), _p(x), _p(b), alpha, g, order, _p(f), _p(v))
+    with _cell(box) as b:
+        e = lib().oracle_pme_recip(q.size, _p(q), _p(x), _p(b), alpha, g, order, _p(f), _p(v))
     return e, f.reshape(-1, 3), v.reshape(3, 3)
 
 
@@ -145,8 +176,8 @@ def pairs_need_rebuild(coords, coords_ref, box, skin) -> bool:
     """The Verlet rebuild criterion (NonBondedUtils.cpp:87-100)."""
     x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
     r = np.ascontiguousarray(coords_ref, dtype=np.float64).reshape(-1)
-    b = np.ascontiguousarray(box, dtype=np.float64)
-    return bool(lib().oracle_pairs_need_rebuild(x.size // 3, _p(x), _p(r), _p(b), float(skin)))
+    with _cell(box) as b:
+        return bool(lib().oracle_pairs_need_rebuild(x.size // 3, _p(x), _p(r), _p(b), float(skin)))
 
 
 def md_run(desc: SystemDesc, params: Params, masses, x, v, f, box, dt, nsteps):
@@ -155,10 +186,10 @@ def md_run(desc: SystemDesc, params: Params, masses, x, v, f, box, dt, nsteps):
     x = np.array(x, dtype=np.float64).reshape(-1)
     v = np.array(v, dtype=np.float64).reshape(-1)
     f = np.array(f, dtype=np.float64).reshape(-1)
-    b = np.ascontiguousarray(box, dtype=np.float64)
     tr = np.zeros(2 * max(nsteps, 1))
-    st = lib().oracle_md_run(C.byref(desc), C.byref(params), _p(m), _p(x), _p(v), _p(f), _p(b), float(dt),
-                             int(nsteps), _p(tr))
+    with _cell(box) as b:
+        st = lib().oracle_md_run(C.byref(desc), C.byref(params), _p(m), _p(x), _p(v), _p(f), _p(b), float(dt),
+                                 int(nsteps), _p(tr))
     if st != 0:
         raise RuntimeError("oracle_md_run failed")
     return x.reshape(-1, 3), v.reshape(-1, 3), f.reshape(-1, 3), tr[:2 * nsteps].reshape(-1, 2)
@@ -167,11 +198,11 @@ def md_run(desc: SystemDesc, params: Params, masses, x, v, f, box, dt, nsteps):
 def minimize(desc: SystemDesc, params: Params, x, box, max_iter=500, rms_tol=1e-2, max_step=0.2, history=8):
     """L-BFGS; returns (x, energies per accepted iteration, result dict)."""
     x = np.array(x, dtype=np.float64).reshape(-1)
-    b = np.ascontiguousarray(box, dtype=np.float64)
     en = np.zeros(max_iter + 2)
     out = np.zeros(5)
-    st = lib().oracle_minimize(C.byref(desc), C.byref(params), _p(x), _p(b), int(max_iter), float(rms_tol),
-                               float(max_step), int(history), _p(en), _p(out))
+    with _cell(box) as b:
+        st = lib().oracle_minimize(C.byref(desc), C.byref(params), _p(x), _p(b), int(max_iter), float(rms_tol),
+                                   float(max_step), int(history), _p(en), _p(out))
     if st != 0:
         raise RuntimeError("oracle_minimize failed")
     it = int(out[0])

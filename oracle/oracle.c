@@ -61,6 +61,110 @@ static void min_image(double* d, const double box[3], int pbc) {
     d[2] = min_image_1d(d[2], box[2]);
 }
 
+/* ---------------------------------------------------- triclinic cells ---- */
+
+/* The reference's periodic boundary is orthorhombic only (PBC.cpp:72-81).
+ * A triclinic cell has edge vectors a = (ax,0,0), b = (bx,by,0), c =
+ * (cx,cy,cz), built from the edge lengths and the angles alpha (b,c), beta
+ * (a,c), gamma (a,b) as on an rst7 box line; H has a, b, c as columns and
+ * R = H^-1 is upper triangular, so s = R x are fractional coordinates.  An
+ * angle of exactly 90 degrees has cosine 0; three of them make the cell
+ * orthorhombic and every routine takes its orthorhombic path. */
+typedef struct {
+    int tri;
+    double L[3];
+    double ax, bx, by, cx, cy, cz;
+    double r00, r01, r02, r11, r12, r22;
+    double face[3]; /* distances between opposite faces */
+} ocell;
+
+static double g_angles[3] = {90.0, 90.0, 90.0};
+
+int oracle_set_box_angles(const double ang[3]) {
+    for (int d = 0; d < 3; ++d)
+        if (!(ang[d] > 0.0 && ang[d] < 180.0)) return 1;
+    for (int d = 0; d < 3; ++d) g_angles[d] = ang[d];
+    return 0;
+}
+
+static double cos_deg(double a) { return a == 90.0 ? 0.0 : cos(a * (M_PI / 180.0)); }
+
+/* Returns 0 when the lengths and angles form a cell. */
+static int ocell_make(ocell* c, const double box[3]) {
+    memset(c, 0, sizeof *c);
+    for (int d = 0; d < 3; ++d) c->L[d] = c->face[d] = box[d];
+    c->tri = !(g_angles[0] == 90.0 && g_angles[1] == 90.0 && g_angles[2] == 90.0);
+    if (!c->tri) return 0;
+    const double ca = cos_deg(g_angles[0]), cb = cos_deg(g_angles[1]), cg = cos_deg(g_angles[2]);
+    const double sg = g_angles[2] == 90.0 ? 1.0 : sin(g_angles[2] * (M_PI / 180.0));
+    if (!(sg > 1e-6)) return 1;
+    c->ax = box[0];
+    c->bx = box[1] * cg;
+    c->by = box[1] * sg;
+    c->cx = box[2] * cb;
+    c->cy = box[2] * (ca - cb * cg) / sg;
+    const double cz2 = box[2] * box[2] - c->cx * c->cx - c->cy * c->cy;
+    if (!(cz2 > 1e-12 * box[2] * box[2])) return 1;
+    c->cz = sqrt(cz2);
+    c->r00 = 1.0 / c->ax;
+    c->r11 = 1.0 / c->by;
+    c->r22 = 1.0 / c->cz;
+    c->r01 = -c->bx / (c->ax * c->by);
+    c->r12 = -c->cy / (c->by * c->cz);
+    c->r02 = (c->bx * c->cy - c->by * c->cx) / (c->ax * c->by * c->cz);
+    c->face[0] = 1.0 / sqrt(c->r00 * c->r00 + c->r01 * c->r01 + c->r02 * c->r02);
+    c->face[1] = 1.0 / sqrt(c->r11 * c->r11 + c->r12 * c->r12);
+    c->face[2] = c->cz;
+    return 0;
+}
+
+int oracle_cell_matrix(const double box[3], double h[9], double face[3]) {
+    ocell c;
+    if (ocell_make(&c, box)) return 1;
+    const double m[9] = {c.tri ? c.ax : c.L[0], c.bx, c.cx, 0.0, c.tri ? c.by : c.L[1], c.cy, 0.0, 0.0,
+                         c.tri ? c.cz : c.L[2]};
+    for (int k = 0; k < 9; ++k) h[k] = m[k];
+    for (int d = 0; d < 3; ++d) face[d] = c.face[d];
+    return 0;
+}
+
+static double ocell_volume(const ocell* c) { return c->tri ? c->ax * c->by * c->cz : c->L[0] * c->L[1] * c->L[2]; }
+
+static void ocell_frac(const ocell* c, const double* x, double* s) {
+    s[2] = c->r22 * x[2];
+    s[1] = c->r11 * x[1] + c->r12 * x[2];
+    s[0] = c->r00 * x[0] + c->r01 * x[1] + c->r02 * x[2];
+}
+
+/* Minimum image d - H rint(R d) (triclinic) or the reference's per-axis
+ * rounding (orthorhombic).  With the cutoff below half the smallest face
+ * distance no other image of a pair can lie within it. */
+static void ocell_min_image(const ocell* c, double* d) {
+    if (!c->tri) {
+        min_image(d, c->L, 1);
+        return;
+    }
+    double s[3];
+    ocell_frac(c, d, s);
+    const double n0 = rint(s[0]), n1 = rint(s[1]), n2 = rint(s[2]);
+    d[0] = d[0] - (c->ax * n0 + c->bx * n1 + c->cx * n2);
+    d[1] = d[1] - (c->by * n1 + c->cy * n2);
+    d[2] = d[2] - c->cz * n2;
+}
+
+/* Reciprocal vector m R (Cartesian) of the integer vector m. */
+static void ocell_recip(const ocell* c, double ma, double mb, double mc, double* m) {
+    if (!c->tri) {
+        m[0] = ma / c->L[0];
+        m[1] = mb / c->L[1];
+        m[2] = mc / c->L[2];
+        return;
+    }
+    m[0] = ma * c->r00;
+    m[1] = ma * c->r01 + mb * c->r11;
+    m[2] = ma * c->r02 + mb * c->r12 + mc * c->r22;
+}
+
 /* ------------------------------------------------------- Verlet list ---- */
 
 /* setup_neighbor_rebuild_threshold_lj (NonBondedUtils.cpp:65-69) sets the
@@ -70,9 +174,11 @@ static void min_image(double* d, const double box[3], int pbc) {
 int oracle_pairs_need_rebuild(int n, const double* x, const double* xref, const double box[3], double skin) {
     const double half = 0.5 * skin, lim2 = half * half;
     double max_dr2 = 0.0;
+    ocell cell;
+    if (ocell_make(&cell, box)) return -1;
     for (int i = 0; i < n; ++i) {
         double d[3] = {x[3 * i] - xref[3 * i], x[3 * i + 1] - xref[3 * i + 1], x[3 * i + 2] - xref[3 * i + 2]};
-        min_image(d, box, 1);
+        ocell_min_image(&cell, d);
         const double dr2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         if (dr2 > max_dr2) max_dr2 = dr2;
         if (max_dr2 > lim2) return 1;
@@ -658,8 +764,9 @@ static void ref_nonbonded(const mdgpu_system_desc* s, const mdgpu_params* p, con
 
 /* ------------------------------------------------------ AMBER nonbonded -- */
 
-static void amber_nonbonded(const mdgpu_system_desc* s, const mdgpu_params* p, const double* x, const double box[3],
+static void amber_nonbonded(const mdgpu_system_desc* s, const mdgpu_params* p, const double* x, const ocell* cell,
                             mdgpu_energies* e, double* f) {
+    const double* box = cell->L;
     const double kc = (double)(SQRT_KC * SQRT_KC);
     int n = s->n_atoms;
     double rc = p->lj_cutoff > p->ee_cutoff ? p->lj_cutoff : p->ee_cutoff;
@@ -667,10 +774,12 @@ static void amber_nonbonded(const mdgpu_system_desc* s, const mdgpu_params* p, c
     double alpha = p->ewald_alpha, a2 = alpha * alpha, tsp = 2.0 * alpha / sqrt(M_PI);
     int use_sw = p->lj_switch > 0.0;
     /* cell grid over the wrapped coordinates; cells >= rc per side so the 27
-     * neighbour cells contain every minimum-image partner within rc */
+     * neighbour cells contain every minimum-image partner within rc (a
+     * triclinic cell is cut along its fractional coordinates, into slices at
+     * least rc apart between faces) */
     int nc[3];
     for (int d = 0; d < 3; ++d) {
-        nc[d] = (int)floor(box[d] / rc);
+        nc[d] = (int)floor(cell->face[d] / rc);
         if (nc[d] < 1) nc[d] = 1;
     }
     int ncell = nc[0] * nc[1] * nc[2];
@@ -679,9 +788,18 @@ static void amber_nonbonded(const mdgpu_system_desc* s, const mdgpu_params* p, c
     int* cidx = (int*)malloc(sizeof(int) * n);
     for (int c = 0; c < ncell; ++c) head[c] = -1;
     for (int i = 0; i < n; ++i) {
-        int ix = cell_of(wrap1(x[3 * i], box[0]), box[0], nc[0]);
-        int iy = cell_of(wrap1(x[3 * i + 1], box[1]), box[1], nc[1]);
-        int iz = cell_of(wrap1(x[3 * i + 2], box[2]), box[2], nc[2]);
+        int ix, iy, iz;
+        if (!cell->tri) {
+            ix = cell_of(wrap1(x[3 * i], box[0]), box[0], nc[0]);
+            iy = cell_of(wrap1(x[3 * i + 1], box[1]), box[1], nc[1]);
+            iz = cell_of(wrap1(x[3 * i + 2], box[2]), box[2], nc[2]);
+        } else {
+            double sf[3];
+            ocell_frac(cell, x + 3 * i, sf);
+            ix = cell_of(wrap1(sf[0], 1.0), 1.0, nc[0]);
+            iy = cell_of(wrap1(sf[1], 1.0), 1.0, nc[1]);
+            iz = cell_of(wrap1(sf[2], 1.0), 1.0, nc[2]);
+        }
         cidx[i] = ix + nc[0] * (iy + nc[1] * iz);
         next[i] = head[cidx[i]];
         head[cidx[i]] = i;
@@ -702,7 +820,7 @@ static void amber_nonbonded(const mdgpu_system_desc* s, const mdgpu_params* p, c
                 if (j <= i) continue;
                 if (excluded(s, i, j)) continue;
                 double d[3] = {x[3 * i] - x[3 * j], x[3 * i + 1] - x[3 * j + 1], x[3 * i + 2] - x[3 * j + 2]};
-                min_image(d, box, 1);
+                ocell_min_image(cell, d);
                 double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
                 if (r2 < 1e-12) continue;
                 double g = 0.0;
@@ -745,7 +863,7 @@ static void amber_nonbonded(const mdgpu_system_desc* s, const mdgpu_params* p, c
     for (int t = 0; t < s->n_excl_pairs; ++t) {
         int i = s->excl_pairs[2 * t], j = s->excl_pairs[2 * t + 1];
         double d[3] = {x[3 * i] - x[3 * j], x[3 * i + 1] - x[3 * j + 1], x[3 * i + 2] - x[3 * j + 2]};
-        min_image(d, box, 1);
+        ocell_min_image(cell, d);
         double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2], r = sqrt(r2);
         double qq = kc * s->charge[i] * s->charge[j];
         double ef = erf(alpha * r);
@@ -762,7 +880,7 @@ static void amber_nonbonded(const mdgpu_system_desc* s, const mdgpu_params* p, c
     for (int t = 0; t < s->n_pairs14; ++t) {
         int i = s->pairs14[2 * t], j = s->pairs14[2 * t + 1];
         double d[3] = {x[3 * i] - x[3 * j], x[3 * i + 1] - x[3 * j + 1], x[3 * i + 2] - x[3 * j + 2]};
-        min_image(d, box, 1);
+        ocell_min_image(cell, d);
         double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2], r = sqrt(r2);
         int pidx = s->nb_index[s->type[i] * s->n_types + s->type[j]] - 1;
         double g = 0.0;
@@ -875,9 +993,20 @@ double oracle_pme_potential(int n, const double* q, const double* x, const doubl
     return pme_core(n, q, x, box, alpha, K, order, NULL, NULL, pot);
 }
 
+/* Fractional grid coordinate u = K s of atom i along dimension d (u = K x / L
+ * wrapped, in an orthorhombic cell). */
+static double pme_u(const ocell* cell, const double* xi, int d, int K) {
+    if (!cell->tri) return K * wrap1(xi[d], cell->L[d]) / cell->L[d];
+    double sf[3];
+    ocell_frac(cell, xi, sf);
+    return K * wrap1(sf[d], 1.0);
+}
+
 static double pme_core(int n, const double* q, const double* x, const double box[3], double alpha, const int K[3],
                        int order, double* f, double* vir, double* pot) {
     const double kc = (double)(SQRT_KC * SQRT_KC);
+    ocell cell;
+    if (ocell_make(&cell, box)) return NAN;
     size_t G = (size_t)K[0] * K[1] * K[2];
     double* re = (double*)calloc(G, sizeof(double));
     double* im = (double*)calloc(G, sizeof(double));
@@ -885,7 +1014,7 @@ static double pme_core(int n, const double* q, const double* x, const double box
     int base[3];
     for (int i = 0; i < n; ++i) {
         for (int d = 0; d < 3; ++d) {
-            double u = K[d] * wrap1(x[3 * i + d], box[d]) / box[d];
+            double u = pme_u(&cell, x + 3 * i, d, K[d]);
             int fl = (int)floor(u);
             oracle_bspline(order, u - fl, th[d], dth[d]);
             base[d] = fl;
@@ -918,7 +1047,7 @@ static double pme_core(int n, const double* q, const double* x, const double box
             bm[d][m] = 1.0 / (sr * sr + si * si);
         }
     }
-    double V = box[0] * box[1] * box[2];
+    double V = ocell_volume(&cell);
     double e = 0.0, W[6] = {0, 0, 0, 0, 0, 0};
     for (int a = 0; a < K[0]; ++a) {
         int ma = a <= K[0] / 2 ? a : a - K[0];
@@ -929,7 +1058,18 @@ static double pme_core(int n, const double* q, const double* x, const double box
                 int mc = c <= K[2] / 2 ? c : c - K[2];
                 size_t id = ((size_t)a * K[1] + b) * K[2] + c;
                 if (a == 0 && b == 0 && c == 0) { re[id] = 0; im[id] = 0; continue; }
-                double mx = ma / box[0], my = mb / box[1], mz = mc / box[2];
+                double mv[3];
+                if (!cell.tri || mc >= 0 || 2 * c == K[2]) {
+                    ocell_recip(&cell, ma, mb, mc, mv);
+                } else {
+                    /* m_z < 0: the negative of the conjugate point's vector,
+                     * so that m and -m pair up on the Nyquist planes too */
+                    const int a2 = (K[0] - a) % K[0], b2 = (K[1] - b) % K[1];
+                    int ma2 = a2 <= K[0] / 2 ? a2 : a2 - K[0];
+                    int mb2 = b2 <= K[1] / 2 ? b2 : b2 - K[1];
+                    ocell_recip(&cell, -ma2, -mb2, mc, mv);
+                }
+                double mx = mv[0], my = mv[1], mz = mv[2];
                 double m2 = mx * mx + my * my + mz * mz;
                 double g = kc * exp(-M_PI * M_PI * m2 / (alpha * alpha)) / (M_PI * V * m2) * bm[0][a] * bm[1][b] * bm[2][c];
                 double s2 = re[id] * re[id] + im[id] * im[id];
@@ -957,7 +1097,7 @@ static double pme_core(int n, const double* q, const double* x, const double box
         for (int ax = 0; ax < 3; ++ax) dft_axis(re, im, K, ax, +1);
         for (int i = 0; i < n; ++i) {
             for (int d = 0; d < 3; ++d) {
-                double u = K[d] * wrap1(x[3 * i + d], box[d]) / box[d];
+                double u = pme_u(&cell, x + 3 * i, d, K[d]);
                 int fl = (int)floor(u);
                 oracle_bspline(order, u - fl, th[d], dth[d]);
                 base[d] = fl;
@@ -978,10 +1118,16 @@ static double pme_core(int n, const double* q, const double* x, const double box
                 }
             }
             if (pot) pot[i] = ph;
-            if (f) {
+            if (f && !cell.tri) {
                 f[3 * i] -= q[i] * fx * K[0] / box[0];
                 f[3 * i + 1] -= q[i] * fy * K[1] / box[1];
                 f[3 * i + 2] -= q[i] * fz * K[2] / box[2];
+            } else if (f) {
+                /* dE/dx = R^T dE/ds, dE/ds_d = K_d dE/du_d */
+                const double ux = fx * K[0], uy = fy * K[1], uz = fz * K[2];
+                f[3 * i] -= q[i] * (ux * cell.r00);
+                f[3 * i + 1] -= q[i] * (ux * cell.r01 + uy * cell.r11);
+                f[3 * i + 2] -= q[i] * (ux * cell.r02 + uy * cell.r12 + uz * cell.r22);
             }
         }
     }
@@ -994,7 +1140,9 @@ static double pme_core(int n, const double* q, const double* x, const double box
 double oracle_ewald_recip_dense(int n, const double* q, const double* x, const double box[3], double alpha, int mmax,
                                 double* f, double* vir) {
     const double kc = (double)(SQRT_KC * SQRT_KC);
-    double V = box[0] * box[1] * box[2];
+    ocell cell;
+    if (ocell_make(&cell, box)) return NAN;
+    double V = ocell_volume(&cell);
     double e = 0.0, W[6] = {0, 0, 0, 0, 0, 0};
     double* c = (double*)malloc(sizeof(double) * n);
     double* s = (double*)malloc(sizeof(double) * n);
@@ -1002,7 +1150,9 @@ double oracle_ewald_recip_dense(int n, const double* q, const double* x, const d
         for (int b = -mmax; b <= mmax; ++b)
             for (int cc = -mmax; cc <= mmax; ++cc) {
                 if (a == 0 && b == 0 && cc == 0) continue;
-                double mx = a / box[0], my = b / box[1], mz = cc / box[2];
+                double mv[3];
+                ocell_recip(&cell, a, b, cc, mv);
+                double mx = mv[0], my = mv[1], mz = mv[2];
                 double m2 = mx * mx + my * my + mz * mz;
                 double g = kc * exp(-M_PI * M_PI * m2 / (alpha * alpha)) / (M_PI * V * m2);
                 double sr = 0, si = 0;
@@ -1054,6 +1204,9 @@ int oracle_eval(const mdgpu_system_desc* s, const mdgpu_params* p, const double*
     if (f) memset(f, 0, sizeof(double) * 3 * (size_t)n);
     if (vir) memset(vir, 0, sizeof(double) * 9);
     int amber = p->semantics == MDGPU_SEM_AMBER;
+    ocell cell;
+    if (ocell_make(&cell, box)) return 2;
+    if (cell.tri && !amber) return 3; /* triclinic cells: AMBER semantics only */
     g_amber_torsion = amber;
     /* full virial: bonded forces into their own array (each term's forces sum
      * to zero, so sum_i x_i f_i is its virial with the coordinates as given),
@@ -1089,7 +1242,7 @@ int oracle_eval(const mdgpu_system_desc* s, const mdgpu_params* p, const double*
         ref_nonbonded(s, p, x, box, opt, e, f);
         e->total = bonded + e->vdw + e->ee_pairlist;
     } else {
-        amber_nonbonded(s, p, x, box, e, f);
+        amber_nonbonded(s, p, x, &cell, e, f);
         if (p->pme_grid[0] > 0) {
             double wr[9];
             e->ewald_recip = oracle_pme_recip(n, s->charge, x, box, p->ewald_alpha, p->pme_grid, p->pme_order, f,
@@ -1142,6 +1295,7 @@ static void lj_unit(const mdgpu_params* p, double r2, double* ea, double* eb) {
 int oracle_param_grads(const mdgpu_system_desc* s, const mdgpu_params* p, const double* x, const double box[3],
                        const mdgpu_energies* weights, mdgpu_param_grads* g) {
     if (!s || !p || !x || !weights || !g || p->semantics != MDGPU_SEM_AMBER) return 1;
+    if (!(g_angles[0] == 90.0 && g_angles[1] == 90.0 && g_angles[2] == 90.0)) return 1; /* orthorhombic only */
     const double kc = (double)(SQRT_KC * SQRT_KC);
     mdgpu_energies w;
     eff_weights(weights, &w);

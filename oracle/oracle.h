@@ -24,6 +24,17 @@ typedef struct {
     int full_virial;       /* 1: virial = bonded + pair + correction + recip   */
 } oracle_opts;
 
+/* Cell angles alpha, beta, gamma (degrees) for every later call, with the
+ * box arguments as edge lengths (default 90, 90, 90: orthorhombic).  A
+ * triclinic cell is AMBER semantics only; oracle_param_grads refuses it.
+ * File-scope state: the oracle is single-threaded.  Returns 0, or 1 for an
+ * angle outside (0, 180). */
+int oracle_set_box_angles(const double angles[3]);
+/* The cell matrix H (row-major, edge vectors as columns) and the distances
+ * between opposite faces for edge lengths box and the current angles.
+ * Returns 1 when they form no cell. */
+int oracle_cell_matrix(const double box[3], double h[9], double face[3]);
+
 /* One evaluation.  Returns 0 on success. */
 int oracle_eval(const mdgpu_system_desc* sys, const mdgpu_params* p, const double* x,
                 const double box[3], const oracle_opts* opt, mdgpu_energies* e,

@@ -1,0 +1,205 @@
+"""Triclinic cells on the GPU (mdgpu_set_box_angles) against the oracle.
+
+Two cells hold the 2.1k-atom polymer: the genuinely triclinic cell of edges
+30 A and angles 80, 85, 75 degrees (the cubic system carried into it by
+x' = H x / 30), and its own cubic box written in the sheared basis a, a + b,
+a + b + c (every cell-matrix entry non-zero).  Bars, as for orthorhombic
+boxes:
+  * FP64 vs oracle: per-term |dE| <= 1e-9 |E|, force relative RMS <= 1e-8;
+    the full virial within 1e-9 of its largest component;
+  * mixed vs FP64: force relative RMS <= 1e-5;
+  * a Verlet list kept across moves below skin / 2 gives the oracle's result
+    at the moved positions (FP64 1e-9) and is built once;
+  * the rebuild criterion takes triclinic minimum images: a move of 0.99 and
+    1.01 times skin / 2 plus a lattice vector answers as the oracle does;
+  * twelve FP64 velocity-Verlet steps with a Verlet list follow the oracle's
+    trajectory (positions 1e-9 A);
+  * REF semantics, the adjoint and faces closer than twice the cutoff are
+    refused with an error.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from md_amd import native as N
+
+from md_systems import thermal_velocities
+from test_gpu_parity import TERMS_AMBER, check_terms, rel_rms
+from tri_systems import cell_vectors, mapped, sheared_box
+
+pytestmark = pytest.mark.gpu
+
+TRI_ANGLES = [80.0, 85.0, 75.0]
+KW = dict(lj_cutoff=10.0, lj_switch=9.0, ewald_tol=1e-5, pme_grid=(32, 32, 32))
+
+
+def _cell(polymer_2k, kind):
+    prm_path, rst_path = polymer_2k
+    top = N.Topology(prm_path, rst_path, N.SEM_AMBER)
+    edge = float(top.box[0])
+    if kind == "triclinic":
+        box6 = [edge, edge, edge] + TRI_ANGLES
+        x = mapped(top.coords, edge, box6)
+    else:
+        box6 = sheared_box(top.box)
+        x = np.asarray(top.coords, dtype=np.float64).reshape(-1, 3)
+    return top, np.ascontiguousarray(x), [float(v) for v in box6]
+
+
+def _ev(top, prm, box6, skin=0.0, virial=False):
+    ev = N.Evaluator(0)
+    ev.upload(top.desc)
+    ev.set_params(prm)
+    ev.set_box_angles(box6[3:])
+    ev.set_verlet(skin)
+    ev.set_virial(virial)
+    return ev
+
+
+def _dev(x):
+    t = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64).reshape(-1)).to("cuda:0")
+    torch.cuda.synchronize()
+    return t
+
+
+@pytest.mark.parametrize("kind", ["triclinic", "sheared"])
+def test_fp64_matches_oracle(polymer_2k, kind):
+    print(f"case polymer_2k {kind} cell FP64 vs oracle", flush=True)
+    top, x, box6 = _cell(polymer_2k, kind)
+    prm = N.make_params(N.SEM_AMBER, N.PREC_FP64, **KW)
+    ev = _ev(top, prm, box6, virial=True)
+    try:
+        assert np.array_equal(ev.box_angles(), box6[3:])
+        e, f, _ = ev.evaluate(x, box6[:3])
+        w = ev.virial()
+    finally:
+        ev.close()
+    eo, fo, wo = oracle.evaluate(top.desc, prm, x, box6, skip_all_pairs=True, full_virial=True)
+    check_terms(e, eo, TERMS_AMBER, 1e-9)
+    r = rel_rms(f, fo)
+    dw = np.abs(w - wo).max() / np.abs(wo).max()
+    print(f"  force rel-RMS {r:.2e}, virial {dw:.2e}")
+    assert r <= 1e-8
+    assert dw <= 1e-9
+
+
+@pytest.mark.parametrize("kind", ["triclinic", "sheared"])
+def test_mixed_matches_fp64(polymer_2k, kind):
+    print(f"case polymer_2k {kind} cell mixed vs FP64", flush=True)
+    top, x, box6 = _cell(polymer_2k, kind)
+    out = []
+    for prec in (N.PREC_FP64, N.PREC_MIXED):
+        ev = _ev(top, N.make_params(N.SEM_AMBER, prec, **KW), box6)
+        try:
+            out.append(ev.evaluate(x, box6[:3]))
+        finally:
+            ev.close()
+    (e64, f64, _), (e32, f32, _) = out
+    r = rel_rms(f32, f64)
+    print(f"  force rel-RMS {r:.2e}, total {e32['total']:.6f} vs {e64['total']:.6f}")
+    assert r <= 1e-5
+    assert abs(e32["total"] - e64["total"]) <= 1e-5 * sum(abs(e64[k]) for k in TERMS_AMBER if k != "total")
+
+
+def test_verlet_list_follows_moves(polymer_2k):
+    print("case polymer_2k triclinic cell, Verlet list kept across moves", flush=True)
+    top, x0, box6 = _cell(polymer_2k, "triclinic")
+    prm = N.make_params(N.SEM_AMBER, N.PREC_FP64, **KW)
+    rng = np.random.default_rng(17)
+    moves = rng.normal(0.0, 1.0, x0.shape)
+    moves *= 0.45 / np.linalg.norm(moves, axis=1).max()  # largest move 0.45 A < skin / 2
+    x1 = x0 + moves
+    ev = _ev(top, prm, box6, skin=1.0)
+    try:
+        ev.evaluate(x0, box6[:3])
+        e, f, _ = ev.evaluate(x1, box6[:3])
+        builds = ev.list_builds()
+    finally:
+        ev.close()
+    print(f"  builds {builds}")
+    assert builds == (1, 2)
+    eo, fo, _ = oracle.evaluate(top.desc, prm, x1, box6, skip_all_pairs=True)
+    check_terms(e, eo, TERMS_AMBER, 1e-9)
+    assert rel_rms(f, fo) <= 1e-8
+
+
+def test_rebuild_criterion_takes_triclinic_images(polymer_2k):
+    print("case polymer_2k triclinic cell rebuild criterion", flush=True)
+    top, x0, box6 = _cell(polymer_2k, "triclinic")
+    prm = N.make_params(N.SEM_AMBER, N.PREC_FP64, **KW)
+    skin = 1.0
+    h = cell_vectors(box6)
+    u = np.array([0.3, -0.5, 0.81])
+    u /= np.linalg.norm(u)
+    ev = _ev(top, prm, box6, skin=skin)
+    try:
+        xd = _dev(x0)
+        fd = torch.zeros_like(xd)
+        ev.evaluate_device(xd.data_ptr(), box6[:3], fd.data_ptr())
+        for frac in (0.99, 1.01):
+            x = x0.copy()
+            x[123] += frac * 0.5 * skin * u + h @ np.array([1.0, -1.0, 2.0])
+            got = ev.needs_rebuild(_dev(x).data_ptr())
+            want = oracle.pairs_need_rebuild(x, x0, box6, skin)
+            print(f"  move {frac} skin/2 plus a lattice vector: gpu {got} oracle {want}")
+            assert got == want == (frac > 1.0)
+    finally:
+        ev.close()
+
+
+def test_trajectory_matches_oracle(polymer_2k):
+    print("case polymer_2k triclinic cell FP64 trajectory vs oracle", flush=True)
+    top, x0, box6 = _cell(polymer_2k, "triclinic")
+    prm = N.make_params(N.SEM_AMBER, N.PREC_FP64, **KW)
+    v0 = thermal_velocities(top.masses, 7, temperature=50.0)
+    dt, n = 0.0002, 12
+    ev = _ev(top, prm, box6, skin=0.8)
+    try:
+        ev.md_set_masses(top.masses)
+        xd, vd = _dev(x0), _dev(v0)
+        fd = torch.zeros_like(xd)
+        ev.evaluate_device(xd.data_ptr(), box6[:3], fd.data_ptr())
+        tr, _ = ev.md_run(xd.data_ptr(), vd.data_ptr(), fd.data_ptr(), box6[:3], dt, n)
+        x = xd.cpu().numpy().reshape(-1, 3)
+    finally:
+        ev.close()
+    _, f0, _ = oracle.evaluate(top.desc, prm, x0, box6, skip_all_pairs=True)
+    xo, _, _, tro = oracle.md_run(top.desc, prm, top.masses, x0, v0, f0, box6, dt, n)
+    dx = np.abs(x - xo).max()
+    print(f"  positions {dx:.2e} A")
+    assert dx <= 1e-9
+    assert np.all(np.abs(tr[:, 0] - tro[:, 0]) <= 1e-9 * np.abs(tro[:, 0]))
+
+
+def test_refusals(polymer_2k):
+    print("case triclinic refusals", flush=True)
+    top, x, box6 = _cell(polymer_2k, "triclinic")
+    prm_ref = N.make_params(N.SEM_REF, N.PREC_FP64, lj_cutoff=10.0, lj_switch=9.0, ewald_tol=1e-5,
+                            pme_grid=(0, 0, 0))
+    ev = _ev(top, prm_ref, box6)
+    try:
+        with pytest.raises(N.GpuError, match="AMBER"):
+            ev.evaluate(x, box6[:3])
+    finally:
+        ev.close()
+    prm = N.make_params(N.SEM_AMBER, N.PREC_FP64, **KW)
+    ev = _ev(top, prm, box6)
+    try:
+        with pytest.raises(N.GpuError, match="orthorhombic"):
+            ev.adjoint(x, box6[:3], {"total": 1.0}, top.desc)
+        with pytest.raises(N.GpuError):
+            ev.set_box_angles([90.0, 90.0, 180.0])
+        big = N.make_params(N.SEM_AMBER, N.PREC_FP64, lj_cutoff=14.5, lj_switch=13.0, ewald_tol=1e-5,
+                            pme_grid=(32, 32, 32))
+        ev.set_params(big)
+        with pytest.raises(N.GpuError, match="faces"):
+            ev.evaluate(x, box6[:3])
+        # back to a right-angled cell: the orthorhombic path again
+        ev.set_params(prm)
+        ev.set_box_angles([90.0, 90.0, 90.0])
+        e, _, _ = ev.evaluate(top.coords, top.box)
+    finally:
+        ev.close()
+    eo, _, _ = oracle.evaluate(top.desc, prm, top.coords, top.box, skip_all_pairs=True)
+    assert abs(e["total"] - eo["total"]) <= 1e-9 * abs(eo["total"])
