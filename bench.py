@@ -180,6 +180,49 @@ def bench_md(torch, N, systems, top, p32, box, steps=200, warmup=20, dt=0.0005):
             "dynamics": legs}
 
 
+def bench_triclinic(torch, N, top, p32, box, reps=200, warmup=20):
+    """The polymer_29k's own cubic lattice written as a triclinic cell with
+    edges a, a + b, a + b + c (mixed form, AMBER, rc 12 A, PME 72^3): the same
+    periodic system through the triclinic path (fractional binning and the
+    lattice-shift search).  Evaluations/s over `reps` after `warmup`, and the
+    relative differences of the direct-space energies from the orthorhombic
+    cell (the reciprocal energy differs by the PME grid, whose lines now run
+    along the sheared edges)."""
+    l1, l2, l3 = (float(v) for v in box)
+    a, b, c = np.array([l1, 0.0, 0.0]), np.array([l1, l2, 0.0]), np.array([l1, l2, l3])
+
+    def ang(u, v):
+        return float(np.degrees(np.arccos(np.dot(u, v) / (np.linalg.norm(u) * np.linalg.norm(v)))))
+
+    lengths = [float(np.linalg.norm(u)) for u in (a, b, c)]
+    angles = [ang(b, c), ang(a, c), ang(a, b)]
+    dev = torch.device("cuda:0")
+    x = torch.tensor(top.coords.reshape(-1), dtype=torch.float64, device=dev)
+    f = torch.zeros_like(x)
+    ev = N.Evaluator(0)
+    try:
+        ev.upload(top.desc)
+        ev.set_params(p32)
+        e_o, _ = ev.evaluate_device(x.data_ptr(), box, f.data_ptr())
+        ev.set_box_angles(angles)
+        for _ in range(warmup):
+            ev.evaluate_device(x.data_ptr(), lengths, f.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            e_t, _ = ev.evaluate_device(x.data_ptr(), lengths, f.data_ptr())
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+    finally:
+        ev.close()
+    return {"workload": "polymer_29k lattice in the sheared basis a, a+b, a+b+c: edges %.2f, %.2f, %.2f A, "
+                        "angles %.2f, %.2f, %.2f deg; mixed form, AMBER, rc 12 A, PME 72^3" % tuple(lengths + angles),
+            "evals_per_s": reps / t,
+            "vdw_rel_diff_vs_orthorhombic": abs(e_t["vdw"] - e_o["vdw"]) / abs(e_o["vdw"]),
+            "ewald_direct_rel_diff_vs_orthorhombic":
+                abs(e_t["ewald_direct"] - e_o["ewald_direct"]) / abs(e_o["ewald_direct"])}
+
+
 def bench_io(N, top, reps=5):
     """Amber restart write and read (coordinates, velocities and box of the
     28,800-atom workload, 6F12.7) through the host library, in MB/s of file."""
@@ -248,6 +291,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-system batched run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 64-frame parameter-gradient run")
     ap.add_argument("--no-md", action="store_true", help="skip the minimisation + dynamics run")
+    ap.add_argument("--triclinic", action="store_true",
+                    help="also time the same system written as a triclinic cell (sheared basis)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy three times (the reference "
                          "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -372,6 +417,7 @@ def main():
 
     md = None if args.no_md else bench_md(torch, N, systems, top, p32, box)
     io = bench_io(N, top)
+    tri = bench_triclinic(torch, N, top, p32, box) if args.triclinic else None
     batched = None if args.no_batched else bench_batched(torch, N, systems)
     adjoint = None if args.no_adjoint else bench_adjoint(torch, N, systems)
 
@@ -430,6 +476,7 @@ def main():
         "cpu_baseline": cpu,
         "md": md,
         "io": io,
+        "triclinic": tri,
         "batched": batched,
         "adjoint": adjoint,
     }

@@ -29,6 +29,16 @@
  *                        skin/2, across a periodic face (minimum image), a NaN
  *                        coordinate, and agreement with an exact-rational
  *                        brute force on random displacement sets
+ *   triclinic cells      cell matrix lengths / angles / face distances; right
+ *                        angles reproduce the orthorhombic path bit for bit;
+ *                        an orthorhombic lattice in a sheared basis gives the
+ *                        orthorhombic energies, forces and structure-factor
+ *                        sum; triclinic PME converges to that sum at fourth
+ *                        order, its forces are its FD gradient; alpha-
+ *                        independence of the total Ewald energy; a cell
+ *                        doubled along a doubles every term; diagonal virial
+ *                        = FD strain derivative; rebuild criterion across
+ *                        lattice vectors
  */
 #include "oracle.h"
 
