@@ -15,7 +15,9 @@ boxes:
   * twelve FP64 velocity-Verlet steps with a Verlet list follow the oracle's
     trajectory (positions 1e-9 A);
   * REF semantics, the adjoint and faces closer than twice the cutoff are
-    refused with an error.
+    refused with an error;
+  * the C++ driver (md_main --minimize) takes the angles of a triclinic rst7
+    and minimises as the C ABI does (1e-5, its six printed digits).
 """
 import numpy as np
 import pytest
@@ -203,3 +205,39 @@ def test_refusals(polymer_2k):
         ev.close()
     eo, _, _ = oracle.evaluate(top.desc, prm, top.coords, top.box, skip_all_pairs=True)
     assert abs(e["total"] - eo["total"]) <= 1e-9 * abs(eo["total"])
+
+
+def test_cpp_driver_reads_cell_angles(polymer_2k, tmp_path):
+    """md_main --minimize on a triclinic rst7: the C++ System passes the file's
+    angles to the GPU (AMBER semantics), so its minimisation equals the C ABI
+    one with mdgpu_set_box_angles (same settings as the orthorhombic driver
+    test: rc 10 A, switch 9 A, skin 2 A, Ewald tolerance 1e-6, the reference's
+    PME grid formula rounded up to 2^a 3^b)."""
+    import math
+    import os
+    import re
+    import subprocess
+    from md_amd import systems
+    print("case md_main --minimize 10 on a triclinic rst7", flush=True)
+    prm_path, _ = polymer_2k
+    top, x, box6 = _cell(polymer_2k, "triclinic")
+    rst = str(tmp_path / "tri.rst7")
+    N.write_rst7(rst, x, box=box6)
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "md_amd", "md_main")
+    out = subprocess.run([exe, prm_path, rst, "--minimize", "10"], capture_output=True, text=True, timeout=120)
+    print(out.stdout[-400:], out.stderr[-300:])
+    assert out.returncode == 0
+    e_cpp = float(re.search(r"Minimized:.*\nEnergy: (\S+)", out.stdout).group(1))
+    xr = N.read_rst7(rst)["coords"]
+    tol, rc = 1e-6, 10.0
+    alpha = math.sqrt(-math.log(2.0 * tol)) / rc
+    grid = tuple(systems.smallest_2a3b(int(2.0 * alpha * L / (3.0 * tol ** 0.2))) for L in box6[:3])
+    prm = N.make_params(N.SEM_AMBER, N.PREC_FP64, lj_cutoff=rc, lj_switch=9.0, ewald_alpha=alpha, pme_grid=grid)
+    ev = _ev(top, prm, box6, skin=2.0)
+    try:
+        xd = _dev(xr)
+        res = ev.minimize(xd.data_ptr(), box6[:3], max_iter=10, rms_force_tol=1e-2, max_step=0.2)
+    finally:
+        ev.close()
+    print(f"  C++ driver {e_cpp:.6f}; C ABI {res}")
+    assert abs(e_cpp - res["energy"]) <= 1e-5 * abs(res["energy"])
