@@ -181,13 +181,16 @@ def bench_md(torch, N, systems, top, p32, box, steps=200, warmup=20, dt=0.0005):
 
 
 def bench_triclinic(torch, N, top, p32, box, reps=200, warmup=20):
-    """The polymer_29k's own cubic lattice written as a triclinic cell with
-    edges a, a + b, a + b + c (mixed form, AMBER, rc 12 A, PME 72^3): the same
-    periodic system through the triclinic path (fractional binning and the
-    lattice-shift search).  Evaluations/s over `reps` after `warmup`, and the
-    relative differences of the direct-space energies from the orthorhombic
-    cell (the reciprocal energy differs by the PME grid, whose lines now run
-    along the sheared edges)."""
+    """Two triclinic cells of the polymer_29k (mixed form, AMBER, rc 12 A,
+    PME 72^3), evaluations/s over `reps` after `warmup` each:
+      * `sheared`: its own cubic lattice written with edges a, a + b, a + b + c,
+        the same periodic system through the triclinic path, with the relative
+        differences of the direct-space energies from the orthorhombic cell
+        (the reciprocal energy differs by the PME grid, whose lines now run
+        along the sheared edges); its slanted columns are too wide for the
+        minimum-image search;
+      * `angles_80_85_75`: the system carried into a 66 A cell of those angles
+        (x' = H x / 66), which keeps the minimum-image search."""
     l1, l2, l3 = (float(v) for v in box)
     a, b, c = np.array([l1, 0.0, 0.0]), np.array([l1, l2, 0.0]), np.array([l1, l2, l3])
 
@@ -197,30 +200,44 @@ def bench_triclinic(torch, N, top, p32, box, reps=200, warmup=20):
     lengths = [float(np.linalg.norm(u)) for u in (a, b, c)]
     angles = [ang(b, c), ang(a, c), ang(a, b)]
     dev = torch.device("cuda:0")
-    x = torch.tensor(top.coords.reshape(-1), dtype=torch.float64, device=dev)
-    f = torch.zeros_like(x)
-    ev = N.Evaluator(0)
-    try:
-        ev.upload(top.desc)
-        ev.set_params(p32)
-        e_o, _ = ev.evaluate_device(x.data_ptr(), box, f.data_ptr())
-        ev.set_box_angles(angles)
-        for _ in range(warmup):
-            ev.evaluate_device(x.data_ptr(), lengths, f.data_ptr())
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            e_t, _ = ev.evaluate_device(x.data_ptr(), lengths, f.data_ptr())
-        torch.cuda.synchronize()
-        t = time.perf_counter() - t0
-    finally:
-        ev.close()
-    return {"workload": "polymer_29k lattice in the sheared basis a, a+b, a+b+c: edges %.2f, %.2f, %.2f A, "
-                        "angles %.2f, %.2f, %.2f deg; mixed form, AMBER, rc 12 A, PME 72^3" % tuple(lengths + angles),
-            "evals_per_s": reps / t,
-            "vdw_rel_diff_vs_orthorhombic": abs(e_t["vdw"] - e_o["vdw"]) / abs(e_o["vdw"]),
-            "ewald_direct_rel_diff_vs_orthorhombic":
-                abs(e_t["ewald_direct"] - e_o["ewald_direct"]) / abs(e_o["ewald_direct"])}
+
+    def run(coords, lengths, angles):
+        x = torch.tensor(np.asarray(coords).reshape(-1), dtype=torch.float64, device=dev)
+        f = torch.zeros_like(x)
+        ev = N.Evaluator(0)
+        try:
+            ev.upload(top.desc)
+            ev.set_params(p32)
+            e_o, _ = ev.evaluate_device(x.data_ptr(), box, f.data_ptr())
+            ev.set_box_angles(angles)
+            for _ in range(warmup):
+                ev.evaluate_device(x.data_ptr(), lengths, f.data_ptr())
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                e_t, _ = ev.evaluate_device(x.data_ptr(), lengths, f.data_ptr())
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            mode = ev.search_mode()
+        finally:
+            ev.close()
+        return e_o, e_t, {"evals_per_s": reps / t, "cluster_search": "general" if mode else "minimum image"}
+
+    e_o, e_t, sheared = run(top.coords, lengths, angles)
+    sheared["cell"] = "edges %.2f, %.2f, %.2f A, angles %.2f, %.2f, %.2f deg" % tuple(lengths + angles)
+    sheared["vdw_rel_diff_vs_orthorhombic"] = abs(e_t["vdw"] - e_o["vdw"]) / abs(e_o["vdw"])
+    sheared["ewald_direct_rel_diff_vs_orthorhombic"] = \
+        abs(e_t["ewald_direct"] - e_o["ewald_direct"]) / abs(e_o["ewald_direct"])
+    # x' = H x / L for the cell of edges L and angles 80, 85, 75
+    ca, cb, cg = np.cos(np.radians([80.0, 85.0, 75.0]))
+    sg = np.sin(np.radians(75.0))
+    cy = (ca - cb * cg) / sg
+    h = l1 * np.array([[1.0, cg, cb], [0.0, sg, cy], [0.0, 0.0, np.sqrt(1.0 - cb * cb - cy * cy)]])
+    xm = (top.coords.reshape(-1, 3) / l1) @ h.T
+    _, _, tilted = run(xm, [l1, l1, l1], [80.0, 85.0, 75.0])
+    tilted["cell"] = "edges %.2f A, angles 80, 85, 75 deg (x' = H x / %.0f)" % (l1, l1)
+    return {"workload": "polymer_29k in two triclinic cells; mixed form, AMBER, rc 12 A, PME 72^3",
+            "sheared": sheared, "angles_80_85_75": tilted}
 
 
 def bench_io(N, top, reps=5):

@@ -365,6 +365,11 @@ int mdgpu_set_verlet(mdgpu_ctx* ctx, double skin);
  * MDGPU_ERR_ARG.  A change of angles forces the next list build. */
 int mdgpu_set_box_angles(mdgpu_ctx* ctx, const double angles[3]);
 int mdgpu_get_box_angles(const mdgpu_ctx* ctx, double angles[3]);
+/* Which cluster search the context uses (no counterpart in the reference):
+ * 0 the minimum-image search, 1 the general per-shift search, which a
+ * context switches to for good once a cluster is too wide for the first
+ * (a small box, or a strongly sheared cell); -1 for a null context. */
+int mdgpu_get_search_mode(const mdgpu_ctx* ctx);
 /* List builds and successful evaluations since the system upload. */
 int mdgpu_get_list_builds(const mdgpu_ctx* ctx, int64_t* builds, int64_t* evals);
 /* ljpairs_needs_rebuild (NonBondedUtils.cpp:87-100) on the GPU for device

@@ -550,6 +550,7 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     g.box[1] = box[1];
     g.box[2] = box[2];
     g.cell = cell;
+    for (int d = 0; d < 3; ++d) g.face[d] = face[d];
     const double dens = n / cell_volume(cell);
     const double s = std::cbrt(mdg::kCluster / std::max(dens, 1e-6));
     g.ncx = std::max(1, static_cast<int>(face[0] / s));
@@ -1389,6 +1390,8 @@ int mdgpu_set_box_angles(mdgpu_ctx* c, const double angles[3]) {
     c->verlet_force = true;  // the kept list holds shifts of the old cell
     return MDGPU_OK;
 }
+
+int mdgpu_get_search_mode(const mdgpu_ctx* c) { return c ? (c->search_general ? 1 : 0) : -1; }
 
 int mdgpu_get_box_angles(const mdgpu_ctx* c, double angles[3]) {
     if (!c || !angles) return MDGPU_ERR_ARG;

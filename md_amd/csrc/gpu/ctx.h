@@ -170,6 +170,7 @@ struct Grid {
     // orthorhombic); frac_scale: the largest edge length over face distance
     // ratio, which converts a distance into a bound on s_x |a| and s_y |b|.
     Cell cell{};
+    double face[3] = {0, 0, 0};  // distances between opposite faces (the lengths when orthorhombic)
     double wx = 0, wy = 0;
     double frac_scale = 1.0;
 };

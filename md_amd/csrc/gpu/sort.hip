@@ -672,7 +672,9 @@ __device__ __forceinline__ double bcast_lane(double v, int l) {
 // exact when every cluster half-extent h satisfies 2h <= L/2 - r_list in each
 // dimension (then no second image of a cluster can come within r_list);
 // a cluster that breaks the bound sets flags[4] and the host falls back to
-// k_search.
+// k_search.  In a triclinic cell the separation is rounded in fractional
+// coordinates (d - H rint(R d)) and the bound holds each cluster's half
+// diagonal to half of (half the smallest face distance - r_list).
 //
 // The candidate clusters of the rows in reach form at most two contiguous
 // index ranges per row (the row's x window, split where it wraps).  Lane r
@@ -727,7 +729,8 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
             const double gy = fmax(0.0, (abs(dy) - 1) * a.sy);
             const double rem = a.rl2 - gy * gy;
             if (rem <= 0.0) use = false;
-            else dxr = min(a.dcx, static_cast<int>(sqrt(rem) / a.sx) + 1);
+            // a triclinic cell's face normals are not orthogonal: no narrowing
+            else if (!a.cell.tri) dxr = min(a.dcx, static_cast<int>(sqrt(rem) / a.sx) + 1);
         }
         if (use) {
             int jy = (y0 + ry) % a.ncy;
@@ -807,9 +810,13 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
             ddx = oj.x - oi.x;
             ddy = oj.y - oi.y;
             ddz = oj.z - oi.z;
-            ddx -= a.lx * rint(ddx * ilx);
-            ddy -= a.ly * rint(ddy * ily);
-            ddz -= a.lz * rint(ddz * ilz);
+            if (!a.cell.tri) {
+                ddx -= a.lx * rint(ddx * ilx);
+                ddy -= a.ly * rint(ddy * ily);
+                ddz -= a.lz * rint(ddz * ilz);
+            } else {
+                cell_min_image_tri(a.cell, ddx, ddy, ddz);
+            }
             const double qx = fmax(0.0, fabs(ddx) - (double)(hi.x + hj.x));
             const double qy = fmax(0.0, fabs(ddy) - (double)(hi.y + hj.y));
             const double qz = fmax(0.0, fabs(ddz) - (double)(hi.z + hj.z));
@@ -1079,11 +1086,15 @@ hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber, const VerletS
     a.excl_list = c->topo.excl_list;
     a.excl_rmax = w.excl_rmax;
     const int nb = (c->n_clusters + kSearchWaves - 1) / kSearchWaves;
-    // half-extent bound of the minimum-image search in each dimension
+    // half-extent bound of the minimum-image search in each dimension.  A
+    // triclinic cell bounds each cluster's half diagonal by half of
+    // (half the smallest face distance - r_list): two clusters whose boxes
+    // come within r_list then have origins closer than half every face
+    // distance, and the fractional rounding of their separation is that image.
     auto hlim = [&](double L) { return 0.5 * (0.5 * L - rl); };
-    const double hx = hlim(g.box[0]), hy = hlim(g.box[1]), hz = hlim(g.box[2]);
-    // the fast search takes minimum images along the axes: orthorhombic cells only
-    const bool fast = !c->search_general && !g.cell.tri && hx > 1.0 && hy > 1.0 && hz > 1.0;
+    double hx = hlim(g.box[0]), hy = hlim(g.box[1]), hz = hlim(g.box[2]);
+    if (g.cell.tri) hx = hy = hz = hlim(std::min({g.face[0], g.face[1], g.face[2]})) / std::sqrt(3.0);
+    const bool fast = !c->search_general && hx > 1.0 && hy > 1.0 && hz > 1.0;
     if (fast) {
         if (mixed)
             hipLaunchKernelGGL(k_search_mi<float>, dim3(nb), dim3(kSearchWaves * kWave), 0, c->stream, a, hx, hy, hz);

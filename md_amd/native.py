@@ -145,6 +145,7 @@ def gpu_lib() -> C.CDLL:
         lib.mdgpu_set_verlet.argtypes = [C.c_void_p, C.c_double]
         lib.mdgpu_set_box_angles.argtypes = [C.c_void_p, P_F64]
         lib.mdgpu_get_box_angles.argtypes = [C.c_void_p, P_F64]
+        lib.mdgpu_get_search_mode.argtypes = [C.c_void_p]
         lib.mdgpu_set_virial.argtypes = [C.c_void_p, C.c_int]
         lib.mdgpu_get_virial.argtypes = [C.c_void_p, P_F64]
         lib.mdgpu_md_set_masses.argtypes = [C.c_void_p, P_F64]
@@ -544,6 +545,10 @@ class Evaluator:
         box arguments then give the edge lengths); 90, 90, 90 is orthorhombic."""
         a = np.ascontiguousarray(angles, dtype=np.float64).reshape(3)
         self._check(self.lib.mdgpu_set_box_angles(self.ctx, a.ctypes.data_as(P_F64)))
+
+    def search_mode(self) -> int:
+        """0: minimum-image cluster search; 1: the general per-shift search."""
+        return int(self.lib.mdgpu_get_search_mode(self.ctx))
 
     def box_angles(self) -> np.ndarray:
         a = np.zeros(3)

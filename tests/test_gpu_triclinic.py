@@ -17,7 +17,9 @@ boxes:
   * REF semantics, the adjoint and faces closer than twice the cutoff are
     refused with an error;
   * the C++ driver (md_main --minimize) takes the angles of a triclinic rst7
-    and minimises as the C ABI does (1e-5, its six printed digits).
+    and minimises as the C ABI does (1e-5, its six printed digits);
+  * the 29k-atom polymer in a 66 A cell of the same angles keeps the
+    minimum-image cluster search and matches the oracle (FP64 bars above).
 """
 import numpy as np
 import pytest
@@ -241,3 +243,31 @@ def test_cpp_driver_reads_cell_angles(polymer_2k, tmp_path):
         ev.close()
     print(f"  C++ driver {e_cpp:.6f}; C ABI {res}")
     assert abs(e_cpp - res["energy"]) <= 1e-5 * abs(res["energy"])
+
+
+def test_minimum_image_search_in_a_triclinic_cell(polymer_29k):
+    """The 29k-atom polymer carried into a 66 A cell of angles 80, 85, 75: its
+    clusters are narrow enough for the minimum-image search (fractional
+    rounding of cluster separations), which stays in use; FP64 matches the
+    oracle as above."""
+    print("case polymer_29k triclinic cell, minimum-image search, FP64 vs oracle", flush=True)
+    prm_path, rst_path = polymer_29k
+    top = N.Topology(prm_path, rst_path, N.SEM_AMBER)
+    edge = float(top.box[0])
+    box6 = [edge, edge, edge] + TRI_ANGLES
+    x = np.ascontiguousarray(mapped(top.coords, edge, box6))
+    prm = N.make_params(N.SEM_AMBER, N.PREC_FP64, lj_cutoff=12.0, lj_switch=11.0, ewald_tol=1e-5,
+                        pme_grid=(72, 72, 72))
+    ev = _ev(top, prm, box6)
+    try:
+        e, f, _ = ev.evaluate(x, box6[:3])
+        mode = ev.search_mode()
+    finally:
+        ev.close()
+    print(f"  search mode {mode}")
+    assert mode == 0
+    eo, fo, _ = oracle.evaluate(top.desc, prm, x, box6, skip_all_pairs=True)
+    check_terms(e, eo, TERMS_AMBER, 1e-9)
+    r = rel_rms(f, fo)
+    print(f"  force rel-RMS {r:.2e}")
+    assert r <= 1e-8
