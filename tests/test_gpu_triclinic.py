@@ -1,10 +1,10 @@
 """Triclinic cells on the GPU (mdgpu_set_box_angles) against the oracle.
 
-Two cells hold the 2.1k-atom polymer: the genuinely triclinic cell of edges
-30 A and angles 80, 85, 75 degrees (the cubic system carried into it by
-x' = H x / 30), and its own cubic box written in the sheared basis a, a + b,
-a + b + c (every cell-matrix entry non-zero).  Bars, as for orthorhombic
-boxes:
+Three cells hold the 2.1k-atom polymer: the genuinely triclinic cells of
+edges 30 A and angles 80, 85, 75 degrees and of the truncated octahedron
+(109.47 degrees; the cubic system carried into each by x' = H x / 30), and
+its own cubic box written in the sheared basis a, a + b, a + b + c (every
+cell-matrix entry non-zero).  Bars, as for orthorhombic boxes:
   * FP64 vs oracle: per-term |dE| <= 1e-9 |E|, force relative RMS <= 1e-8;
     the full virial within 1e-9 of its largest component;
   * mixed vs FP64: force relative RMS <= 1e-5;
@@ -35,6 +35,7 @@ from tri_systems import cell_vectors, mapped, sheared_box
 pytestmark = pytest.mark.gpu
 
 TRI_ANGLES = [80.0, 85.0, 75.0]
+OCTAHEDRON = [109.4712206, 109.4712206, 109.4712206]  # the truncated octahedron's cell (faces 24.5 A at 30 A)
 KW = dict(lj_cutoff=10.0, lj_switch=9.0, ewald_tol=1e-5, pme_grid=(32, 32, 32))
 
 
@@ -42,8 +43,8 @@ def _cell(polymer_2k, kind):
     prm_path, rst_path = polymer_2k
     top = N.Topology(prm_path, rst_path, N.SEM_AMBER)
     edge = float(top.box[0])
-    if kind == "triclinic":
-        box6 = [edge, edge, edge] + TRI_ANGLES
+    if kind in ("triclinic", "octahedron"):
+        box6 = [edge, edge, edge] + (TRI_ANGLES if kind == "triclinic" else OCTAHEDRON)
         x = mapped(top.coords, edge, box6)
     else:
         box6 = sheared_box(top.box)
@@ -67,7 +68,7 @@ def _dev(x):
     return t
 
 
-@pytest.mark.parametrize("kind", ["triclinic", "sheared"])
+@pytest.mark.parametrize("kind", ["triclinic", "sheared", "octahedron"])
 def test_fp64_matches_oracle(polymer_2k, kind):
     print(f"case polymer_2k {kind} cell FP64 vs oracle", flush=True)
     top, x, box6 = _cell(polymer_2k, kind)
@@ -88,7 +89,7 @@ def test_fp64_matches_oracle(polymer_2k, kind):
     assert dw <= 1e-9
 
 
-@pytest.mark.parametrize("kind", ["triclinic", "sheared"])
+@pytest.mark.parametrize("kind", ["triclinic", "sheared", "octahedron"])
 def test_mixed_matches_fp64(polymer_2k, kind):
     print(f"case polymer_2k {kind} cell mixed vs FP64", flush=True)
     top, x, box6 = _cell(polymer_2k, kind)
