@@ -65,7 +65,8 @@ void free_work(mdgpu_ctx* c) {
     void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_count, w.block_emax, w.block_off, w.col_start, w.col_cstart, w.col_key,
                     w.slot_atom, w.slot_meta, w.slot_pos_d, w.slot_pos_f, w.slot_pme_fl, w.slot_pme_wd,
                     w.slot_pme_wf, w.cluster_col, w.cl_origin, w.cl_half,
-                    w.list_count, w.list_j, w.list_mask, w.list_delta_d, w.list_delta_f, w.force_d, w.force_fx,
+                    w.list_count, w.list_j, w.list_mask, w.list_delta_d, w.list_delta_f, w.inner_count, w.inner_j,
+                    w.inner_mask, w.inner_delta_d, w.inner_delta_f, w.force_d, w.force_fx,
                     w.force_d_s, w.force_fx_s, w.atom_slot,
                     w.partial, w.energies, w.force_out, w.flags, w.excl_rmax, w.grid_f, w.grid_phi_d,
                     w.cgrid, w.bsp_mod, w.bsp_mod_f, w.twiddle_d, w.twiddle_f, w.x_ref, w.verlet};
@@ -655,6 +656,19 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.list_delta_f), nl * sizeof(float4)), "alloc list");
         w.cap_list = nl;
     }
+    if (c->skin > 0 && (w.cap_inner < nl || !w.inner_count || w.cap_inner_cl < ncl)) {
+        // the pruned copy of a kept Verlet list
+        void* ptrs[] = {w.inner_count, w.inner_j, w.inner_mask, w.inner_delta_d, w.inner_delta_f};
+        for (void* q : ptrs)
+            if (q) (void)hipFree(q);
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.inner_count), ncl * sizeof(int)), "alloc list");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.inner_j), nl * sizeof(int)), "alloc list");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.inner_mask), nl * sizeof(unsigned long long)), "alloc list");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.inner_delta_d), nl * sizeof(double4)), "alloc list");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.inner_delta_f), nl * sizeof(float4)), "alloc list");
+        w.cap_inner = nl;
+        w.cap_inner_cl = ncl;
+    }
     int st = ensure_pme(c, box);
     if (st != MDGPU_OK) return st;
     const size_t rows = static_cast<size_t>(mdg::nonbonded_rows(c) + mdg::bonded_rows(c) +
@@ -797,6 +811,7 @@ int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out, bool fork) {
     // the search and the pair kernel already run
     mark(MDGPU_K_PAIRLIST, true);
     HIP_TRY(mdg::launch_pairsearch(c, mixed, amber, vs), "pair search");
+    if (vs) HIP_TRY(mdg::launch_prune(c, mixed), "prune");
     mark(MDGPU_K_PAIRLIST, false);
     mark(MDGPU_K_NONBONDED, true);
     HIP_TRY(mdg::launch_nonbonded(c, mixed, amber, row_nb, x), "nonbonded");
@@ -815,7 +830,8 @@ int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out, bool fork) {
     hipLaunchKernelGGL(k_reduce, dim3(mdg::E_NSLOTS + 1 + fblocks), dim3(kReduceThreads), 0, c->stream, w.partial,
                        static_cast<int>(w.cap_partial), rows, w.energies, c->d_host_energies, w.flags, w.excl_rmax,
                        mixed ? 1 : 0, w.force_fx, w.force_d, 1.0 / mdg::fixed_scale(p.semantics), 3 * n, d_out,
-                       w.list_count, w.col_cstart + c->n_cols, w.force_fx_s, w.force_d_s, w.atom_slot,
+                       vs ? w.inner_count : w.list_count, w.col_cstart + c->n_cols, w.force_fx_s, w.force_d_s,
+                       w.atom_slot,
                        static_cast<int>(w.cap_slots));
     HIP_TRY(hipGetLastError(), "reduce");
     mark(MDGPU_K_REDUCE, false);
@@ -919,7 +935,7 @@ mdg::GraphKey graph_key(const mdgpu_ctx* c, const double* x, const double* out) 
     k.caps[0] = w.cap_atoms;
     k.caps[1] = w.cap_cols;
     k.caps[2] = w.cap_slots;
-    k.caps[3] = w.cap_list;
+    k.caps[3] = w.cap_list + w.cap_inner;
     k.caps[4] = w.cap_partial;
     k.caps[5] = w.cap_grid;
     return k;

@@ -123,6 +123,15 @@ struct Work {
     unsigned long long* list_mask = nullptr;
     double4* list_delta_d = nullptr;
     float4* list_delta_f = nullptr;
+    // Verlet mode: the kept list (built at cutoff + skin) pruned every
+    // evaluation to the tiles with a pair inside the cutoff, same layout
+    size_t cap_inner = 0;
+    size_t cap_inner_cl = 0;
+    int* inner_count = nullptr;      // [clusters]
+    int* inner_j = nullptr;          // [clusters * cap]
+    unsigned long long* inner_mask = nullptr;
+    double4* inner_delta_d = nullptr;
+    float4* inner_delta_f = nullptr;
     // forces
     double* force_d = nullptr;                 // [3N]
     unsigned long long* force_fx = nullptr;    // [3N] fixed point
@@ -285,6 +294,9 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl
 hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber, const VerletState* verlet);
 // displacement criterion alone: the decision goes to verlet->query
 hipError_t launch_disp_query(mdgpu_ctx* c, const double* coords);
+// Verlet mode: prunes the kept list to the tiles that hold a pair within the
+// cutoff at the current positions (the pair kernel then reads that list).
+hipError_t launch_prune(mdgpu_ctx* c, bool mixed);
 // pair kernel, with the bonded terms and (amber) the FP64 excluded / 1-4
 // corrections as further blocks of the same launch; energy rows partial_row0
 // .. + nonbonded_rows + bonded_rows + corrections_rows

@@ -410,16 +410,18 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0, const double* coords) {
     a.max_clusters = c->n_clusters;
     a.split = c->split;
     a.cap = w.list_cap;
-    a.list_count = w.list_count;
-    a.list_j = w.list_j;
-    a.list_mask = w.list_mask;
+    // Verlet mode: the kept list pruned to the cutoff for this evaluation
+    const bool pruned = c->skin > 0;
+    a.list_count = pruned ? w.inner_count : w.list_count;
+    a.list_j = pruned ? w.inner_j : w.list_j;
+    a.list_mask = pruned ? w.inner_mask : w.list_mask;
     a.ntp = t.n_types + 1;
     if constexpr (sizeof(real) == 4) {
-        a.list_delta = w.list_delta_f;
+        a.list_delta = pruned ? w.inner_delta_f : w.list_delta_f;
         a.pos = w.slot_pos_f;
         a.lj = t.lj_f;
     } else {
-        a.list_delta = w.list_delta_d;
+        a.list_delta = pruned ? w.inner_delta_d : w.list_delta_d;
         a.pos = w.slot_pos_d;
         a.lj = t.lj_d;
     }

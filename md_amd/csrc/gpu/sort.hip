@@ -982,7 +982,116 @@ __global__ __launch_bounds__(kRefreshThreads) void k_refresh(const VerletState* 
     }
 }
 
+// Verlet mode: one wave per i-cluster copies, in order, the entries of its
+// kept list whose tile holds an unmasked pair within the cutoff (plus 1e-3 A
+// for rounding) at the current slot positions, computed as the pair kernel
+// computes them.  The pair kernel tests every pair against the cutoff, so the
+// pruned list gives the same energies and forces with fewer tiles.  Each lane
+// takes one entry: its eight j-atom loads are in flight together, and the
+// i-cluster's atoms are broadcast from registers.
+struct PruneArgs {
+    const int* ncl_dev;
+    int max_clusters;
+    int cap;
+    const int* count;
+    const int* list_j;
+    const unsigned long long* list_mask;
+    const void* list_delta;
+    const void* pos;
+    int* out_count;
+    int* out_j;
+    unsigned long long* out_mask;
+    void* out_delta;
+    double rc2;
+};
+
+constexpr int kPruneWaves = 4;
+
+template <typename real>
+__global__ __launch_bounds__(kPruneWaves * kWave) void k_prune(const PruneArgs a) {
+    using R4 = typename Vec4<real>::T;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int ci = __builtin_amdgcn_readfirstlane(blockIdx.x * kPruneWaves + (threadIdx.x >> 6));
+    const int ncl = min(*a.ncl_dev, a.max_clusters);
+    if (ci >= ncl) return;  // whole wave leaves; no block barrier follows
+    const R4* pos = static_cast<const R4*>(a.pos);
+    const R4* din = static_cast<const R4*>(a.list_delta);
+    R4* dout = static_cast<R4*>(a.out_delta);
+    const R4 pi_l = pos[ci * kCluster + (lane & 7)];
+    real pix[kCluster], piy[kCluster], piz[kCluster];
+#pragma unroll
+    for (int k = 0; k < kCluster; ++k) {
+        pix[k] = bcast_lane(pi_l.x, k);
+        piy[k] = bcast_lane(pi_l.y, k);
+        piz[k] = bcast_lane(pi_l.z, k);
+    }
+    const real rc2 = static_cast<real>(a.rc2);
+    const int cnt = min(a.count[ci], a.cap);
+    const size_t base = static_cast<size_t>(ci) * a.cap;
+    int n = 0;
+    for (int kb = 0; kb < cnt; kb += kWave) {
+        const bool have = kb + lane < cnt;
+        int e_j = 0;
+        unsigned long long e_m = 0;
+        R4 e_d = {0, 0, 0, 0};
+        bool live = false;
+        if (have) {
+            e_j = a.list_j[base + kb + lane];
+            e_m = a.list_mask[base + kb + lane];
+            e_d = din[base + kb + lane];
+            R4 pj[kCluster];
+#pragma unroll
+            for (int u = 0; u < kCluster; ++u) pj[u] = pos[e_j * kCluster + u];
+#pragma unroll
+            for (int u = 0; u < kCluster; ++u) {
+#pragma unroll
+                for (int k = 0; k < kCluster; ++k) {
+                    // bit 8 i + j of the mask, i the i-slot and j the j-slot
+                    const real dx = pix[k] - pj[u].x + e_d.x;
+                    const real dy = piy[k] - pj[u].y + e_d.y;
+                    const real dz = piz[k] - pj[u].z + e_d.z;
+                    const bool near = dx * dx + dy * dy + dz * dz <= rc2;
+                    live |= near & static_cast<bool>((e_m >> (kCluster * k + u)) & 1ull);
+                }
+            }
+        }
+        const unsigned long long keep = __ballot(live);
+        if (live) {
+            const size_t o = base + n + __popcll(keep & ((1ull << lane) - 1ull));
+            a.out_j[o] = e_j;
+            a.out_mask[o] = e_m;
+            dout[o] = e_d;
+        }
+        n += __popcll(keep);
+    }
+    if (lane == 0) a.out_count[ci] = n;
+}
+
 }  // namespace
+
+hipError_t launch_prune(mdgpu_ctx* c, bool mixed) {
+    Work& w = c->work;
+    const Grid& g = c->grid;
+    PruneArgs a{};
+    a.ncl_dev = w.col_cstart + g.ncx * g.ncy;
+    a.max_clusters = c->n_clusters;
+    a.cap = w.list_cap;
+    a.count = w.list_count;
+    a.list_j = w.list_j;
+    a.list_mask = w.list_mask;
+    a.list_delta = mixed ? static_cast<const void*>(w.list_delta_f) : static_cast<const void*>(w.list_delta_d);
+    a.pos = mixed ? static_cast<const void*>(w.slot_pos_f) : static_cast<const void*>(w.slot_pos_d);
+    a.out_count = w.inner_count;
+    a.out_j = w.inner_j;
+    a.out_mask = w.inner_mask;
+    a.out_delta = mixed ? static_cast<void*>(w.inner_delta_f) : static_cast<void*>(w.inner_delta_d);
+    const double rc = std::max(c->params.lj_cutoff, c->params.ee_cutoff) + 1e-3;
+    a.rc2 = rc * rc;
+    const int nb = (c->n_clusters + kPruneWaves - 1) / kPruneWaves;
+    if (mixed) hipLaunchKernelGGL(k_prune<float>, dim3(nb), dim3(kPruneWaves * kWave), 0, c->stream, a);
+    else hipLaunchKernelGGL(k_prune<double>, dim3(nb), dim3(kPruneWaves * kWave), 0, c->stream, a);
+    return hipGetLastError();
+}
 
 int sort_blocks(int n) {
     return std::max(1, std::min(kMaxSortBlocks, (n + kAtomsPerSortBlock - 1) / kAtomsPerSortBlock));
