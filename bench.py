@@ -240,6 +240,34 @@ def bench_triclinic(torch, N, top, p32, box, reps=200, warmup=20):
             "sheared": sheared, "angles_80_85_75": tilted}
 
 
+def bench_scale(torch, N, systems, reps=100, warmup=20):
+    """The ~100k-atom system (16,000-atom polymer in ~28,000 waters, 100 A
+    box; AMBER, rc 10 A, switch 9 A, Ewald tol 1e-5, PME 108^3), mixed form:
+    evaluations/s over `reps` graph replays after `warmup`."""
+    prm, rst = systems.cached_system("scale_100k")
+    top = N.Topology(prm, rst, N.SEM_AMBER)
+    p32 = N.make_params(N.SEM_AMBER, N.PREC_MIXED, lj_cutoff=10.0, lj_switch=9.0, ewald_tol=1e-5,
+                        pme_grid=(108, 108, 108))
+    x = torch.tensor(top.coords.reshape(-1), dtype=torch.float64, device="cuda:0")
+    f = torch.zeros_like(x)
+    ev = N.Evaluator(0)
+    try:
+        ev.upload(top.desc)
+        ev.set_params(p32)
+        for _ in range(warmup):
+            ev.evaluate_device(x.data_ptr(), top.box, f.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ev.evaluate_device(x.data_ptr(), top.box, f.data_ptr())
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+    finally:
+        ev.close()
+    return {"workload": "scale_100k: %d atoms, 100 A box, AMBER, rc 10 A, PME 108^3, mixed form" % top.n_atoms,
+            "evals_per_s": reps / t, "atom_evals_per_s": top.n_atoms * reps / t}
+
+
 def bench_io(N, top, reps=5):
     """Amber restart write and read (coordinates, velocities and box of the
     28,800-atom workload, 6F12.7) through the host library, in MB/s of file."""
@@ -308,6 +336,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-system batched run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 64-frame parameter-gradient run")
     ap.add_argument("--no-md", action="store_true", help="skip the minimisation + dynamics run")
+    ap.add_argument("--scale", action="store_true", help="also time the ~100k-atom system")
     ap.add_argument("--triclinic", action="store_true",
                     help="also time the same system written as a triclinic cell (sheared basis)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -435,6 +464,7 @@ def main():
     md = None if args.no_md else bench_md(torch, N, systems, top, p32, box)
     io = bench_io(N, top)
     tri = bench_triclinic(torch, N, top, p32, box) if args.triclinic else None
+    scale = bench_scale(torch, N, systems) if args.scale else None
     batched = None if args.no_batched else bench_batched(torch, N, systems)
     adjoint = None if args.no_adjoint else bench_adjoint(torch, N, systems)
 
@@ -494,6 +524,7 @@ def main():
         "md": md,
         "io": io,
         "triclinic": tri,
+        "scale_100k": scale,
         "batched": batched,
         "adjoint": adjoint,
     }
