@@ -12,7 +12,9 @@ GPU (System::ljpairs_needs_rebuild, NonBondedUtils.cpp:87-100).  Bars:
     rel-RMS 1e-5) of the FP64 result and are bit-reproducible;
   * along a trajectory, captured-graph replay gives what direct launches
     give (mixed bit for bit, FP64 to 1e-12), the number of list builds equals
-    the oracle criterion's count, and the replay continues across rebuilds.
+    the oracle criterion's count, and the replay continues across rebuilds;
+  * the kept list is pruned to the cutoff at every evaluation: no more tiles
+    than a list built at the cutoff, the same pair count and FP64 energies.
 """
 import numpy as np
 import pytest
@@ -253,3 +255,35 @@ def test_trajectory_replay_and_build_count(prec, polymer_2k):
             assert np.abs(fg - fdd).max() <= 1e-11 * np.abs(fdd).max(), k
             assert rel_rms(fg, ff) <= 1e-10, k
             check_terms(eg, ef, TERMS_AMBER, 1e-10)
+
+
+def test_kept_list_is_pruned_to_the_cutoff(polymer_2k):
+    """Each evaluation with a skin prunes the kept list (radius cutoff + skin)
+    to the tiles holding a pair within the cutoff: the pair kernel then runs
+    no more tiles than a list built at the cutoff alone, the same pairs, and
+    the same FP64 energies (1e-12)."""
+    print("case polymer_2k pruned Verlet list", flush=True)
+    prm_path, rst_path = polymer_2k
+    top = N.Topology(prm_path, rst_path, N.SEM_AMBER)
+    prm = N.make_params(N.SEM_AMBER, N.PREC_FP64, lj_cutoff=10.0, lj_switch=9.0, ewald_tol=1e-5,
+                        pme_grid=(32, 32, 32))
+    out = {}
+    for skin in (0.0, 1.5):
+        ev = N.Evaluator(0)
+        try:
+            ev.upload(top.desc)
+            ev.set_params(prm)
+            ev.set_verlet(skin)
+            ev.set_counting(True)
+            e, _, _ = ev.evaluate(top.coords, top.box)
+            out[skin] = (e, ev.counts())
+        finally:
+            ev.close()
+    (e0, c0), (e1, c1) = out[0.0], out[1.5]
+    print(f"  tiles without skin {c0['tiles']}, pruned from cutoff + 1.5 A {c1['tiles']}; pairs {c0['pairs']}")
+    assert c1["pairs"] == c0["pairs"]
+    # (a tile with a pair just beyond the cutoff, inside the prune's 1e-3 A
+    # rounding margin, may be kept by the prune and not by the search)
+    assert c1["tiles"] <= 1.001 * c0["tiles"]
+    for k in TERMS_AMBER:
+        assert abs(e1[k] - e0[k]) <= 1e-12 * max(abs(e0[k]), 1.0), k
