@@ -523,7 +523,7 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
                     // shifts (the other holds the same atom pairs), all 64 pairs
                     const bool self_up = kz > 0 || (kz == 0 && (ky > 0 || (ky == 0 && kx > 0)));
                     const bool cand = valid && (cj > ci || (cj == ci && (noshift || self_up))) && d2 < a.rl2;
-                    const bool tri_mask = cj == ci && noshift;
+                    const bool triangle = cj == ci && noshift;  // the cluster with itself: j > i only
                     const unsigned long long bal = __ballot(cand);
                     if (bal == 0) continue;
                     const int rank = __popcll(bal & ((1ull << lane) - 1ull));
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
                         const size_t e = base_i + pos;
                         a.list_j[e] = cj;
                         // masks of tiles that may hold excluded pairs are written below
-                        if (!near) a.list_mask[e] = tri_mask ? self_mask : ~0ull;
+                        if (!near) a.list_mask[e] = triangle ? self_mask : ~0ull;
                         R4 dl;
                         dl.x = static_cast<real>(-ddx);
                         dl.y = static_cast<real>(-ddy);
