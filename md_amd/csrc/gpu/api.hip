@@ -69,7 +69,7 @@ void free_work(mdgpu_ctx* c) {
                     w.inner_mask, w.inner_delta_d, w.inner_delta_f, w.force_d, w.force_fx,
                     w.force_d_s, w.force_fx_s, w.atom_slot,
                     w.partial, w.energies, w.force_out, w.flags, w.excl_rmax, w.grid_f, w.grid_phi_d,
-                    w.cgrid, w.bsp_mod, w.bsp_mod_f, w.twiddle_d, w.twiddle_f, w.x_ref, w.verlet};
+                    w.cgrid, w.bsp_mod, w.bsp_mod_f, w.twiddle_d, w.twiddle_f, w.x_ref, w.x_prune, w.verlet};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c->work = Work{};
@@ -571,10 +571,10 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     cap = w.cap_atoms;
     if (w.cap_atoms < static_cast<size_t>(n)) {
         void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_key, w.force_d, w.force_fx, w.force_out, w.x_ref,
-                        w.atom_slot};
+                        w.x_prune, w.atom_slot};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
-        w.coords = w.wrapped = w.force_d = w.force_out = w.x_ref = nullptr;
+        w.coords = w.wrapped = w.force_d = w.force_out = w.x_ref = w.x_prune = nullptr;
         w.atom_slot = nullptr;
         w.col_of = nullptr;
         w.col_key = nullptr;
@@ -587,6 +587,7 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_fx), 3 * n * sizeof(unsigned long long)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_out), 3 * n * sizeof(double)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.x_ref), 3 * n * sizeof(double)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.x_prune), 3 * n * sizeof(double)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.atom_slot), n * sizeof(int)), "alloc");
         HIP_TRY(hipMemset(w.atom_slot, 0xff, n * sizeof(int)), "clear");  // -1: no slot yet
         c->verlet_force = true;  // no list positions to compare with
@@ -811,7 +812,7 @@ int enqueue_eval(mdgpu_ctx* c, const double* x, double* d_out, bool fork) {
     // the search and the pair kernel already run
     mark(MDGPU_K_PAIRLIST, true);
     HIP_TRY(mdg::launch_pairsearch(c, mixed, amber, vs), "pair search");
-    if (vs) HIP_TRY(mdg::launch_prune(c, mixed), "prune");
+    if (vs) HIP_TRY(mdg::launch_prune(c, mixed, x, vs), "prune");
     mark(MDGPU_K_PAIRLIST, false);
     mark(MDGPU_K_NONBONDED, true);
     HIP_TRY(mdg::launch_nonbonded(c, mixed, amber, row_nb, x), "nonbonded");

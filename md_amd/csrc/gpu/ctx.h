@@ -87,7 +87,15 @@ struct VerletState {
     unsigned int build;    // 1: this evaluation builds the list
     unsigned int force;    // set by the host: the next evaluation builds
     unsigned int query;    // decision of the last query (mdgpu_pairs_need_rebuild)
+    unsigned long long max_p2;  // largest squared displacement since the last prune
+    unsigned int prune;    // 1: this evaluation prunes the kept list again
+    unsigned int pad;
 };
+
+// The pruned list keeps the tiles with a pair within cutoff + kPruneBuffer and
+// is pruned again once some atom has moved more than kPruneBuffer / 2 since
+// the last prune (two atoms then come at most kPruneBuffer closer).
+constexpr double kPruneBuffer = 0.2;
 
 // Work arrays that depend on the atom count / box.
 struct Work {
@@ -151,6 +159,7 @@ struct Work {
                                      // 5 the list was built (Verlet mode; not an error)
     // Verlet mode: raw coordinates at the last list build, and the state
     double* x_ref = nullptr;         // [3N]
+    double* x_prune = nullptr;       // [3N] positions of the last prune of the kept list
     VerletState* verlet = nullptr;
     float* excl_rmax = nullptr;      // [1] largest excluded-pair distance (as float)
     // PME
@@ -294,9 +303,10 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl
 hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber, const VerletState* verlet);
 // displacement criterion alone: the decision goes to verlet->query
 hipError_t launch_disp_query(mdgpu_ctx* c, const double* coords);
-// Verlet mode: prunes the kept list to the tiles that hold a pair within the
-// cutoff at the current positions (the pair kernel then reads that list).
-hipError_t launch_prune(mdgpu_ctx* c, bool mixed);
+// Verlet mode: when the displacement check asks for it, prunes the kept list
+// to the tiles that hold a pair within cutoff + kPruneBuffer at the current
+// positions (the pair kernel reads that list) and records those positions.
+hipError_t launch_prune(mdgpu_ctx* c, bool mixed, const double* coords, const VerletState* verlet);
 // pair kernel, with the bonded terms and (amber) the FP64 excluded / 1-4
 // corrections as further blocks of the same launch; energy rows partial_row0
 // .. + nonbonded_rows + bonded_rows + corrections_rows

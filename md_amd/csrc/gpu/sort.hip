@@ -890,16 +890,13 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
 constexpr int kDispThreads = 256;
 constexpr int kDispMaxBlocks = 1024;
 __global__ __launch_bounds__(kDispThreads) void k_disp(const double* __restrict__ x, const double* __restrict__ xref,
-                                                       int n, const Cell cell, double lim2, int query, VerletState* vs,
+                                                       const double* __restrict__ xprune, int n, const Cell cell,
+                                                       double lim2, double limp2, int query, VerletState* vs,
                                                        unsigned int* flags) {
 #pragma clang fp contract(off)
-    __shared__ double wmax[kDispThreads / kWave];
+    __shared__ double wmax[kDispThreads / kWave], wmaxp[kDispThreads / kWave];
     const double lx = cell.l[0], ly = cell.l[1], lz = cell.l[2];
-    double m = 0.0;
-    for (int i = blockIdx.x * kDispThreads + threadIdx.x; i < n; i += gridDim.x * kDispThreads) {
-        double dx = x[3 * i] - xref[3 * i];
-        double dy = x[3 * i + 1] - xref[3 * i + 1];
-        double dz = x[3 * i + 2] - xref[3 * i + 2];
+    auto min_image = [&](double& dx, double& dy, double& dz) {
         if (!cell.tri) {
             dx = dx - lx * round(dx / lx);
             dy = dy - ly * round(dy / ly);
@@ -907,30 +904,55 @@ __global__ __launch_bounds__(kDispThreads) void k_disp(const double* __restrict_
         } else {
             cell_min_image_tri(cell, dx, dy, dz);
         }
+    };
+    double m = 0.0, mp = 0.0;
+    for (int i = blockIdx.x * kDispThreads + threadIdx.x; i < n; i += gridDim.x * kDispThreads) {
+        double dx = x[3 * i] - xref[3 * i];
+        double dy = x[3 * i + 1] - xref[3 * i + 1];
+        double dz = x[3 * i + 2] - xref[3 * i + 2];
+        min_image(dx, dy, dz);
         const double d2 = dx * dx + dy * dy + dz * dz;
         if (d2 > m) m = d2;
+        // displacement since the last prune of the kept list
+        double px = x[3 * i] - xprune[3 * i];
+        double py = x[3 * i + 1] - xprune[3 * i + 1];
+        double pz = x[3 * i + 2] - xprune[3 * i + 2];
+        min_image(px, py, pz);
+        const double p2 = px * px + py * py + pz * pz;
+        if (p2 > mp) mp = p2;
     }
     for (int k = 32; k >= 1; k >>= 1) {
         const double o = __shfl_xor(m, k, kWave);
         if (o > m) m = o;
+        const double op = __shfl_xor(mp, k, kWave);
+        if (op > mp) mp = op;
     }
-    if ((threadIdx.x & (kWave - 1)) == 0) wmax[threadIdx.x / kWave] = m;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        wmax[threadIdx.x / kWave] = m;
+        wmaxp[threadIdx.x / kWave] = mp;
+    }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    for (int q = 1; q < kDispThreads / kWave; ++q)
+    for (int q = 1; q < kDispThreads / kWave; ++q) {
         if (wmax[q] > m) m = wmax[q];
+        if (wmaxp[q] > mp) mp = wmaxp[q];
+    }
     if (m > 0.0) atomicMax(&vs->max_d2, static_cast<unsigned long long>(__double_as_longlong(m)));
+    if (mp > 0.0) atomicMax(&vs->max_p2, static_cast<unsigned long long>(__double_as_longlong(mp)));
     __threadfence();
     if (atomicAdd(&vs->ticket, 1u) != gridDim.x - 1) return;
     __threadfence();
     const double mx = __longlong_as_double(static_cast<long long>(atomicAdd(&vs->max_d2, 0ull)));
+    const double mpx = __longlong_as_double(static_cast<long long>(atomicAdd(&vs->max_p2, 0ull)));
     const unsigned int build = (atomicAdd(&vs->force, 0u) != 0u || mx > lim2) ? 1u : 0u;
     atomicExch(&vs->max_d2, 0ull);
+    atomicExch(&vs->max_p2, 0ull);
     atomicExch(&vs->ticket, 0u);
     if (query) {
         atomicExch(&vs->query, build);
     } else {
         atomicExch(&vs->build, build);
+        atomicExch(&vs->prune, (build || mpx > limp2) ? 1u : 0u);
         atomicExch(&vs->force, 0u);
         if (build) atomicOr(flags + 5, 1u);
     }
@@ -982,13 +1004,16 @@ __global__ __launch_bounds__(kRefreshThreads) void k_refresh(const VerletState* 
     }
 }
 
-// Verlet mode: one wave per i-cluster copies, in order, the entries of its
-// kept list whose tile holds an unmasked pair within the cutoff (plus 1e-3 A
-// for rounding) at the current slot positions, computed as the pair kernel
-// computes them.  The pair kernel tests every pair against the cutoff, so the
-// pruned list gives the same energies and forces with fewer tiles.  Each lane
-// takes one entry: its eight j-atom loads are in flight together, and the
-// i-cluster's atoms are broadcast from registers.
+// Verlet mode, when the displacement check set vs->prune (a build, or some
+// atom moved more than kPruneBuffer / 2 since the last prune): one wave per
+// i-cluster copies, in order, the entries of its kept list whose tile holds
+// an unmasked pair within cutoff + kPruneBuffer at the current slot
+// positions, computed as the pair kernel computes them, and records the
+// cluster's atom positions.  Until the next prune no pair can close in by
+// more than kPruneBuffer, and the pair kernel tests every pair against the
+// cutoff, so the pruned list gives the same energies and forces with fewer
+// tiles.  Each lane takes one entry: its eight j-atom loads are in flight
+// together, and the i-cluster's atoms are broadcast from registers.
 struct PruneArgs {
     const int* ncl_dev;
     int max_clusters;
@@ -1003,6 +1028,10 @@ struct PruneArgs {
     unsigned long long* out_mask;
     void* out_delta;
     double rc2;
+    const VerletState* vs;   // prune only when vs->prune is set
+    const double* x;         // raw coordinates, recorded as the prune's positions
+    double* x_prune;
+    const int* slot_atom;
 };
 
 constexpr int kPruneWaves = 4;
@@ -1013,7 +1042,13 @@ __global__ __launch_bounds__(kPruneWaves * kWave) void k_prune(const PruneArgs a
     const int lane = threadIdx.x & (kWave - 1);
     const int ci = __builtin_amdgcn_readfirstlane(blockIdx.x * kPruneWaves + (threadIdx.x >> 6));
     const int ncl = min(*a.ncl_dev, a.max_clusters);
-    if (ci >= ncl) return;  // whole wave leaves; no block barrier follows
+    if (ci >= ncl || !a.vs->prune) return;  // whole wave leaves; no block barrier follows
+    if (lane < kCluster) {
+        // the positions this prune is valid for
+        const int atom = a.slot_atom[ci * kCluster + lane];
+        if (atom >= 0)
+            for (int d = 0; d < 3; ++d) a.x_prune[3 * atom + d] = a.x[3 * atom + d];
+    }
     const R4* pos = static_cast<const R4*>(a.pos);
     const R4* din = static_cast<const R4*>(a.list_delta);
     R4* dout = static_cast<R4*>(a.out_delta);
@@ -1069,7 +1104,7 @@ __global__ __launch_bounds__(kPruneWaves * kWave) void k_prune(const PruneArgs a
 
 }  // namespace
 
-hipError_t launch_prune(mdgpu_ctx* c, bool mixed) {
+hipError_t launch_prune(mdgpu_ctx* c, bool mixed, const double* coords, const VerletState* vs) {
     Work& w = c->work;
     const Grid& g = c->grid;
     PruneArgs a{};
@@ -1085,8 +1120,12 @@ hipError_t launch_prune(mdgpu_ctx* c, bool mixed) {
     a.out_j = w.inner_j;
     a.out_mask = w.inner_mask;
     a.out_delta = mixed ? static_cast<void*>(w.inner_delta_f) : static_cast<void*>(w.inner_delta_d);
-    const double rc = std::max(c->params.lj_cutoff, c->params.ee_cutoff) + 1e-3;
+    const double rc = std::max(c->params.lj_cutoff, c->params.ee_cutoff) + kPruneBuffer;
     a.rc2 = rc * rc;
+    a.vs = vs;
+    a.x = coords;
+    a.x_prune = w.x_prune;
+    a.slot_atom = w.slot_atom;
     const int nb = (c->n_clusters + kPruneWaves - 1) / kPruneWaves;
     if (mixed) hipLaunchKernelGGL(k_prune<float>, dim3(nb), dim3(kPruneWaves * kWave), 0, c->stream, a);
     else hipLaunchKernelGGL(k_prune<double>, dim3(nb), dim3(kPruneWaves * kWave), 0, c->stream, a);
@@ -1110,8 +1149,9 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl
     ex.chunk = (ex.n1 + ex.n2 + nblk - 1) / nblk;
     if (vs) {
         const int db = std::max(1, std::min(kDispMaxBlocks, (n + kDispThreads - 1) / kDispThreads));
-        hipLaunchKernelGGL(k_disp, dim3(db), dim3(kDispThreads), 0, c->stream, coords, w.x_ref, n, g.cell,
-                           (0.5 * c->skin) * (0.5 * c->skin), 0, vs, w.flags);
+        hipLaunchKernelGGL(k_disp, dim3(db), dim3(kDispThreads), 0, c->stream, coords, w.x_ref, w.x_prune, n,
+                           g.cell, (0.5 * c->skin) * (0.5 * c->skin), (0.5 * kPruneBuffer) * (0.5 * kPruneBuffer), 0,
+                           vs, w.flags);
     }
     hipLaunchKernelGGL(k_bin, dim3(nblk), dim3(kBinThreads), 0, c->stream, coords, n, chunk, g.cell, g.ncx, g.ncy,
                        g.sx, g.sy, w.wrapped, w.col_of, w.col_count, w.block_off, ex, vs ? w.x_ref : nullptr, vs);
@@ -1152,8 +1192,9 @@ hipError_t launch_disp_query(mdgpu_ctx* c, const double* coords) {
     const Grid& g = c->grid;
     const int n = c->topo.n_atoms;
     const int db = std::max(1, std::min(kDispMaxBlocks, (n + kDispThreads - 1) / kDispThreads));
-    hipLaunchKernelGGL(k_disp, dim3(db), dim3(kDispThreads), 0, c->stream, coords, w.x_ref, n, g.cell,
-                       (0.5 * c->skin) * (0.5 * c->skin), 1, w.verlet, w.flags);
+    hipLaunchKernelGGL(k_disp, dim3(db), dim3(kDispThreads), 0, c->stream, coords, w.x_ref, w.x_prune, n, g.cell,
+                       (0.5 * c->skin) * (0.5 * c->skin), (0.5 * kPruneBuffer) * (0.5 * kPruneBuffer), 1, w.verlet,
+                       w.flags);
     return hipGetLastError();
 }
 

@@ -13,8 +13,9 @@ GPU (System::ljpairs_needs_rebuild, NonBondedUtils.cpp:87-100).  Bars:
   * along a trajectory, captured-graph replay gives what direct launches
     give (mixed bit for bit, FP64 to 1e-12), the number of list builds equals
     the oracle criterion's count, and the replay continues across rebuilds;
-  * the kept list is pruned to the cutoff at every evaluation: no more tiles
-    than a list built at the cutoff, the same pair count and FP64 energies.
+  * the kept list is pruned to cutoff + 0.2 A (again once an atom has moved
+    0.1 A since the last prune): about the tiles of a list built at the
+    cutoff, the same pair count and FP64 energies.
 """
 import numpy as np
 import pytest
@@ -258,10 +259,10 @@ def test_trajectory_replay_and_build_count(prec, polymer_2k):
 
 
 def test_kept_list_is_pruned_to_the_cutoff(polymer_2k):
-    """Each evaluation with a skin prunes the kept list (radius cutoff + skin)
-    to the tiles holding a pair within the cutoff: the pair kernel then runs
-    no more tiles than a list built at the cutoff alone, the same pairs, and
-    the same FP64 energies (1e-12)."""
+    """A kept list (radius cutoff + skin) is pruned to the tiles holding a pair
+    within cutoff + 0.2 A: the pair kernel then runs about as many tiles as a
+    list built at the cutoff alone (within 10 %), the same pairs, and the
+    same FP64 energies (1e-12)."""
     print("case polymer_2k pruned Verlet list", flush=True)
     prm_path, rst_path = polymer_2k
     top = N.Topology(prm_path, rst_path, N.SEM_AMBER)
@@ -282,8 +283,6 @@ def test_kept_list_is_pruned_to_the_cutoff(polymer_2k):
     (e0, c0), (e1, c1) = out[0.0], out[1.5]
     print(f"  tiles without skin {c0['tiles']}, pruned from cutoff + 1.5 A {c1['tiles']}; pairs {c0['pairs']}")
     assert c1["pairs"] == c0["pairs"]
-    # (a tile with a pair just beyond the cutoff, inside the prune's 1e-3 A
-    # rounding margin, may be kept by the prune and not by the search)
-    assert c1["tiles"] <= 1.001 * c0["tiles"]
+    assert c1["tiles"] <= 1.1 * c0["tiles"]
     for k in TERMS_AMBER:
         assert abs(e1[k] - e0[k]) <= 1e-12 * max(abs(e0[k]), 1.0), k
