@@ -343,8 +343,9 @@ int mdgpu_get_virial(const mdgpu_ctx* ctx, double virial[9]);
  * and kept across evaluations; each evaluation first measures, on the GPU,
  * every atom's minimum-image displacement since the build and rebuilds the
  * list only when some atom has moved more than skin / 2 (the reference's
- * test dr^2 > (skin/2)^2); every evaluation then prunes the kept list to the
- * tiles with a pair inside the cutoff.  Kept clusters follow their atoms; energies and
+ * test dr^2 > (skin/2)^2).  The pair kernel reads the kept list pruned to the
+ * tiles with a pair within cutoff + 0.2 A, pruned again once an atom has
+ * moved 0.1 A since the last prune.  Kept clusters follow their atoms; energies and
  * forces are those of a fresh list (every pair within the cutoff is in the
  * list).  A new box, system, parameter set or skin, a redo and an adjoint
  * call force the next evaluation to build.  The decision is made on the
