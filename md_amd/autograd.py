@@ -108,7 +108,10 @@ def energy(model: Model, values: dict, frames: torch.Tensor, box) -> torch.Tenso
 def lj_tables(sigma: torch.Tensor, epsilon: torch.Tensor, nb_index: np.ndarray, n_lj: int):
     """Lorentz-Berthelot A, B per LJ table entry from per-type sigma, epsilon
     (A = 4 eps s^12, B = 4 eps s^6, s = (s_i + s_j)/2, eps = sqrt(eps_i eps_j));
-    entries no type pair maps to stay 0."""
+    entries no type pair maps to stay 0.  A pair with eps_i eps_j = 0 (a type
+    without LJ, such as a TIP3P hydrogen) gets eps = 0 and a zero gradient: the
+    square root is taken only of positive products, so its derivative at 0
+    cannot turn the other type's gradient into NaN."""
     t = sigma.numel()
     idx = np.asarray(nb_index).reshape(t, t) - 1
     a = torch.zeros(n_lj, dtype=torch.float64)
@@ -121,7 +124,9 @@ def lj_tables(sigma: torch.Tensor, epsilon: torch.Tensor, nb_index: np.ndarray, 
     ti = torch.tensor([first[k][0] for k in sorted(first)], dtype=torch.long)
     tj = torch.tensor([first[k][1] for k in sorted(first)], dtype=torch.long)
     s = 0.5 * (sigma[ti] + sigma[tj])
-    e = torch.sqrt(epsilon[ti] * epsilon[tj])
+    prod = epsilon[ti] * epsilon[tj]
+    pos = prod > 0
+    e = torch.where(pos, torch.sqrt(torch.where(pos, prod, torch.ones_like(prod))), torch.zeros_like(prod))
     s6 = s ** 6
     a = a.index_put((ent,), 4.0 * e * s6 * s6)
     b = b.index_put((ent,), 4.0 * e * s6)

@@ -19,6 +19,7 @@
 // its 1/r^12 then exceeds every fixed-point range fine enough for the FP64
 // bar (the mixed form uses 2^32).
 #include "ctx.h"
+#include "erfc_f32.h"
 
 namespace mdg {
 
@@ -39,17 +40,8 @@ __device__ __forceinline__ void fx_add(unsigned long long* acc, double v, double
     if (q != 0) atomicAdd(acc, static_cast<unsigned long long>(q));
 }
 
-// erfc by Abramowitz & Stegun 7.1.26 (the pair kernel's mixed-form form)
-__device__ __forceinline__ float erfc_as(float x, float e) {
-    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, x, 1.0f));
-    float p = 1.061405429f;
-    p = fmaf(p, t, -1.453152027f);
-    p = fmaf(p, t, 1.421413741f);
-    p = fmaf(p, t, -0.284496736f);
-    p = fmaf(p, t, 0.254829592f);
-    return p * t * e;
-}
-__device__ __forceinline__ float erfc_r(float x) { return erfc_as(x, __expf(-x * x)); }
+// erfc of the mixed form: the pair kernel's relative-accuracy fit (erfc_f32.h)
+__device__ __forceinline__ float erfc_r(float x) { return erfc_f32(x, __expf(-x * x)); }
 __device__ __forceinline__ double erfc_r(double x) { return erfc(x); }
 __device__ __forceinline__ float rsqrt_r(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double rsqrt_r(double x) { return 1.0 / sqrt(x); }
@@ -86,28 +78,56 @@ struct PairAdjArgs {
     unsigned int* ovf;
 };
 
-// LJ derivative bins: FP64 in the FP64 form, 64-bit fixed point (2^32) in the
-// mixed form.
+// LJ derivative bins: FP64 in the FP64 form; in the mixed form two 64-bit
+// fixed-point words per entry, deterministic like every other sum of the
+// mixed adjoint.  A contribution v is split into hi = floor(v 2^16) and the
+// remainder lo = round((v 2^16 - hi) 2^32), so the entry is
+// sum hi / 2^16 + sum lo / 2^48: the resolution of a 2^48 fixed point with
+// the range of a 2^16 one (|v| < 1.4e14, r > 0.066 A for 1/r^12).  The sums of
+// the A and B entries cancel in the derivatives with respect to sigma and
+// epsilon, so a single 2^32 word (the resolution of the other sums) would
+// quantise the nearest pairs' 1/r^12 to ~5e-5 each.
 template <typename real>
 struct LjBin;
 template <>
 struct LjBin<double> {
     using T = double;
+    static constexpr int kWords = 1;
     static __device__ __forceinline__ void add(double* s, double v, unsigned int*) { atomicAdd(s, v); }
-    static __device__ __forceinline__ void flush(double v, double* gd, unsigned long long*) {
-        if (v != 0.0) atomicAdd(gd, v);
+    static __device__ __forceinline__ void flush(const double* v, double* gd, unsigned long long*) {
+        if (*v != 0.0) atomicAdd(gd, *v);
     }
 };
 template <>
 struct LjBin<float> {
     using T = unsigned long long;
+    static constexpr int kWords = 2;
     static __device__ __forceinline__ void add(unsigned long long* s, double v, unsigned int* ovf) {
-        fx_add(s, v, kAdjScaleLJm, ovf);
+        const double x = v * kAdjScaleLJhi;
+        if (!(fabs(x) < 9.0e18)) {  // also catches NaN
+            atomicOr(ovf, 1u);
+            return;
+        }
+        const double h = floor(x);
+        const long long hi = static_cast<long long>(h);
+        const unsigned long long lo = static_cast<unsigned long long>(__double2ll_rn((x - h) * 4294967296.0));
+        if (hi) atomicAdd(s, static_cast<unsigned long long>(hi));
+        if (lo) atomicAdd(s + 1, lo);
     }
-    static __device__ __forceinline__ void flush(unsigned long long v, double*, unsigned long long* gf) {
-        if (v) atomicAdd(gf, v);
+    static __device__ __forceinline__ void flush(const unsigned long long* v, double*, unsigned long long* gf) {
+        if (v[0]) atomicAdd(gf, v[0]);
+        if (v[1]) atomicAdd(gf + 1, v[1]);
     }
 };
+
+// 1/r^2 from r^2 in FP32: the hardware reciprocal and one Newton step, so
+// that 1/r^6 and 1/r^12 carry unbiased rounding only (their sums over the
+// nearest pairs set the sigma and epsilon derivatives)
+__device__ __forceinline__ float inv_r2(float r2) {
+    const float y = __builtin_amdgcn_rcpf(r2);
+    return fmaf(y, fmaf(-r2, y, 1.0f), y);
+}
+__device__ __forceinline__ double inv_r2(double r2) { return 1.0 / r2; }
 
 // One wave per (i-cluster, list part) as in the pair kernel, lane = 8 tj + ti.
 // LJ derivative sums go to per-block LDS bins, flushed once per block.
@@ -117,11 +137,12 @@ __global__ __launch_bounds__(kWaves * kWave) void k_pair_adjoint(const PairAdjAr
     using Bin = LjBin<real>;
     using BT = typename Bin::T;
     extern __shared__ unsigned char smem[];
+    constexpr int kW = Bin::kWords;
     BT* s_a = reinterpret_cast<BT*>(smem);
-    BT* s_b = s_a + a.n_lj;
-    int* s_p = reinterpret_cast<int*>(s_b + a.n_lj);
+    BT* s_b = s_a + kW * a.n_lj;
+    int* s_p = reinterpret_cast<int*>(s_b + kW * a.n_lj);
     const int ntp2 = a.ntp * a.ntp;
-    for (int k = threadIdx.x; k < a.n_lj; k += blockDim.x) {
+    for (int k = threadIdx.x; k < kW * a.n_lj; k += blockDim.x) {
         s_a[k] = 0;
         s_b[k] = 0;
     }
@@ -159,7 +180,7 @@ __global__ __launch_bounds__(kWaves * kWave) void k_pair_adjoint(const PairAdjAr
                 if (r2 <= a.rc2_lj) {
                     const int p = s_p[tyi + mj.x];
                     if (p >= 0) {
-                        const real i2 = rinv * rinv, i6 = i2 * i2 * i2, i12 = i6 * i6;
+                        const real i2 = inv_r2(r2), i6 = i2 * i2 * i2, i12 = i6 * i6;
                         real ea, eb;
                         if (kSwitch) {
                             real xs = (r - a.rs) * a.inv_sw;
@@ -171,8 +192,8 @@ __global__ __launch_bounds__(kWaves * kWave) void k_pair_adjoint(const PairAdjAr
                             ea = i12 - a.c12 + (r - a.rc_lj) * a.g12;
                             eb = -(i6 - a.c6) - (r - a.rc_lj) * a.g6;
                         }
-                        Bin::add(s_a + p, static_cast<double>(ea), a.ovf);
-                        Bin::add(s_b + p, static_cast<double>(eb), a.ovf);
+                        Bin::add(s_a + kW * p, static_cast<double>(ea), a.ovf);
+                        Bin::add(s_b + kW * p, static_cast<double>(eb), a.ovf);
                     }
                 }
                 if (r2 <= a.rc2_ee) {
@@ -190,8 +211,8 @@ __global__ __launch_bounds__(kWaves * kWave) void k_pair_adjoint(const PairAdjAr
     }
     __syncthreads();
     for (int k = threadIdx.x; k < a.n_lj; k += blockDim.x) {
-        Bin::flush(s_a[k], a.lj_ad + k, a.lj_a + k);
-        Bin::flush(s_b[k], a.lj_bd + k, a.lj_b + k);
+        Bin::flush(s_a + kW * k, a.lj_ad + k, a.lj_a + kW * k);
+        Bin::flush(s_b + kW * k, a.lj_bd + k, a.lj_b + kW * k);
     }
 }
 
@@ -245,7 +266,6 @@ __device__ __forceinline__ real wrap_pi(real d) {
     return d;
 }
 
-template <typename real>
 struct BondAdjArgs {
     const double* x;
     Topo t;
@@ -256,9 +276,13 @@ struct BondAdjArgs {
 };
 
 // One thread per term (bonds, UB, angles, dihedrals, impropers, CMAP, in that
-// order), as the bonded kernel walks them.
-template <typename real>
-__global__ __launch_bounds__(kBlock) void k_bonded_adjoint(const BondAdjArgs<real> a) {
+// order), as the bonded kernel walks them.  FP64 in both precision forms: each
+// term forms only a few scalars (a length, angle or torsion and its difference
+// from the equilibrium), and in FP32 r - r0 or theta - theta0 would keep only
+// about 6e-8 r / (r - r0) of relative precision; the per-type sums then cancel
+// further.  The bonded terms are a negligible share of the adjoint's work.
+__global__ __launch_bounds__(kBlock) void k_bonded_adjoint(const BondAdjArgs a) {
+    using real = double;
     const Topo& t = a.t;
     const double* x = a.x;
     const int nb = t.n_bonds, nu = nb + t.n_ubs, na = nu + t.n_angles, nd = na + t.n_dihedrals,
@@ -435,7 +459,7 @@ hipError_t run_pair(mdgpu_ctx* c) {
     a.lj_ad = b.lj_d;
     a.lj_bd = b.lj_d + t.n_lj;
     a.ovf = reinterpret_cast<unsigned int*>(b.acc + b.flag);
-    const size_t shm = 2 * sizeof(unsigned long long) * t.n_lj + sizeof(int) * a.ntp * a.ntp;
+    const size_t shm = 2 * LjBin<real>::kWords * sizeof(unsigned long long) * t.n_lj + sizeof(int) * a.ntp * a.ntp;
     if (shm > 60 * 1024) return hipErrorInvalidValue;
     const int nwork = c->n_clusters * c->split;
     const int nblk = std::max(1, std::min(1024, (nwork + kWaves - 1) / kWaves));
@@ -459,13 +483,9 @@ hipError_t launch_bonded_adjoint(mdgpu_ctx* c, const double* coords, bool mixed)
               off_imp = off_dih + 2 * t.n_dih_types;
     unsigned int* ovf = reinterpret_cast<unsigned int*>(b.acc + b.flag);
     const dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
-    if (mixed) {
-        BondAdjArgs<float> a{coords, t, b.acc + b.bonded, off_ub, off_ang, off_dih, off_imp, b.acc + b.cmap, ovf};
-        hipLaunchKernelGGL(k_bonded_adjoint<float>, grid, block, 0, c->stream, a);
-    } else {
-        BondAdjArgs<double> a{coords, t, b.acc + b.bonded, off_ub, off_ang, off_dih, off_imp, b.acc + b.cmap, ovf};
-        hipLaunchKernelGGL(k_bonded_adjoint<double>, grid, block, 0, c->stream, a);
-    }
+    (void)mixed;  // one FP64 kernel serves both forms
+    const BondAdjArgs a{coords, t, b.acc + b.bonded, off_ub, off_ang, off_dih, off_imp, b.acc + b.cmap, ovf};
+    hipLaunchKernelGGL(k_bonded_adjoint, grid, block, 0, c->stream, a);
     return hipGetLastError();
 }
 

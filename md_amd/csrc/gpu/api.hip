@@ -346,13 +346,10 @@ extern "C" int mdgpu_upload_system(mdgpu_ctx* c, const mdgpu_system_desc* s) {
     t.cmap_n_sets = s->cmap_n_sets;
     if (s->n_cmaps > 0) {
         const size_t nc = static_cast<size_t>(s->cmap_n_sets) * s->cmap_resolution * s->cmap_resolution * 16;
-        std::vector<float> cf(nc);
-        for (size_t k = 0; k < nc; ++k) cf[k] = static_cast<float>(s->cmap_coeff[k]);
         HIP_TRY(upload(c, &t.cmap_atoms, reinterpret_cast<const int*>(s->cmap_atoms), 5 * static_cast<size_t>(s->n_cmaps)),
                 "upload cmap");
         HIP_TRY(upload(c, &t.cmap_set, reinterpret_cast<const int*>(s->cmap_set), s->n_cmaps), "upload cmap");
         HIP_TRY(upload(c, &t.cmap_coeff, s->cmap_coeff, nc), "upload cmap");
-        HIP_TRY(upload(c, &t.cmap_coeff_f, cf.data(), nc), "upload cmap");
     }
     // exclusions
     t.has_excl = s->excl_start != nullptr && s->excl_start[n] > 0;
@@ -1182,8 +1179,8 @@ int ensure_adjoint(mdgpu_ctx* c) {
     b.q_dir = o; o += n;
     b.q_excl = o; o += n;
     b.q_14 = o; o += n;
-    b.lj_a = o; o += nlj;
-    b.lj_b = o; o += nlj;
+    b.lj_a = o; o += 2 * nlj;
+    b.lj_b = o; o += 2 * nlj;
     b.lj14_a = o; o += nlj;
     b.lj14_b = o; o += nlj;
     b.bonded = o;
@@ -1257,7 +1254,15 @@ int mdgpu_adjoint(mdgpu_ctx* c, const double* coords, const double box[3], const
         HIP_TRY(mdg::launch_pair_adjoint(c, mixed), "pair adjoint");
         HIP_TRY(mdg::launch_corrections_adjoint(c), "corrections adjoint");
         HIP_TRY(mdg::launch_bonded_adjoint(c, w.coords, mixed), "bonded adjoint");
-        if (pme) HIP_TRY(mdg::launch_pme_potential(c, mixed, 0, b.pot_rec), "pme potential");
+        if (pme) {
+            // the reciprocal potential in FP64 in both forms: the FP32 grid
+            // transforms leave ~1e-6 of the potential, which the charge
+            // gradient (a potential, with cancellation between the direct and
+            // reciprocal parts and between frames) would inherit; the mixed
+            // form first sets up the FP64 spline records with a second sort
+            if (mixed) HIP_TRY(mdg::launch_sort(c, w.coords, false, t.has_excl, nullptr), "sort");
+            HIP_TRY(mdg::launch_pme_potential(c, false, 0, b.pot_rec), "pme potential");
+        }
         unsigned int flags[mdg::kFlags];
         HIP_TRY(hipMemcpyAsync(flags, w.flags, sizeof flags, hipMemcpyDeviceToHost, c->stream), "read flags");
         HIP_TRY(hipMemcpyAsync(acc.data(), b.acc, b.total * sizeof(unsigned long long), hipMemcpyDeviceToHost,
@@ -1306,8 +1311,15 @@ int mdgpu_adjoint(mdgpu_ctx* c, const double* coords, const double box[3], const
         for (int k = 0; k < cnt; ++k) dst[k] = wgt * val(off + static_cast<size_t>(stride) * k, scale);
     };
     if (mixed) {
-        fill(out->lj_a, b.lj_a, t.n_lj, 1, wt.vdw, mdg::kAdjScaleLJm);
-        fill(out->lj_b, b.lj_b, t.n_lj, 1, wt.vdw, mdg::kAdjScaleLJm);
+        // two words per entry: signed high word at 2^16, remainder at 2^48
+        auto two = [&](double* dst, size_t off) {
+            if (!dst) return;
+            for (int k = 0; k < t.n_lj; ++k)
+                dst[k] = wt.vdw * (val(off + 2 * static_cast<size_t>(k), mdg::kAdjScaleLJhi) +
+                                   static_cast<double>(acc[off + 2 * static_cast<size_t>(k) + 1]) / mdg::kAdjScaleLJlo);
+        };
+        two(out->lj_a, b.lj_a);
+        two(out->lj_b, b.lj_b);
     } else {
         for (int k = 0; k < t.n_lj; ++k) {
             if (out->lj_a) out->lj_a[k] = wt.vdw * ljd[k];

@@ -1,20 +1,15 @@
-// Fused bonded kernel and the FP64 exclusion / 1-4 correction kernel.
+// Block counts of the bonded terms and the FP64 exclusion / 1-4 corrections.
 //
-// k_bonded replaces, in one launch with one thread per term, the reference's
-// calculate_{bond,angle,dihedral,UB,improper,CMAP}_energy
-// (src/System/BondedEnergy.cpp:10-166) and calculate_forces_{bonds,UBbonds,
-// angles,cosinedihedrals,harmonicImpropers,cmap} (src/System/BondedForces.cpp:
-// 10-479).  The threads of a block walk the term lists in the order bonds,
-// Urey-Bradley, angles, dihedrals, impropers, CMAP.  Coordinate differences are
-// taken in FP64 from the raw coordinates (the reference applies no minimum
-// image to bonded terms); the mixed form does the rest in FP32.
-//
-// The corrections (AMBER semantics, FP64 in both forms; blocks after the
-// bonded ones in the same launch) add, with minimum
-// image and no cutoff, -k q_i q_j erf(a r)/r for every excluded pair (the
-// reciprocal sum counts it; cf. the exclusion loop of NonBondedEnergy.cpp:
-// 271-304) and, for every 1-4 pair, the 1-4 LJ (LENNARD_JONES_14 tables) plus
-// k q_i q_j (1/(SCEE r) - erf(a r)/r).
+// Both run as the last blocks of the pair launch (k_nonbonded): the bonded
+// blocks after the pair blocks, one thread per term, walking the concatenated
+// term lists (bonds, Urey-Bradley, angles, dihedrals, impropers, CMAP; the
+// device code is bonded_terms() in bonded_terms.h, replacing
+// calculate_{bond,angle,dihedral,UB,improper,CMAP}_energy,
+// src/System/BondedEnergy.cpp:10-166, and calculate_forces_*,
+// src/System/BondedForces.cpp:10-479), then the correction blocks, one thread
+// per excluded or 1-4 pair (correction_pairs(); the exclusion loop of
+// NonBondedEnergy.cpp:271-304 and the 1-4 terms).  These helpers size those
+// block ranges.
 #include "bonded_terms.h"
 
 namespace mdg {

@@ -32,10 +32,14 @@ __device__ __forceinline__ real dot(const R3<real>& a, const R3<real>& b) {
 }
 
 template <typename real>
+__device__ __forceinline__ R3<real> cvt(const R3<double>& v) {
+    return {static_cast<real>(v.x), static_cast<real>(v.y), static_cast<real>(v.z)};
+}
+
+template <typename real>
 struct BArgs {
     const double* x;
     Topo t;
-    const real* cmap_coeff;
     double* force_d;
     FixedAcc fx;
     double* partial;
@@ -139,18 +143,23 @@ __device__ __forceinline__ void bonded_terms(const BArgs<real>& a, int k, int ro
               ni = nd + t.n_impropers, nc = ni + t.n_cmaps;
     double e[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // bond, angle, dihedral, UB, improper, CMAP, virial
     double* w = e + 6;
+    // The geometry of a term (bond length, angle, torsion), its difference
+    // from the equilibrium and the energy are formed in FP64 in both forms:
+    // in FP32, r - r0 keeps only about 6e-8 r / (r - r0) of relative
+    // precision.  The mixed form assembles the force vectors in FP32.
     if (k < nu) {
         // harmonic bond / Urey-Bradley: E = k (r - r0)^2, F_a = -2k (r - r0) d / r
         const bool ub = k >= nb;
         const int q = ub ? k - nb : k;
         const int2 at = ub ? t.ub_atoms[q] : t.bond_atoms[q];
         const double2 par = ub ? t.ub_par[t.ub_type[q]] : t.bond_par[t.bond_type[q]];
-        const R3<real> d = diff<real>(x, at.x, at.y);
-        const real r = sqrt(dot(d, d));
-        const real dr = r - static_cast<real>(par.y);
-        e[ub ? 3 : 0] = static_cast<double>(static_cast<real>(par.x) * dr * dr);
-        if (r >= static_cast<real>(1e-12)) {
-            const real s = static_cast<real>(-2.0 * par.x) * dr / r;
+        const R3<double> dd = diff<double>(x, at.x, at.y);
+        const double r = sqrt(dot(dd, dd));
+        const double dr = r - par.y;
+        e[ub ? 3 : 0] = par.x * dr * dr;
+        if (r >= 1e-12) {
+            const real s = static_cast<real>(-2.0 * par.x * dr / r);
+            const R3<real> d = cvt<real>(dd);
             addf(a, at.x, s * d.x, s * d.y, s * d.z);
             addf(a, at.y, -s * d.x, -s * d.y, -s * d.z);
             if (a.virial) vir_add(w, d, R3<real>{s * d.x, s * d.y, s * d.z});
@@ -160,25 +169,27 @@ __device__ __forceinline__ void bonded_terms(const BArgs<real>& a, int k, int ro
         const int q = k - nu;
         const int4 at = t.angle_atoms[q];
         const double2 par = t.angle_par[t.angle_type[q]];
-        const R3<real> ba = diff<real>(x, at.x, at.y), bc = diff<real>(x, at.z, at.y);
-        const real ab2 = dot(ba, ba), bc2 = dot(bc, bc), dt = dot(ba, bc);
-        real th;
-        if (ab2 < static_cast<real>(1e-14) || bc2 < static_cast<real>(1e-14)) {
-            th = static_cast<real>(NAN);
+        const R3<double> bad = diff<double>(x, at.x, at.y), bcd = diff<double>(x, at.z, at.y);
+        const double ab2d = dot(bad, bad), bc2d = dot(bcd, bcd);
+        double th;
+        if (ab2d < 1e-14 || bc2d < 1e-14) {
+            th = NAN;
         } else {
-            real cs = dt * (static_cast<real>(1) / sqrt(ab2 * bc2));
-            cs = cs < -1 ? static_cast<real>(-1) : (cs > 1 ? static_cast<real>(1) : cs);
+            double cs = dot(bad, bcd) * (1.0 / sqrt(ab2d * bc2d));
+            cs = cs < -1 ? -1.0 : (cs > 1 ? 1.0 : cs);
             th = acos(cs);
         }
-        const real dth = th - static_cast<real>(par.y);
-        e[1] = static_cast<double>(static_cast<real>(par.x) * dth * dth);
+        const double dth = th - par.y;
+        e[1] = par.x * dth * dth;
+        const R3<real> ba = cvt<real>(bad), bc = cvt<real>(bcd);
+        const real ab2 = dot(ba, ba), bc2 = dot(bc, bc), dt = dot(ba, bc);
         const real ab = sqrt(ab2), cb = sqrt(bc2);
         R3<real> pa = {dt * ba.x - ab2 * bc.x, dt * ba.y - ab2 * bc.y, dt * ba.z - ab2 * bc.z};
         R3<real> pc = {dt * bc.x - bc2 * ba.x, dt * bc.y - bc2 * ba.y, dt * bc.z - bc2 * ba.z};
         const real mpa = sqrt(dot(pa, pa)), mpc = sqrt(dot(pc, pc));
         const real tiny = static_cast<real>(1e-12);
         if (ab >= tiny && mpa >= tiny && cb >= tiny && mpc >= tiny) {
-            const real fm = static_cast<real>(-2.0 * par.x) * dth;
+            const real fm = static_cast<real>(-2.0 * par.x * dth);
             const real sa = fm / ab / mpa, sc = fm / cb / mpc;
             const R3<real> fa = {sa * pa.x, sa * pa.y, sa * pa.z}, fc = {sc * pc.x, sc * pc.y, sc * pc.z};
             addf(a, at.x, fa.x, fa.y, fa.z);
@@ -194,62 +205,64 @@ __device__ __forceinline__ void bonded_terms(const BArgs<real>& a, int k, int ro
         const bool imp = k >= nd;
         const int q = imp ? k - nd : k - na;
         const int4 at = imp ? t.imp_atoms[q] : t.dih_atoms[q];
-        const R3<real> b1 = diff<real>(x, at.y, at.x), b2 = diff<real>(x, at.z, at.y), b3 = diff<real>(x, at.w, at.z);
-        const real phi = dihedral_angle(b1, b2, b3);
+        const R3<double> b1 = diff<double>(x, at.y, at.x), b2 = diff<double>(x, at.z, at.y),
+                         b3 = diff<double>(x, at.w, at.z);
+        const double phi = dihedral_angle(b1, b2, b3);
         real tau;
         if (!imp) {
             const double4 par = t.dih_par[t.dih_type[q]];
-            const real arg = static_cast<real>(par.y) * phi - static_cast<real>(par.z);
-            e[2] = static_cast<double>(static_cast<real>(par.x) * (1 + cos(arg)));
-            tau = static_cast<real>(par.x * par.y) * sin(arg);
+            const double arg = par.y * phi - par.z;
+            e[2] = par.x * (1 + cos(arg));
+            tau = static_cast<real>(par.x * par.y * sin(arg));
         } else {
             const double2 par = t.imp_par[t.imp_type[q]];
-            const real dl = phi - static_cast<real>(par.y);
-            const real dw = wrap_pi(dl);
-            e[4] = static_cast<double>(static_cast<real>(par.x) * (a.amber ? dw * dw : dl * dl));
-            tau = static_cast<real>(-2.0 * par.x) * dw;
+            const double dl = phi - par.y;
+            const double dw = wrap_pi(dl);
+            e[4] = par.x * (a.amber ? dw * dw : dl * dl);
+            tau = static_cast<real>(-2.0 * par.x * dw);
         }
-        const R3<real> ba = {-b1.x, -b1.y, -b1.z}, cb = {-b2.x, -b2.y, -b2.z}, dc = {-b3.x, -b3.y, -b3.z};
+        const R3<real> ba = cvt<real>({-b1.x, -b1.y, -b1.z}), cb = cvt<real>({-b2.x, -b2.y, -b2.z}),
+                       dc = cvt<real>({-b3.x, -b3.y, -b3.z});
         torsion(a, at.x, at.y, at.z, at.w, ba, cb, dc, tau, true, w);
     } else if (k < nc) {
         // CMAP (BondedEnergy.cpp:124-166, CMapGroup.cpp:82-237,
-        // BondedForces.cpp:334-479)
+        // BondedForces.cpp:334-479); the patch from the FP64 coefficients
         const int q = k - ni;
         const int* at = t.cmap_atoms + 5 * q;
-        const R3<real> b1 = diff<real>(x, at[1], at[0]), b2 = diff<real>(x, at[2], at[1]),
-                       b3 = diff<real>(x, at[3], at[2]), b4 = diff<real>(x, at[4], at[3]);
-        const real phi = dihedral_angle(b1, b2, b3), psi = dihedral_angle(b2, b3, b4);
+        const R3<double> b1 = diff<double>(x, at[1], at[0]), b2 = diff<double>(x, at[2], at[1]),
+                         b3 = diff<double>(x, at[3], at[2]), b4 = diff<double>(x, at[4], at[3]);
+        const double phi = dihedral_angle(b1, b2, b3), psi = dihedral_angle(b2, b3, b4);
         const int res = t.cmap_res;
-        const real sp = static_cast<real>(2 * kPi) / static_cast<real>(res);
-        const real pp = wrap_pi(phi) + static_cast<real>(kPi), ss = wrap_pi(psi) + static_cast<real>(kPi);
-        const real gp = pp / sp, gs = ss / sp;
+        const double sp = (2 * kPi) / static_cast<double>(res);
+        const double pp = wrap_pi(phi) + kPi, ss = wrap_pi(psi) + kPi;
+        const double gp = pp / sp, gs = ss / sp;
         int row = static_cast<int>(gp), col = static_cast<int>(gs);
-        const real u = gp - static_cast<real>(row), v = gs - static_cast<real>(col);
+        const double u = gp - static_cast<double>(row), v = gs - static_cast<double>(col);
         row %= res;
         col %= res;
         if (row < 0) row += res;
         if (col < 0) col += res;
-        real en = 0, du = 0, dv = 0;
+        double en = 0, du = 0, dv = 0;
         if (phi == phi && psi == psi) {
-            const real* c = a.cmap_coeff + (static_cast<size_t>(t.cmap_set[q] - 1) * res * res + row * res + col) * 16;
-            const real up[4] = {1, u, u * u, u * u * u}, vp[4] = {1, v, v * v, v * v * v};
-            const real dup[4] = {0, 1, 2 * u, 3 * u * u}, dvp[4] = {0, 1, 2 * v, 3 * v * v};
+            const double* c = t.cmap_coeff + (static_cast<size_t>(t.cmap_set[q] - 1) * res * res + row * res + col) * 16;
+            const double up[4] = {1, u, u * u, u * u * u}, vp[4] = {1, v, v * v, v * v * v};
+            const double dup[4] = {0, 1, 2 * u, 3 * u * u}, dvp[4] = {0, 1, 2 * v, 3 * v * v};
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const real cc = c[4 * i + j];
+                    const double cc = c[4 * i + j];
                     en += cc * up[i] * vp[j];
                     du += cc * dup[i] * vp[j];
                     dv += cc * up[i] * dvp[j];
                 }
         } else {
-            en = static_cast<real>(NAN);
+            en = NAN;
         }
-        e[5] = static_cast<double>(en);
-        const real dphi = du / sp, dpsi = dv / sp;
-        const R3<real> ba = {-b1.x, -b1.y, -b1.z}, cb = {-b2.x, -b2.y, -b2.z}, dc = {-b3.x, -b3.y, -b3.z},
-                       ed = {-b4.x, -b4.y, -b4.z};
+        e[5] = en;
+        const real dphi = static_cast<real>(du / sp), dpsi = static_cast<real>(dv / sp);
+        const R3<real> ba = cvt<real>({-b1.x, -b1.y, -b1.z}), cb = cvt<real>({-b2.x, -b2.y, -b2.z}),
+                       dc = cvt<real>({-b3.x, -b3.y, -b3.z}), ed = cvt<real>({-b4.x, -b4.y, -b4.z});
         // the reference skips both halves if any of the five products is degenerate
         const bool ok = torsion(a, at[0], at[1], at[2], at[3], ba, cb, dc, -dphi, false, w) &&
                         torsion(a, at[1], at[2], at[3], at[4], cb, dc, ed, -dpsi, false, w);

@@ -39,7 +39,7 @@ struct Topo {
     int4* dih_atoms = nullptr;   int* dih_type = nullptr;   double4* dih_par = nullptr;   // (k, n, phase)
     int4* imp_atoms = nullptr;   int* imp_type = nullptr;   double2* imp_par = nullptr;   // (k, psi0)
     int* cmap_atoms = nullptr;   int* cmap_set = nullptr;                                // [5n], [n]
-    double* cmap_coeff = nullptr; float* cmap_coeff_f = nullptr;
+    double* cmap_coeff = nullptr;
     int cmap_res = 0, cmap_n_sets = 0;
     // exclusions
     int* excl_start = nullptr;  int* excl_list = nullptr; int has_excl = 0;
@@ -65,7 +65,8 @@ struct AdjointBuf {
     double* lj_d = nullptr;
     size_t cap_lj_d = 0;
     size_t q_dir = 0, q_excl = 0, q_14 = 0;          // [N] each
-    size_t lj_a = 0, lj_b = 0, lj14_a = 0, lj14_b = 0;  // [n_lj] each
+    size_t lj_a = 0, lj_b = 0;                       // [2 n_lj] each: (high, remainder) word pairs (mixed form)
+    size_t lj14_a = 0, lj14_b = 0;                   // [n_lj] each
     size_t bonded = 0;                               // [2 x (bond, UB, angle, dihedral, improper types)]
     size_t cmap = 0;                                 // [sets x res^2 x 16]
     size_t flag = 0;                                 // overflow flag (as an element)
@@ -73,7 +74,8 @@ struct AdjointBuf {
 };
 constexpr double kAdjScaleQ = 68719476736.0;        // 2^36: potentials
 constexpr double kAdjScaleLJ = 17592186044416.0;    // 2^44: 1-4 LJ and CMAP coefficient sums
-constexpr double kAdjScaleLJm = 4294967296.0;       // 2^32: LJ sums of the mixed form
+constexpr double kAdjScaleLJhi = 65536.0;           // 2^16: LJ sums of the mixed form, high word
+constexpr double kAdjScaleLJlo = 281474976710656.0; // 2^48: their remainder word
 constexpr double kAdjScaleB = 1099511627776.0;      // 2^40: bonded constants and equilibria
 
 // Verlet-list state of a context on the device.  k_disp folds every atom's

@@ -208,16 +208,37 @@ int mdgpu_md_run(mdgpu_ctx* c, double* d_x, double* d_v, double* d_f, const doub
         MD_TRY(hipMemcpyAsync(m->save + 2 * n3, d_f, bytes, hipMemcpyDeviceToDevice, s), "save");
         MD_TRY(hipMemsetAsync(m->step, 0, sizeof(int), s), "step counter");
         c->timing.record = false;
+        // back to the saved entry state (the stream's earlier work included)
+        auto restore = [&]() -> hipError_t {
+            hipError_t e = hipStreamSynchronize(s);
+            if (e == hipSuccess) e = hipMemcpyAsync(d_x, m->save, bytes, hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(d_v, m->save + n3, bytes, hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(d_f, m->save + 2 * n3, bytes, hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            return e;
+        };
         for (int k = 0; k < nsteps; ++k) {
             hipLaunchKernelGGL(mdg::k_kick_drift, dim3(nb), dim3(mdg::kMdThreads), 0, s, d_x, d_v, d_f, m->inv_mass,
                                hdt, dt, n3);
             st = mdg::ctx_enqueue_step(c, d_x, d_f);
-            if (st != MDGPU_OK) return st;
-            hipLaunchKernelGGL(mdg::k_kick, dim3(nb), dim3(mdg::kMdThreads), 0, s, d_v, d_f, m->inv_mass, m->mass,
-                               hdt, n3, m->ke_part);
-            hipLaunchKernelGGL(mdg::k_record, dim3(1), dim3(mdg::kMdThreads), 0, s, m->ke_part, nb,
-                               c->work.energies, m->trace, m->step, nsteps);
-            MD_TRY(hipGetLastError(), "md step");
+            if (st == MDGPU_OK) {
+                hipLaunchKernelGGL(mdg::k_kick, dim3(nb), dim3(mdg::kMdThreads), 0, s, d_v, d_f, m->inv_mass,
+                                   m->mass, hdt, n3, m->ke_part);
+                hipLaunchKernelGGL(mdg::k_record, dim3(1), dim3(mdg::kMdThreads), 0, s, m->ke_part, nb,
+                                   c->work.energies, m->trace, m->step, nsteps);
+                const hipError_t e = hipGetLastError();
+                if (e != hipSuccess) st = mdg::ctx_error(c, MDGPU_ERR_HIP, std::string("md step: ") + hipGetErrorString(e));
+            }
+            if (st != MDGPU_OK) {
+                // a step that could not be enqueued: the steps before it may
+                // have moved d_x, d_v and d_f, so put the entry state back
+                // (the context keeps the error message of the failing call)
+                const std::string msg = c->error;
+                (void)restore();
+                c->verlet_force = true;
+                c->error = msg;
+                return st;
+            }
         }
         MD_TRY(hipMemcpyAsync(tr.data(), m->trace, tr.size() * sizeof(double), hipMemcpyDeviceToHost, s), "trace");
         MD_TRY(hipStreamSynchronize(s), "md run");
@@ -231,10 +252,7 @@ int mdgpu_md_run(mdgpu_ctx* c, double* d_x, double* d_v, double* d_f, const doub
         if (redo || err != MDGPU_OK) {
             // back to the saved state: a redo runs again with the adjusted
             // context (which builds a new list); an error leaves the entry state
-            MD_TRY(hipMemcpyAsync(d_x, m->save, bytes, hipMemcpyDeviceToDevice, s), "restore");
-            MD_TRY(hipMemcpyAsync(d_v, m->save + n3, bytes, hipMemcpyDeviceToDevice, s), "restore");
-            MD_TRY(hipMemcpyAsync(d_f, m->save + 2 * n3, bytes, hipMemcpyDeviceToDevice, s), "restore");
-            MD_TRY(hipStreamSynchronize(s), "restore");
+            MD_TRY(restore(), "restore");
             c->verlet_force = true;
             if (err != MDGPU_OK) return err;
             continue;

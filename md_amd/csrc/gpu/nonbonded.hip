@@ -21,6 +21,7 @@
 // Ewald real-space energy and force.
 #include "bonded_terms.h"
 #include "ctx.h"
+#include "erfc_f32.h"
 
 #include <type_traits>
 
@@ -66,18 +67,6 @@ struct NbArgs {
     int n3;                            // 3 x slots: entries of the force arrays
 };
 
-// erfc(x) for x >= 0 from e = exp(-x^2): Abramowitz & Stegun 7.1.26
-// (|error| <= 1.5e-7), t = 1/(1 + p x).
-__device__ __forceinline__ float erfc_as(float x, float e) {
-    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, x, 1.0f));
-    float p = 1.061405429f;
-    p = fmaf(p, t, -1.453152027f);
-    p = fmaf(p, t, 1.421413741f);
-    p = fmaf(p, t, -0.284496736f);
-    p = fmaf(p, t, 0.254829592f);
-    return p * t * e;
-}
-
 template <typename real>
 struct Math;
 template <>
@@ -86,7 +75,7 @@ struct Math<float> {
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
     static __device__ __forceinline__ void erfc_exp(float x, float& ec, float& ex) {
         ex = __expf(-x * x);
-        ec = erfc_as(x, ex);
+        ec = erfc_f32(x, ex);
     }
 };
 template <>
@@ -450,9 +439,7 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0, const double* coords) {
     const FixedAcc fxa{w.force_fx, fixed_scale(p.semantics), w.flags};
     // statistics runs (pair counting, virial) use the counting instance of the pair kernel
     const bool stats = c->counting || c->virial;
-    const BArgs<real> ba{coords, t, sizeof(real) == 4 ? reinterpret_cast<const real*>(t.cmap_coeff_f)
-                                                      : reinterpret_cast<const real*>(t.cmap_coeff),
-                         w.force_d, fxa, w.partial, static_cast<int>(w.cap_partial), row0 + npb, amber ? 1 : 0,
+    const BArgs<real> ba{coords, t, w.force_d, fxa, w.partial, static_cast<int>(w.cap_partial), row0 + npb, amber ? 1 : 0,
                          stats ? 1 : 0};
     CArgs ca{};
     ca.x = coords;

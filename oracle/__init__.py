@@ -15,6 +15,8 @@ from md_amd.native import Energies, ParamGrads, Params, SystemDesc, P_F64, alloc
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _lib = None
+_fast = None
+FAST_FLAGS = "gcc -std=gnu11 -O3 -march=x86-64-v3 -mtune=generic -fno-fast-math"
 
 
 class Opts(C.Structure):
@@ -61,6 +63,39 @@ def lib() -> C.CDLL:
 
 def _p(a):
     return a.ctypes.data_as(P_F64)
+
+
+def fast_lib() -> C.CDLL:
+    """liboracle_fast.so: the same restatement built for timing (FAST_FLAGS),
+    used by the CPU baseline only."""
+    global _fast
+    if _fast is None:
+        path = os.path.join(_HERE, "liboracle_fast.so")
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is not built (make -C oracle)")
+        L = C.CDLL(path)
+        L.oracle_eval.argtypes = [C.POINTER(SystemDesc), C.POINTER(Params), P_F64, P_F64, C.POINTER(Opts),
+                                  C.POINTER(Energies), P_F64, P_F64]
+        _fast = L
+    return _fast
+
+
+def timed_evaluate(desc: SystemDesc, params: Params, coords, box, skip_all_pairs=False, list_skin=2.0) -> float:
+    """One full evaluation (energies and forces) of an orthorhombic box by the
+    timing build; returns the wall time in seconds."""
+    import time
+    x = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+    b = np.ascontiguousarray(box, dtype=np.float64).reshape(-1)[:3]
+    f = np.zeros(x.size)
+    v = np.zeros(9)
+    e = Energies()
+    o = Opts(list_skin, 1 if skip_all_pairs else 0, 0)
+    t0 = time.perf_counter()
+    st = fast_lib().oracle_eval(C.byref(desc), C.byref(params), _p(x), _p(b), C.byref(o), C.byref(e), _p(f), _p(v))
+    dt = time.perf_counter() - t0
+    if st != 0:
+        raise RuntimeError("oracle_eval failed")
+    return dt
 
 
 @contextlib.contextmanager
