@@ -381,6 +381,32 @@ int mdgpu_get_list_builds(const mdgpu_ctx* ctx, int64_t* builds, int64_t* evals)
  * none built yet, or a build pending).  Changes nothing. */
 int mdgpu_pairs_need_rebuild(mdgpu_ctx* ctx, const double* d_coords, int* needs);
 
+/* Allocation-free form of mdgpu_pairs_need_rebuild.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops that keep their own reference positions),
+ * and it allocates nothing and takes no context.  On `device`, it enqueues on
+ * `stream` (a hipStream_t; NULL: the null stream) the test of
+ * ljpairs_needs_rebuild (NonBondedUtils.cpp:87-100), with the arithmetic of
+ * the context's query, so the decision is the same bit for bit:
+ *   d_coords, d_coords_ref  device, 3 * n_atoms doubles: the current and the
+ *                           reference positions (e.g. those of the last build);
+ *   box                     host, orthorhombic edge lengths;
+ *   d_needs                 device int32: 1 when some atom's squared
+ *                           minimum-image displacement exceeds (skin / 2)^2;
+ *   d_max_d2                device double (or NULL): that largest squared
+ *                           displacement.
+ * The outputs are written when the stream reaches the work; nothing is
+ * synchronised.  Scratch: a fixed device array per GPU (one partial maximum
+ * per 256 atoms and a counter, 32 KiB), held until the process ends, so
+ * n_atoms is at most MDGPU_NOALLOC_MAX_ATOMS; larger, negative or NULL
+ * arguments, a non-positive box edge, a negative skin or a bad device give
+ * MDGPU_ERR_ARG and enqueue nothing.  Calls that overlap on one GPU (other
+ * streams or threads) share that array and race. */
+#define MDGPU_NOALLOC_MAX_ATOMS 1048576
+int mdgpu_pairs_need_rebuild_noalloc(int device, const double* d_coords, const double* d_coords_ref, int32_t n_atoms,
+                                     const double box[3], double skin, void* stream, int32_t* d_needs,
+                                     double* d_max_d2);
+
 /* Molecular dynamics: velocity Verlet, NVE, every step on the GPU.  No
  * counterpart in the reference, whose README describes integrating Newton's
  * equations but whose driver stops after one evaluation (apps/main.cpp:23-28).

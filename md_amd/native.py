@@ -154,6 +154,8 @@ def gpu_lib() -> C.CDLL:
                                      C.c_void_p, C.POINTER(Energies)]
         lib.mdgpu_get_list_builds.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         lib.mdgpu_pairs_need_rebuild.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
+        lib.mdgpu_pairs_need_rebuild_noalloc.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int32, P_F64,
+                                                         C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.mdgpu_adjoint.argtypes = [C.c_void_p, P_F64, P_F64, C.POINTER(Energies), C.POINTER(ParamGrads)]
         lib.mdgpu_batch_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
         lib.mdgpu_batch_destroy.argtypes = [C.c_void_p]
@@ -432,6 +434,25 @@ class GpuError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{ERRORS.get(status, status)}: {msg}")
         self.status = status
+
+
+NOALLOC_MAX_ATOMS = 1048576  # MDGPU_NOALLOC_MAX_ATOMS
+
+
+def pairs_need_rebuild_noalloc(device: int, x_ptr: int, xref_ptr: int, n_atoms: int, box, skin: float,
+                               stream_ptr: int, needs_ptr: int, max_d2_ptr: int = 0) -> None:
+    """mdgpu_pairs_need_rebuild_noalloc: enqueue the rebuild test of device
+    coordinates x against xref (3 n_atoms doubles each) on the stream; the
+    int32 decision and (if max_d2_ptr) the largest squared displacement are
+    written to device memory.  Allocates nothing; no context."""
+    b = np.ascontiguousarray(box, dtype=np.float64).reshape(-1)[:3]
+    st = gpu_lib().mdgpu_pairs_need_rebuild_noalloc(int(device), C.c_void_p(x_ptr), C.c_void_p(xref_ptr),
+                                                   int(n_atoms), b.ctypes.data_as(P_F64), float(skin),
+                                                   C.c_void_p(stream_ptr) if stream_ptr else None,
+                                                   C.c_void_p(needs_ptr),
+                                                   C.c_void_p(max_d2_ptr) if max_d2_ptr else None)
+    if st != 0:
+        raise GpuError(st, "mdgpu_pairs_need_rebuild_noalloc")
 
 
 class Evaluator:

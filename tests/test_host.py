@@ -74,3 +74,22 @@ def test_amber_semantics_fills_exclusions(polymer_2k):
     assert amb.arrays["bond_type"].size == ref.arrays["bond_type"].size + 2   # H and non-H lists
     assert amb.arrays["cmap_set"].size == ref.arrays["cmap_set"].size + 1
     assert amb.arrays["improper_type"].size == ref.arrays["improper_type"].size
+
+
+def test_noalloc_query_rejects_bad_arguments_before_any_launch():
+    """mdgpu_pairs_need_rebuild_noalloc validates on the host (no device
+    access): sizes beyond MDGPU_NOALLOC_MAX_ATOMS, negative sizes, NULL
+    outputs, bad boxes and skins give MDGPU_ERR_ARG."""
+    import ctypes
+    lib = N.gpu_lib()
+    box = np.array([30.0, 30.0, 30.0])
+    bp = box.ctypes.data_as(N.P_F64)
+    fake = ctypes.c_void_p(0x1000)   # never dereferenced: the checks come first
+    f = lib.mdgpu_pairs_need_rebuild_noalloc
+    assert f(0, fake, fake, N.NOALLOC_MAX_ATOMS + 1, bp, 1.0, None, fake, None) == 1
+    assert f(0, fake, fake, -1, bp, 1.0, None, fake, None) == 1
+    assert f(0, fake, fake, 10, bp, 1.0, None, None, None) == 1
+    assert f(0, fake, fake, 10, bp, -1.0, None, fake, None) == 1
+    bad = np.array([30.0, 0.0, 30.0])
+    assert f(0, fake, fake, 10, bad.ctypes.data_as(N.P_F64), 1.0, None, fake, None) == 1
+    assert f(10 ** 6, fake, fake, 10, bp, 1.0, None, fake, None) == 1
