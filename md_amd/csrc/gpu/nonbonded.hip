@@ -388,6 +388,225 @@ __attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 && kAmber && !kCount ? 6 : 
     }
 }
 
+// The FP32 AMBER pair kernel without statistics, with in-range pairs
+// compacted.  38 % of a tile's 64 pairs lie within the cutoff on average, and
+// k_nonbonded runs the whole LJ + switch + erfc path for a tile once any of
+// its pairs does.  Here a wave takes two tiles per step: every lane forms the
+// separation of its pair in both tiles (as in k_nonbonded), the in-range pairs
+// of the two tiles are packed into a per-wave LDS buffer (r^2, q_i q_j, type
+// pair), the pair terms run once per 64 packed pairs (one pass for up to 64,
+// two for up to 128), and each pair's force factor returns to the lane that
+// owns it through the same buffer.  The pair arithmetic is k_nonbonded's, so
+// each pair's force is the same; the i- and j-force paths are unchanged.
+// Energies are summed by the lane that computes the pair.
+constexpr int kPackSlots = 2 * kWave;  // two tiles of packed pairs per wave
+// LDS of one block of the compacted kernel at most (else k_nonbonded runs:
+// about 55 LJ types)
+constexpr size_t kPackMaxLds = 64 * 1024;
+
+// LDS bytes of a block of the compacted kernel for ntp LJ types (padding type
+// included): the pair table and one pack buffer per wave.
+constexpr size_t compact_lds(int ntp) {
+    return sizeof(float4) * (static_cast<size_t>(ntp) * ntp + kWavesOf<float> * kPackSlots);
+}
+
+// Orders this wave's LDS stores before its later LDS loads by other lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool kSwitch>
+__global__ __launch_bounds__(kWavesOf<float> * kWave) __attribute__((amdgpu_waves_per_eu(6))) void k_nonbonded_c(
+    const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int npb, int nbb) {
+    constexpr int kWaves = kWavesOf<float>;
+    if (static_cast<int>(blockIdx.x) >= npb) {
+        const int b = static_cast<int>(blockIdx.x) - npb;
+        const int k = b * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+        if (b < nbb) bonded_terms(ba, k, ba.row0 + b);
+        else correction_pairs(ca, k - nbb * static_cast<int>(blockDim.x), ca.row0 + (b - nbb));
+        return;
+    }
+    extern __shared__ float4 smem4[];
+    const int ntp2 = a.ntp * a.ntp;
+    float4* s_lj = smem4;
+    for (int k = threadIdx.x; k < ntp2; k += blockDim.x) s_lj[k] = a.lj[k];
+    // (r^2, q_i q_j, type-pair index, force factor) per packed pair
+    float4* s_pack = smem4 + ntp2 + (threadIdx.x >> 6) * kPackSlots;
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int ti = lane & 7, tj = lane >> 3;
+    const int ncl = min(*a.ncl_dev, a.max_clusters);
+    const int mbit = 8 * ti + tj;
+    double e_vdw = 0, e_ew = 0;
+    float s_vdw = 0, s_ew = 0;
+    const float tsp = 1.1283791670955126f * a.alpha;
+    const float rc2 = a.rc2_lj > a.rc2_ee ? a.rc2_lj : a.rc2_ee;
+    int flush = 0;
+    const BufRsrc rsrc = buf_rsrc(a.fx.f, 8u * a.n3);
+    const int blk = xcd_block(blockIdx.x, npb);
+    const int item = __builtin_amdgcn_readfirstlane(blk * kWaves + (threadIdx.x >> 6));
+    if (item < ncl * a.split) {
+        const int ci = item / a.split, part = item % a.split;
+        const int cnt = a.list_count[ci];
+        const int k0 = static_cast<int>((static_cast<long long>(cnt) * part) / a.split);
+        const int k1 = static_cast<int>((static_cast<long long>(cnt) * (part + 1)) / a.split);
+        const float4 pi = a.pos[ci * kCluster + ti];
+        const int2 mi = a.meta[ci * kCluster + ti];
+        const int tyi = mi.x * a.ntp;
+        const float qi = pi.w * a.kc;
+        float fix = 0, fiy = 0, fiz = 0;
+        const size_t base = static_cast<size_t>(ci) * a.cap;
+        for (int kb = k0; kb < k1; kb += kWave) {
+            const int nb = min(kWave, k1 - kb);
+            int e_j = 0;
+            unsigned long long e_m = 0;
+            float4 e_d = {0, 0, 0, 0};
+            if (lane < nb) {
+                e_j = a.list_j[base + kb + lane];
+                e_m = a.list_mask[base + kb + lane];
+                e_d = a.list_delta[base + kb + lane];
+            }
+            const int e_mlo = static_cast<int>(e_m & 0xffffffffull), e_mhi = static_cast<int>(e_m >> 32);
+            auto fetch = [&](int t, float4& p, int2& m) {
+                const int cj = __builtin_amdgcn_readlane(e_j, min(t, nb - 1));
+                p = a.pos[cj * kCluster + tj];
+                m = a.meta[cj * kCluster + tj];
+            };
+            struct Geo {
+                float dx, dy, dz;
+                bool act;
+            };
+            // separation of this lane's pair in tile t; packs it if in range
+            auto geom = [&](const float4& pc, const int2& mc, int t, bool live, float& r2, int& tp, float& qq) {
+                unsigned long long mask =
+                    (static_cast<unsigned long long>(static_cast<unsigned int>(__builtin_amdgcn_readlane(e_mhi, t))) << 32) |
+                    static_cast<unsigned int>(__builtin_amdgcn_readlane(e_mlo, t));
+                mask = live ? mask : 0ull;
+                Geo g;
+                g.dx = pi.x - pc.x + bcast<float>(e_d.x, t);
+                g.dy = pi.y - pc.y + bcast<float>(e_d.y, t);
+                g.dz = pi.z - pc.z + bcast<float>(e_d.z, t);
+                r2 = g.dx * g.dx + g.dy * g.dy + g.dz * g.dz;
+                g.act = ((mask >> mbit) & 1ull) && r2 <= rc2 && r2 >= 1e-12f;
+                tp = tyi + mc.x;
+                qq = qi * pc.w;
+                return g;
+            };
+            // the pair terms of packed pair v; returns the force factor
+            auto pair = [&](const float4 v) {
+                const float r2 = v.x, qq = v.y;
+                const int tp = __float_as_int(v.z);
+                const float rinv = __builtin_amdgcn_rsqf(r2);
+                const float r = r2 * rinv;
+                const float r2inv = rinv * rinv;
+                const float4 lj = s_lj[tp];
+                float f = 0;
+                if (r2 <= a.rc2_lj) {
+                    const float r6 = r2inv * r2inv * r2inv;
+                    const float ar6 = lj.x * r6;
+                    const float u = (ar6 - lj.y) * r6;
+                    const float g = (12.0f * ar6 - 6.0f * lj.y) * r6 * r2inv;
+                    float el, fl;
+                    if (kSwitch) {
+                        float xs = (r - a.rs) * a.inv_sw;
+                        xs = xs < 0 ? 0.0f : xs;
+                        const float x2 = xs * xs;
+                        const float sw = 1 + x2 * xs * (-10.0f + xs * (15.0f - 6 * xs));
+                        const float dsw = x2 * (-30.0f + xs * (60.0f - 30 * xs)) * a.inv_sw;
+                        el = u * sw;
+                        fl = sw * g - u * dsw * rinv;
+                    } else {
+                        el = u - lj.z + (r - a.rc_lj) * (lj.w * a.rc_lj);
+                        fl = g - lj.w * a.rc_lj * rinv;
+                    }
+                    s_vdw += el;
+                    f += fl;
+                }
+                if (r2 <= a.rc2_ee) {
+                    float ec, ex;
+                    Math<float>::erfc_exp(a.alpha * r, ec, ex);
+                    const float e_c = qq * rinv;
+                    s_ew += e_c * ec;
+                    f += qq * (ec * rinv + tsp * ex) * r2inv;
+                }
+                return f;
+            };
+            // force of tile t from its factor: i-forces, j-force sums and atomics
+            auto finish = [&](const Geo& g, float f, int t, const int2& mc) {
+                const float fx = f * g.dx, fy = f * g.dy, fz = f * g.dz;
+                fix += fx;
+                fiy += fy;
+                fiz += fz;
+                long long q = 0;
+                unsigned int off = kOffOOB;
+                if (__ballot(g.act) != 0ull) {
+                    const float gx = sum8(fx), gy = sum8(fy), gz = sum8(fz);
+                    const int cjt = __builtin_amdgcn_readlane(e_j, t);
+                    q = force_payload<float, true>(a, ti < 3 && mc.y >= 0, 3 * (cjt * kCluster + tj) + ti,
+                                                   -(ti == 0 ? gx : (ti == 1 ? gy : gz)), off);
+                }
+                buf_add(rsrc, off, q);
+            };
+
+            float4 pA, pB;
+            int2 mA, mB;
+            fetch(0, pA, mA);
+            fetch(1, pB, mB);
+            for (int t = 0; t < nb; t += 2) {
+                float r2A, r2B, qqA, qqB;
+                int tpA, tpB;
+                const Geo gA = geom(pA, mA, t, true, r2A, tpA, qqA);
+                const int2 mcA = mA;
+                fetch(t + 2, pA, mA);
+                const Geo gB = geom(pB, mB, t + 1, t + 1 < nb, r2B, tpB, qqB);
+                const int2 mcB = mB;
+                fetch(t + 3, pB, mB);
+                const unsigned long long bA = __ballot(gA.act), bB = __ballot(gB.act);
+                const int nA = __popcll(bA);
+                const int total = nA + __popcll(bB);
+                float fA = 0, fB = 0;
+                if (total > 0) {
+                    const int sA = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned int>(bA >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(bA), 0u));
+                    const int sB = nA + __builtin_amdgcn_mbcnt_hi(static_cast<unsigned int>(bB >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(bB), 0u));
+                    if (gA.act) s_pack[sA] = make_float4(r2A, qqA, __int_as_float(tpA), 0.0f);
+                    if (gB.act) s_pack[sB] = make_float4(r2B, qqB, __int_as_float(tpB), 0.0f);
+                    wave_sync();
+                    if (lane < total) s_pack[lane].w = pair(s_pack[lane]);
+                    if (total > kWave && lane + kWave < total) s_pack[lane + kWave].w = pair(s_pack[lane + kWave]);
+                    wave_sync();
+                    fA = gA.act ? s_pack[sA].w : 0.0f;
+                    fB = gB.act ? s_pack[sB].w : 0.0f;
+                    wave_sync();
+                }
+                finish(gA, fA, t, mcA);
+                finish(gB, fB, min(t + 1, nb - 1), mcB);
+                flush += 2;
+                if (flush >= 32) {
+                    flush = 0;
+                    e_vdw += s_vdw;
+                    e_ew += s_ew;
+                    s_vdw = s_ew = 0;
+                }
+            }
+        }
+        fix += shfl_xor(fix, 8);  fiy += shfl_xor(fiy, 8);  fiz += shfl_xor(fiz, 8);
+        fix += shfl_xor(fix, 16); fiy += shfl_xor(fiy, 16); fiz += shfl_xor(fiz, 16);
+        fix += shfl_xor(fix, 32); fiy += shfl_xor(fiy, 32); fiz += shfl_xor(fiz, 32);
+        add_force<float, true>(a, rsrc, tj < 3 && k1 > k0 && mi.y >= 0, 3 * (ci * kCluster + ti) + tj,
+                               tj == 0 ? fix : (tj == 1 ? fiy : fiz));
+    }
+    e_vdw += s_vdw;
+    e_ew += s_ew;
+    double v[6] = {e_vdw, 0.0, 0.0, 0.0, e_ew, 0.0};
+    const int slots[6] = {E_VDW, E_EE_ALL, E_EE_LIST, E_EE_RF, E_EW_DIRECT, E_PAIRS};
+    block_sum_store<6>(v, a.partial, a.ld, a.row0 + blk, slots);
+}
+
 template <typename real>
 hipError_t run(mdgpu_ctx* c, bool amber, int row0, const double* coords) {
     Work& w = c->work;
@@ -467,6 +686,17 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0, const double* coords) {
     ca.virial = stats ? 1 : 0;
     const size_t shm = (sizeof(typename V4<real>::T) + 1) * a.ntp * a.ntp;
     dim3 grid(npb + nbb + ncb), block(kWavesOf<real> * kWave);
+    if constexpr (sizeof(real) == 4) {
+        if (amber && !stats) {
+            // the compacted kernel: LJ table, then one pack buffer per wave
+            const size_t shc = compact_lds(a.ntp);
+            if (shc <= kPackMaxLds) {
+                if (sw) hipLaunchKernelGGL(k_nonbonded_c<true>, grid, block, shc, c->stream, a, ba, ca, npb, nbb);
+                else hipLaunchKernelGGL(k_nonbonded_c<false>, grid, block, shc, c->stream, a, ba, ca, npb, nbb);
+                return hipGetLastError();
+            }
+        }
+    }
 #define MDG_NB_LAUNCH(AM, SW)                                                                                  \
     do {                                                                                                      \
         if (stats)                                                                                            \
@@ -504,8 +734,11 @@ const void* pair_fn(bool amber, bool sw, bool counting) {
 const void* pair_kernel_fn(const mdgpu_ctx* c) {
     const bool amber = c->params.semantics == MDGPU_SEM_AMBER, sw = c->params.lj_switch > 0.0;
     const bool stats = c->counting || c->virial;
-    return c->params.precision == MDGPU_PREC_MIXED ? pair_fn<float>(amber, sw, stats)
-                                                   : pair_fn<double>(amber, sw, stats);
+    const bool mixed = c->params.precision == MDGPU_PREC_MIXED;
+    if (mixed && amber && !stats && compact_lds(c->topo.n_types + 1) <= kPackMaxLds)
+        return sw ? reinterpret_cast<const void*>(&k_nonbonded_c<true>)
+                  : reinterpret_cast<const void*>(&k_nonbonded_c<false>);
+    return mixed ? pair_fn<float>(amber, sw, stats) : pair_fn<double>(amber, sw, stats);
 }
 
 int pair_block_threads(const mdgpu_ctx* c) {
