@@ -66,6 +66,55 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 3}), flush=True)
 
 
+def pme_roofline(kernels_ms, n_at, kg):
+    """HBM roofline of the PME kernel groups: algorithmic bytes per launch
+    (FP32 form) / HIP-event time of the group."""
+    G, Gc = kg ** 3, kg * kg * (kg // 2 + 1)
+    pme_bytes = {
+        "spread_z_r2c": (16 * n_at + 8 * Gc, "16N (x, y, z, q) read + 8 Gc complex z-transformed grid written"),
+        "fft_y_y_inverse_z_c2r": (2 * 16 * Gc + 8 * Gc + 4 * G,
+                                  "y pass and inverse y pass 16 Gc each, z c2r 8 Gc read + 4 G written"),
+        "x_conv": (16 * Gc, "x pass + influence function + inverse x pass: 8 Gc read + 8 Gc written"),
+        "gather": (16 * n_at + 4 * G + 12 * n_at, "16N + 4G read, 12N written"),
+    }
+    group = {"spread_z_r2c": "spread", "fft_y_y_inverse_z_c2r": "fft", "x_conv": "conv", "gather": "gather"}
+    out = {}
+    for k, (nbytes, what) in pme_bytes.items():
+        ms = kernels_ms.get(group[k])
+        if ms:
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            out[k] = {"bound": "hbm", "bytes": nbytes, "ms": ms, "achieved": gbs, "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "algorithmic": what}
+    return out
+
+
+def kernel_breakdown(torch, ev, xp, fp, box, n):
+    """Per-kernel-group HIP-event times (ms per launch) over n direct-launch
+    evaluations (timing mode 1: twenty events per evaluation, so a pass of its
+    own, apart from the throughput figure)."""
+    ev.set_timing(1)
+    for _ in range(n):
+        ev.evaluate_device(xp, box, fp)
+    torch.cuda.synchronize()
+    tim = ev.timing()
+    ev.set_timing(0)
+    return {k: round(v[0] / max(v[1], 1), 5) for k, v in tim.items() if v[1] > 0}
+
+
+def timed_steps(torch, ev, xp, fp, box, steps, warmup):
+    """Seconds for `steps` evaluations (the time-step form of the device-buffer
+    call: no Python objects per step) after `warmup` untimed ones."""
+    ev.set_box(box)
+    for _ in range(warmup):
+        ev.step_device(xp, fp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ev.step_device(xp, fp)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
 def host_description():
     model = ""
     try:
@@ -294,29 +343,67 @@ def bench_triclinic(torch, N, top, p32, box, reps=200, warmup=20):
 def bench_scale(torch, N, systems, reps=100, warmup=20):
     """The ~100k-atom system (16,000-atom polymer in ~28,000 waters, 100 A
     box; AMBER, rc 10 A, switch 9 A, Ewald tol 1e-5, PME 108^3), mixed form:
-    evaluations/s over `reps` graph replays after `warmup`."""
+    evaluations/s over `reps` graph replays after `warmup`, the pair kernel's
+    launch time sampled by event nodes inside that timed region (one
+    evaluation in 16), its VALU roofline, and the PME groups' HBM rooflines
+    from a per-kernel-group pass."""
     prm, rst = systems.cached_system("scale_100k")
     top = N.Topology(prm, rst, N.SEM_AMBER)
+    kg = 108
     p32 = N.make_params(N.SEM_AMBER, N.PREC_MIXED, lj_cutoff=10.0, lj_switch=9.0, ewald_tol=1e-5,
-                        pme_grid=(108, 108, 108))
+                        pme_grid=(kg, kg, kg))
     x = torch.tensor(top.coords.reshape(-1), dtype=torch.float64, device="cuda:0")
     f = torch.zeros_like(x)
     ev = N.Evaluator(0)
     try:
         ev.upload(top.desc)
         ev.set_params(p32)
-        for _ in range(warmup):
-            ev.evaluate_device(x.data_ptr(), top.box, f.data_ptr())
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            ev.evaluate_device(x.data_ptr(), top.box, f.data_ptr())
-        torch.cuda.synchronize()
-        t = time.perf_counter() - t0
+        ev.set_counting(True)
+        ev.evaluate_device(x.data_ptr(), top.box, f.data_ptr())
+        counts = ev.counts()
+        ev.set_counting(False)
+        ev.set_timing(3)
+        t = timed_steps(torch, ev, x.data_ptr(), f.data_ptr(), top.box, reps, warmup)
+        nb_total, nb_launches = ev.timing()["nonbonded"]
+        ev.set_timing(0)
+        kernels_ms = kernel_breakdown(torch, ev, x.data_ptr(), f.data_ptr(), top.box, 20)
     finally:
         ev.close()
-    return {"workload": "scale_100k: %d atoms, 100 A box, AMBER, rc 10 A, PME 108^3, mixed form" % top.n_atoms,
-            "evals_per_s": reps / t, "atom_evals_per_s": top.n_atoms * reps / t}
+    nb_ms = nb_total / max(nb_launches, 1)
+    achieved = PAIR_FLOP * counts["pairs"] / (nb_ms * 1e-3) / 1e12
+    return {"workload": "scale_100k: %d atoms, 100 A box, AMBER, rc 10 A (switch 9 A), tol 1e-5, PME 108^3, "
+                        "mixed form" % top.n_atoms,
+            "evals_per_s": reps / t, "atom_evals_per_s": top.n_atoms * reps / t, "work": counts,
+            "kernel_ms": kernels_ms,
+            "roofline": {"bound": "valu", "kernel": "pair kernel (mixed, AMBER, switched LJ + erfc Ewald)",
+                         "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / FP32_PEAK_TFLOPS, "avg_launch_ms": nb_ms,
+                         "timed_launches": int(nb_launches),
+                         "algorithmic": f"{PAIR_FLOP} flop x {counts['pairs']} pairs within 10 A per launch"},
+            "pme_roofline": pme_roofline(kernels_ms, top.n_atoms, kg)}
+
+
+def profile_workload(name, steps, warmup):
+    """Only `steps` mixed-form evaluations of one workload after `warmup`
+    (graph replay), for profiler runs: polymer_29k or scale_100k."""
+    import torch
+    from md_amd import native as N
+    from md_amd import systems
+    if name == "polymer_29k":
+        kw = dict(lj_cutoff=12.0, lj_switch=11.0, ewald_tol=1e-5, pme_grid=(72, 72, 72))
+    else:
+        kw = dict(lj_cutoff=10.0, lj_switch=9.0, ewald_tol=1e-5, pme_grid=(108, 108, 108))
+    prm, rst = systems.cached_system(name)
+    top = N.Topology(prm, rst, N.SEM_AMBER)
+    x = torch.tensor(top.coords.reshape(-1), dtype=torch.float64, device="cuda:0")
+    f = torch.zeros_like(x)
+    ev = N.Evaluator(0)
+    ev.upload(top.desc)
+    ev.set_params(N.make_params(N.SEM_AMBER, N.PREC_MIXED, **kw))
+    t = timed_steps(torch, ev, x.data_ptr(), f.data_ptr(), top.box, steps, warmup)
+    print(json.dumps({"profile_workload": name, "atoms": top.n_atoms, "steps": steps, "warmup": warmup,
+                      "evals_per_s": steps / t}), flush=True)
+    ev.close()
 
 
 def bench_io(N, top, reps=5):
@@ -411,7 +498,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-system batched run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 64-frame parameter-gradient run")
     ap.add_argument("--no-md", action="store_true", help="skip the minimisation + dynamics run")
-    ap.add_argument("--scale", action="store_true", help="also time the ~100k-atom system")
+    ap.add_argument("--no-scale", action="store_true", help="skip the ~100k-atom system")
+    ap.add_argument("--profile-workload", choices=["polymer_29k", "scale_100k"], default=None,
+                    help="only time --steps mixed evaluations of this workload (for profiler runs)")
     ap.add_argument("--triclinic", action="store_true",
                     help="also time the same system written as a triclinic cell (sheared basis)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -420,6 +509,9 @@ def main():
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.profile_workload:
+        profile_workload(args.profile_workload, args.steps, args.warmup)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X; "
@@ -464,25 +556,12 @@ def main():
     counts = ev32.counts()
     ev32.set_counting(False)
 
-    def timed(ev, forces, steps, warmup):
-        # the time-step form of the device-buffer call (no Python objects per step)
-        ev.set_box(box)
-        xp, fp = x.data_ptr(), forces.data_ptr()
-        for _ in range(warmup):
-            ev.step_device(xp, fp)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            ev.step_device(xp, fp)
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0
-
     # the timed region: in one step of every 16 the pair kernel is bracketed by
     # two HIP events on the evaluator's stream, so its launch duration is
     # measured live; the steps replay a captured graph of the evaluation (the
     # sampled steps the instance with the two event nodes)
     ev32.set_timing(3)
-    t32 = timed(ev32, f32, args.steps, args.warmup)
+    t32 = timed_steps(torch, ev32, x.data_ptr(), f32.data_ptr(), box, args.steps, args.warmup)
     nb_total, nb_launches = ev32.timing()["nonbonded"]
     graph_replay = ev32.graph_state() == 1
     if nb_launches < (args.warmup + args.steps) // 16:
@@ -490,7 +569,7 @@ def main():
     ev32.set_timing(0)
     nb_ms = nb_total / max(nb_launches, 1)
     steps64 = args.fp64_steps or max(10, args.steps // 5)
-    t64 = timed(ev64, f64, steps64, max(2, args.warmup // 5))
+    t64 = timed_steps(torch, ev64, x.data_ptr(), f64.data_ptr(), box, steps64, max(2, args.warmup // 5))
     rate32 = args.steps / t32
     rate64 = steps64 / t64
 
@@ -504,42 +583,16 @@ def main():
         ev32.evaluate(xh, box)
     rate_host = nh / (time.perf_counter() - t0)
 
-    # per-kernel-group HIP-event breakdown (separate pass: twenty events per
-    # step would perturb the throughput figure)
-    ev32.set_timing(1)
-    nt = max(20, args.steps // 4)
-    for _ in range(nt):
-        ev32.evaluate_device(x.data_ptr(), box, f32.data_ptr())
-    torch.cuda.synchronize()
-    tim = ev32.timing()
-    ev32.set_timing(0)
+    kernels_ms = kernel_breakdown(torch, ev32, x.data_ptr(), f32.data_ptr(), box, max(20, args.steps // 4))
     flops = PAIR_FLOP * counts["pairs"]
     achieved = flops / (nb_ms * 1e-3) / 1e12
     traffic = read_traffic(os.path.join(ROOT, "profiles"))
-    kernels_ms = {k: round(v[0] / max(v[1], 1), 5) for k, v in tim.items() if v[1] > 0}
-    # HBM-bound PME kernels: algorithmic bytes per launch (FP32 form) / HIP-event time
-    n_at, kg = WORKLOAD["atoms"], WORKLOAD["pme_grid"]
-    G, Gc = kg ** 3, kg * kg * (kg // 2 + 1)
-    pme_bytes = {
-        "spread_z_r2c": (16 * n_at + 8 * Gc, "16N (x, y, z, q) read + 8 Gc complex z-transformed grid written"),
-        "fft_y_y_inverse_z_c2r": (2 * 16 * Gc + 8 * Gc + 4 * G,
-                                  "y pass and inverse y pass 16 Gc each, z c2r 8 Gc read + 4 G written"),
-        "x_conv": (16 * Gc, "x pass + influence function + inverse x pass: 8 Gc read + 8 Gc written"),
-        "gather": (16 * n_at + 4 * G + 12 * n_at, "16N + 4G read, 12N written"),
-    }
-    group = {"spread_z_r2c": "spread", "fft_y_y_inverse_z_c2r": "fft", "x_conv": "conv", "gather": "gather"}
-    pme_roofline = {}
-    for k, (nbytes, what) in pme_bytes.items():
-        ms = kernels_ms.get(group[k])
-        if ms:
-            gbs = nbytes / (ms * 1e-3) / 1e9
-            pme_roofline[k] = {"bound": "hbm", "bytes": nbytes, "ms": ms, "achieved": gbs, "peak": HBM_PEAK_GBS,
-                               "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "algorithmic": what}
+    pme_rl = pme_roofline(kernels_ms, WORKLOAD["atoms"], WORKLOAD["pme_grid"])
 
     md = None if args.no_md else bench_md(torch, N, systems, top, p32, box)
     io = bench_io(N, top)
     tri = bench_triclinic(torch, N, top, p32, box) if args.triclinic else None
-    scale = bench_scale(torch, N, systems) if args.scale else None
+    scale = None if args.no_scale else bench_scale(torch, N, systems)
     batched = None if args.no_batched else bench_batched(torch, N, systems)
     adjoint = None if args.no_adjoint else bench_adjoint(torch, N, systems)
 
@@ -567,11 +620,11 @@ def main():
         "energy_rel_err_mixed_vs_fp64": term_err,
         "work": counts,
         "kernel_ms": kernels_ms,
-        "pme_roofline": pme_roofline,
+        "pme_roofline": pme_rl,
         "roofline": {
             "bound": "valu",
             "engine": "VALU FP32 (no MFMA: the pair kernel is not a dense contraction)",
-            "kernel": "k_nonbonded (mixed, AMBER, switched LJ + erfc Ewald)",
+            "kernel": "k_nonbonded_c (mixed, AMBER, switched LJ + erfc Ewald, in-range pairs packed)",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",

@@ -388,8 +388,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 && kAmber && !kCount ? 6 : 
     }
 }
 
-// The FP32 AMBER pair kernel without statistics, with in-range pairs
-// compacted.  38 % of a tile's 64 pairs lie within the cutoff on average, and
+// The FP32 AMBER pair kernel with in-range pairs compacted.  38 % of a tile's 64 pairs lie within the cutoff on average, and
 // k_nonbonded runs the whole LJ + switch + erfc path for a tile once any of
 // its pairs does.  Here a wave takes two tiles per step: every lane forms the
 // separation of its pair in both tiles (as in k_nonbonded), the in-range pairs
@@ -398,7 +397,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 && kAmber && !kCount ? 6 : 
 // two for up to 128), and each pair's force factor returns to the lane that
 // owns it through the same buffer.  The pair arithmetic is k_nonbonded's, so
 // each pair's force is the same; the i- and j-force paths are unchanged.
-// Energies are summed by the lane that computes the pair.
+// Energies are summed by the lane that computes the pair.  The statistics
+// instance (kCount: pair count and virial, formed by the lane that owns the
+// pair) sums the energies in the same order, so forming the virial leaves
+// energies and forces bit-identical.
 constexpr int kPackSlots = 2 * kWave;  // two tiles of packed pairs per wave
 // LDS of one block of the compacted kernel at most (else k_nonbonded runs:
 // about 55 LJ types)
@@ -417,9 +419,9 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <bool kSwitch>
-__global__ __launch_bounds__(kWavesOf<float> * kWave) __attribute__((amdgpu_waves_per_eu(6))) void k_nonbonded_c(
-    const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int npb, int nbb) {
+template <bool kSwitch, bool kCount>
+__global__ __launch_bounds__(kWavesOf<float> * kWave) __attribute__((amdgpu_waves_per_eu(kCount ? 1 : 6))) void
+k_nonbonded_c(const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int npb, int nbb) {
     constexpr int kWaves = kWavesOf<float>;
     if (static_cast<int>(blockIdx.x) >= npb) {
         const int b = static_cast<int>(blockIdx.x) - npb;
@@ -440,8 +442,10 @@ __global__ __launch_bounds__(kWavesOf<float> * kWave) __attribute__((amdgpu_wave
     const int ti = lane & 7, tj = lane >> 3;
     const int ncl = min(*a.ncl_dev, a.max_clusters);
     const int mbit = 8 * ti + tj;
-    double e_vdw = 0, e_ew = 0;
+    double e_vdw = 0, e_ew = 0, n_pairs = 0;
     float s_vdw = 0, s_ew = 0;
+    double w_d[6] = {0, 0, 0, 0, 0, 0};
+    float w_s[6] = {0, 0, 0, 0, 0, 0};
     const float tsp = 1.1283791670955126f * a.alpha;
     const float rc2 = a.rc2_lj > a.rc2_ee ? a.rc2_lj : a.rc2_ee;
     int flush = 0;
@@ -537,6 +541,15 @@ __global__ __launch_bounds__(kWavesOf<float> * kWave) __attribute__((amdgpu_wave
             // force of tile t from its factor: i-forces, j-force sums and atomics
             auto finish = [&](const Geo& g, float f, int t, const int2& mc) {
                 const float fx = f * g.dx, fy = f * g.dy, fz = f * g.dz;
+                if constexpr (kCount) {
+                    if (g.act && mc.y >= 0 && mi.y >= 0) n_pairs += 1.0;
+                    w_s[0] += fx * g.dx;
+                    w_s[1] += fy * g.dy;
+                    w_s[2] += fz * g.dz;
+                    w_s[3] += fx * g.dy;
+                    w_s[4] += fx * g.dz;
+                    w_s[5] += fy * g.dz;
+                }
                 fix += fx;
                 fiy += fy;
                 fiz += fz;
@@ -591,6 +604,12 @@ __global__ __launch_bounds__(kWavesOf<float> * kWave) __attribute__((amdgpu_wave
                     e_vdw += s_vdw;
                     e_ew += s_ew;
                     s_vdw = s_ew = 0;
+                    if constexpr (kCount) {
+                        for (int q = 0; q < 6; ++q) {
+                            w_d[q] += static_cast<double>(w_s[q]);
+                            w_s[q] = 0;
+                        }
+                    }
                 }
             }
         }
@@ -602,9 +621,17 @@ __global__ __launch_bounds__(kWavesOf<float> * kWave) __attribute__((amdgpu_wave
     }
     e_vdw += s_vdw;
     e_ew += s_ew;
-    double v[6] = {e_vdw, 0.0, 0.0, 0.0, e_ew, 0.0};
-    const int slots[6] = {E_VDW, E_EE_ALL, E_EE_LIST, E_EE_RF, E_EW_DIRECT, E_PAIRS};
-    block_sum_store<6>(v, a.partial, a.ld, a.row0 + blk, slots);
+    if constexpr (kCount) {
+        double v[12] = {e_vdw, 0.0, 0.0, 0.0, e_ew, n_pairs};
+        for (int q = 0; q < 6; ++q) v[6 + q] = w_d[q] + static_cast<double>(w_s[q]);
+        const int slots[12] = {E_VDW, E_EE_ALL, E_EE_LIST, E_EE_RF, E_EW_DIRECT, E_PAIRS,
+                               E_VSR_XX, E_VSR_YY, E_VSR_ZZ, E_VSR_XY, E_VSR_XZ, E_VSR_YZ};
+        block_sum_store<12>(v, a.partial, a.ld, a.row0 + blk, slots);
+    } else {
+        double v[6] = {e_vdw, 0.0, 0.0, 0.0, e_ew, n_pairs};
+        const int slots[6] = {E_VDW, E_EE_ALL, E_EE_LIST, E_EE_RF, E_EW_DIRECT, E_PAIRS};
+        block_sum_store<6>(v, a.partial, a.ld, a.row0 + blk, slots);
+    }
 }
 
 template <typename real>
@@ -687,12 +714,17 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0, const double* coords) {
     const size_t shm = (sizeof(typename V4<real>::T) + 1) * a.ntp * a.ntp;
     dim3 grid(npb + nbb + ncb), block(kWavesOf<real> * kWave);
     if constexpr (sizeof(real) == 4) {
-        if (amber && !stats) {
+        if (amber) {
             // the compacted kernel: LJ table, then one pack buffer per wave
             const size_t shc = compact_lds(a.ntp);
             if (shc <= kPackMaxLds) {
-                if (sw) hipLaunchKernelGGL(k_nonbonded_c<true>, grid, block, shc, c->stream, a, ba, ca, npb, nbb);
-                else hipLaunchKernelGGL(k_nonbonded_c<false>, grid, block, shc, c->stream, a, ba, ca, npb, nbb);
+                if (stats) {
+                    if (sw) hipLaunchKernelGGL((k_nonbonded_c<true, true>), grid, block, shc, c->stream, a, ba, ca, npb, nbb);
+                    else hipLaunchKernelGGL((k_nonbonded_c<false, true>), grid, block, shc, c->stream, a, ba, ca, npb, nbb);
+                } else {
+                    if (sw) hipLaunchKernelGGL((k_nonbonded_c<true, false>), grid, block, shc, c->stream, a, ba, ca, npb, nbb);
+                    else hipLaunchKernelGGL((k_nonbonded_c<false, false>), grid, block, shc, c->stream, a, ba, ca, npb, nbb);
+                }
                 return hipGetLastError();
             }
         }
@@ -735,9 +767,13 @@ const void* pair_kernel_fn(const mdgpu_ctx* c) {
     const bool amber = c->params.semantics == MDGPU_SEM_AMBER, sw = c->params.lj_switch > 0.0;
     const bool stats = c->counting || c->virial;
     const bool mixed = c->params.precision == MDGPU_PREC_MIXED;
-    if (mixed && amber && !stats && compact_lds(c->topo.n_types + 1) <= kPackMaxLds)
-        return sw ? reinterpret_cast<const void*>(&k_nonbonded_c<true>)
-                  : reinterpret_cast<const void*>(&k_nonbonded_c<false>);
+    if (mixed && amber && compact_lds(c->topo.n_types + 1) <= kPackMaxLds) {
+        if (stats)
+            return sw ? reinterpret_cast<const void*>(&k_nonbonded_c<true, true>)
+                      : reinterpret_cast<const void*>(&k_nonbonded_c<false, true>);
+        return sw ? reinterpret_cast<const void*>(&k_nonbonded_c<true, false>)
+                  : reinterpret_cast<const void*>(&k_nonbonded_c<false, false>);
+    }
     return mixed ? pair_fn<float>(amber, sw, stats) : pair_fn<double>(amber, sw, stats);
 }
 

@@ -72,7 +72,9 @@ struct PmeGeom {
 // already transformed.  The twiddle table (e^{-2 pi i m/n}, m < n) is in LDS.
 // Divisions by the runtime p use a float reciprocal with an exact integer
 // correction (i < 2^16).  Each pass reads src and writes dst; the result ends
-// in the returned buffer.
+// in the returned buffer.  The radix is a template parameter of the pass, so
+// a butterfly's inputs and outputs stay in registers (indexed by a runtime
+// radix they would live in scratch memory).
 __device__ __forceinline__ int div_small(int i, int p, float inv_p) {
     int q = static_cast<int>(static_cast<float>(i) * inv_p);
     q -= (q * p > i);
@@ -80,55 +82,68 @@ __device__ __forceinline__ int div_small(int i, int p, float inv_p) {
     return q;
 }
 
+template <typename real, int R>
+__device__ __forceinline__ void lds_fft_pass(const typename C2<real>::T* src, typename C2<real>::T* dst, int n,
+                                             int lognl, const typename C2<real>::T* __restrict__ tw, int sign,
+                                             int p) {
+    using c2 = typename C2<real>::T;
+    const int nl = 1 << lognl;
+    const int nr = n / R, stride_tw = n / (p * R);
+    const float inv_p = 1.0f / static_cast<float>(p);
+    for (int t = threadIdx.x; t < nr * nl; t += blockDim.x) {
+        const int l = t & (nl - 1), i = t >> lognl;
+        const int k = i - div_small(i, p, inv_p) * p;
+        c2 u[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            c2 v = src[(i + q * nr) * nl + l];
+            if (q > 0 && k > 0) {
+                const c2 w = tw[q * k * stride_tw];   // q k n/(p r) < n
+                const real wi = sign < 0 ? w.y : -w.y;
+                v = {v.x * w.x - v.y * wi, v.x * wi + v.y * w.x};
+            }
+            u[q] = v;
+        }
+        c2 o[R];
+        if constexpr (R == 2) {
+            o[0] = {u[0].x + u[1].x, u[0].y + u[1].y};
+            o[1] = {u[0].x - u[1].x, u[0].y - u[1].y};
+        } else if constexpr (R == 4) {
+            const c2 s02 = {u[0].x + u[2].x, u[0].y + u[2].y}, d02 = {u[0].x - u[2].x, u[0].y - u[2].y};
+            const c2 s13 = {u[1].x + u[3].x, u[1].y + u[3].y}, d13 = {u[1].x - u[3].x, u[1].y - u[3].y};
+            // i * sign * d13
+            const c2 jd = {-static_cast<real>(sign) * d13.y, static_cast<real>(sign) * d13.x};
+            o[0] = {s02.x + s13.x, s02.y + s13.y};
+            o[2] = {s02.x - s13.x, s02.y - s13.y};
+            o[1] = {d02.x + jd.x, d02.y + jd.y};
+            o[3] = {d02.x - jd.x, d02.y - jd.y};
+        } else {
+            static_assert(R == 3, "radix 2, 3 or 4");
+            const real c = static_cast<real>(-0.5);
+            const real s = static_cast<real>(sign) * static_cast<real>(0.86602540378443864676);
+            const c2 a = {u[1].x + u[2].x, u[1].y + u[2].y}, b = {u[1].x - u[2].x, u[1].y - u[2].y};
+            o[0] = {u[0].x + a.x, u[0].y + a.y};
+            const c2 m = {u[0].x + c * a.x, u[0].y + c * a.y};
+            const c2 isb = {-s * b.y, s * b.x};  // i s b
+            o[1] = {m.x + isb.x, m.y + isb.y};
+            o[2] = {m.x - isb.x, m.y - isb.y};
+        }
+        const int j = (i - k) * R + k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) dst[(j + q * p) * nl + l] = o[q];
+    }
+}
+
 template <typename real>
 __device__ typename C2<real>::T* lds_fft(typename C2<real>::T* src, typename C2<real>::T* dst, int n, int lognl,
                                          const typename C2<real>::T* __restrict__ tw, int sign) {
     using c2 = typename C2<real>::T;
-    const int nl = 1 << lognl;
     int rem = n, p = 1;
     while (rem > 1) {
         const int r = (rem % 4 == 0) ? 4 : ((rem % 2 == 0) ? 2 : 3);
-        const int nr = n / r, stride_tw = n / (p * r);
-        const float inv_p = 1.0f / static_cast<float>(p);
-        for (int t = threadIdx.x; t < nr * nl; t += blockDim.x) {
-            const int l = t & (nl - 1), i = t >> lognl;
-            const int k = i - div_small(i, p, inv_p) * p;
-            c2 u[4];
-            for (int q = 0; q < r; ++q) {
-                c2 v = src[(i + q * nr) * nl + l];
-                if (q > 0 && k > 0) {
-                    const c2 w = tw[q * k * stride_tw];   // q k n/(p r) < n
-                    const real wi = sign < 0 ? w.y : -w.y;
-                    v = {v.x * w.x - v.y * wi, v.x * wi + v.y * w.x};
-                }
-                u[q] = v;
-            }
-            c2 o[4];
-            if (r == 2) {
-                o[0] = {u[0].x + u[1].x, u[0].y + u[1].y};
-                o[1] = {u[0].x - u[1].x, u[0].y - u[1].y};
-            } else if (r == 4) {
-                const c2 s02 = {u[0].x + u[2].x, u[0].y + u[2].y}, d02 = {u[0].x - u[2].x, u[0].y - u[2].y};
-                const c2 s13 = {u[1].x + u[3].x, u[1].y + u[3].y}, d13 = {u[1].x - u[3].x, u[1].y - u[3].y};
-                // i * sign * d13
-                const c2 jd = {-static_cast<real>(sign) * d13.y, static_cast<real>(sign) * d13.x};
-                o[0] = {s02.x + s13.x, s02.y + s13.y};
-                o[2] = {s02.x - s13.x, s02.y - s13.y};
-                o[1] = {d02.x + jd.x, d02.y + jd.y};
-                o[3] = {d02.x - jd.x, d02.y - jd.y};
-            } else {
-                const real c = static_cast<real>(-0.5);
-                const real s = static_cast<real>(sign) * static_cast<real>(0.86602540378443864676);
-                const c2 a = {u[1].x + u[2].x, u[1].y + u[2].y}, b = {u[1].x - u[2].x, u[1].y - u[2].y};
-                o[0] = {u[0].x + a.x, u[0].y + a.y};
-                const c2 m = {u[0].x + c * a.x, u[0].y + c * a.y};
-                const c2 isb = {-s * b.y, s * b.x};  // i s b
-                o[1] = {m.x + isb.x, m.y + isb.y};
-                o[2] = {m.x - isb.x, m.y - isb.y};
-            }
-            const int j = (i - k) * r + k;
-            for (int q = 0; q < r; ++q) dst[(j + q * p) * nl + l] = o[q];
-        }
+        if (r == 4) lds_fft_pass<real, 4>(src, dst, n, lognl, tw, sign, p);
+        else if (r == 2) lds_fft_pass<real, 2>(src, dst, n, lognl, tw, sign, p);
+        else lds_fft_pass<real, 3>(src, dst, n, lognl, tw, sign, p);
         __syncthreads();
         c2* tmp = src;
         src = dst;
@@ -453,23 +468,35 @@ __global__ void k_gather(const int* ncl_dev, int max_slots, const int* __restric
     bspline4<real>(wq.x, th[0], dth[0]);
     bspline4<real>(wq.y, th[1], dth[1]);
     bspline4<real>(wq.z, th[2], dth[2]);
+    // per (x, y) row the z sums sum_c theta_z phi and sum_c theta_z' phi, then
+    // the x and y weights; one x plane at a time keeps 16 loads in flight
+    int iz[4];
+    for (int c = 0; c < 4; ++c) {
+        iz[c] = flz - c;
+        iz[c] += iz[c] < 0 ? g.kz : 0;
+    }
     real gx = 0, gy = 0, gz = 0, ph = 0;
+#pragma unroll 1
     for (int a = 0; a < 4; ++a) {
         int ix = flx - a;
         ix += ix < 0 ? g.kx : 0;
+#pragma unroll
         for (int b = 0; b < 4; ++b) {
             int iy = fly - b;
             iy += iy < 0 ? g.ky : 0;
             const real* row = phi + (static_cast<size_t>(ix) * g.ky + iy) * g.kz;
+            real sz = 0, dz = 0;
+#pragma unroll
             for (int c = 0; c < 4; ++c) {
-                int iz = flz - c;
-                iz += iz < 0 ? g.kz : 0;
-                const real p = row[iz];
-                gx += dth[0][a] * th[1][b] * th[2][c] * p;
-                gy += th[0][a] * dth[1][b] * th[2][c] * p;
-                gz += th[0][a] * th[1][b] * dth[2][c] * p;
-                ph += th[0][a] * th[1][b] * th[2][c] * p;
+                const real p = row[iz[c]];
+                sz += th[2][c] * p;
+                dz += dth[2][c] * p;
             }
+            const real wxy = th[0][a] * th[1][b];
+            gx += dth[0][a] * th[1][b] * sz;
+            gy += th[0][a] * dth[1][b] * sz;
+            gz += wxy * dz;
+            ph += wxy * sz;
         }
     }
     if (pot) {
