@@ -680,13 +680,17 @@ __device__ __forceinline__ double bcast_lane(double v, int l) {
 // index ranges per row (the row's x window, split where it wraps).  Lane r
 // loads the bounds of range r, clipped to j >= i for the half list; the
 // ranges are then walked as one stream of candidates, 64 per step, with the
-// next step's cluster boxes loaded while the current step is tested.
+// next step's cluster boxes loaded while the current step is tested.  The
+// stream's cluster indices are written once into LDS (streams of up to
+// kCandCap candidates; a longer one maps each index through the ranges).
+constexpr int kCandCap = 1024;
 template <typename real>
 __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const SearchArgs a, double hlx, double hly,
                                                                      double hlz) {
     using R4 = typename Vec4<real>::T;
     __shared__ int2 s_near[kSearchWaves][kNearCap];
     __shared__ int s_ex[kSearchWaves][kCluster][kExclCap];
+    __shared__ int s_cand[kSearchWaves][kCandCap];
     if (a.vs && !a.vs->build) return;
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
     const int ci = __builtin_amdgcn_readfirstlane(blockIdx.x * kSearchWaves + wv);
@@ -760,10 +764,24 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     const int r_pre = r_inc - r_len;
     const int total = __shfl(r_inc, kWave - 1, kWave);
     const int nr = 2 * ny;
+    // the stream in LDS: range k occupies [r_pre_k, r_pre_k + r_len_k)
+    int* cand_s = s_cand[wv];
+    const bool staged = total <= kCandCap;
+    if (staged) {
+        for (int k = 0; k < nr; ++k) {
+            const int b0 = __builtin_amdgcn_readlane(r_b0, k), len = __builtin_amdgcn_readlane(r_len, k);
+            const int pre = __builtin_amdgcn_readlane(r_pre, k);
+            for (int q = lane; q < len; q += kWave) cand_s[pre + q] = b0 + q;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
     // candidate q of the stream -> cluster index (-1 past the end).  Called
     // with every lane active: the shuffles read the range lanes, which must
     // not be masked off.
     auto cand_of = [&](int q) {
+        if (staged) return q < total ? cand_s[q] : -1;
         int r = 0;
         for (int k = 1; k < nr; ++k) {
             const int pk = __builtin_amdgcn_readlane(r_pre, k), lk = __builtin_amdgcn_readlane(r_len, k);
