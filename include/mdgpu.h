@@ -367,13 +367,20 @@ int mdgpu_set_verlet(mdgpu_ctx* ctx, double skin);
  * MDGPU_ERR_ARG.  A change of angles forces the next list build. */
 int mdgpu_set_box_angles(mdgpu_ctx* ctx, const double angles[3]);
 int mdgpu_get_box_angles(const mdgpu_ctx* ctx, double angles[3]);
-/* Which cluster search the context uses (no counterpart in the reference):
- * 0 the minimum-image search, 1 the general per-shift search, which a
- * context switches to for good once a cluster is too wide for the first
- * (a small box, or a strongly sheared cell); -1 for a null context. */
+/* Which cluster search the context's last list build ran (no counterpart in
+ * the reference): 0 the minimum-image search, 1 the general per-shift search,
+ * which runs when a cluster could be within the cutoff of a second image of
+ * another (the box or the cell's face distances are small for cutoff + skin),
+ * and for good once a cluster was too wide for the minimum-image search (a
+ * strongly sheared cell); -1 for a null context. */
 int mdgpu_get_search_mode(const mdgpu_ctx* ctx);
 /* List builds and successful evaluations since the system upload. */
 int mdgpu_get_list_builds(const mdgpu_ctx* ctx, int64_t* builds, int64_t* evals);
+/* Prunes of the kept list since the system upload, Verlet mode only (every
+ * build is followed by a prune; between builds the list is pruned again once
+ * some atom has moved 0.1 A since the last prune).  No counterpart in the
+ * reference, which keeps no list. */
+int mdgpu_get_list_prunes(const mdgpu_ctx* ctx, int64_t* prunes);
 /* ljpairs_needs_rebuild (NonBondedUtils.cpp:87-100) on the GPU for device
  * coordinates d_coords (3N doubles): *needs = 1 when some atom's squared
  * minimum-image displacement from the positions of the last list build, in

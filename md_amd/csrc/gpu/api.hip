@@ -389,7 +389,7 @@ extern "C" int mdgpu_upload_system(mdgpu_ctx* c, const mdgpu_system_desc* s) {
     c->have_system = true;
     ++c->gen;  // a captured evaluation graph no longer applies
     c->verlet_force = true;
-    c->list_builds = c->list_evals = 0;
+    c->list_builds = c->list_evals = c->list_prunes = 0;
     if (c->have_params) return build_lj_table(c);
     return MDGPU_OK;
 }
@@ -1085,6 +1085,7 @@ int eval_impl(mdgpu_ctx* c, const double* d_coords, const double box[3], mdgpu_e
         c->verlet_force = false;
         ++c->list_evals;
         if (c->skin <= 0 || e[mdg::E_NSLOTS + 5] != 0.0) ++c->list_builds;
+        if (c->skin > 0 && e[mdg::E_NSLOTS + 6] != 0.0) ++c->list_prunes;
         c->last_key = key;
         c->have_last_key = true;
         ++T.tick;
@@ -1136,8 +1137,10 @@ int ctx_enqueue_step(mdgpu_ctx* c, const double* d_coords, double* d_forces) {
 void ctx_steps_done(mdgpu_ctx* c, const double* rows, int stride, int nsteps) {
     c->verlet_force = false;
     c->list_evals += nsteps;
-    for (int k = 0; k < nsteps; ++k)
+    for (int k = 0; k < nsteps; ++k) {
         if (c->skin <= 0 || rows[static_cast<size_t>(k) * stride + mdg::E_NSLOTS + 5] != 0.0) ++c->list_builds;
+        if (c->skin > 0 && rows[static_cast<size_t>(k) * stride + mdg::E_NSLOTS + 6] != 0.0) ++c->list_prunes;
+    }
 }
 }  // namespace mdg
 
@@ -1420,11 +1423,20 @@ int mdgpu_set_box_angles(mdgpu_ctx* c, const double angles[3]) {
     return MDGPU_OK;
 }
 
-int mdgpu_get_search_mode(const mdgpu_ctx* c) { return c ? (c->search_general ? 1 : 0) : -1; }
+int mdgpu_get_search_mode(const mdgpu_ctx* c) {
+    if (!c) return -1;
+    return c->search_general || !c->search_fast ? 1 : 0;
+}
 
 int mdgpu_get_box_angles(const mdgpu_ctx* c, double angles[3]) {
     if (!c || !angles) return MDGPU_ERR_ARG;
     for (int d = 0; d < 3; ++d) angles[d] = c->angles[d];
+    return MDGPU_OK;
+}
+
+int mdgpu_get_list_prunes(const mdgpu_ctx* c, int64_t* prunes) {
+    if (!c || !prunes) return MDGPU_ERR_ARG;
+    *prunes = c->list_prunes;
     return MDGPU_OK;
 }
 

@@ -245,6 +245,7 @@ struct mdgpu_ctx {
     int fft_lines = 8;              // cap on FFT lines per block (power of two; 0 = 16)
     const double* cur_x = nullptr;  // raw coordinates of the evaluation in progress (device)
     bool search_general = false;    // use the general (per-shift) cluster search
+    bool search_fast = true;        // the last list build ran the minimum-image search
     std::vector<void*> topo_allocs;
     // host copies of the per-type-pair LJ parameters (the device table also
     // holds cutoff-dependent shift constants, rebuilt by mdgpu_set_params)
@@ -273,7 +274,7 @@ struct mdgpu_ctx {
     // cell angles alpha, beta, gamma in degrees (all 90: orthorhombic)
     double angles[3] = {90.0, 90.0, 90.0};
     bool verlet_force = true;
-    long long list_builds = 0, list_evals = 0;
+    long long list_builds = 0, list_evals = 0, list_prunes = 0;
     void* md = nullptr;             // molecular-dynamics buffers (md.hip), after mdgpu_md_set_masses
     void* minbuf = nullptr;         // minimiser buffers (minimize.hip)
     // concurrent PME chain (single-context evaluations)

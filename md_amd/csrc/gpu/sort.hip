@@ -969,10 +969,12 @@ __global__ __launch_bounds__(kDispThreads) void k_disp(const double* __restrict_
     if (query) {
         atomicExch(&vs->query, build);
     } else {
+        const unsigned int prune = (build || mpx > limp2) ? 1u : 0u;
         atomicExch(&vs->build, build);
-        atomicExch(&vs->prune, (build || mpx > limp2) ? 1u : 0u);
+        atomicExch(&vs->prune, prune);
         atomicExch(&vs->force, 0u);
         if (build) atomicOr(flags + 5, 1u);
+        if (prune) atomicOr(flags + 6, 1u);
     }
 }
 
@@ -1263,6 +1265,7 @@ hipError_t launch_pairsearch(mdgpu_ctx* c, bool mixed, bool amber, const VerletS
     double hx = hlim(g.box[0]), hy = hlim(g.box[1]), hz = hlim(g.box[2]);
     if (g.cell.tri) hx = hy = hz = hlim(std::min({g.face[0], g.face[1], g.face[2]})) / std::sqrt(3.0);
     const bool fast = !c->search_general && hx > 1.0 && hy > 1.0 && hz > 1.0;
+    c->search_fast = fast;
     if (fast) {
         if (mixed)
             hipLaunchKernelGGL(k_search_mi<float>, dim3(nb), dim3(kSearchWaves * kWave), 0, c->stream, a, hx, hy, hz);

@@ -153,6 +153,7 @@ def gpu_lib() -> C.CDLL:
         lib.mdgpu_md_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, P_F64, C.c_double, C.c_int32,
                                      C.c_void_p, C.POINTER(Energies)]
         lib.mdgpu_get_list_builds.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        lib.mdgpu_get_list_prunes.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         lib.mdgpu_pairs_need_rebuild.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
         lib.mdgpu_pairs_need_rebuild_noalloc.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int32, P_F64,
                                                          C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -575,6 +576,12 @@ class Evaluator:
         a = np.zeros(3)
         self._check(self.lib.mdgpu_get_box_angles(self.ctx, a.ctypes.data_as(P_F64)))
         return a
+
+    def list_prunes(self) -> int:
+        """Prunes of the kept Verlet list since the upload."""
+        p = C.c_int64()
+        self._check(self.lib.mdgpu_get_list_prunes(self.ctx, C.byref(p)))
+        return p.value
 
     def list_builds(self) -> tuple:
         """(list builds, successful evaluations) since the system upload."""

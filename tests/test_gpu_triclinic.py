@@ -78,8 +78,14 @@ def test_fp64_matches_oracle(polymer_2k, kind):
         assert np.array_equal(ev.box_angles(), box6[3:])
         e, f, _ = ev.evaluate(x, box6[:3])
         w = ev.virial()
+        mode = ev.search_mode()
     finally:
         ev.close()
+    # a 30 A cell with a 10 A cutoff: the minimum-image search's bound on the
+    # cluster half-extents (half of half the smallest face distance minus the
+    # cutoff, over sqrt 3) is at most 0.6 A here, so the general search runs
+    print(f"  search mode {mode}")
+    assert mode == 1
     eo, fo, wo = oracle.evaluate(top.desc, prm, x, box6, skip_all_pairs=True, full_virial=True)
     check_terms(e, eo, TERMS_AMBER, 1e-9)
     r = rel_rms(f, fo)
@@ -249,26 +255,69 @@ def test_cpp_driver_reads_cell_angles(polymer_2k, tmp_path):
 def test_minimum_image_search_in_a_triclinic_cell(polymer_29k):
     """The 29k-atom polymer carried into a 66 A cell of angles 80, 85, 75: its
     clusters are narrow enough for the minimum-image search (fractional
-    rounding of cluster separations), which stays in use; FP64 matches the
-    oracle as above."""
-    print("case polymer_29k triclinic cell, minimum-image search, FP64 vs oracle", flush=True)
+    rounding of cluster separations), which stays in use (search mode 0) in
+    every evaluation below:
+      * FP64 matches the oracle as above;
+      * mixed vs FP64: force rel-RMS <= 1e-5 and the per-term bars of the
+        cubic 29k workload (test_gpu_cfg2.MIXED_BARS);
+      * kept lists (skins 1, 0.5 and 0.25 A) built at x and pruned, then used
+        at x + moves of at most 0.45 skin (FP64), match the oracle there.  The
+        minimum-image search's bound tightens with the skin (half of half the
+        smallest face distance minus cutoff + skin, over sqrt 3: 5.62 A without
+        a skin, 5.33 A with 1 A), and the clusters of a kept list grow as their
+        atoms move; the search each kept list ends with is printed."""
+    from test_gpu_cfg2 import MIXED_BARS
+    print("case polymer_29k triclinic cell, minimum-image search", flush=True)
     prm_path, rst_path = polymer_29k
     top = N.Topology(prm_path, rst_path, N.SEM_AMBER)
     edge = float(top.box[0])
     box6 = [edge, edge, edge] + TRI_ANGLES
     x = np.ascontiguousarray(mapped(top.coords, edge, box6))
-    prm = N.make_params(N.SEM_AMBER, N.PREC_FP64, lj_cutoff=12.0, lj_switch=11.0, ewald_tol=1e-5,
-                        pme_grid=(72, 72, 72))
-    ev = _ev(top, prm, box6)
-    try:
-        e, f, _ = ev.evaluate(x, box6[:3])
-        mode = ev.search_mode()
-    finally:
-        ev.close()
-    print(f"  search mode {mode}")
-    assert mode == 0
+    kw = dict(lj_cutoff=12.0, lj_switch=11.0, ewald_tol=1e-5, pme_grid=(72, 72, 72))
+    prm = N.make_params(N.SEM_AMBER, N.PREC_FP64, **kw)
+    res = {}
+    for name, prec in (("fp64", N.PREC_FP64), ("mixed", N.PREC_MIXED)):
+        ev = _ev(top, N.make_params(N.SEM_AMBER, prec, **kw), box6)
+        try:
+            res[name] = ev.evaluate(x, box6[:3])
+            mode = ev.search_mode()
+        finally:
+            ev.close()
+        print(f"  {name}: search mode {mode}")
+        assert mode == 0
+    e, f, _ = res["fp64"]
     eo, fo, _ = oracle.evaluate(top.desc, prm, x, box6, skip_all_pairs=True)
     check_terms(e, eo, TERMS_AMBER, 1e-9)
     r = rel_rms(f, fo)
     print(f"  force rel-RMS {r:.2e}")
     assert r <= 1e-8
+    e32, f32, _ = res["mixed"]
+    bad = []
+    for k in TERMS_AMBER:
+        rel = abs(e32[k] - e[k]) / max(abs(e[k]), 1e-300)
+        print(f"  {k:14s} mixed vs fp64 rel {rel:.2e} (bar {MIXED_BARS[k]:.0e})")
+        if not rel <= MIXED_BARS[k]:
+            bad.append((k, rel))
+    r32 = rel_rms(f32, f)
+    print(f"  mixed force rel-RMS {r32:.2e}")
+    assert r32 <= 1e-5
+    assert not bad, bad
+    # kept lists across moves (FP64)
+    modes = {}
+    for skin in (1.0, 0.5, 0.25):
+        rng = np.random.default_rng(29)
+        moves = rng.normal(0.0, 1.0, x.shape)
+        moves *= 0.45 * skin / np.linalg.norm(moves, axis=1).max()
+        x1 = x + moves
+        ev = _ev(top, prm, box6, skin=skin)
+        try:
+            ev.evaluate(x, box6[:3])
+            e1, f1, _ = ev.evaluate(x1, box6[:3])
+            builds, modes[skin], prunes = ev.list_builds(), ev.search_mode(), ev.list_prunes()
+        finally:
+            ev.close()
+        print(f"  kept list, skin {skin}: builds {builds}, prunes {prunes}, search mode {modes[skin]}")
+        assert builds == (1, 2)
+        eo1, fo1, _ = oracle.evaluate(top.desc, prm, x1, box6, skip_all_pairs=True)
+        check_terms(e1, eo1, TERMS_AMBER, 1e-9)
+        assert rel_rms(f1, fo1) <= 1e-8

@@ -286,3 +286,65 @@ def test_kept_list_is_pruned_to_the_cutoff(polymer_2k):
     assert c1["tiles"] <= 1.1 * c0["tiles"]
     for k in TERMS_AMBER:
         assert abs(e1[k] - e0[k]) <= 1e-12 * max(abs(e0[k]), 1.0), k
+
+
+def _close_pair(top, rc, lo, hi):
+    """Two atoms of the system, neither excluded nor 1-4 with the other, at a
+    minimum-image distance in (rc + lo, rc + hi); and their unit separation."""
+    x = np.asarray(top.coords, dtype=np.float64).reshape(-1, 3)
+    box = np.asarray(top.box, dtype=np.float64)
+    a = top.arrays
+    st, ex = a["excl_start"], a["excl_list"]
+    for i in range(x.shape[0]):
+        d = x - x[i]
+        d -= box * np.round(d / box)
+        r = np.sqrt(np.sum(d * d, axis=1))
+        near = set(ex[st[i]:st[i + 1]].tolist())
+        for j in np.nonzero((r > rc + lo) & (r < rc + hi))[0]:
+            if j > i and int(j) not in near:
+                return i, int(j), d[j] / r[j], r[j]
+    raise AssertionError("no pair in the shell")
+
+
+def test_prune_interval(polymer_2k):
+    """Reuse of a pruned list without a fresh prune.  Build and prune at x0
+    (skin 1 A: tiles with a pair within cutoff + 0.2 A).  A pair at cutoff
+    + 0.12..0.18 A is moved 0.09 A closer at each end (now inside the
+    cutoff); no atom has moved 0.1 A since the prune, so the same pruned list
+    is used (the prune count stays at 1), and FP64 still matches the oracle.
+    Then one atom moves 0.12 A: the list is pruned again (count 2, still one
+    build) and FP64 matches the oracle."""
+    print("case polymer_2k prune interval", flush=True)
+    prm_path, rst_path = polymer_2k
+    top = N.Topology(prm_path, rst_path, N.SEM_AMBER)
+    rc = 10.0
+    prm = N.make_params(N.SEM_AMBER, N.PREC_FP64, lj_cutoff=rc, lj_switch=9.0, ewald_tol=1e-5,
+                        pme_grid=(32, 32, 32))
+    i, j, u, r0 = _close_pair(top, rc, 0.12, 0.18)
+    x0 = np.asarray(top.coords, dtype=np.float64).reshape(-1, 3)
+    x1 = x0.copy()
+    x1[i] += 0.09 * u
+    x1[j] -= 0.09 * u
+    x2 = x1.copy()
+    k = (i + x0.shape[0] // 2) % x0.shape[0]
+    x2[k] += np.array([0.12, 0.0, 0.0])
+    print(f"  pair ({i}, {j}) at {r0:.4f} A -> {r0 - 0.18:.4f} A; atom {k} moves 0.12 A")
+    ev = N.Evaluator(0)
+    try:
+        ev.upload(top.desc)
+        ev.set_params(prm)
+        ev.set_verlet(1.0)
+        ev.evaluate(x0, top.box)
+        assert ev.list_builds()[0] == 1 and ev.list_prunes() == 1
+        e1, f1, _ = ev.evaluate(x1, top.box)
+        p1 = ev.list_prunes()
+        e2, f2, _ = ev.evaluate(x2, top.box)
+        p2, b2 = ev.list_prunes(), ev.list_builds()[0]
+    finally:
+        ev.close()
+    print(f"  prunes after the 0.09 A moves {p1}, after the 0.12 A move {p2}; builds {b2}")
+    assert p1 == 1 and p2 == 2 and b2 == 1
+    for x, e, f in ((x1, e1, f1), (x2, e2, f2)):
+        eo, fo, _ = oracle.evaluate(top.desc, prm, x, top.box, skip_all_pairs=True)
+        check_terms(e, eo, TERMS_AMBER, 1e-9)
+        assert rel_rms(f, fo) <= 1e-8
