@@ -113,6 +113,8 @@ def energies_struct(d: dict) -> Energies:
 
 
 def _load(name: str) -> C.CDLL:
+    if name == "libmdhost.so" and os.environ.get("MD_AMD_SANITIZE") == "1":
+        name = "libmdhost_asan.so"   # the CPU sanitizer build (make -C md_amd/csrc sanitize)
     path = os.path.join(_HERE, name)
     if not os.path.exists(path):
         raise ImportError(f"{path} is not built; run __graft_entry__.build() (make -C md_amd/csrc)")

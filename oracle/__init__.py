@@ -26,7 +26,8 @@ class Opts(C.Structure):
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        name = "liboracle_asan.so" if os.environ.get("MD_AMD_SANITIZE") == "1" else "liboracle.so"
+        path = os.path.join(_HERE, name)
         if not os.path.exists(path):
             raise ImportError(f"{path} is not built (make -C oracle)")
         L = C.CDLL(path)
