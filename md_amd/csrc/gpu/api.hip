@@ -62,7 +62,7 @@ hipError_t ensure(T** p, size_t* cap, size_t n) {
 
 void free_work(mdgpu_ctx* c) {
     Work& w = c->work;
-    void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_count, w.block_emax, w.block_off, w.col_start, w.col_cstart, w.col_key,
+    void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_count, w.block_emax, w.col_ticket, w.col_cstart, w.col_key,
                     w.slot_atom, w.slot_meta, w.slot_pos_d, w.slot_pos_f, w.slot_pme_fl, w.slot_pme_wd,
                     w.slot_pme_wf, w.cluster_col, w.cl_origin, w.cl_half,
                     w.list_count, w.list_j, w.list_mask, w.list_delta_d, w.list_delta_f, w.inner_count, w.inner_j,
@@ -567,19 +567,17 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     size_t cap;
     cap = w.cap_atoms;
     if (w.cap_atoms < static_cast<size_t>(n)) {
-        void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.col_key, w.force_d, w.force_fx, w.force_out, w.x_ref,
+        void* ptrs[] = {w.coords, w.wrapped, w.col_of, w.force_d, w.force_fx, w.force_out, w.x_ref,
                         w.x_prune, w.atom_slot};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         w.coords = w.wrapped = w.force_d = w.force_out = w.x_ref = w.x_prune = nullptr;
         w.atom_slot = nullptr;
         w.col_of = nullptr;
-        w.col_key = nullptr;
         w.force_fx = nullptr;
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.coords), 3 * n * sizeof(double)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.wrapped), 3 * n * sizeof(double)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_of), n * sizeof(int)), "alloc");
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_key), n * sizeof(unsigned long long)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_d), 3 * n * sizeof(double)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_fx), 3 * n * sizeof(unsigned long long)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.force_out), 3 * n * sizeof(double)), "alloc");
@@ -596,17 +594,25 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     (void)cap;
     const size_t nbc = static_cast<size_t>(mdg::sort_blocks(n)) * (ncol + 1);
     if (w.cap_cols < nbc) {
-        void* ptrs[] = {w.col_count, w.block_emax, w.block_off, w.col_start, w.col_cstart};
+        void* ptrs[] = {w.col_count, w.block_emax, w.col_ticket, w.col_cstart};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_count), nbc * sizeof(int)), "alloc");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.block_emax), nbc * sizeof(float)), "alloc");
-        // column totals start at zero; the column sort re-zeroes them every evaluation
+        // column totals and the ticket start at zero; the last block of the
+        // column sort re-zeroes them every evaluation
         HIP_TRY(hipMemset(w.col_count, 0, nbc * sizeof(int)), "clear");
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.block_off), nbc * sizeof(int)), "alloc");
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_start), nbc * sizeof(int)), "alloc");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_ticket), sizeof(unsigned int)), "alloc");
+        HIP_TRY(hipMemset(w.col_ticket, 0, sizeof(unsigned int)), "clear");
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_cstart), nbc * sizeof(int)), "alloc");
         w.cap_cols = nbc;
+    }
+    const size_t nkey = static_cast<size_t>(ncol) * mdg::kColBucket;
+    if (w.cap_colkey < nkey) {
+        if (w.col_key) (void)hipFree(w.col_key);
+        w.col_key = nullptr;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.col_key), nkey * sizeof(unsigned long long)), "alloc");
+        w.cap_colkey = nkey;
     }
     const size_t ncl = c->n_clusters, nslots = ncl * mdg::kCluster;
     if (w.cap_slots < nslots) {
@@ -936,6 +942,7 @@ mdg::GraphKey graph_key(const mdgpu_ctx* c, const double* x, const double* out) 
     k.caps[3] = w.cap_list + w.cap_inner;
     k.caps[4] = w.cap_partial;
     k.caps[5] = w.cap_grid;
+    k.caps[6] = w.cap_colkey;
     return k;
 }
 
@@ -946,7 +953,7 @@ bool same_key(const mdg::GraphKey& a, const mdg::GraphKey& b) {
               a.n_cols == b.n_cols;
     for (int d = 0; d < 3; ++d)
         eq = eq && a.box[d] == b.box[d] && a.angles[d] == b.angles[d] && a.pme_k[d] == b.pme_k[d];
-    for (int k = 0; k < 6; ++k) eq = eq && a.caps[k] == b.caps[k];
+    for (int k = 0; k < 7; ++k) eq = eq && a.caps[k] == b.caps[k];
     return eq;
 }
 

@@ -99,6 +99,10 @@ struct VerletState {
 // the last prune (two atoms then come at most kPruneBuffer closer).
 constexpr double kPruneBuffer = 0.2;
 
+// atoms per column at most (the in-LDS column sort's capacity, and the
+// stride of the per-column key buckets)
+constexpr int kColBucket = 2048;
+
 // Work arrays that depend on the atom count / box.
 struct Work {
     size_t cap_atoms = 0;
@@ -106,11 +110,13 @@ struct Work {
     double* wrapped = nullptr;       // [3N] wrapped into the box
     int* col_of = nullptr;           // [N] column of each atom
     int* col_count = nullptr;        // [ncol] atoms per column (zero between evaluations)
-    int* block_off = nullptr;        // [sort blocks x ncol] start of each block's run in a column
     float* block_emax = nullptr;     // [sort blocks] largest excluded-pair distance per sort block
-    int* col_start = nullptr;        // [ncol + 1] atom offsets
     int* col_cstart = nullptr;       // [ncol + 1] cluster offsets
-    unsigned long long* col_key = nullptr;  // [N] atoms bucketed by column as (z key << 32 | atom)
+    // atoms bucketed by column as (z key << 32 | atom): column c holds its
+    // atoms at [c * kColBucket, c * kColBucket + col_count[c])
+    unsigned long long* col_key = nullptr;
+    size_t cap_colkey = 0;           // entries of col_key
+    unsigned int* col_ticket = nullptr;  // [1] blocks of the column sort done (zero between evaluations)
     size_t cap_cols = 0;
     size_t cap_slots = 0;            // clusters * 8
     int* slot_atom = nullptr;        // [slots] original atom or -1
@@ -217,7 +223,7 @@ struct GraphKey {
     int pme_k[3];
     unsigned long long gen;
     int list_cap, search_general, split, fft_lines, counting, timing, n_clusters, n_cols;
-    size_t caps[6];
+    size_t caps[7];
 };
 }  // namespace mdg
 

@@ -26,21 +26,23 @@ namespace mdg {
 
 namespace {
 
-constexpr int kColMax = 2048;  // atoms per column the in-LDS sort handles
+constexpr int kColMax = kColBucket;  // atoms per column the in-LDS sort handles
 
-// Column binning in three launches.
+// Column binning in two launches.
 //   k_bin:     each block wraps a contiguous chunk of atoms, histograms their
-//              columns in LDS and reserves its run in every column it touches
-//              with one global atomic per (block, column); the order of the
-//              runs inside a column is whatever the atomics give, and the
-//              column sort below makes the result independent of it.  The
-//              same blocks also measure the excluded and 1-4 pair distances.
-//   k_scatter: every block scans the column totals in LDS (block 0 also stores
-//              the atom and cluster offsets and the exclusion extent), then
-//              places its atoms.
-//   k_col_sort: one block per column ranks its atoms by (z, index), writes the
-//              slot map and sets up the column's clusters; it also clears the
-//              column total for the next evaluation.
+//              columns in LDS, reserves its run in every column it touches
+//              with one global atomic per (block, column) and writes its
+//              atoms' sort keys into the runs.  Each column owns a bucket of
+//              kColMax keys, so no scan over the columns is needed first; the
+//              order of the runs inside a bucket is whatever the atomics give,
+//              and the column sort below makes the result independent of it.
+//              The same blocks also measure the excluded and 1-4 pair
+//              distances.
+//   k_col_sort: one block per column sums the totals of the columns before
+//              its own (its cluster offset), ranks its atoms by (z, index),
+//              writes the slot map and sets up the column's clusters.  The
+//              last block to have read the totals clears them for the next
+//              evaluation; block 0 also folds the exclusion extent.
 constexpr int kBinThreads = 1024;
 constexpr int kMaxCols = 8192;
 constexpr int kMaxSortBlocks = 256;
@@ -58,57 +60,20 @@ struct ExclPairs {
     float* out;        // largest distance x 1.0001
 };
 
-// Exclusive scan of s[0..n) in LDS by the whole block (blockDim.x = 1024,
-// n <= 8 * 1024); returns the total.  Each thread scans 8 consecutive
-// entries in registers, the 1024 thread totals are scanned with wave
-// shuffles and one LDS round.
-__device__ int lds_exclusive_scan(int* s, int n) {
-    __shared__ int wsum[kBinThreads / kWave];
-    __shared__ int total;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    int v[8];
-    int run = 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int i = 8 * t + k;
-        v[k] = i < n ? s[i] : 0;
-        run += v[k];
-    }
-    // inclusive scan of the thread totals within the wave
-    int inc = run;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int o = __shfl_up(inc, d, kWave);
-        if (lane >= d) inc += o;
-    }
-    if (lane == 63) wsum[w] = inc;
-    __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int q = 0; q < kBinThreads / kWave; ++q) {
-            const int x = wsum[q];
-            wsum[q] = acc;
-            acc += x;
-        }
-        total = acc;
-    }
-    __syncthreads();
-    int base = wsum[w] + inc - run;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int i = 8 * t + k;
-        if (i < n) s[i] = base;
-        base += v[k];
-    }
-    const int tot = total;
-    __syncthreads();
-    return tot;
+// The sort key of an atom: its wrapped z as 32-bit fixed point of the box
+// height (zscale = 2^32 / L_z) above its index, so one unsigned comparison
+// orders atoms by z and breaks ties by index.
+__device__ __forceinline__ unsigned long long z_key(double z, double zscale, int atom) {
+    const double q = floor(z * zscale);
+    const unsigned long long zq = q <= 0.0 ? 0ull : (q >= 4294967295.0 ? 4294967295ull : static_cast<unsigned long long>(q));
+    return (zq << 32) | static_cast<unsigned int>(atom);
 }
 
 __global__ __launch_bounds__(kBinThreads) void k_bin(const double* __restrict__ x, int n, int chunk, const Cell cell,
-                                                     int ncx, int ncy, double sx, double sy,
+                                                     int ncx, int ncy, double sx, double sy, double zscale,
                                                      double* __restrict__ wrapped, int* __restrict__ col_of,
-                                                     int* __restrict__ col_count, int* __restrict__ block_off,
+                                                     int* __restrict__ col_count,
+                                                     unsigned long long* __restrict__ col_key, unsigned int* flags,
                                                      const ExclPairs ex, double* __restrict__ xref,
                                                      const VerletState* __restrict__ vs) {
     __shared__ int hist[kMaxCols];
@@ -180,63 +145,23 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(const double* __restrict__ 
         for (int q = 0; q < kBinThreads / kWave; ++q) m = fmaxf(m, wmax[q]);
         ex.block_max[blockIdx.x] = m;
     }
-    // reserve this block's run in each column it touches
+    // reserve this block's run in each column it touches; hist[c] becomes
+    // the run's start in the column
     for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
         const int h = hist[c];
-        block_off[static_cast<size_t>(blockIdx.x) * ncol + c] = h > 0 ? atomicAdd(col_count + c, h) : 0;
+        hist[c] = h > 0 ? atomicAdd(col_count + c, h) : 0;
     }
-}
-
-// The sort key of an atom: its wrapped z as 32-bit fixed point of the box
-// height (zscale = 2^32 / L_z) above its index, so one unsigned comparison
-// orders atoms by z and breaks ties by index.
-__device__ __forceinline__ unsigned long long z_key(double z, double zscale, int atom) {
-    const double q = floor(z * zscale);
-    const unsigned long long zq = q <= 0.0 ? 0ull : (q >= 4294967295.0 ? 4294967295ull : static_cast<unsigned long long>(q));
-    return (zq << 32) | static_cast<unsigned int>(atom);
-}
-
-__global__ __launch_bounds__(kBinThreads) void k_scatter(int n, int chunk, int ncol, const int* __restrict__ col_of,
-                                                         const double* __restrict__ wrapped, double zscale,
-                                                         const int* __restrict__ col_count,
-                                                         const int* __restrict__ block_off,
-                                                         unsigned long long* __restrict__ col_key,
-                                                         int* __restrict__ col_start, int* __restrict__ col_cstart,
-                                                         unsigned int* flags, const ExclPairs ex,
-                                                         const VerletState* __restrict__ vs) {
-    __shared__ int off[kMaxCols];
-    if (vs && !vs->build) return;
-    if (blockIdx.x == 0) {
-        // cluster offsets (8 atoms per cluster, the last one padded), the
-        // column-size check and the exclusion extent
-        for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
-            const int a = col_count[c];
-            if (a > kColMax) atomicOr(flags + 2, 1u);
-            off[c] = (a + kCluster - 1) / kCluster;
-        }
-        __syncthreads();
-        const int ncl = lds_exclusive_scan(off, ncol);
-        for (int c = threadIdx.x; c < ncol; c += blockDim.x) col_cstart[c] = off[c];
-        if (threadIdx.x == 0) col_cstart[ncol] = ncl;
-        if (threadIdx.x == 0 && ex.n1 + ex.n2 > 0) {
-            float m = 0.f;
-            for (int b = 0; b < gridDim.x; ++b) m = fmaxf(m, ex.block_max[b]);
-            *ex.out = m;
-        }
-        __syncthreads();
-    }
-    for (int c = threadIdx.x; c < ncol; c += blockDim.x) off[c] = col_count[c];
     __syncthreads();
-    const int na = lds_exclusive_scan(off, ncol);
-    if (blockIdx.x == 0) {
-        for (int c = threadIdx.x; c < ncol; c += blockDim.x) col_start[c] = off[c];
-        if (threadIdx.x == 0) col_start[ncol] = na;
+    // place this block's atoms in their columns' buckets (each thread walks
+    // the atoms it wrapped above, so it reads back its own writes)
+    bool over = false;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const int c = col_of[i];
+        const int p = atomicAdd(hist + c, 1);
+        if (p < kColMax) col_key[static_cast<size_t>(c) * kColMax + p] = z_key(wrapped[3 * i + 2], zscale, i);
+        else over = true;
     }
-    for (int c = threadIdx.x; c < ncol; c += blockDim.x) off[c] += block_off[static_cast<size_t>(blockIdx.x) * ncol + c];
-    __syncthreads();
-    const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
-    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x)
-        col_key[atomicAdd(off + col_of[i], 1)] = z_key(wrapped[3 * i + 2], zscale, i);
+    if (over) atomicOr(flags + 2, 1u);  // a column holds more atoms than the column sort handles
 }
 
 template <typename real>
@@ -300,29 +225,61 @@ constexpr int kSortThreads = 256;
 // and the PME spline record.  Padding slots get -1, the padding type, zero
 // charge and an offset of kFarAway.
 template <typename real>
-__global__ __launch_bounds__(kSortThreads) void k_col_sort(const int* __restrict__ col_start,
-                                                           const int* __restrict__ col_cstart,
+__global__ __launch_bounds__(kSortThreads) void k_col_sort(int ncol, int* __restrict__ col_count,
+                                                           unsigned int* __restrict__ ticket,
+                                                           int* __restrict__ col_cstart,
                                                            const unsigned long long* __restrict__ col_key,
-                                                           const double* __restrict__ wrapped,
-                                                           int* __restrict__ col_count, const ClusterOut<real> o,
+                                                           const double* __restrict__ wrapped, const ExclPairs ex,
+                                                           int nbin, const ClusterOut<real> o,
                                                            const VerletState* __restrict__ vs) {
     constexpr int kPer = kColMax / kSortThreads;  // atoms per thread at most
     __shared__ unsigned long long s_key[kColMax];  // keys; after ranking, x by rank
     __shared__ double s_y[kColMax], s_z[kColMax];
     __shared__ int s_sorted[kColMax];
+    __shared__ int s_red[kSortThreads / kWave];
+    __shared__ int s_n, s_last;
     if (vs && !vs->build) return;
     const int c = blockIdx.x;
-    const int a0 = col_start[c], n = min(col_start[c + 1] - a0, kColMax);
-    const int cl0 = col_cstart[c], ncl = col_cstart[c + 1] - cl0;
+    // cluster offset of this column: clusters of the columns before it (a
+    // column beyond kColMax atoms keeps kColMax of them; k_bin has flagged it)
+    int before = 0;
+    for (int q = threadIdx.x; q < c; q += kSortThreads) before += (min(col_count[q], kColMax) + kCluster - 1) / kCluster;
+    for (int m = kWave / 2; m >= 1; m >>= 1) before += __shfl_xor(before, m, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_red[threadIdx.x / kWave] = before;
+    // read into LDS, so the load has completed before the ticket below
+    if (threadIdx.x == 0) s_n = min(col_count[c], kColMax);
+    __syncthreads();
+    int cl0 = 0;
+    for (int w = 0; w < kSortThreads / kWave; ++w) cl0 += s_red[w];
+    const int n = s_n;
+    const int ncl = (n + kCluster - 1) / kCluster;
+    if (threadIdx.x == 0) {
+        col_cstart[c] = cl0;
+        if (c == ncol - 1) col_cstart[ncol] = cl0 + ncl;
+        if (c == 0 && ex.n1 + ex.n2 > 0) {
+            float m = 0.f;
+            for (int b = 0; b < nbin; ++b) m = fmaxf(m, ex.block_max[b]);
+            *ex.out = m;
+        }
+        // every block has read the totals once it has taken its ticket; the
+        // last one clears them (and the ticket) for the next evaluation
+        __threadfence();
+        s_last = atomicAdd(ticket, 1u) == gridDim.x - 1 ? 1 : 0;
+    }
+    __syncthreads();
+    if (s_last) {
+        for (int q = threadIdx.x; q < ncol; q += kSortThreads) col_count[q] = 0;
+        if (threadIdx.x == 0) atomicExch(ticket, 0u);
+    }
+    const unsigned long long* bucket = col_key + static_cast<size_t>(c) * kColMax;
     unsigned long long key[kPer];
 #pragma unroll
     for (int p = 0; p < kPer; ++p) {
         const int k = static_cast<int>(threadIdx.x) + p * kSortThreads;
-        key[p] = k < n ? col_key[a0 + k] : 0ull;
+        key[p] = k < n ? bucket[k] : 0ull;
         if (k < n) s_key[k] = key[p];
     }
     __syncthreads();
-    if (threadIdx.x == 0) col_count[c] = 0;  // read by nobody after the scatter; zero for the next evaluation
     int rank[kPer];
 #pragma unroll
     for (int p = 0; p < kPer; ++p) {
@@ -612,23 +569,33 @@ __device__ __forceinline__ ExclRegs excl_lists_load(const SearchArgs& a, int ci,
     return r;
 }
 
+// The j atoms of the first kNearPre groups of eight near tiles are loaded at
+// once, so their latency is paid once rather than once per group.
+constexpr int kNearPre = 6;
+
 __device__ void near_tile_masks(const SearchArgs& a, int ci, int lane, const ExclRegs& xr, int (*s_ex)[kExclCap],
                                 const int2* near, int nnear, size_t base_i) {
     const unsigned long long self_mask = 0x0080C0E0F0F8FCFEull;  // bit 8*ti+tj set iff tj > ti
     const int a_l = xr.a_l, en = xr.en;
+    const int ti = lane >> 3, tj = lane & 7;
+    // lane 8u + tj: atom tj of near tile 8g + u
+    int aj_pre[kNearPre];
+#pragma unroll
+    for (int g = 0; g < kNearPre; ++g) {
+        const int qq = kCluster * g + (lane >> 3);
+        aj_pre[g] = qq < nnear ? a.slot_atom[near[qq].y * kCluster + tj] : -1;
+    }
 #pragma unroll
     for (int t = 0; t < kCluster; ++t)
         if (lane < kExclCap) s_ex[t][lane] = xr.v[t];
     __builtin_amdgcn_wave_barrier();
-    const int ti = lane >> 3, tj = lane & 7;
     const int ai = __shfl(a_l, ti, kWave), my_n = __shfl(en, ti, kWave);
     const bool in_lds = my_n <= kExclCap;
     // longest staged list of the cluster (uniform loop bound)
     int nmax = in_lds ? my_n : 0;
     for (int m = 32; m >= 1; m >>= 1) nmax = max(nmax, __shfl_xor(nmax, m, kWave));
-    for (int q0 = 0; q0 < nnear; q0 += kCluster) {
-        const int qq = q0 + (lane >> 3);
-        const int aj_l = qq < nnear ? a.slot_atom[near[qq].y * kCluster + tj] : -1;
+    // the masks of near tiles q0 .. q0 + 7, from this lane's j atom aj_l
+    auto group = [&](int q0, int aj_l) {
         int aj[kCluster];
 #pragma unroll
         for (int u = 0; u < kCluster; ++u) aj[u] = __shfl(aj_l, kCluster * u + tj, kWave);
@@ -653,6 +620,13 @@ __device__ void near_tile_masks(const SearchArgs& a, int ci, int lane, const Exc
             const int2 en_q = near[q0 + u];
             if (lane == 0) a.list_mask[base_i + en_q.x] = ((en_q.y == ci) ? self_mask : ~0ull) & ~bal;
         }
+    };
+#pragma unroll
+    for (int g = 0; g < kNearPre; ++g)
+        if (kCluster * g < nnear) group(kCluster * g, aj_pre[g]);
+    for (int q0 = kCluster * kNearPre; q0 < nnear; q0 += kCluster) {
+        const int qq = q0 + (lane >> 3);
+        group(q0, qq < nnear ? a.slot_atom[near[qq].y * kCluster + tj] : -1);
     }
 }
 
@@ -717,6 +691,9 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     const unsigned long long self_mask = 0x0080C0E0F0F8FCFEull;  // bit 8*ti+tj set iff tj > ti
     ExclRegs xr{};
     if (a.amber) xr = excl_lists_load(a, ci, lane);
+    // the i-cluster's atoms relative to its origin (used after the ranges)
+    const R4* pos = static_cast<const R4*>(a.pos);
+    const R4 pi_l = pos[ci * kCluster + (lane & 7)];
 
     // ranges: lane r = 2 * row + segment
     int r_b0 = 0, r_len = 0;
@@ -792,8 +769,6 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     };
 
     // the i-cluster's atoms relative to its origin, in scalar registers
-    const R4* pos = static_cast<const R4*>(a.pos);
-    const R4 pi_l = pos[ci * kCluster + (lane & 7)];
     real pix[kCluster], piy[kCluster], piz[kCluster];
 #pragma unroll
     for (int k = 0; k < kCluster; ++k) {
@@ -804,24 +779,8 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     const real rp2 = static_cast<real>(a.rp2);
 
     int count = 0, nnear = 0;
-    int cj_n = cand_of(lane);
-    double4 oj_n = make_double4(0, 0, 0, 0);
-    float4 hj_n = make_float4(0, 0, 0, 0);
-    if (cj_n >= 0) {
-        oj_n = a.origin[cj_n];
-        hj_n = a.half[cj_n];
-    }
-    for (int b = 0; b < total; b += kWave) {
-        const int cj = cj_n;
-        const double4 oj = oj_n;
-        const float4 hj = hj_n;
-        if (b + kWave < total) {
-            cj_n = cand_of(b + kWave + lane);
-            if (cj_n >= 0) {
-                oj_n = a.origin[cj_n];
-                hj_n = a.half[cj_n];
-            }
-        }
+    // one step of 64 candidates (cj, origin, half of this lane's candidate)
+    auto step = [&](const int cj, const double4& oj, const float4& hj) {
         const bool valid = cj >= 0;
         double ddx = 0, ddy = 0, ddz = 0, d2 = 1e300;
         if (valid) {
@@ -842,7 +801,7 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
         }
         bool cand = valid && d2 < a.rl2;
         unsigned long long bal = __ballot(cand);
-        if (bal == 0) continue;
+        if (bal == 0) return;
         // second test: a tile can only hold a pair within r_list if one of the
         // i-atoms lies within r_list (plus a margin) of the j-cluster's box
         if (cand) {
@@ -859,7 +818,7 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
             cand = any;
         }
         bal = __ballot(cand);
-        if (bal == 0) continue;
+        if (bal == 0) return;
         const int pos = count + __popcll(bal & ((1ull << lane) - 1ull));
         const bool near = a.amber && cand && d2 < rex2;
         if (cand && pos < a.cap) {
@@ -880,6 +839,29 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
         }
         nnear += __popcll(nbal);
         count += __popcll(bal);
+    };
+    // the boxes of candidate q of the stream (nothing past the end)
+    auto load = [&](int q, int& cj, double4& oj, float4& hj) {
+        cj = cand_of(q);
+        if (cj >= 0) {
+            oj = a.origin[cj];
+            hj = a.half[cj];
+        }
+    };
+    // Two register sets, each loaded two steps before its use, so that a
+    // step's loads have a whole step to arrive.
+    int cjA = -1, cjB = -1;
+    double4 oA = make_double4(0, 0, 0, 0), oB = oA;
+    float4 hA = make_float4(0, 0, 0, 0), hB = hA;
+    load(lane, cjA, oA, hA);
+    load(kWave + lane, cjB, oB, hB);
+    for (int b = 0; b < total; b += 2 * kWave) {
+        step(cjA, oA, hA);
+        load(b + 2 * kWave + lane, cjA, oA, hA);
+        if (b + kWave < total) {
+            step(cjB, oB, hB);
+            load(b + 3 * kWave + lane, cjB, oB, hB);
+        }
     }
     if (count > a.cap) {
         if (lane == 0) atomicOr(a.flags + 1, 1u);
@@ -1173,12 +1155,11 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl
                            g.cell, (0.5 * c->skin) * (0.5 * c->skin), (0.5 * kPruneBuffer) * (0.5 * kPruneBuffer), 0,
                            vs, w.flags);
     }
-    hipLaunchKernelGGL(k_bin, dim3(nblk), dim3(kBinThreads), 0, c->stream, coords, n, chunk, g.cell, g.ncx, g.ncy,
-                       g.sx, g.sy, w.wrapped, w.col_of, w.col_count, w.block_off, ex, vs ? w.x_ref : nullptr, vs);
     // the sort key is the wrapped z over the cell height
     const double zscale = 4294967296.0 / (g.cell.tri ? g.cell.cz : g.box[2]);
-    hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kBinThreads), 0, c->stream, n, chunk, ncol, w.col_of, w.wrapped,
-                       zscale, w.col_count, w.block_off, w.col_key, w.col_start, w.col_cstart, w.flags, ex, vs);
+    hipLaunchKernelGGL(k_bin, dim3(nblk), dim3(kBinThreads), 0, c->stream, coords, n, chunk, g.cell, g.ncx, g.ncy,
+                       g.sx, g.sy, zscale, w.wrapped, w.col_of, w.col_count, w.col_key, w.flags, ex,
+                       vs ? w.x_ref : nullptr, vs);
     PmeSlots pme{};
     const bool with_pme = c->params.semantics == MDGPU_SEM_AMBER && w.pme_k[0] > 0;
     for (int d = 0; d < 3; ++d) pme.k[d] = with_pme ? w.pme_k[d] : 0;
@@ -1189,8 +1170,8 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl
         pme.w = w.slot_pme_wf;
         const ClusterOut<float> o{c->n_clusters, t.charge, t.type, t.n_types, w.slot_atom, w.atom_slot, w.cluster_col,
                                   w.slot_pos_f, w.slot_meta, w.cl_origin, w.cl_half, pme};
-        hipLaunchKernelGGL(k_col_sort<float>, dim3(ncol), dim3(kSortThreads), 0, c->stream, w.col_start, w.col_cstart,
-                           w.col_key, w.wrapped, w.col_count, o, vs);
+        hipLaunchKernelGGL(k_col_sort<float>, dim3(ncol), dim3(kSortThreads), 0, c->stream, ncol, w.col_count,
+                           w.col_ticket, w.col_cstart, w.col_key, w.wrapped, ex, nblk, o, vs);
         if (vs)
             hipLaunchKernelGGL(k_refresh<float>, dim3(rb), dim3(kRefreshThreads), 0, c->stream, vs,
                                w.col_cstart + ncol, coords, w.x_ref, w.wrapped, o, g.cell);
@@ -1198,8 +1179,8 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl
         pme.w = w.slot_pme_wd;
         const ClusterOut<double> o{c->n_clusters, t.charge, t.type, t.n_types, w.slot_atom, w.atom_slot, w.cluster_col,
                                    w.slot_pos_d, w.slot_meta, w.cl_origin, w.cl_half, pme};
-        hipLaunchKernelGGL(k_col_sort<double>, dim3(ncol), dim3(kSortThreads), 0, c->stream, w.col_start,
-                           w.col_cstart, w.col_key, w.wrapped, w.col_count, o, vs);
+        hipLaunchKernelGGL(k_col_sort<double>, dim3(ncol), dim3(kSortThreads), 0, c->stream, ncol, w.col_count,
+                           w.col_ticket, w.col_cstart, w.col_key, w.wrapped, ex, nblk, o, vs);
         if (vs)
             hipLaunchKernelGGL(k_refresh<double>, dim3(rb), dim3(kRefreshThreads), 0, c->stream, vs,
                                w.col_cstart + ncol, coords, w.x_ref, w.wrapped, o, g.cell);
