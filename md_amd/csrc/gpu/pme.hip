@@ -165,16 +165,24 @@ __device__ __forceinline__ const typename C2<real>::T* stage_twiddles(typename C
 
 // ------------------------------------------------- spread + z r2c --------
 // Owner-computes charge spreading fused with the z transform.  A block owns
-// bx x by z-lines (x in [X0, X0+bx), y in [Y0, Y0+by), all z) and builds them
-// in LDS.  It visits the slots of the sort columns that can hold atoms
-// reaching its lines (an atom with base index f contributes to points
-// f - 0..3, so these lines receive f_x in [X0, X0+bx+2], i.e. x in
-// [X0 hx, (X0+bx+3) hx)), reads each slot's spline record (base indices,
-// fractions, charge; written once per evaluation by the cluster set-up), and
-// adds q theta_x theta_y theta_z with 64-bit integer LDS atomics (2^40 fixed
-// point: the sum does not depend on their order).  It then converts the lines
-// and transforms them along z in place.  Every grid point is written by one
-// block, so the real grid never goes to memory and needs no clearing.
+// by z-lines (x = X0, y in [Y0, Y0+by), all z) and builds them in LDS.  It
+// visits the slots of the sort columns that can hold atoms reaching its lines
+// (an atom with base index f contributes to points f - 0..3, so these lines
+// receive f_x in [X0, X0+3] and f_y in [Y0, Y0+by+2], i.e. x in
+// [X0 hx, (X0+4) hx) and y in [Y0 hy, (Y0+by+3) hy)), reads each slot's
+// spline record (base indices, fractions, charge; written once per evaluation
+// by the cluster set-up), and adds q theta_x theta_y theta_z with 64-bit
+// integer LDS atomics (2^40 fixed point: the sum does not depend on their
+// order).  It then converts the lines and transforms them along z in place.
+// Every grid point is written by one block, so the real grid never goes to
+// memory and needs no clearing.
+//
+// Each wave takes 64 candidate slots at a time, in two phases.  (A) Each
+// lane tests its slot; a slot that reaches the block's lines gets a record in
+// the wave's LDS buffer: q theta_x(a) for its one x point a, theta_y(0..3),
+// theta_z(0..3), f_y - Y0 and f_z.  (B) 16 lanes per record, four records
+// per step, lane (b, c) adds the point (a, b, c) when f_y - b lies in the
+// block.  The products are formed as (q theta_x) theta_y, then theta_z.
 struct SpreadCols {
     int ncx, ncy;
     double sx, sy;
@@ -186,10 +194,11 @@ struct SpreadCols {
 };
 
 constexpr int kSpreadCols = 256;  // columns per pass of the candidate loop
-constexpr int kSpreadLines = 8;   // z-lines per spreading block
+constexpr int kSpreadLines = 8;   // z-lines per spreading block (one x, eight y)
+constexpr int kSpreadRec = 9;     // reals of an LDS spreading record
 
 template <typename real>
-__global__ __launch_bounds__(kThreads) void k_spread_fft_z(PmeGeom g, SpreadCols sc, int bx, int by, int lognl,
+__global__ __launch_bounds__(kThreads) void k_spread_fft_z(PmeGeom g, SpreadCols sc, int by, int lognl,
                                                            typename C2<real>::T* __restrict__ cg,
                                                            const typename C2<real>::T* __restrict__ tw_g,
                                                            unsigned int* overflow) {
@@ -197,25 +206,34 @@ __global__ __launch_bounds__(kThreads) void k_spread_fft_z(PmeGeom g, SpreadCols
     using R4 = typename std::conditional<sizeof(real) == 8, double4, float4>::type;
     extern __shared__ unsigned char smem[];
     __shared__ int s_first[kSpreadCols], s_pre[kSpreadCols + 1];
-    const int nl = 1 << lognl;  // == bx * by
+    // records per candidate of a wave's chunk: [qa, theta_y 0..3, theta_z 0..3]
+    // and dyp | flz << 16
+    __shared__ real s_rec[kThreads / kWave][kWave * kSpreadRec];
+    __shared__ int s_pk[kThreads / kWave][kWave];
+    const int nl = 1 << lognl;  // == by
     c2* A = reinterpret_cast<c2*>(smem);
     c2* B = A + g.kz * nl;
     unsigned long long* acc = reinterpret_cast<unsigned long long*>(B);  // [kz][nl], aliases B
     const c2* tw = stage_twiddles<real>(B + g.kz * nl, tw_g, g.kz);
     for (int t = threadIdx.x; t < nl * g.kz; t += blockDim.x) acc[t] = 0ull;
-    const int X0 = blockIdx.x * bx, Y0 = blockIdx.y * by;
+    const int X0 = blockIdx.x, Y0 = blockIdx.y * by;
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    real* rec = s_rec[wv];
+    int* pks = s_pk[wv];
     const double hx = g.lx / g.kx, hy = g.ly / g.ky, tol = 1e-6 + sc.margin;
-    // columns overlapping [X0 hx, (X0+bx+3) hx) x [Y0 hy, (Y0+by+3) hy), with
+    // columns overlapping [X0 hx, (X0+4) hx) x [Y0 hy, (Y0+by+3) hy), with
     // 1e-6 A for the rounding of floor(u) against the column binning, grown
     // by how far an atom of a kept Verlet list may have left its column
     int cx0 = static_cast<int>(floor((X0 * hx - tol) / sc.sx));
-    int ncxr = static_cast<int>(floor(((X0 + bx + 3) * hx + tol) / sc.sx)) - cx0 + 1;
+    int ncxr = static_cast<int>(floor(((X0 + 4) * hx + tol) / sc.sx)) - cx0 + 1;
     int cy0 = static_cast<int>(floor((Y0 * hy - tol) / sc.sy));
     int ncyr = static_cast<int>(floor(((Y0 + by + 3) * hy + tol) / sc.sy)) - cy0 + 1;
     if (ncxr >= sc.ncx) { cx0 = 0; ncxr = sc.ncx; }
     if (ncyr >= sc.ncy) { cy0 = 0; ncyr = sc.ncy; }
     const int ncand = ncxr * ncyr;
     const R4* wrec = static_cast<const R4*>(sc.w);
+    // phase B lane roles
+    const int rk = lane >> 4, rb = (lane >> 2) & 3, rc = lane & 3;
     for (int cb = 0; cb < ncand; cb += kSpreadCols) {
         const int nc = min(kSpreadCols, ncand - cb);
         __syncthreads();  // the previous pass is done with s_first / s_pre
@@ -233,69 +251,87 @@ __global__ __launch_bounds__(kThreads) void k_spread_fft_z(PmeGeom g, SpreadCols
             for (int k = 0; k < nc; ++k) s_pre[k + 1] += s_pre[k];
         __syncthreads();
         const int total = s_pre[nc];
-        for (int t = threadIdx.x; t < total; t += blockDim.x) {
-            // column of candidate t: the last k with s_pre[k] <= t
-            int lo = 0, hi = nc - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_pre[mid] <= t) lo = mid;
-                else hi = mid - 1;
-            }
-            const int s = s_first[lo] + (t - s_pre[lo]);
-            const int fl = sc.fl[s];
-            const R4 wq = wrec[s];
-            if (wq.w == 0) continue;  // padding slot or uncharged atom
-            const int flx = fl & 1023, fly = (fl >> 10) & 1023, flz = fl >> 20;
-            int dxp = flx - X0, dyp = fly - Y0;
-            dxp += dxp < 0 ? g.kx : 0;
-            dyp += dyp < 0 ? g.ky : 0;
-            if (dxp > bx + 2 || dyp > by + 2) continue;
+        // each wave takes chunks of 64 candidates, round robin
+        for (int t0 = wv * kWave; t0 < total; t0 += blockDim.x) {
+            // phase A: this lane's candidate
+            const int t = t0 + lane;
+            bool take = false;
             real th[3][4], dth[3][4];
-            bspline4<real>(wq.x, th[0], dth[0]);
-            bspline4<real>(wq.y, th[1], dth[1]);
-            bspline4<real>(wq.z, th[2], dth[2]);
-            if constexpr (sizeof(real) == 4) {
-                // FP32 products and conversion (charges far below 2^23 e;
-                // a non-finite record raises the overflow flag instead)
-                if (!(fabsf(wq.w) < 8.0e6f) || !(fabsf(wq.x) + fabsf(wq.y) + fabsf(wq.z) < 4.0f)) {
+            real qa = 0;
+            int dyp = 0, flz = 0;
+            if (t < total) {
+                // column of candidate t: the last k with s_pre[k] <= t
+                int lo = 0, hi = nc - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_pre[mid] <= t) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int s = s_first[lo] + (t - s_pre[lo]);
+                const int fl = sc.fl[s];
+                const R4 wq = wrec[s];
+                const int flx = fl & 1023, fly = (fl >> 10) & 1023;
+                flz = fl >> 20;
+                int dxp = flx - X0;
+                dyp = fly - Y0;
+                dxp += dxp < 0 ? g.kx : 0;
+                dyp += dyp < 0 ? g.ky : 0;
+                // padding slots and uncharged atoms add nothing
+                take = wq.w != 0 && dxp <= 3 && dyp <= by + 2;
+                if (take && sizeof(real) == 4 &&
+                    (!(fabs(wq.w) < static_cast<real>(8.0e6)) ||
+                     !(fabs(wq.x) + fabs(wq.y) + fabs(wq.z) < static_cast<real>(4.0)))) {
+                    // FP32 products and conversion need charges far below
+                    // 2^23 e; a non-finite record raises the overflow flag
                     atomicOr(overflow, 1u);
-                    continue;
+                    take = false;
                 }
-                for (int a = 0; a < 4; ++a) {
-                    const int li = dxp - a;
-                    if (li < 0 || li >= bx || X0 + li >= g.kx) continue;
-                    for (int b = 0; b < 4; ++b) {
-                        const int lj = dyp - b;
-                        if (lj < 0 || lj >= by || Y0 + lj >= g.ky) continue;
-                        const int l = li * by + lj;
-                        const float qab = wq.w * th[0][a] * th[1][b];
-                        for (int c = 0; c < 4; ++c) {
-                            int iz = flz - c;
-                            iz += iz < 0 ? g.kz : 0;
-                            const long long v = fixed40_f32(qab * th[2][c]);
-                            if (v != 0) atomicAdd(acc + iz * nl + l, static_cast<unsigned long long>(v));
-                        }
-                    }
+                if (take) {
+                    bspline4<real>(wq.x, th[0], dth[0]);
+                    bspline4<real>(wq.y, th[1], dth[1]);
+                    bspline4<real>(wq.z, th[2], dth[2]);
+                    const real tx = dxp == 0 ? th[0][0] : (dxp == 1 ? th[0][1] : (dxp == 2 ? th[0][2] : th[0][3]));
+                    qa = wq.w * tx;
                 }
-            } else {
-                const double q = static_cast<double>(wq.w);
-                for (int a = 0; a < 4; ++a) {
-                    const int li = dxp - a;
-                    if (li < 0 || li >= bx || X0 + li >= g.kx) continue;
-                    for (int b = 0; b < 4; ++b) {
-                        const int lj = dyp - b;
-                        if (lj < 0 || lj >= by || Y0 + lj >= g.ky) continue;
-                        const int l = li * by + lj;
-                        const double qab = q * static_cast<double>(th[0][a] * th[1][b]);
-                        for (int c = 0; c < 4; ++c) {
-                            int iz = flz - c;
-                            iz += iz < 0 ? g.kz : 0;
-                            const long long v = to_fixed(qab * static_cast<double>(th[2][c]), kGridScale, overflow);
-                            if (v != 0) atomicAdd(acc + iz * nl + l, static_cast<unsigned long long>(v));
-                        }
+            }
+            const unsigned long long bal = __ballot(take);
+            const int nrec = __popcll(bal);
+            if (nrec == 0) continue;
+            if (take) {
+                const int r = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned int>(bal >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(bal), 0u));
+                real* e = rec + r * kSpreadRec;
+                e[0] = qa;
+                for (int q = 0; q < 4; ++q) {
+                    e[1 + q] = th[1][q];
+                    e[5 + q] = th[2][q];
+                }
+                pks[r] = dyp | (flz << 16);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // phase B: four records per step, lane (rk, rb, rc)
+            for (int r0 = 0; r0 < nrec; r0 += 4) {
+                const int r = r0 + rk;
+                if (r < nrec) {
+                    const real* e = rec + r * kSpreadRec;
+                    const int pk = pks[r];
+                    const int lj = (pk & 0xffff) - rb;
+                    if (lj >= 0 && lj < by && Y0 + lj < g.ky) {
+                        int iz = (pk >> 16) - rc;
+                        iz += iz < 0 ? g.kz : 0;
+                        const real qab = e[0] * e[1 + rb];
+                        long long v;
+                        if constexpr (sizeof(real) == 4) v = fixed40_f32(qab * e[5 + rc]);
+                        else v = to_fixed(qab * static_cast<double>(e[5 + rc]), kGridScale, overflow);
+                        if (v != 0) atomicAdd(acc + iz * nl + lj, static_cast<unsigned long long>(v));
                     }
                 }
             }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
     __syncthreads();
@@ -305,10 +341,8 @@ __global__ __launch_bounds__(kThreads) void k_spread_fft_z(PmeGeom g, SpreadCols
     __syncthreads();
     c2* R = lds_fft<real>(A, B, g.kz, lognl, tw, -1);
     for (int t = threadIdx.x; t < nl * g.hz; t += blockDim.x) {
-        const int l = t / g.hz, k = t - (t / g.hz) * g.hz;
-        const int li = l / by, lj = l - li * by;
-        if (X0 + li < g.kx && Y0 + lj < g.ky)
-            cg[(static_cast<size_t>(X0 + li) * g.ky + Y0 + lj) * g.hz + k] = R[k * nl + l];
+        const int lj = t / g.hz, k = t - (t / g.hz) * g.hz;
+        if (Y0 + lj < g.ky) cg[(static_cast<size_t>(X0) * g.ky + Y0 + lj) * g.hz + k] = R[k * nl + lj];
     }
 }
 
@@ -566,15 +600,14 @@ hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used, double* pot) {
     // spreading fused with the z transform (timed as the spread)
     mark(MDGPU_K_SPREAD, true);
     {
-        // 8 z-lines per block (1 x 8 in x-y): measured faster than 4 or 16
-        // next to the concurrent pair kernel
+        // up to 8 z-lines per block (1 x 8 in x-y): measured faster than 4 or
+        // 16 next to the concurrent pair kernel
         const int nl = lines_per_block(g.kz, sizeof(c2), kSpreadLines);
-        const int by = std::min(nl, 8), bx = nl / by;
         const size_t shm = (2 * static_cast<size_t>(g.kz) * nl + g.kz) * sizeof(c2);
         const SpreadCols sc{c->grid.ncx, c->grid.ncy, c->grid.sx, c->grid.sy, w.col_cstart, w.slot_pme_fl, wrec,
                             c->skin > 0 ? 0.5 * c->skin * c->grid.frac_scale : 0.0};
-        const dim3 grid((g.kx + bx - 1) / bx, (g.ky + by - 1) / by);
-        hipLaunchKernelGGL(k_spread_fft_z<real>, grid, dim3(kThreads), shm, c->stream, g, sc, bx, by, ilog2(nl), cg,
+        const dim3 grid(g.kx, (g.ky + nl - 1) / nl);
+        hipLaunchKernelGGL(k_spread_fft_z<real>, grid, dim3(kThreads), shm, c->stream, g, sc, nl, ilog2(nl), cg,
                            twz, w.flags);
     }
     mark(MDGPU_K_SPREAD, false);
