@@ -169,7 +169,8 @@ def cpu_baseline(N, prm, rst, kw):
 def bench_batched(torch, N, systems, n_systems=256, reps=3):
     """256 uneven water boxes (3k-30k atoms, AMBER, rc 9 A, switch 8 A, Ewald tol
     1e-5, PME grid = smallest 2^a 3^b >= L / 1 A), mixed form: systems/s of one
-    graph launch against one synchronous evaluation per system."""
+    graph launch, of the same launches on the same streams issued directly,
+    and of one synchronous evaluation per system."""
     t0 = time.perf_counter()
     work = systems.batch_workload(n_systems)
     keep = []
@@ -191,6 +192,14 @@ def bench_batched(torch, N, systems, n_systems=256, reps=3):
         batch.evaluate_device(x.data_ptr(), f.data_ptr())
     torch.cuda.synchronize()
     serial_s = (time.perf_counter() - t0) / reps
+    batch.configure(mode=2, streams=cfg["streams"], split=cfg["split"], fft_lines=cfg["fft_lines"])
+    batch.evaluate_device(x.data_ptr(), f.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.evaluate_device(x.data_ptr(), f.data_ptr())
+    torch.cuda.synchronize()
+    streams_s = (time.perf_counter() - t0) / reps
     batch.configure(mode=0, streams=cfg["streams"], split=cfg["split"], fft_lines=cfg["fft_lines"])
     batch.evaluate_device(x.data_ptr(), f.data_ptr())  # sizes lists, captures the graph, first launch
     torch.cuda.synchronize()
@@ -201,11 +210,24 @@ def bench_batched(torch, N, systems, n_systems=256, reps=3):
     graph_s = (time.perf_counter() - t0) / reps
     state = batch.graph_state()
     n_atoms = int(sum(w[1].shape[0] for w in work))
+    # the tuner's choice per size class (systems by atom count, four classes of equal count)
+    sizes = [w[1].shape[0] for w in work]
+    order = sorted(range(n_systems), key=lambda i: sizes[i])
+    classes = []
+    for k in range(4):
+        members = order[k * n_systems // 4:(k + 1) * n_systems // 4]
+        if members:
+            classes.append({"atoms": [int(sizes[members[0]]), int(sizes[members[-1]])],
+                            "split_fft_lines": sorted({batch.system_config(i) for i in members})})
     batch.close()
-    return {"systems": n_systems, "atoms": n_atoms, "setup_s": setup_s, "tuned": cfg,
+    return {"systems": n_systems, "atoms": n_atoms, "setup_s": setup_s, "tuned": cfg, "tuned_classes": classes,
+            "form": "one hipGraph launch: every system's kernel chain (sort, search, pair + bonded, PME, "
+                    "reduction) captured once, dealt largest first over 8 streams, then one gathering kernel",
             "graph_state": state,
             "graph_systems_per_s": n_systems / graph_s, "serial_systems_per_s": n_systems / serial_s,
+            "streams_systems_per_s": n_systems / streams_s,
             "graph_ms_per_batch": 1e3 * graph_s, "serial_ms_per_batch": 1e3 * serial_s,
+            "streams_ms_per_batch": 1e3 * streams_s,
             "speedup_graph_vs_serial": serial_s / graph_s,
             "graph_atoms_per_s": n_atoms / graph_s,
             "workload": "256 water boxes, atoms U[3000, 30000] (seeds 4000+i), 0.1 atoms/A^3, AMBER, "

@@ -275,19 +275,28 @@ int mdgpu_batch_eval_device(mdgpu_batch* b, const double* d_coords, mdgpu_energi
 /* Same from host buffers (copied in and out around the graph launch). */
 int mdgpu_batch_eval(mdgpu_batch* b, const double* coords, mdgpu_energies* out_e, double* out_forces);
 /* Execution settings: mode 0 = one graph launch (default), 1 = one
- * synchronous evaluation per system in turn (the serial baseline);
- * streams = concurrent streams in the graph (1..8); split = waves per
- * i-cluster list in the pair kernel (1..8); fft_lines = cap on FFT lines per
- * block (0 = automatic).  streams, split and fft_lines are fixed once the
- * graph exists (MDGPU_ERR_STATE); mode can always change. */
+ * synchronous evaluation per system in turn (the serial baseline), 2 = the
+ * graph's launches on the graph's streams issued directly, without a graph;
+ * streams = concurrent streams (1..8); split = waves per i-cluster list in
+ * the pair kernel (1..8); fft_lines = cap on FFT lines per block (0 =
+ * automatic).  A changed split or fft_lines applies to every system (it
+ * replaces the tuner's per-class choice).  streams, split and fft_lines are
+ * fixed once the graph exists (MDGPU_ERR_STATE); mode can always change. */
 int mdgpu_batch_configure(mdgpu_batch* b, int mode, int streams, int split, int fft_lines);
-/* Autotuner, before the first graph evaluation: times `reps` serial passes
- * over all systems for each candidate pair-tile split (1, 2, 4, 8) and FFT
- * block shape (16, 8, 4 lines) on the given device buffers and keeps the
- * fastest; streams become min(8, systems).  Returns the settings and the mean
- * serial pass time (ms). */
+/* Autotuner, before the first graph evaluation.  Streams become
+ * min(8, systems).  The systems are ordered by atom count and cut into up to
+ * four size classes of equal count; for each class in turn (largest first)
+ * every pair-tile split (1, 2, 4, 8) and FFT block shape (16, 8, 4 lines) is
+ * timed over `reps` passes of the whole batch with the graph's launches and
+ * streams issued directly (mode 2), the other classes at their current
+ * choice, and the fastest is kept for that class.  Returns the streams, the
+ * largest class's settings and the best pass time (ms);
+ * mdgpu_batch_system_config gives every system's. */
 int mdgpu_batch_tune(mdgpu_batch* b, const double* d_coords, double* d_forces, int reps, int* streams, int* split,
                      int* fft_lines, double* ms);
+/* Pair-tile split and FFT lines of system `index` (no counterpart in the
+ * reference). */
+int mdgpu_batch_system_config(const mdgpu_batch* b, int index, int* split, int* fft_lines);
 /* 0: no graph yet; 1: the captured graph is in use; 2: the graph went stale
  * and the batch evaluates serially. */
 int mdgpu_batch_graph_state(const mdgpu_batch* b, int* state);

@@ -9,7 +9,12 @@
 // system's energy row and flags into one buffer, read back with one copy.
 // Replays need no host work per system.  A system whose flags ask for a redo
 // (pair list over capacity, cluster too wide for the minimum-image search) is
-// evaluated again on its own and the graph is captured anew next time.
+// evaluated again on its own, and the batch evaluates serially from then on.
+//
+// Each system keeps its own pair-tile split and FFT block shape.  The
+// autotuner sets them per size class (systems ordered by atom count, cut into
+// up to four classes of equal count), timing for each candidate the same
+// launches on the same streams as the graph, issued directly (mode 2).
 #include "ctx.h"
 
 #include <array>
@@ -42,6 +47,7 @@ struct mdgpu_batch {
     std::vector<int64_t> off;
     int64_t n_atoms = 0;
     int mode = 0, streams = 4, split = 4, fft_lines = 8;  // split and fft_lines as a single context
+    std::vector<int> sys_split, sys_fft;                  // per system (set by configure or the tuner)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const double* cap_x = nullptr;
@@ -100,37 +106,100 @@ int ensure_rows(mdgpu_batch* b) {
     return MDGPU_OK;
 }
 
-// Captures every system's evaluation on x / f into one graph.
-int capture(mdgpu_batch* b, const double* x, double* f) {
-    drop_graph(b);
+// The launch layout of a multi-stream pass: every system prepared with its
+// own settings, the row table uploaded, and the systems in issue order
+// (largest first) with their streams (each to the least-loaded stream).
+int layout(mdgpu_batch* b, std::vector<int>& order, std::vector<int>& stream_of, int& ns) {
     const int n = static_cast<int>(b->sys.size());
     int st = ensure_rows(b);
     if (st != MDGPU_OK) return st;
     std::vector<const double*> src(n);
     for (int i = 0; i < n; ++i) {
         mdgpu_ctx* c = b->sys[i];
-        c->split = b->split;
-        c->fft_lines = b->fft_lines;
+        c->split = b->sys_split[i];
+        c->fft_lines = b->sys_fft[i];
         c->timing.mode = 0;
         st = mdg::ctx_prepare(c, b->box[i].data());
         if (st != MDGPU_OK) return berr(b, st, "system " + std::to_string(i) + ": " + c->error);
         src[i] = c->work.energies;
     }
     B_TRY(hipMemcpy(b->d_src, src.data(), n * sizeof(double*), hipMemcpyHostToDevice), "upload row table");
-    // largest systems first, each to the least-loaded stream
-    std::vector<int> order(n);
+    order.resize(n);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
                      [&](int a, int c) { return b->sys[a]->topo.n_atoms > b->sys[c]->topo.n_atoms; });
-    // never more streams than systems: every branch of the graph holds kernels
-    const int ns = std::max(1, std::min({b->streams, kMaxStreams, n}));
+    // never more streams than systems: every branch of a graph holds kernels
+    ns = std::max(1, std::min({b->streams, kMaxStreams, n}));
     std::vector<int64_t> load(ns, 0);
-    std::vector<int> stream_of(n, 0);
+    stream_of.assign(n, 0);
     for (int i : order) {
         const int k = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
         load[k] += b->sys[i]->topo.n_atoms;
         stream_of[i] = k;
     }
+    return MDGPU_OK;
+}
+
+// Acts on the gathered rows of a finished multi-stream pass: outputs, or a
+// redo alone for a system whose flags ask for one.
+int finish_rows(mdgpu_batch* b, const double* x, mdgpu_energies* out_e, double* f) {
+    for (size_t i = 0; i < b->sys.size(); ++i) {
+        mdgpu_ctx* c = b->sys[i];
+        const double* row = b->h_all + i * kRow;
+        int st = mdg::ctx_check(c, row);
+        if (st == mdg::kEvalRedo) {
+            // adjusted (larger list or general search): redo alone; a graph is stale
+            b->dirty = true;
+            st = mdg::ctx_eval_device(c, x + 3 * b->off[i], b->box[i].data(), out_e ? &out_e[i] : nullptr,
+                                      f ? f + 3 * b->off[i] : nullptr);
+            if (st != MDGPU_OK) return berr(b, st, "system " + std::to_string(i) + ": " + c->error);
+            continue;
+        }
+        if (st != MDGPU_OK) return berr(b, st, "system " + std::to_string(i) + ": " + c->error);
+        mdg::ctx_outputs(c, row, out_e ? &out_e[i] : nullptr, nullptr);
+    }
+    return MDGPU_OK;
+}
+
+// One pass with the graph's launches and streams, issued directly (mode 2).
+int run_streams(mdgpu_batch* b, const double* x, mdgpu_energies* out_e, double* f) {
+    std::vector<int> order, stream_of;
+    int ns = 1;
+    int st = layout(b, order, stream_of, ns);
+    if (st != MDGPU_OK) return st;
+    const int n = static_cast<int>(b->sys.size());
+    B_TRY(hipEventRecord(b->fork, b->s0), "fork");
+    for (int k = 0; k < ns; ++k) B_TRY(hipStreamWaitEvent(b->workers[k], b->fork, 0), "fork");
+    for (int i : order) {
+        mdgpu_ctx* c = b->sys[i];
+        hipStream_t own = c->stream;
+        c->stream = b->workers[stream_of[i]];
+        st = mdg::ctx_enqueue(c, x + 3 * b->off[i], f ? f + 3 * b->off[i] : nullptr);
+        c->stream = own;
+        if (st != MDGPU_OK) {
+            (void)hipDeviceSynchronize();
+            return berr(b, st, "system " + std::to_string(i) + ": " + c->error);
+        }
+    }
+    for (int k = 0; k < ns; ++k) {
+        B_TRY(hipEventRecord(b->join[k], b->workers[k]), "join");
+        B_TRY(hipStreamWaitEvent(b->s0, b->join[k], 0), "join");
+    }
+    hipLaunchKernelGGL(k_collect, dim3(n), dim3(64), 0, b->s0, b->d_src, b->d_all, n);
+    B_TRY(hipGetLastError(), "collect");
+    B_TRY(hipMemcpyAsync(b->h_all, b->d_all, n * kRow * sizeof(double), hipMemcpyDeviceToHost, b->s0), "read rows");
+    B_TRY(hipStreamSynchronize(b->s0), "batch");
+    return finish_rows(b, x, out_e, f);
+}
+
+// Captures every system's evaluation on x / f into one graph.
+int capture(mdgpu_batch* b, const double* x, double* f) {
+    drop_graph(b);
+    const int n = static_cast<int>(b->sys.size());
+    std::vector<int> order, stream_of;
+    int ns = 1;
+    int st = layout(b, order, stream_of, ns);
+    if (st != MDGPU_OK) return st;
     B_TRY(hipStreamBeginCapture(b->s0, hipStreamCaptureModeThreadLocal), "begin capture");
     auto abort_capture = [&](int code, const std::string& msg) {
         hipGraph_t g = nullptr;
@@ -167,8 +236,8 @@ int capture(mdgpu_batch* b, const double* x, double* f) {
 int run_serial(mdgpu_batch* b, const double* x, mdgpu_energies* out_e, double* f) {
     for (size_t i = 0; i < b->sys.size(); ++i) {
         mdgpu_ctx* c = b->sys[i];
-        c->split = b->split;
-        c->fft_lines = b->fft_lines;
+        c->split = b->sys_split[i];
+        c->fft_lines = b->sys_fft[i];
         const int st = mdg::ctx_eval_device(c, x + 3 * b->off[i], b->box[i].data(), out_e ? &out_e[i] : nullptr,
                                             f ? f + 3 * b->off[i] : nullptr);
         if (st != MDGPU_OK) return berr(b, st, "system " + std::to_string(i) + ": " + c->error);
@@ -190,22 +259,7 @@ int run_graph(mdgpu_batch* b, const double* x, mdgpu_energies* out_e, double* f)
     B_TRY(hipGraphLaunch(b->exec, b->s0), "graph launch");
     B_TRY(hipMemcpyAsync(b->h_all, b->d_all, n * kRow * sizeof(double), hipMemcpyDeviceToHost, b->s0), "read rows");
     B_TRY(hipStreamSynchronize(b->s0), "batch");
-    for (size_t i = 0; i < n; ++i) {
-        mdgpu_ctx* c = b->sys[i];
-        const double* row = b->h_all + i * kRow;
-        int st = mdg::ctx_check(c, row);
-        if (st == mdg::kEvalRedo) {
-            // adjusted (larger list or general search): redo alone; the graph is stale
-            b->dirty = true;
-            st = mdg::ctx_eval_device(c, x + 3 * b->off[i], b->box[i].data(), out_e ? &out_e[i] : nullptr,
-                                      f ? f + 3 * b->off[i] : nullptr);
-            if (st != MDGPU_OK) return berr(b, st, "system " + std::to_string(i) + ": " + c->error);
-            continue;
-        }
-        if (st != MDGPU_OK) return berr(b, st, "system " + std::to_string(i) + ": " + c->error);
-        mdg::ctx_outputs(c, row, out_e ? &out_e[i] : nullptr, nullptr);
-    }
-    return MDGPU_OK;
+    return finish_rows(b, x, out_e, f);
 }
 
 }  // namespace
@@ -285,6 +339,8 @@ int mdgpu_batch_add(mdgpu_batch* b, const mdgpu_system_desc* sys, const mdgpu_pa
     }
     if (index) *index = static_cast<int>(b->sys.size());
     b->sys.push_back(c);
+    b->sys_split.push_back(b->split);
+    b->sys_fft.push_back(b->fft_lines);
     b->box.push_back({box[0], box[1], box[2]});
     b->off.push_back(b->n_atoms);
     b->n_atoms += sys->n_atoms;
@@ -296,7 +352,9 @@ int mdgpu_batch_eval_device(mdgpu_batch* b, const double* d_coords, mdgpu_energi
     if (!b || !d_coords) return berr(b, MDGPU_ERR_ARG, "null argument");
     if (b->sys.empty()) return berr(b, MDGPU_ERR_STATE, "the batch has no systems");
     B_TRY(hipSetDevice(b->device), "hipSetDevice");
-    return b->mode == 1 ? run_serial(b, d_coords, out_e, d_forces) : run_graph(b, d_coords, out_e, d_forces);
+    if (b->mode == 1) return run_serial(b, d_coords, out_e, d_forces);
+    if (b->mode == 2) return run_streams(b, d_coords, out_e, d_forces);
+    return run_graph(b, d_coords, out_e, d_forces);
 }
 
 int mdgpu_batch_eval(mdgpu_batch* b, const double* coords, mdgpu_energies* out_e, double* out_forces) {
@@ -324,15 +382,21 @@ int mdgpu_batch_eval(mdgpu_batch* b, const double* coords, mdgpu_energies* out_e
 
 int mdgpu_batch_configure(mdgpu_batch* b, int mode, int streams, int split, int fft_lines) {
     if (!b) return MDGPU_ERR_ARG;
-    if (mode < 0 || mode > 1 || streams < 1 || streams > kMaxStreams || split < 1 || split > 8 || fft_lines < 0 ||
+    if (mode < 0 || mode > 2 || streams < 1 || streams > kMaxStreams || split < 1 || split > 8 || fft_lines < 0 ||
         fft_lines > 16 || (fft_lines & (fft_lines - 1)) != 0)
-        return berr(b, MDGPU_ERR_ARG, "mode 0..1, streams 1..8, split 1..8, fft_lines 0 or a power of two <= 16");
+        return berr(b, MDGPU_ERR_ARG, "mode 0..2, streams 1..8, split 1..8, fft_lines 0 or a power of two <= 16");
     if (b->graph_state != 0 && (streams != b->streams || split != b->split || fft_lines != b->fft_lines))
         return berr(b, MDGPU_ERR_STATE, "launch settings are fixed once the batch graph exists; create a new batch");
+    const bool reset = split != b->split || fft_lines != b->fft_lines;
     b->mode = mode;
     b->streams = streams;
     b->split = split;
     b->fft_lines = fft_lines;
+    // new batch-wide settings replace per-class ones; unchanged ones keep them
+    if (reset) {
+        std::fill(b->sys_split.begin(), b->sys_split.end(), split);
+        std::fill(b->sys_fft.begin(), b->sys_fft.end(), fft_lines);
+    }
     return MDGPU_OK;
 }
 
@@ -342,37 +406,74 @@ int mdgpu_batch_tune(mdgpu_batch* b, const double* d_coords, double* d_forces, i
     if (b->sys.empty()) return berr(b, MDGPU_ERR_STATE, "the batch has no systems");
     if (b->graph_state != 0) return berr(b, MDGPU_ERR_STATE, "tune before the first graph evaluation");
     B_TRY(hipSetDevice(b->device), "hipSetDevice");
-    // pair-tile split and FFT block shape: each candidate times `reps` serial
-    // passes over all systems (no graph); the graph then runs one branch per
-    // stream with the systems dealt over min(8, systems) streams
+    const int n = static_cast<int>(b->sys.size());
+    b->streams = std::min(kMaxStreams, n);
+    // size classes: systems by atom count, cut into up to kClasses of equal count
+    constexpr int kClasses = 4;
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int c) { return b->sys[a]->topo.n_atoms < b->sys[c]->topo.n_atoms; });
+    const int nclass = std::min(kClasses, n);
+    std::vector<std::vector<int>> cls(nclass);
+    for (int k = 0; k < n; ++k) cls[static_cast<size_t>(k) * nclass / n].push_back(order[k]);
+    // one pass with the graph's launches and streams, timed
+    int st = run_serial(b, d_coords, nullptr, d_forces);  // settles list sizes
+    if (st != MDGPU_OK) return st;
+    auto timed = [&](double& per) {
+        int s = run_streams(b, d_coords, nullptr, d_forces);  // settles lists for these settings
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int r = 0; r < reps && s == MDGPU_OK; ++r) s = run_streams(b, d_coords, nullptr, d_forces);
+        per = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
+        return s;
+    };
+    double best = 0;
+    st = timed(best);
+    if (st != MDGPU_OK) return st;
+    // one sweep over the classes, largest first: each tries every pair-tile
+    // split (1, 2, 4, 8) and FFT block shape (16, 8, 4 lines) with the other
+    // classes at their current choice, and keeps the fastest
     const int cand_split[] = {1, 2, 4, 8};
     const int cand_fft[] = {0, 8, 4};
-    double best = 1e300;
-    int bp = b->split, bf = b->fft_lines;
-    for (int p : cand_split)
-        for (int f : cand_fft) {
-            b->split = p;
-            b->fft_lines = f;
-            int st = run_serial(b, d_coords, nullptr, d_forces);  // settles list sizes
-            if (st != MDGPU_OK) return st;
-            const auto t0 = std::chrono::steady_clock::now();
-            for (int r = 0; r < reps && st == MDGPU_OK; ++r) st = run_serial(b, d_coords, nullptr, d_forces);
-            if (st != MDGPU_OK) return st;
-            const double per =
-                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
-            if (per < best) {
-                best = per;
-                bp = p;
-                bf = f;
+    for (int k = nclass - 1; k >= 0; --k) {
+        const int bp0 = b->sys_split[cls[k][0]], bf0 = b->sys_fft[cls[k][0]];
+        int bp = bp0, bf = bf0;
+        for (int p : cand_split)
+            for (int f : cand_fft) {
+                if (p == bp0 && f == bf0) continue;
+                for (int i : cls[k]) {
+                    b->sys_split[i] = p;
+                    b->sys_fft[i] = f;
+                }
+                double per = 0;
+                st = timed(per);
+                if (st != MDGPU_OK) return st;
+                if (per < best) {
+                    best = per;
+                    bp = p;
+                    bf = f;
+                }
             }
+        for (int i : cls[k]) {
+            b->sys_split[i] = bp;
+            b->sys_fft[i] = bf;
         }
-    b->split = bp;
-    b->fft_lines = bf;
-    b->streams = std::min(kMaxStreams, static_cast<int>(b->sys.size()));
+    }
+    // the largest class's choice is reported as the batch's
+    const int big = cls[nclass - 1][0];
+    b->split = b->sys_split[big];
+    b->fft_lines = b->sys_fft[big];
     if (streams) *streams = b->streams;
-    if (split) *split = bp;
-    if (fft_lines) *fft_lines = bf;
+    if (split) *split = b->split;
+    if (fft_lines) *fft_lines = b->fft_lines;
     if (ms) *ms = best;
+    return MDGPU_OK;
+}
+
+int mdgpu_batch_system_config(const mdgpu_batch* b, int index, int* split, int* fft_lines) {
+    if (!b || index < 0 || index >= static_cast<int>(b->sys.size())) return MDGPU_ERR_ARG;
+    if (split) *split = b->sys_split[index];
+    if (fft_lines) *fft_lines = b->sys_fft[index];
     return MDGPU_OK;
 }
 

@@ -171,6 +171,7 @@ def gpu_lib() -> C.CDLL:
         lib.mdgpu_batch_configure.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
         lib.mdgpu_batch_tune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                          C.POINTER(C.c_int), C.POINTER(C.c_int), P_F64]
+        lib.mdgpu_batch_system_config.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         lib.mdgpu_batch_size.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
         lib.mdgpu_batch_graph_state.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         _gpu = lib
@@ -711,7 +712,13 @@ class Batch:
                                               C.c_void_p(forces_ptr) if forces_ptr else None, reps,
                                               C.byref(s), C.byref(p), C.byref(fl), C.byref(ms)))
         self._settings = {"streams": s.value, "split": p.value, "fft_lines": fl.value}
-        return {"streams": s.value, "split": p.value, "fft_lines": fl.value, "ms_serial_pass": ms.value}
+        return {"streams": s.value, "split": p.value, "fft_lines": fl.value, "ms_pass": ms.value}
+
+    def system_config(self, index: int) -> tuple:
+        """(pair-tile split, FFT lines) of system `index`."""
+        p, f = C.c_int(), C.c_int()
+        self._check(self.lib.mdgpu_batch_system_config(self.b, index, C.byref(p), C.byref(f)))
+        return p.value, f.value
 
     def graph_state(self) -> int:
         """0 no graph yet, 1 graph in use, 2 graph stale (serial evaluation)."""

@@ -93,9 +93,10 @@ def test_graph_batch_matches_separate_contexts(prec):
     batch.close()
 
 
-@pytest.mark.skip(reason="re-capturing the graph of a batch (the former graph-timing tuner did it) ended the "
+@pytest.mark.skip(reason="re-capturing the graph of a batch (a former graph-timing tuner did it) ended the "
                          "process with a host segmentation fault on the MI355X twice; cause not found, the "
-                         "re-capture path is switched off (a batch captures once) and the tuner times serial passes")
+                         "re-capture path is switched off (a batch captures once) and the tuner times directly "
+                         "issued passes")
 def test_tuned_batch_and_host_path():
     dev = torch.device("cuda:0")
     work, descs, keep, batch, x = _setup(N.PREC_MIXED, n_sys=5, seed0=7100)
@@ -116,24 +117,33 @@ def test_tuned_batch_and_host_path():
     batch.close()
 
 
-def test_serial_tuner_then_single_capture_host_path():
-    """Tune before the capture, capture once from the host path, replay, then a
-    call with other buffers: the graph goes stale and the batch evaluates
-    serially.  With the tuned launch shape the graph equals the serial mode bit
-    for bit; against separate contexts (default shape) the FP32 partial sums of
-    the i-forces are grouped differently, so the bar there is 2e-6."""
+def test_class_tuner_then_single_capture_host_path():
+    """Tune per size class before the capture, capture once from the host path,
+    replay, then a call with other buffers: the graph goes stale and the batch
+    evaluates serially.  With the tuned per-system launch shapes the graph,
+    the directly issued passes (mode 2) and the serial mode agree bit for bit;
+    against separate contexts (default shape) the FP32 partial sums of the
+    i-forces are grouped differently, so the bar there is 2e-6."""
     dev = torch.device("cuda:0")
     work, descs, keep, batch, x = _setup(N.PREC_MIXED, n_sys=5, seed0=7100)
     xd = torch.tensor(x, dtype=torch.float64, device=dev)
     fd = torch.zeros_like(xd)
-    print("case serial tuner, 5 systems", flush=True)
+    print("case class tuner, 5 systems", flush=True)
     cfg = batch.tune(xd.data_ptr(), fd.data_ptr(), reps=1)
-    print(f"  tuned {cfg}")
+    per_sys = [batch.system_config(i) for i in range(len(batch))]
+    print(f"  tuned {cfg}, per system {per_sys}")
     assert cfg["split"] in (1, 2, 4, 8) and cfg["fft_lines"] in (0, 4, 8) and cfg["streams"] == 5
+    assert all(p in (1, 2, 4, 8) and f in (0, 4, 8) for p, f in per_sys)
     assert batch.graph_state() == 0
     batch.configure(mode=1, streams=cfg["streams"], split=cfg["split"], fft_lines=cfg["fft_lines"])
+    assert [batch.system_config(i) for i in range(len(batch))] == per_sys  # unchanged settings keep the classes'
     e_ser, f_ser = batch.evaluate(x)
     assert batch.graph_state() == 0
+    batch.configure(mode=2, streams=cfg["streams"], split=cfg["split"], fft_lines=cfg["fft_lines"])
+    e_str, f_str = batch.evaluate(x)
+    assert batch.graph_state() == 0
+    assert np.array_equal(f_str, f_ser)
+    _same(e_str, e_ser, f_ser, f_ser, exact=True)
     batch.configure(mode=0, streams=cfg["streams"], split=cfg["split"], fft_lines=cfg["fft_lines"])
     e1, f1 = batch.evaluate(x)
     assert batch.graph_state() == 1
