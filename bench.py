@@ -223,7 +223,8 @@ def bench_batched(torch, N, systems, n_systems=256, reps=3):
     return {"systems": n_systems, "atoms": n_atoms, "setup_s": setup_s, "tuned": cfg, "tuned_classes": classes,
             "form": "one hipGraph launch: every system's kernel chain (sort, search, pair + bonded, PME, "
                     "reduction) captured once, dealt largest first over 8 streams, then one gathering kernel",
-            "graph_state": state,
+            "graph_state": state, "tuned_mode": {0: "graph", 2: "direct launches"}.get(cfg["mode"], "serial"),
+            "tuned_mode_systems_per_s": n_systems / (graph_s if cfg["mode"] == 0 else streams_s),
             "graph_systems_per_s": n_systems / graph_s, "serial_systems_per_s": n_systems / serial_s,
             "streams_systems_per_s": n_systems / streams_s,
             "graph_ms_per_batch": 1e3 * graph_s, "serial_ms_per_batch": 1e3 * serial_s,

@@ -289,11 +289,16 @@ int mdgpu_batch_configure(mdgpu_batch* b, int mode, int streams, int split, int 
  * every pair-tile split (1, 2, 4, 8) and FFT block shape (16, 8, 4 lines) is
  * timed over `reps` passes of the whole batch with the graph's launches and
  * streams issued directly (mode 2), the other classes at their current
- * choice, and the fastest is kept for that class.  Returns the streams, the
+ * choice, and the fastest is kept for that class.  Unless the batch is in
+ * the serial mode, it then captures the graph (once, on these buffers), times
+ * `reps` graph launches against `reps` direct passes and keeps the faster form
+ * as the batch's mode (0 or 2; mdgpu_batch_mode).  Returns the streams, the
  * largest class's settings and the best pass time (ms);
  * mdgpu_batch_system_config gives every system's. */
 int mdgpu_batch_tune(mdgpu_batch* b, const double* d_coords, double* d_forces, int reps, int* streams, int* split,
                      int* fft_lines, double* ms);
+/* The batch's execution mode (0 graph, 1 serial, 2 direct launches). */
+int mdgpu_batch_mode(const mdgpu_batch* b, int* mode);
 /* Pair-tile split and FFT lines of system `index` (no counterpart in the
  * reference). */
 int mdgpu_batch_system_config(const mdgpu_batch* b, int index, int* split, int* fft_lines);

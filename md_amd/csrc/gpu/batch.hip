@@ -463,10 +463,37 @@ int mdgpu_batch_tune(mdgpu_batch* b, const double* d_coords, double* d_forces, i
     const int big = cls[nclass - 1][0];
     b->split = b->sys_split[big];
     b->fft_lines = b->sys_fft[big];
+    // graph or direct launches: capture the graph once with these shapes,
+    // time both forms and keep the faster as the batch's mode
+    if (b->mode != 1) {
+        st = capture(b, d_coords, d_forces);
+        if (st != MDGPU_OK) return st;
+        b->graph_state = 1;
+        auto timed_mode = [&](int mode, double& per) {
+            b->mode = mode;
+            int s = mdgpu_batch_eval_device(b, d_coords, nullptr, d_forces);
+            const auto t0 = std::chrono::steady_clock::now();
+            for (int r = 0; r < reps && s == MDGPU_OK; ++r) s = mdgpu_batch_eval_device(b, d_coords, nullptr, d_forces);
+            per = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
+            return s;
+        };
+        double t_graph = 0, t_direct = 0;
+        st = timed_mode(0, t_graph);
+        if (st == MDGPU_OK) st = timed_mode(2, t_direct);
+        if (st != MDGPU_OK) return st;
+        b->mode = t_graph <= t_direct ? 0 : 2;
+        best = std::min(t_graph, t_direct);
+    }
     if (streams) *streams = b->streams;
     if (split) *split = b->split;
     if (fft_lines) *fft_lines = b->fft_lines;
     if (ms) *ms = best;
+    return MDGPU_OK;
+}
+
+int mdgpu_batch_mode(const mdgpu_batch* b, int* mode) {
+    if (!b || !mode) return MDGPU_ERR_ARG;
+    *mode = b->mode;
     return MDGPU_OK;
 }
 

@@ -172,6 +172,7 @@ def gpu_lib() -> C.CDLL:
         lib.mdgpu_batch_tune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                          C.POINTER(C.c_int), C.POINTER(C.c_int), P_F64]
         lib.mdgpu_batch_system_config.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        lib.mdgpu_batch_mode.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         lib.mdgpu_batch_size.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
         lib.mdgpu_batch_graph_state.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         _gpu = lib
@@ -712,7 +713,14 @@ class Batch:
                                               C.c_void_p(forces_ptr) if forces_ptr else None, reps,
                                               C.byref(s), C.byref(p), C.byref(fl), C.byref(ms)))
         self._settings = {"streams": s.value, "split": p.value, "fft_lines": fl.value}
-        return {"streams": s.value, "split": p.value, "fft_lines": fl.value, "ms_pass": ms.value}
+        return {"streams": s.value, "split": p.value, "fft_lines": fl.value, "ms_pass": ms.value,
+                "mode": self.mode()}
+
+    def mode(self) -> int:
+        """0 graph, 1 serial, 2 the graph's launches issued directly."""
+        m = C.c_int()
+        self._check(self.lib.mdgpu_batch_mode(self.b, C.byref(m)))
+        return m.value
 
     def system_config(self, index: int) -> tuple:
         """(pair-tile split, FFT lines) of system `index`."""
