@@ -215,6 +215,7 @@ struct ClusterOut {
     double4* origin;
     float4* half;
     PmeSlots pme;
+    int n_atoms;             // atoms of the system (slot records outside [0, n_atoms) are padding)
 };
 
 constexpr int kSortThreads = 256;
@@ -313,7 +314,10 @@ __global__ __launch_bounds__(kSortThreads) void k_col_sort(int ncol, int* __rest
         const int slot = cl0 * kCluster + k;
         const int cl = slot / kCluster;
         const bool live = k < nslots && cl < o.max_clusters;
-        const int atom = live && k < n ? s_sorted[k] : -1;
+        int atom = live && k < n ? s_sorted[k] : -1;
+        // the ranks are a permutation of the column's keys (each atom's key
+        // is placed once); an index outside the system never reaches memory
+        if (atom >= o.n_atoms) atom = -1;
         double x = 0, y = 0, z = 0;
         if (atom >= 0) {
             x = s_x[k];
@@ -1169,7 +1173,7 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl
     if (mixed) {
         pme.w = w.slot_pme_wf;
         const ClusterOut<float> o{c->n_clusters, t.charge, t.type, t.n_types, w.slot_atom, w.atom_slot, w.cluster_col,
-                                  w.slot_pos_f, w.slot_meta, w.cl_origin, w.cl_half, pme};
+                                  w.slot_pos_f, w.slot_meta, w.cl_origin, w.cl_half, pme, n};
         hipLaunchKernelGGL(k_col_sort<float>, dim3(ncol), dim3(kSortThreads), 0, c->stream, ncol, w.col_count,
                            w.col_ticket, w.col_cstart, w.col_key, w.wrapped, ex, nblk, o, vs);
         if (vs)
@@ -1178,7 +1182,7 @@ hipError_t launch_sort(mdgpu_ctx* c, const double* coords, bool mixed, bool excl
     } else {
         pme.w = w.slot_pme_wd;
         const ClusterOut<double> o{c->n_clusters, t.charge, t.type, t.n_types, w.slot_atom, w.atom_slot, w.cluster_col,
-                                   w.slot_pos_d, w.slot_meta, w.cl_origin, w.cl_half, pme};
+                                   w.slot_pos_d, w.slot_meta, w.cl_origin, w.cl_half, pme, n};
         hipLaunchKernelGGL(k_col_sort<double>, dim3(ncol), dim3(kSortThreads), 0, c->stream, ncol, w.col_count,
                            w.col_ticket, w.col_cstart, w.col_key, w.wrapped, ex, nblk, o, vs);
         if (vs)
