@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kWaves * kWave) void k_pair_adjoint(const PairAdjAr
     const int lane = threadIdx.x & 63, ti = lane & 7, tj = lane >> 3;
     const int ncl = min(*a.ncl_dev, a.max_clusters);
     const int nwork = ncl * a.split;
-    const int mbit = 8 * ti + tj;
+    const int mbit = mask_bit(ti, tj);
     for (int gw0 = blockIdx.x * kWaves; gw0 < nwork; gw0 += gridDim.x * kWaves) {
         const int gw = __builtin_amdgcn_readfirstlane(gw0 + (threadIdx.x >> 6));
         if (gw >= nwork) continue;  // uniform over the wave; no barrier inside the loop

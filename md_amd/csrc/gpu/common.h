@@ -16,6 +16,20 @@ constexpr double kPi = 3.14159265358979323846;
 // cutoff from it, and it has zero charge and LJ parameters.
 constexpr double kFarAway = 1.0e5;
 
+// Bit of a tile's 64-bit pair mask for i-slot ti and j-slot tj.  It is the
+// lane that computes the pair in the pair kernels (lane = 8 tj + ti), so a
+// tile's mask is a lane mask of the wave and is applied with scalar ANDs.
+__host__ __device__ constexpr int mask_bit(int ti, int tj) { return kCluster * tj + ti; }
+// Mask of a cluster's tile with itself: the pairs with tj > ti.
+constexpr unsigned long long self_tile_mask() {
+    unsigned long long m = 0;
+    for (int ti = 0; ti < kCluster; ++ti)
+        for (int tj = ti + 1; tj < kCluster; ++tj) m |= 1ull << mask_bit(ti, tj);
+    return m;
+}
+constexpr unsigned long long kSelfMask = self_tile_mask();
+static_assert(kSelfMask == 0x7F3F1F0F07030100ull, "self-tile mask");
+
 // Periodic cell.  Orthorhombic (tri == 0): edge lengths l.  Triclinic: edge
 // vectors a = (ax, 0, 0), b = (bx, by, 0), c = (cx, cy, cz) (the columns of
 // H) and the upper-triangular R = H^-1, so that s = R x are the fractional
@@ -133,6 +147,24 @@ __device__ double llvm_buf_atomic_fadd_f64(double v, BufRsrc r, unsigned int off
 // Resource over `bytes` bytes from p; bytes must stay below kOffOOB.
 __device__ __forceinline__ BufRsrc buf_rsrc(void* p, unsigned int bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(p, 0, static_cast<int>(bytes), kBufWord3);
+}
+// s_waitcnt operand of gfx9: vmcnt(0), expcnt and lgkmcnt not waited for
+constexpr int kWaitVmcnt0 = 0x0F70;
+// Loads through a buffer resource with a scalar byte offset (soff, uniform
+// over the wave) and a vector one (off).
+typedef float mdg_v4f __attribute__((ext_vector_type(4)));
+typedef int mdg_v2i __attribute__((ext_vector_type(2)));
+__device__ mdg_v4f llvm_buf_load_v4f32(BufRsrc r, unsigned int off, int soff, int aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.load.v4f32");
+__device__ mdg_v2i llvm_buf_load_v2i32(BufRsrc r, unsigned int off, int soff, int aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.load.v2i32");
+__device__ __forceinline__ float4 buf_load_f4(BufRsrc r, unsigned int off, int soff) {
+    const mdg_v4f v = llvm_buf_load_v4f32(r, off, soff, 0);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ int2 buf_load_i2(BufRsrc r, unsigned int off, int soff) {
+    const mdg_v2i v = llvm_buf_load_v2i32(r, off, soff, 0);
+    return make_int2(v.x, v.y);
 }
 __device__ __forceinline__ void buf_add(BufRsrc r, unsigned int off, long long q) {
     llvm_buf_atomic_add_i64(q, r, off, 0, 0);

@@ -148,6 +148,50 @@ __device__ __forceinline__ double sum8(double v) {
     return v;
 }
 
+// j-force sums of a tile by reduce-scatter over the eight lanes of a row
+// (lanes 8 tj .. 8 tj + 7): lanes ti = 0, 1, 2 of the row end with the x, y
+// and z totals, the other lanes with partial sums.  Round 1 pairs lanes
+// l, l ^ 1 (lanes with e keep x and z, the others y), round 2 pairs l, l ^ 2
+// (one value kept each), round 3 adds the mirrored lane 7 - l of the row's
+// half, which holds the same component: the roles in the upper four lanes of
+// the eight are swapped (the e and h patterns below) so that lanes 0, 1, 2
+// meet lanes 7, 6, 5.  Four DPP adds and four selects instead of nine DPP
+// adds and two selects for three full sums.
+struct JSum {
+    bool e, h;
+};
+__device__ __forceinline__ JSum jsum_lanes(int lane) {
+    return {((lane ^ (lane >> 2)) & 1) == 0, (((lane >> 1) ^ (lane >> 2)) & 1) != 0};
+}
+__device__ __forceinline__ float dpp_f(float v, int which) {
+    const int b = __float_as_int(v);
+    return __int_as_float(which == 0 ? dpp_xor1(b) : (which == 1 ? dpp_xor2(b) : dpp_hmir(b)));
+}
+__device__ __forceinline__ double dpp_f(double v, int which) { return dpp_d(v, which); }
+template <typename real>
+__device__ __forceinline__ real jsum3(real x, real y, real z, const JSum& s) {
+    const real k1 = s.e ? x : y, s1 = s.e ? y : x;
+    const real a = k1 + dpp_f(s1, 0);
+    const real b = z + dpp_f(z, 0);
+    const real k2 = s.h ? b : a, s2 = s.h ? a : b;
+    const real c = k2 + dpp_f(s2, 1);
+    return c + dpp_f(c, 2);
+}
+// lanes ti = 0, 1, 2 of every row: the lanes that add a j-force component
+constexpr unsigned long long kJLanes = 0x0707070707070707ull;
+
+// fixed32_f32 without a branch: a contribution that does not fit adds 0 and
+// marks its lane in `bad` (lanes outside `lanes` are not checked).
+__device__ __forceinline__ long long fixed32_f32_flag(float v, unsigned long long& bad, unsigned long long lanes) {
+    const bool ok = fabsf(v) < 2147483520.0f;  // false for NaN
+    bad |= __builtin_amdgcn_ballot_w64(!ok) & lanes;
+    const float w = ok ? v : 0.0f;
+    const float h = floorf(w);
+    const unsigned int lo = static_cast<unsigned int>(fminf((w - h) * 4294967296.0f, 4294967040.0f));
+    const unsigned long long hi = static_cast<unsigned long long>(static_cast<unsigned int>(static_cast<int>(h)));
+    return static_cast<long long>((hi << 32) | lo);
+}
+
 template <typename real>
 __device__ __forceinline__ real bcast(real v, int lane);
 template <>
@@ -207,7 +251,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 && kAmber && !kCount ? 6 : 
     const int lane = threadIdx.x & 63;
     const int ti = lane & 7, tj = lane >> 3;
     const int ncl = min(*a.ncl_dev, a.max_clusters);
-    const int mbit = 8 * ti + tj;  // mask bit of this lane's pair (i-major, as the search writes it)
+    const int mbit = mask_bit(ti, tj);  // mask bit of this lane's pair: the lane itself
+    const JSum js = jsum_lanes(lane);
 
     // energy accumulators: FP64 form in double; mixed form in float, flushed to
     // double every 32 tiles
@@ -333,10 +378,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 && kAmber && !kCount ? 6 : 
                 Payload<real> q = 0;
                 unsigned int off = kOffOOB;
                 if (__ballot(act) != 0ull) {
-                    const real gx = sum8(fx), gy = sum8(fy), gz = sum8(fz);
+                    const real gj = jsum3(fx, fy, fz, js);
                     const int cjt = __builtin_amdgcn_readlane(e_j, t);
-                    q = force_payload<real, kAmber>(a, ti < 3 && mcur.y >= 0, 3 * (cjt * kCluster + tj) + ti,
-                                                    -(ti == 0 ? gx : (ti == 1 ? gy : gz)), off);
+                    q = force_payload<real, kAmber>(a, ti < 3 && mcur.y >= 0, 3 * (cjt * kCluster + tj) + ti, -gj,
+                                                    off);
                 }
                 buf_add(rsrc, off, q);
                 if (++flush == 32) {
@@ -358,6 +403,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 && kAmber && !kCount ? 6 : 
             int2 mA, mB;
             fetch(0, pA, mA);
             fetch(1, pB, mB);
+            // complete before the loop (see k_nonbonded_c)
+            __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
             for (int t = 0; t < nb; t += 2) {
                 tile(pA, mA, t, true);
                 fetch(t + 2, pA, mA);
@@ -441,7 +488,7 @@ k_nonbonded_c(const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int 
     const int lane = threadIdx.x & 63;
     const int ti = lane & 7, tj = lane >> 3;
     const int ncl = min(*a.ncl_dev, a.max_clusters);
-    const int mbit = 8 * ti + tj;
+    const JSum js = jsum_lanes(lane);
     double e_vdw = 0, e_ew = 0, n_pairs = 0;
     float s_vdw = 0, s_ew = 0;
     double w_d[6] = {0, 0, 0, 0, 0, 0};
@@ -450,6 +497,17 @@ k_nonbonded_c(const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int 
     const float rc2 = a.rc2_lj > a.rc2_ee ? a.rc2_lj : a.rc2_ee;
     int flush = 0;
     const BufRsrc rsrc = buf_rsrc(a.fx.f, 8u * a.n3);
+    // j-cluster data through buffer loads: the cluster's offset is a scalar,
+    // the lane's slot a constant vector offset
+    const unsigned int n_slots = static_cast<unsigned int>(a.n3 / 3);
+    const BufRsrc pos_rs = buf_rsrc(const_cast<float4*>(a.pos), 16u * n_slots);
+    const BufRsrc meta_rs = buf_rsrc(const_cast<int2*>(a.meta), 8u * n_slots);
+    const unsigned int pos_off = 16u * tj, meta_off = 8u * tj;
+    // byte offset of this lane's j-force component within its j-cluster's
+    // 24 accumulators (lanes ti = 0, 1, 2 add x, y, z); the other lanes add
+    // nothing
+    const unsigned int joff = ti < 3 ? 8u * static_cast<unsigned int>(3 * tj + ti) : kOffOOB;
+    unsigned long long bad = 0;  // lanes whose j-force did not fit the fixed point
     const int blk = xcd_block(blockIdx.x, npb);
     const int item = __builtin_amdgcn_readfirstlane(blk * kWaves + (threadIdx.x >> 6));
     if (item < ncl * a.split) {
@@ -474,16 +532,20 @@ k_nonbonded_c(const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int 
                 e_d = a.list_delta[base + kb + lane];
             }
             const int e_mlo = static_cast<int>(e_m & 0xffffffffull), e_mhi = static_cast<int>(e_m >> 32);
+            // j-cluster data of tile t (clamped to the batch, so every fetch
+            // is issued and reads a valid cluster)
             auto fetch = [&](int t, float4& p, int2& m) {
                 const int cj = __builtin_amdgcn_readlane(e_j, min(t, nb - 1));
-                p = a.pos[cj * kCluster + tj];
-                m = a.meta[cj * kCluster + tj];
+                p = buf_load_f4(pos_rs, pos_off, cj * static_cast<int>(16 * kCluster));
+                m = buf_load_i2(meta_rs, meta_off, cj * static_cast<int>(8 * kCluster));
             };
             struct Geo {
                 float dx, dy, dz;
-                bool act;
+                unsigned long long act;  // lanes whose pair is unmasked and within the cutoff
             };
-            // separation of this lane's pair in tile t; packs it if in range
+            // separation of this lane's pair in tile t (the tile's mask is a
+            // lane mask, so the in-range test combines with it in scalar
+            // registers)
             auto geom = [&](const float4& pc, const int2& mc, int t, bool live, float& r2, int& tp, float& qq) {
                 unsigned long long mask =
                     (static_cast<unsigned long long>(static_cast<unsigned int>(__builtin_amdgcn_readlane(e_mhi, t))) << 32) |
@@ -494,7 +556,8 @@ k_nonbonded_c(const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int 
                 g.dy = pi.y - pc.y + bcast<float>(e_d.y, t);
                 g.dz = pi.z - pc.z + bcast<float>(e_d.z, t);
                 r2 = g.dx * g.dx + g.dy * g.dy + g.dz * g.dz;
-                g.act = ((mask >> mbit) & 1ull) && r2 <= rc2 && r2 >= 1e-12f;
+                // two ballots of plain compares: the masks stay in scalar registers
+                g.act = __builtin_amdgcn_ballot_w64(r2 <= rc2) & __builtin_amdgcn_ballot_w64(r2 >= 1e-12f) & mask;
                 tp = tyi + mc.x;
                 qq = qi * pc.w;
                 return g;
@@ -538,11 +601,12 @@ k_nonbonded_c(const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int 
                 }
                 return f;
             };
-            // force of tile t from its factor: i-forces, j-force sums and atomics
+            // force of tile t from its factor: i-forces, j-force sums and the
+            // j-force atomic (issued by every tile; an empty tile adds nothing)
             auto finish = [&](const Geo& g, float f, int t, const int2& mc) {
                 const float fx = f * g.dx, fy = f * g.dy, fz = f * g.dz;
                 if constexpr (kCount) {
-                    if (g.act && mc.y >= 0 && mi.y >= 0) n_pairs += 1.0;
+                    if (__builtin_amdgcn_inverse_ballot_w64(g.act) && mc.y >= 0 && mi.y >= 0) n_pairs += 1.0;
                     w_s[0] += fx * g.dx;
                     w_s[1] += fy * g.dy;
                     w_s[2] += fz * g.dz;
@@ -555,11 +619,13 @@ k_nonbonded_c(const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int 
                 fiz += fz;
                 long long q = 0;
                 unsigned int off = kOffOOB;
-                if (__ballot(g.act) != 0ull) {
-                    const float gx = sum8(fx), gy = sum8(fy), gz = sum8(fz);
+                if (g.act != 0ull) {
+                    // a padding j-slot has no pair in range, so its sum is 0
+                    // and it adds nothing
+                    const float gj = jsum3(fx, fy, fz, js);
                     const int cjt = __builtin_amdgcn_readlane(e_j, t);
-                    q = force_payload<float, true>(a, ti < 3 && mc.y >= 0, 3 * (cjt * kCluster + tj) + ti,
-                                                   -(ti == 0 ? gx : (ti == 1 ? gy : gz)), off);
+                    q = fixed32_f32_flag(-gj, bad, kJLanes);
+                    off = q != 0 ? joff + static_cast<unsigned int>(cjt) * (3u * kCluster * 8u) : kOffOOB;
                 }
                 buf_add(rsrc, off, q);
             };
@@ -568,6 +634,13 @@ k_nonbonded_c(const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int 
             int2 mA, mB;
             fetch(0, pA, mA);
             fetch(1, pB, mB);
+            // The first two tiles' data, complete before the loop: the
+            // compiler places its waits for the loop's registers from the
+            // state at the loop header, and a load still in flight there
+            // would put a wait for every vector memory operation but the
+            // newest four (the previous step's atomics included) into every
+            // step.
+            __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
             for (int t = 0; t < nb; t += 2) {
                 float r2A, r2B, qqA, qqB;
                 int tpA, tpB;
@@ -577,23 +650,24 @@ k_nonbonded_c(const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int 
                 const Geo gB = geom(pB, mB, t + 1, t + 1 < nb, r2B, tpB, qqB);
                 const int2 mcB = mB;
                 fetch(t + 3, pB, mB);
-                const unsigned long long bA = __ballot(gA.act), bB = __ballot(gB.act);
-                const int nA = __popcll(bA);
-                const int total = nA + __popcll(bB);
+                const int nA = __popcll(gA.act);
+                const int total = nA + __popcll(gB.act);
                 float fA = 0, fB = 0;
                 if (total > 0) {
-                    const int sA = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned int>(bA >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(bA), 0u));
-                    const int sB = nA + __builtin_amdgcn_mbcnt_hi(static_cast<unsigned int>(bB >> 32),
-                                                                  __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(bB), 0u));
-                    if (gA.act) s_pack[sA] = make_float4(r2A, qqA, __int_as_float(tpA), 0.0f);
-                    if (gB.act) s_pack[sB] = make_float4(r2B, qqB, __int_as_float(tpB), 0.0f);
+                    const bool actA = __builtin_amdgcn_inverse_ballot_w64(gA.act);
+                    const bool actB = __builtin_amdgcn_inverse_ballot_w64(gB.act);
+                    const int sA = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned int>(gA.act >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(gA.act), 0u));
+                    const int sB = nA + __builtin_amdgcn_mbcnt_hi(static_cast<unsigned int>(gB.act >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(gB.act), 0u));
+                    if (actA) s_pack[sA] = make_float4(r2A, qqA, __int_as_float(tpA), 0.0f);
+                    if (actB) s_pack[sB] = make_float4(r2B, qqB, __int_as_float(tpB), 0.0f);
                     wave_sync();
                     if (lane < total) s_pack[lane].w = pair(s_pack[lane]);
                     if (total > kWave && lane + kWave < total) s_pack[lane + kWave].w = pair(s_pack[lane + kWave]);
                     wave_sync();
-                    fA = gA.act ? s_pack[sA].w : 0.0f;
-                    fB = gB.act ? s_pack[sB].w : 0.0f;
+                    fA = actA ? s_pack[sA].w : 0.0f;
+                    fB = actB ? s_pack[sB].w : 0.0f;
                     wave_sync();
                 }
                 finish(gA, fA, t, mcA);
@@ -619,6 +693,7 @@ k_nonbonded_c(const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int 
         add_force<float, true>(a, rsrc, tj < 3 && k1 > k0 && mi.y >= 0, 3 * (ci * kCluster + ti) + tj,
                                tj == 0 ? fix : (tj == 1 ? fiy : fiz));
     }
+    if (bad != 0ull && lane == 0) atomicOr(a.fx.overflow, 1u);
     e_vdw += s_vdw;
     e_ew += s_ew;
     if constexpr (kCount) {

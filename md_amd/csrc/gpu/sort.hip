@@ -446,7 +446,6 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
     if (a.amber) rex = static_cast<double>(*a.excl_rmax) + 0.01;
     const double rex2 = rex * rex;
     int count = 0, nnear = 0;
-    const unsigned long long self_mask = 0x0080C0E0F0F8FCFEull;  // bit 8*ti+tj set iff tj > ti
     for (int dy = -a.dcy; dy <= a.dcy; ++dy) {
         const int jyr = cy + dy, ky = floordiv(jyr, a.ncy), jy = jyr - ky * a.ncy;
         const double gy = fmax(0.0, (abs(dy) - 1) * a.sy);
@@ -494,7 +493,7 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
                         const size_t e = base_i + pos;
                         a.list_j[e] = cj;
                         // masks of tiles that may hold excluded pairs are written below
-                        if (!near) a.list_mask[e] = triangle ? self_mask : ~0ull;
+                        if (!near) a.list_mask[e] = triangle ? kSelfMask : ~0ull;
                         R4 dl;
                         dl.x = static_cast<real>(-ddx);
                         dl.y = static_cast<real>(-ddy);
@@ -527,7 +526,7 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
     // exclusion masks of the tiles whose boxes are close enough to hold an
     // excluded pair: lane (ti, tj) looks its pair up in the CSR list.  The
     // entries come from LDS (written by this wave above), not back from memory.
-    const int ti = lane >> 3, tj = lane & 7;
+    const int ti = lane & 7, tj = lane >> 3;  // the ballot's bit of the pair is mask_bit(ti, tj)
     const int ai = a.slot_atom[ci * kCluster + ti];
     __builtin_amdgcn_wave_barrier();
     for (int q = 0; q < nnear; ++q) {
@@ -537,7 +536,7 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
         const unsigned long long bal = __ballot(ex);
         const bool shifted = (en.x >> 30) & 1;
         const int pos = en.x & ((1 << 30) - 1);
-        if (lane == 0) a.list_mask[base_i + pos] = ((en.y == ci && !shifted) ? self_mask : ~0ull) & ~bal;
+        if (lane == 0) a.list_mask[base_i + pos] = ((en.y == ci && !shifted) ? kSelfMask : ~0ull) & ~bal;
     }
 }
 
@@ -579,15 +578,14 @@ constexpr int kNearPre = 6;
 
 __device__ void near_tile_masks(const SearchArgs& a, int ci, int lane, const ExclRegs& xr, int (*s_ex)[kExclCap],
                                 const int2* near, int nnear, size_t base_i) {
-    const unsigned long long self_mask = 0x0080C0E0F0F8FCFEull;  // bit 8*ti+tj set iff tj > ti
     const int a_l = xr.a_l, en = xr.en;
-    const int ti = lane >> 3, tj = lane & 7;
-    // lane 8u + tj: atom tj of near tile 8g + u
+    const int ti = lane & 7, tj = lane >> 3;  // the ballot's bit of the pair is mask_bit(ti, tj)
+    // lane 8u + t: atom t of near tile 8g + u
     int aj_pre[kNearPre];
 #pragma unroll
     for (int g = 0; g < kNearPre; ++g) {
         const int qq = kCluster * g + (lane >> 3);
-        aj_pre[g] = qq < nnear ? a.slot_atom[near[qq].y * kCluster + tj] : -1;
+        aj_pre[g] = qq < nnear ? a.slot_atom[near[qq].y * kCluster + (lane & 7)] : -1;
     }
 #pragma unroll
     for (int t = 0; t < kCluster; ++t)
@@ -622,7 +620,7 @@ __device__ void near_tile_masks(const SearchArgs& a, int ci, int lane, const Exc
             if (q0 + u >= nnear) break;
             const unsigned long long bal = __ballot((exbits >> u) & 1u);
             const int2 en_q = near[q0 + u];
-            if (lane == 0) a.list_mask[base_i + en_q.x] = ((en_q.y == ci) ? self_mask : ~0ull) & ~bal;
+            if (lane == 0) a.list_mask[base_i + en_q.x] = ((en_q.y == ci) ? kSelfMask : ~0ull) & ~bal;
         }
     };
 #pragma unroll
@@ -630,7 +628,7 @@ __device__ void near_tile_masks(const SearchArgs& a, int ci, int lane, const Exc
         if (kCluster * g < nnear) group(kCluster * g, aj_pre[g]);
     for (int q0 = kCluster * kNearPre; q0 < nnear; q0 += kCluster) {
         const int qq = q0 + (lane >> 3);
-        group(q0, qq < nnear ? a.slot_atom[near[qq].y * kCluster + tj] : -1);
+        group(q0, qq < nnear ? a.slot_atom[near[qq].y * kCluster + (lane & 7)] : -1);
     }
 }
 
@@ -692,7 +690,6 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     double rex = 0.0;
     if (a.amber) rex = static_cast<double>(*a.excl_rmax) + 0.01;
     const double rex2 = rex * rex;
-    const unsigned long long self_mask = 0x0080C0E0F0F8FCFEull;  // bit 8*ti+tj set iff tj > ti
     ExclRegs xr{};
     if (a.amber) xr = excl_lists_load(a, ci, lane);
     // the i-cluster's atoms relative to its origin (used after the ranges)
@@ -828,7 +825,7 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
         if (cand && pos < a.cap) {
             const size_t e = base_i + pos;
             a.list_j[e] = cj;
-            if (!near) a.list_mask[e] = (cj == ci) ? self_mask : ~0ull;
+            if (!near) a.list_mask[e] = (cj == ci) ? kSelfMask : ~0ull;
             R4 dl;
             dl.x = static_cast<real>(-ddx);
             dl.y = static_cast<real>(-ddy);
@@ -1087,12 +1084,12 @@ __global__ __launch_bounds__(kPruneWaves * kWave) void k_prune(const PruneArgs a
             for (int u = 0; u < kCluster; ++u) {
 #pragma unroll
                 for (int k = 0; k < kCluster; ++k) {
-                    // bit 8 i + j of the mask, i the i-slot and j the j-slot
+                    // k the i-slot and u the j-slot of the pair
                     const real dx = pix[k] - pj[u].x + e_d.x;
                     const real dy = piy[k] - pj[u].y + e_d.y;
                     const real dz = piz[k] - pj[u].z + e_d.z;
                     const bool near = dx * dx + dy * dy + dz * dz <= rc2;
-                    live |= near & static_cast<bool>((e_m >> (kCluster * k + u)) & 1ull);
+                    live |= near & static_cast<bool>((e_m >> mask_bit(k, u)) & 1ull);
                 }
             }
         }
