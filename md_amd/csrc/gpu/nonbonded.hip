@@ -435,28 +435,40 @@ __attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 && kAmber && !kCount ? 6 : 
     }
 }
 
-// The FP32 AMBER pair kernel with in-range pairs compacted.  38 % of a tile's 64 pairs lie within the cutoff on average, and
-// k_nonbonded runs the whole LJ + switch + erfc path for a tile once any of
-// its pairs does.  Here a wave takes two tiles per step: every lane forms the
-// separation of its pair in both tiles (as in k_nonbonded), the in-range pairs
-// of the two tiles are packed into a per-wave LDS buffer (r^2, q_i q_j, type
-// pair), the pair terms run once per 64 packed pairs (one pass for up to 64,
-// two for up to 128), and each pair's force factor returns to the lane that
-// owns it through the same buffer.  The pair arithmetic is k_nonbonded's, so
-// each pair's force is the same; the i- and j-force paths are unchanged.
-// Energies are summed by the lane that computes the pair.  The statistics
-// instance (kCount: pair count and virial, formed by the lane that owns the
-// pair) sums the energies in the same order, so forming the virial leaves
-// energies and forces bit-identical.
-constexpr int kPackSlots = 2 * kWave;  // two tiles of packed pairs per wave
-// LDS of one block of the compacted kernel at most (else k_nonbonded runs:
-// about 55 LJ types)
+// The FP32 AMBER pair kernel with in-range pairs queued.  38 % of a tile's 64
+// pairs lie within the cutoff on average, and k_nonbonded runs the whole
+// LJ + switch + erfc path for a tile once any of its pairs does.  Here a wave
+// takes two tiles per step: every lane forms the separation of its pair in
+// both tiles (as in k_nonbonded), and the in-range pairs are appended to a
+// per-wave ring in LDS (separation, q_i q_j, type pair).  The pair terms run
+// over 64 queued pairs at a time, whenever 64 are queued, so nearly every
+// lane of a pass does useful work; the pass writes each pair's force back
+// over its separation.  A step's tiles are finished (i-forces, j-force sums
+// and the j-force atomic) in the next step, whose passes first take what is
+// left of them.  So the ring never holds more than the pairs of two steps
+// (256), and the finish order is the tile order.  Each
+// pair's force is k_nonbonded's: the same separation, r^2 formed by the same
+// fused operations, the same pair arithmetic.  Energies are summed by the lane
+// that runs the pair; the statistics instance (kCount: pair count and virial)
+// forms them in the same pass, so its energies and forces are bit-identical.
+//
+// A wave's share of an i-cluster's list is every split-th entry (part, part
+// + split, ...): the list runs from the near columns to the far ones, so
+// contiguous quarters differed by 2.4x in pairs and the waves of a block
+// waited for the densest.
+constexpr int kRing = 256;  // queued pairs per wave (a power of two)
+// LDS of one block of the queued kernel at most (else k_nonbonded runs:
+// about 40 LJ types)
 constexpr size_t kPackMaxLds = 64 * 1024;
 
-// LDS bytes of a block of the compacted kernel for ntp LJ types (padding type
-// included): the pair table and one pack buffer per wave.
-constexpr size_t compact_lds(int ntp) {
-    return sizeof(float4) * (static_cast<size_t>(ntp) * ntp + kWavesOf<float> * kPackSlots);
+// LDS bytes of a block of the queued kernel for ntp LJ types (padding type
+// included): the pair table, then per wave the ring of (separation, q_i q_j)
+// and its type-pair indices.
+// The switched form reads only (A, B) of each entry, so its table is staged
+// as float2 (three blocks of eight waves then fit one CU's LDS).
+constexpr size_t compact_lds(int ntp, bool sw) {
+    return (sw ? sizeof(float2) : sizeof(float4)) * static_cast<size_t>(ntp) * ntp +
+           kWavesOf<float> * kRing * (sizeof(float4) + sizeof(unsigned short));
 }
 
 // Orders this wave's LDS stores before its later LDS loads by other lanes.
@@ -464,6 +476,12 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// r^2 of a separation, in one fixed order of fused operations (the tile test
+// and the pair pass form it alike).
+__device__ __forceinline__ float sep2(float dx, float dy, float dz) {
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dx, dx, dy * dy));
 }
 
 template <bool kSwitch, bool kCount>
@@ -479,10 +497,20 @@ k_nonbonded_c(const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int 
     }
     extern __shared__ float4 smem4[];
     const int ntp2 = a.ntp * a.ntp;
-    float4* s_lj = smem4;
-    for (int k = threadIdx.x; k < ntp2; k += blockDim.x) s_lj[k] = a.lj[k];
-    // (r^2, q_i q_j, type-pair index, force factor) per packed pair
-    float4* s_pack = smem4 + ntp2 + (threadIdx.x >> 6) * kPackSlots;
+    // the pair table: (A, B) for the switched form, (A, B, U(rc), G(rc)) else
+    float2* s_lj2 = reinterpret_cast<float2*>(smem4);
+    float4* s_lj4 = smem4;
+    for (int k = threadIdx.x; k < ntp2; k += blockDim.x) {
+        const float4 v = a.lj[k];
+        if (kSwitch) s_lj2[k] = make_float2(v.x, v.y);
+        else s_lj4[k] = v;
+    }
+    const int wv = threadIdx.x >> 6;
+    // this wave's ring: (dx, dy, dz, q_i q_j) per queued pair, overwritten by
+    // (fx, fy, fz, -) in its pass; type-pair indices beside it
+    float4* ring0 = smem4 + (kSwitch ? (ntp2 + 1) / 2 : ntp2);
+    float4* s_g = ring0 + wv * kRing;
+    unsigned short* s_t = reinterpret_cast<unsigned short*>(ring0 + kWaves * kRing) + wv * kRing;
     __syncthreads();
 
     const int lane = threadIdx.x & 63;
@@ -509,27 +537,130 @@ k_nonbonded_c(const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int 
     const unsigned int joff = ti < 3 ? 8u * static_cast<unsigned int>(3 * tj + ti) : kOffOOB;
     unsigned long long bad = 0;  // lanes whose j-force did not fit the fixed point
     const int blk = xcd_block(blockIdx.x, npb);
-    const int item = __builtin_amdgcn_readfirstlane(blk * kWaves + (threadIdx.x >> 6));
+    const int item = __builtin_amdgcn_readfirstlane(blk * kWaves + wv);
     if (item < ncl * a.split) {
         const int ci = item / a.split, part = item % a.split;
         const int cnt = a.list_count[ci];
-        const int k0 = static_cast<int>((static_cast<long long>(cnt) * part) / a.split);
-        const int k1 = static_cast<int>((static_cast<long long>(cnt) * (part + 1)) / a.split);
+        const int nmine = cnt > part ? (cnt - part + a.split - 1) / a.split : 0;  // entries part + split k
         const float4 pi = a.pos[ci * kCluster + ti];
         const int2 mi = a.meta[ci * kCluster + ti];
         const int tyi = mi.x * a.ntp;
         const float qi = pi.w * a.kc;
         float fix = 0, fiy = 0, fiz = 0;
-        const size_t base = static_cast<size_t>(ci) * a.cap;
-        for (int kb = k0; kb < k1; kb += kWave) {
-            const int nb = min(kWave, k1 - kb);
+        const size_t base = static_cast<size_t>(ci) * a.cap + part;
+
+        // the pair terms of queued pair p (ring index); its force goes back
+        // over its separation
+        auto pair_at = [&](int p) {
+            const float4 g = s_g[p];
+            const int tp = s_t[p];
+            const float r2 = sep2(g.x, g.y, g.z), qq = g.w;
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            const float r = r2 * rinv;
+            const float r2inv = rinv * rinv;
+            float4 lj;
+            if (kSwitch) {
+                const float2 ab = s_lj2[tp];
+                lj = make_float4(ab.x, ab.y, 0.0f, 0.0f);
+            } else {
+                lj = s_lj4[tp];
+            }
+            float f = 0;
+            if (r2 <= a.rc2_lj) {
+                const float r6 = r2inv * r2inv * r2inv;
+                const float ar6 = lj.x * r6;
+                const float u = (ar6 - lj.y) * r6;
+                const float gg = (12.0f * ar6 - 6.0f * lj.y) * r6 * r2inv;
+                float el, fl;
+                if (kSwitch) {
+                    float xs = (r - a.rs) * a.inv_sw;
+                    xs = xs < 0 ? 0.0f : xs;
+                    const float x2 = xs * xs;
+                    const float sw = 1 + x2 * xs * (-10.0f + xs * (15.0f - 6 * xs));
+                    const float dsw = x2 * (-30.0f + xs * (60.0f - 30 * xs)) * a.inv_sw;
+                    el = u * sw;
+                    fl = sw * gg - u * dsw * rinv;
+                } else {
+                    el = u - lj.z + (r - a.rc_lj) * (lj.w * a.rc_lj);
+                    fl = gg - lj.w * a.rc_lj * rinv;
+                }
+                s_vdw += el;
+                f += fl;
+            }
+            if (r2 <= a.rc2_ee) {
+                float ec, ex;
+                Math<float>::erfc_exp(a.alpha * r, ec, ex);
+                const float e_c = qq * rinv;
+                s_ew += e_c * ec;
+                f += qq * (ec * rinv + tsp * ex) * r2inv;
+            }
+            const float fx = f * g.x, fy = f * g.y, fz = f * g.z;
+            if constexpr (kCount) {
+                n_pairs += 1.0;
+                w_s[0] += fx * g.x;
+                w_s[1] += fy * g.y;
+                w_s[2] += fz * g.z;
+                w_s[3] += fx * g.y;
+                w_s[4] += fx * g.z;
+                w_s[5] += fy * g.z;
+            }
+            s_g[p] = make_float4(fx, fy, fz, qq);
+        };
+        // queue state: pairs [head, tail) wait for a pass (ring index = count
+        // mod kRing); counters are uniform over the wave
+        int head = 0, tail = 0;
+        // one pass over the next n <= 64 queued pairs
+        auto pass = [&](int n) {
+            if (lane < n) pair_at((head + lane) & (kRing - 1));
+            head += n;
+        };
+        // a tile waiting for its pairs' forces: lanes with a queued pair, the
+        // pair's ring index, its j-cluster
+        struct Wait {
+            unsigned long long act;
+            int pos;
+            int cj;
+        };
+        // finish a tile from its queued forces
+        auto finish = [&](const Wait& w) {
+            float fx = 0, fy = 0, fz = 0;
+            if (__builtin_amdgcn_inverse_ballot_w64(w.act)) {
+                const float4 g = s_g[w.pos];
+                fx = g.x;
+                fy = g.y;
+                fz = g.z;
+            }
+            fix += fx;
+            fiy += fy;
+            fiz += fz;
+            long long q = 0;
+            unsigned int off = kOffOOB;
+            if (w.act != 0ull) {
+                // a padding j-slot has no pair in range, so its sum is 0 and
+                // it adds nothing
+                const float gj = jsum3(fx, fy, fz, js);
+                q = fixed32_f32_flag(-gj, bad, kJLanes);
+                off = q != 0 ? joff + static_cast<unsigned int>(w.cj) * (3u * kCluster * 8u) : kOffOOB;
+            }
+            buf_add(rsrc, off, q);
+        };
+        // the previous step's tiles (none before the first step): each step
+        // finishes exactly these two, so every step issues two j-force atomics
+        // and the compiler's waits for the prefetched clusters count exactly
+        // the operations issued after them
+        Wait wA{0ull, 0, 0}, wB{0ull, 0, 0};
+        int prev_end = 0;  // end of their queued pairs
+
+        for (int kb = 0; kb < nmine; kb += kWave) {
+            const int nb = min(kWave, nmine - kb);
             int e_j = 0;
             unsigned long long e_m = 0;
             float4 e_d = {0, 0, 0, 0};
             if (lane < nb) {
-                e_j = a.list_j[base + kb + lane];
-                e_m = a.list_mask[base + kb + lane];
-                e_d = a.list_delta[base + kb + lane];
+                const size_t e = base + static_cast<size_t>(kb + lane) * a.split;
+                e_j = a.list_j[e];
+                e_m = a.list_mask[e];
+                e_d = a.list_delta[e];
             }
             const int e_mlo = static_cast<int>(e_m & 0xffffffffull), e_mhi = static_cast<int>(e_m >> 32);
             // j-cluster data of tile t (clamped to the batch, so every fetch
@@ -539,95 +670,28 @@ k_nonbonded_c(const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int 
                 p = buf_load_f4(pos_rs, pos_off, cj * static_cast<int>(16 * kCluster));
                 m = buf_load_i2(meta_rs, meta_off, cj * static_cast<int>(8 * kCluster));
             };
-            struct Geo {
-                float dx, dy, dz;
-                unsigned long long act;  // lanes whose pair is unmasked and within the cutoff
-            };
-            // separation of this lane's pair in tile t (the tile's mask is a
-            // lane mask, so the in-range test combines with it in scalar
-            // registers)
-            auto geom = [&](const float4& pc, const int2& mc, int t, bool live, float& r2, int& tp, float& qq) {
+            // tile t: the separation of this lane's pair, queued if the pair is
+            // unmasked and within the cutoff (the tile's mask is a lane mask,
+            // so the in-range test combines with it in scalar registers)
+            auto queue = [&](const float4& pc, const int2& mc, int t, bool live, int rank0, Wait& w) {
                 unsigned long long mask =
                     (static_cast<unsigned long long>(static_cast<unsigned int>(__builtin_amdgcn_readlane(e_mhi, t))) << 32) |
                     static_cast<unsigned int>(__builtin_amdgcn_readlane(e_mlo, t));
                 mask = live ? mask : 0ull;
-                Geo g;
-                g.dx = pi.x - pc.x + bcast<float>(e_d.x, t);
-                g.dy = pi.y - pc.y + bcast<float>(e_d.y, t);
-                g.dz = pi.z - pc.z + bcast<float>(e_d.z, t);
-                r2 = g.dx * g.dx + g.dy * g.dy + g.dz * g.dz;
-                // two ballots of plain compares: the masks stay in scalar registers
-                g.act = __builtin_amdgcn_ballot_w64(r2 <= rc2) & __builtin_amdgcn_ballot_w64(r2 >= 1e-12f) & mask;
-                tp = tyi + mc.x;
-                qq = qi * pc.w;
-                return g;
-            };
-            // the pair terms of packed pair v; returns the force factor
-            auto pair = [&](const float4 v) {
-                const float r2 = v.x, qq = v.y;
-                const int tp = __float_as_int(v.z);
-                const float rinv = __builtin_amdgcn_rsqf(r2);
-                const float r = r2 * rinv;
-                const float r2inv = rinv * rinv;
-                const float4 lj = s_lj[tp];
-                float f = 0;
-                if (r2 <= a.rc2_lj) {
-                    const float r6 = r2inv * r2inv * r2inv;
-                    const float ar6 = lj.x * r6;
-                    const float u = (ar6 - lj.y) * r6;
-                    const float g = (12.0f * ar6 - 6.0f * lj.y) * r6 * r2inv;
-                    float el, fl;
-                    if (kSwitch) {
-                        float xs = (r - a.rs) * a.inv_sw;
-                        xs = xs < 0 ? 0.0f : xs;
-                        const float x2 = xs * xs;
-                        const float sw = 1 + x2 * xs * (-10.0f + xs * (15.0f - 6 * xs));
-                        const float dsw = x2 * (-30.0f + xs * (60.0f - 30 * xs)) * a.inv_sw;
-                        el = u * sw;
-                        fl = sw * g - u * dsw * rinv;
-                    } else {
-                        el = u - lj.z + (r - a.rc_lj) * (lj.w * a.rc_lj);
-                        fl = g - lj.w * a.rc_lj * rinv;
-                    }
-                    s_vdw += el;
-                    f += fl;
+                const float dx = pi.x - pc.x + bcast<float>(e_d.x, t);
+                const float dy = pi.y - pc.y + bcast<float>(e_d.y, t);
+                const float dz = pi.z - pc.z + bcast<float>(e_d.z, t);
+                const float r2 = sep2(dx, dy, dz);
+                w.act = __builtin_amdgcn_ballot_w64(r2 <= rc2) & __builtin_amdgcn_ballot_w64(r2 >= 1e-12f) & mask;
+                w.cj = __builtin_amdgcn_readlane(e_j, t);
+                const int rank = rank0 + __builtin_amdgcn_mbcnt_hi(static_cast<unsigned int>(w.act >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(w.act), 0u));
+                w.pos = (tail + rank) & (kRing - 1);
+                if (__builtin_amdgcn_inverse_ballot_w64(w.act)) {
+                    s_g[w.pos] = make_float4(dx, dy, dz, qi * pc.w);
+                    s_t[w.pos] = static_cast<unsigned short>(tyi + mc.x);
                 }
-                if (r2 <= a.rc2_ee) {
-                    float ec, ex;
-                    Math<float>::erfc_exp(a.alpha * r, ec, ex);
-                    const float e_c = qq * rinv;
-                    s_ew += e_c * ec;
-                    f += qq * (ec * rinv + tsp * ex) * r2inv;
-                }
-                return f;
-            };
-            // force of tile t from its factor: i-forces, j-force sums and the
-            // j-force atomic (issued by every tile; an empty tile adds nothing)
-            auto finish = [&](const Geo& g, float f, int t, const int2& mc) {
-                const float fx = f * g.dx, fy = f * g.dy, fz = f * g.dz;
-                if constexpr (kCount) {
-                    if (__builtin_amdgcn_inverse_ballot_w64(g.act) && mc.y >= 0 && mi.y >= 0) n_pairs += 1.0;
-                    w_s[0] += fx * g.dx;
-                    w_s[1] += fy * g.dy;
-                    w_s[2] += fz * g.dz;
-                    w_s[3] += fx * g.dy;
-                    w_s[4] += fx * g.dz;
-                    w_s[5] += fy * g.dz;
-                }
-                fix += fx;
-                fiy += fy;
-                fiz += fz;
-                long long q = 0;
-                unsigned int off = kOffOOB;
-                if (g.act != 0ull) {
-                    // a padding j-slot has no pair in range, so its sum is 0
-                    // and it adds nothing
-                    const float gj = jsum3(fx, fy, fz, js);
-                    const int cjt = __builtin_amdgcn_readlane(e_j, t);
-                    q = fixed32_f32_flag(-gj, bad, kJLanes);
-                    off = q != 0 ? joff + static_cast<unsigned int>(cjt) * (3u * kCluster * 8u) : kOffOOB;
-                }
-                buf_add(rsrc, off, q);
+                return __popcll(w.act);
             };
 
             float4 pA, pB;
@@ -642,36 +706,22 @@ k_nonbonded_c(const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int 
             // step.
             __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
             for (int t = 0; t < nb; t += 2) {
-                float r2A, r2B, qqA, qqB;
-                int tpA, tpB;
-                const Geo gA = geom(pA, mA, t, true, r2A, tpA, qqA);
-                const int2 mcA = mA;
+                Wait cA, cB;
+                const int nA = queue(pA, mA, t, true, 0, cA);
                 fetch(t + 2, pA, mA);
-                const Geo gB = geom(pB, mB, t + 1, t + 1 < nb, r2B, tpB, qqB);
-                const int2 mcB = mB;
+                const int nB = queue(pB, mB, t + 1, t + 1 < nb, nA, cB);
                 fetch(t + 3, pB, mB);
-                const int nA = __popcll(gA.act);
-                const int total = nA + __popcll(gB.act);
-                float fA = 0, fB = 0;
-                if (total > 0) {
-                    const bool actA = __builtin_amdgcn_inverse_ballot_w64(gA.act);
-                    const bool actB = __builtin_amdgcn_inverse_ballot_w64(gB.act);
-                    const int sA = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned int>(gA.act >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(gA.act), 0u));
-                    const int sB = nA + __builtin_amdgcn_mbcnt_hi(static_cast<unsigned int>(gB.act >> 32),
-                                                                  __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(gB.act), 0u));
-                    if (actA) s_pack[sA] = make_float4(r2A, qqA, __int_as_float(tpA), 0.0f);
-                    if (actB) s_pack[sB] = make_float4(r2B, qqB, __int_as_float(tpB), 0.0f);
-                    wave_sync();
-                    if (lane < total) s_pack[lane].w = pair(s_pack[lane]);
-                    if (total > kWave && lane + kWave < total) s_pack[lane + kWave].w = pair(s_pack[lane + kWave]);
-                    wave_sync();
-                    fA = actA ? s_pack[sA].w : 0.0f;
-                    fB = actB ? s_pack[sB].w : 0.0f;
-                    wave_sync();
-                }
-                finish(gA, fA, t, mcA);
-                finish(gB, fB, min(t + 1, nb - 1), mcB);
+                tail += nA + nB;
+                wave_sync();
+                // passes: every full 64, and first whatever the previous
+                // step's tiles still lack
+                while (tail - head >= kWave || head < prev_end) pass(min(kWave, tail - head));
+                wave_sync();
+                finish(wA);
+                finish(wB);
+                wA = cA;
+                wB = cB;
+                prev_end = tail;
                 flush += 2;
                 if (flush >= 32) {
                     flush = 0;
@@ -687,10 +737,17 @@ k_nonbonded_c(const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int 
                 }
             }
         }
+        // the pairs still queued, then the last step's tiles
+        if (head < tail) {
+            pass(tail - head);
+            wave_sync();
+        }
+        finish(wA);
+        finish(wB);
         fix += shfl_xor(fix, 8);  fiy += shfl_xor(fiy, 8);  fiz += shfl_xor(fiz, 8);
         fix += shfl_xor(fix, 16); fiy += shfl_xor(fiy, 16); fiz += shfl_xor(fiz, 16);
         fix += shfl_xor(fix, 32); fiy += shfl_xor(fiy, 32); fiz += shfl_xor(fiz, 32);
-        add_force<float, true>(a, rsrc, tj < 3 && k1 > k0 && mi.y >= 0, 3 * (ci * kCluster + ti) + tj,
+        add_force<float, true>(a, rsrc, tj < 3 && nmine > 0 && mi.y >= 0, 3 * (ci * kCluster + ti) + tj,
                                tj == 0 ? fix : (tj == 1 ? fiy : fiz));
     }
     if (bad != 0ull && lane == 0) atomicOr(a.fx.overflow, 1u);
@@ -791,7 +848,7 @@ hipError_t run(mdgpu_ctx* c, bool amber, int row0, const double* coords) {
     if constexpr (sizeof(real) == 4) {
         if (amber) {
             // the compacted kernel: LJ table, then one pack buffer per wave
-            const size_t shc = compact_lds(a.ntp);
+            const size_t shc = compact_lds(a.ntp, sw);
             if (shc <= kPackMaxLds) {
                 if (stats) {
                     if (sw) hipLaunchKernelGGL((k_nonbonded_c<true, true>), grid, block, shc, c->stream, a, ba, ca, npb, nbb);
@@ -842,7 +899,7 @@ const void* pair_kernel_fn(const mdgpu_ctx* c) {
     const bool amber = c->params.semantics == MDGPU_SEM_AMBER, sw = c->params.lj_switch > 0.0;
     const bool stats = c->counting || c->virial;
     const bool mixed = c->params.precision == MDGPU_PREC_MIXED;
-    if (mixed && amber && compact_lds(c->topo.n_types + 1) <= kPackMaxLds) {
+    if (mixed && amber && compact_lds(c->topo.n_types + 1, sw) <= kPackMaxLds) {
         if (stats)
             return sw ? reinterpret_cast<const void*>(&k_nonbonded_c<true, true>)
                       : reinterpret_cast<const void*>(&k_nonbonded_c<false, true>);

@@ -392,6 +392,24 @@ __device__ __forceinline__ bool pair_excluded(const int* __restrict__ start, con
     return false;
 }
 
+// Half list: a pair of distinct clusters (i, j) is listed with i when j lies
+// in the circular window of indices that starts at i and spans half of the n
+// clusters (for even n, the pair half way round goes to the lower index), so
+// every pair is listed once.  Cluster indices run column by column, so this
+// window holds the columns ahead of i, wrapping past the last column: every
+// cluster gets about half of its neighbours.  (The rule j >= i gave the first
+// columns nearly full lists and the last ones nearly empty lists, since their
+// periodic neighbours wrap to low indices; the pair kernel's waves then
+// differed by 5x in work.)
+__device__ __forceinline__ int half_window(int ci, int n) {
+    return (n & 1) ? (n + 1) / 2 : n / 2 + (ci < n / 2 ? 1 : 0);
+}
+__device__ __forceinline__ bool in_half(int ci, int cj, int n) {
+    int off = cj - ci;
+    off += off < 0 ? n : 0;
+    return off < half_window(ci, n);
+}
+
 struct SearchArgs {
     const int* ncl_dev;
     int max_clusters;
@@ -460,8 +478,8 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
             const int c0 = a.col_cstart[jcol], c1 = a.col_cstart[jcol + 1];
             const double shx = a.cell.tri ? kx * a.cell.ax + ky * a.cell.bx : kx * a.lx;
             const double shy = a.cell.tri ? ky * a.cell.by : ky * a.ly;
-            // half list: only j-clusters >= ci (cluster indices run column by column)
-            for (int b = max(c0, ci); b < c1; b += kWave) {
+            // half list: the j-clusters in the half window after ci
+            for (int b = c0; b < c1; b += kWave) {
                 const int cj = b + lane;
                 const bool valid = cj < c1;
                 double4 oj = make_double4(0, 0, 0, 0);
@@ -482,7 +500,8 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
                     // a cluster with its own image: one of each pair of opposite
                     // shifts (the other holds the same atom pairs), all 64 pairs
                     const bool self_up = kz > 0 || (kz == 0 && (ky > 0 || (ky == 0 && kx > 0)));
-                    const bool cand = valid && (cj > ci || (cj == ci && (noshift || self_up))) && d2 < a.rl2;
+                    const bool cand =
+                        valid && (cj == ci ? (noshift || self_up) : in_half(ci, cj, ncl)) && d2 < a.rl2;
                     const bool triangle = cj == ci && noshift;  // the cluster with itself: j > i only
                     const unsigned long long bal = __ballot(cand);
                     if (bal == 0) continue;
@@ -696,9 +715,12 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     const R4* pos = static_cast<const R4*>(a.pos);
     const R4 pi_l = pos[ci * kCluster + (lane & 7)];
 
-    // ranges: lane r = 2 * row + segment
+    // ranges: lane r = 2 * row + segment, each clipped to the half window
+    // [ci, ci + hw) (circular)
     int r_b0 = 0, r_len = 0;
+    bool straddle = false;
     {
+        const int hw = half_window(ci, ncl);
         const bool yall = ny == a.ncy;
         const int y0 = yall ? 0 : cy - a.dcy;
         const int ry = lane >> 1, seg = lane & 1;
@@ -726,13 +748,35 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
             const int c_n = seg == 0 ? first : nx - first;
             if (c_n > 0) {
                 const int rowc = a.ncx * jy;
-                const int b0 = max(a.col_cstart[rowc + c_lo], ci), b1 = a.col_cstart[rowc + c_lo + c_n];
+                const int c0 = a.col_cstart[rowc + c_lo], c1 = a.col_cstart[rowc + c_lo + c_n];
+                int b0 = max(c0, ci), b1 = min(c1, ci + hw);
+                if (ci + hw > ncl) {
+                    // the window wraps: [ci, ncl) and [0, w2)
+                    const int w2 = ci + hw - ncl;
+                    const bool hi_part = c1 > ci, lo_part = c0 < w2;
+                    straddle = hi_part && lo_part;  // a range as long as half the system
+                    if (lo_part && !hi_part) {
+                        b0 = c0;
+                        b1 = min(c1, w2);
+                    } else {
+                        b1 = c1;
+                    }
+                }
                 if (b1 > b0) {
                     r_b0 = b0;
                     r_len = b1 - b0;
                 }
             }
         }
+    }
+    if (__ballot(straddle) != 0ull) {
+        // a row range reaching both ends of the window (tiny systems): the
+        // general search handles it
+        if (lane == 0) {
+            atomicOr(a.flags + 4, 1u);
+            a.list_count[ci] = 0;
+        }
+        return;
     }
     int r_inc = r_len;
     for (int d = 1; d < kWave; d <<= 1) {
