@@ -308,10 +308,9 @@ def bench_triclinic(torch, N, top, p32, box, reps=200, warmup=20):
     PME 72^3), evaluations/s over `reps` after `warmup` each:
       * `sheared`: its own cubic lattice written with edges a, a + b, a + b + c,
         the same periodic system through the triclinic path, with the relative
-        differences of the direct-space energies from the orthorhombic cell
-        (the reciprocal energy differs by the PME grid, whose lines now run
-        along the sheared edges); its slanted columns are too wide for the
-        minimum-image search;
+        differences of the direct-space energies from the orthorhombic cell;
+        the library evaluates it in the reduced basis of the lattice (the
+        cube), so its columns stay narrow and the minimum-image search runs;
       * `angles_80_85_75`: the system carried into a 66 A cell of those angles
         (x' = H x / 66), which keeps the minimum-image search."""
     l1, l2, l3 = (float(v) for v in box)

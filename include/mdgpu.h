@@ -373,9 +373,13 @@ int mdgpu_set_verlet(mdgpu_ctx* ctx, double skin);
  * arguments then give the edge lengths |a|, |b|, |c|.  The cell is
  * a = (|a|, 0, 0), b = |b| (cos gamma, sin gamma, 0) and c in the half-space
  * z > 0 with the given angles; coordinates are taken in that frame.  All
- * three angles exactly 90 (the default) is the orthorhombic cell.  A
- * triclinic cell needs AMBER semantics, every distance between opposite faces
- * at least twice the cutoff and at least the cutoff plus the Verlet skin
+ * three angles exactly 90 (the default) is the orthorhombic cell.  The
+ * cell is evaluated in its reduced basis: b and c lose whole multiples of a
+ * and b until |c_y| <= b_y / 2 and |b_x|, |c_x| <= a_x / 2 (the same lattice,
+ * so the same periodic system; an already reduced cell is kept as given), and
+ * the PME grid runs along the reduced edges.  A triclinic cell needs AMBER
+ * semantics, every distance between opposite faces of the reduced cell at
+ * least twice the cutoff and at least the cutoff plus the Verlet skin
  * (checked at each evaluation: MDGPU_ERR_UNSUPPORTED), and is not available
  * to mdgpu_adjoint.  Angles outside (0, 180) or that form no cell:
  * MDGPU_ERR_ARG.  A change of angles forces the next list build. */
@@ -385,8 +389,8 @@ int mdgpu_get_box_angles(const mdgpu_ctx* ctx, double angles[3]);
  * the reference): 0 the minimum-image search, 1 the general per-shift search,
  * which runs when a cluster could be within the cutoff of a second image of
  * another (the box or the cell's face distances are small for cutoff + skin),
- * and for good once a cluster was too wide for the minimum-image search (a
- * strongly sheared cell); -1 for a null context. */
+ * and for good once a cluster was too wide for the minimum-image search; -1
+ * for a null context. */
 int mdgpu_get_search_mode(const mdgpu_ctx* ctx);
 /* List builds and successful evaluations since the system upload. */
 int mdgpu_get_list_builds(const mdgpu_ctx* ctx, int64_t* builds, int64_t* evals);

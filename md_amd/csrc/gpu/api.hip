@@ -519,6 +519,43 @@ bool make_cell(const double box[3], const double ang[3], mdg::Cell& cell, double
     return true;
 }
 
+// The same lattice in a reduced basis: c loses whole multiples of b until
+// |c_y| <= b_y / 2, then c and b lose whole multiples of a until |c_x|,
+// |b_x| <= a_x / 2 (a, the diagonal a_x, b_y, c_z and the volume are kept).
+// A strongly sheared cell (say edges a, a + b, a + b + c of a cube) becomes
+// nearly rectangular, so its fractional columns stay narrow and the
+// minimum-image cluster search applies; the PME grid then runs along the
+// reduced edges.  An already reduced cell is returned unchanged, bit for bit.
+// face: the reduced cell's face distances.
+mdg::Cell reduce_cell(const mdg::Cell& in, double face[3]) {
+    mdg::Cell r = in;
+    if (!in.tri) return r;
+    double k = std::nearbyint(r.cy / r.by);
+    if (k != 0.0) {
+        r.cx -= k * r.bx;
+        r.cy -= k * r.by;
+    }
+    k = std::nearbyint(r.cx / r.ax);
+    if (k != 0.0) r.cx -= k * r.ax;
+    k = std::nearbyint(r.bx / r.ax);
+    if (k != 0.0) r.bx -= k * r.ax;
+    if (r.bx == in.bx && r.cx == in.cx && r.cy == in.cy) {
+        face[0] = 1.0 / std::sqrt(in.r00 * in.r00 + in.r01 * in.r01 + in.r02 * in.r02);
+        face[1] = 1.0 / std::sqrt(in.r11 * in.r11 + in.r12 * in.r12);
+        face[2] = in.cz;
+        return r;
+    }
+    r.l[1] = std::sqrt(r.bx * r.bx + r.by * r.by);
+    r.l[2] = std::sqrt(r.cx * r.cx + r.cy * r.cy + r.cz * r.cz);
+    r.r01 = -r.bx / (r.ax * r.by);
+    r.r12 = -r.cy / (r.by * r.cz);
+    r.r02 = (r.bx * r.cy - r.by * r.cx) / (r.ax * r.by * r.cz);
+    face[0] = 1.0 / std::sqrt(r.r00 * r.r00 + r.r01 * r.r01 + r.r02 * r.r02);
+    face[1] = 1.0 / std::sqrt(r.r11 * r.r11 + r.r12 * r.r12);
+    face[2] = r.cz;
+    return r;
+}
+
 int prepare(mdgpu_ctx* c, const double box[3]) {
     const int n = c->topo.n_atoms;
     const mdgpu_params& p = c->params;
@@ -527,10 +564,12 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     // a Verlet list holds shift vectors of the box it was built in
     for (int d = 0; d < 3; ++d)
         if (box[d] != c->grid.box[d]) c->verlet_force = true;
-    mdg::Cell cell{};
+    mdg::Cell given{};
     double face[3];
-    if (!make_cell(box, c->angles, cell, face))
+    if (!make_cell(box, c->angles, given, face))
         return set_error(c, MDGPU_ERR_ARG, "the cell angles do not form a cell");
+    // bin and search in the reduced basis of the same lattice
+    const mdg::Cell cell = reduce_cell(given, face);
     if (cell.tri && p.semantics != MDGPU_SEM_AMBER)
         return set_error(c, MDGPU_ERR_UNSUPPORTED, "triclinic cells are evaluated in AMBER semantics only");
     const double rc = std::max(p.lj_cutoff, p.ee_cutoff);
@@ -553,11 +592,11 @@ int prepare(mdgpu_ctx* c, const double box[3]) {
     const double s = std::cbrt(mdg::kCluster / std::max(dens, 1e-6));
     g.ncx = std::max(1, static_cast<int>(face[0] / s));
     g.ncy = std::max(1, static_cast<int>(face[1] / s));
-    g.sx = box[0] / g.ncx;
-    g.sy = box[1] / g.ncy;
+    g.sx = cell.l[0] / g.ncx;
+    g.sy = cell.l[1] / g.ncy;
     g.wx = cell.tri ? face[0] / g.ncx : g.sx;
     g.wy = cell.tri ? face[1] / g.ncy : g.sy;
-    g.frac_scale = cell.tri ? std::max(box[0] / face[0], box[1] / face[1]) : 1.0;
+    g.frac_scale = cell.tri ? std::max(cell.l[0] / face[0], cell.l[1] / face[1]) : 1.0;
     g.dcx = static_cast<int>(std::floor(rl / g.wx)) + 1;
     g.dcy = static_cast<int>(std::floor(rl / g.wy)) + 1;
     const int ncol = g.ncx * g.ncy;

@@ -195,8 +195,12 @@ struct Grid {
     // wx, wy: the distance between a column's opposite faces (sx, sy when
     // orthorhombic); frac_scale: the largest edge length over face distance
     // ratio, which converts a distance into a bound on s_x |a| and s_y |b|.
+    // the cell of the evaluation: the caller's cell with its off-diagonal edge
+    // components reduced (the same lattice, so the minimum images and the
+    // pairs within the cutoff are the same, but the columns of a strongly
+    // sheared cell stay narrow); the PME grid runs along its edges
     Cell cell{};
-    double face[3] = {0, 0, 0};  // distances between opposite faces (the lengths when orthorhombic)
+    double face[3] = {0, 0, 0};  // distances between opposite faces of cell (the lengths when orthorhombic)
     double wx = 0, wy = 0;
     double frac_scale = 1.0;
 };

@@ -582,8 +582,8 @@ hipError_t run_pme(mdgpu_ctx* c, int row0, int* rows_used, double* pot) {
     using c2 = typename C2<real>::T;
     Work& w = c->work;
     const int n = c->topo.n_atoms;
-    PmeGeom g{w.pme_k[0], w.pme_k[1], w.pme_k[2], w.pme_k[2] / 2 + 1, c->grid.box[0], c->grid.box[1],
-              c->grid.box[2], c->grid.cell};
+    PmeGeom g{w.pme_k[0], w.pme_k[1], w.pme_k[2], w.pme_k[2] / 2 + 1, c->grid.cell.l[0], c->grid.cell.l[1],
+              c->grid.cell.l[2], c->grid.cell};
     const bool dbl = sizeof(real) == 8;
     c2* cg = static_cast<c2*>(w.cgrid);
     const c2* tw = dbl ? reinterpret_cast<const c2*>(w.twiddle_d) : reinterpret_cast<const c2*>(w.twiddle_f);

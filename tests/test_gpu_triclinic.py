@@ -4,7 +4,11 @@ Three cells hold the 2.1k-atom polymer: the genuinely triclinic cells of
 edges 30 A and angles 80, 85, 75 degrees and of the truncated octahedron
 (109.47 degrees; the cubic system carried into each by x' = H x / 30), and
 its own cubic box written in the sheared basis a, a + b, a + b + c (every
-cell-matrix entry non-zero).  Bars, as for orthorhombic boxes:
+cell-matrix entry non-zero).  The library evaluates a cell in its reduced
+basis (the same lattice; the PME grid runs along the reduced edges), so the
+sheared basis is evaluated as the cube and the oracle is evaluated in the
+reduced basis (tri_systems.reduced_box; the other two cells are reduced
+already).  Bars, as for orthorhombic boxes:
   * FP64 vs oracle: per-term |dE| <= 1e-9 |E|, force relative RMS <= 1e-8;
     the full virial within 1e-9 of its largest component;
   * mixed vs FP64: force relative RMS <= 1e-5;
@@ -30,7 +34,7 @@ from md_amd import native as N
 
 from md_systems import thermal_velocities
 from test_gpu_parity import TERMS_AMBER, check_terms, rel_rms
-from tri_systems import cell_vectors, mapped, sheared_box
+from tri_systems import cell_vectors, mapped, reduced_box, sheared_box
 
 pytestmark = pytest.mark.gpu
 
@@ -86,7 +90,7 @@ def test_fp64_matches_oracle(polymer_2k, kind):
     # cutoff, over sqrt 3) is at most 0.6 A here, so the general search runs
     print(f"  search mode {mode}")
     assert mode == 1
-    eo, fo, wo = oracle.evaluate(top.desc, prm, x, box6, skip_all_pairs=True, full_virial=True)
+    eo, fo, wo = oracle.evaluate(top.desc, prm, x, reduced_box(box6), skip_all_pairs=True, full_virial=True)
     check_terms(e, eo, TERMS_AMBER, 1e-9)
     r = rel_rms(f, fo)
     dw = np.abs(w - wo).max() / np.abs(wo).max()
@@ -321,3 +325,56 @@ def test_minimum_image_search_in_a_triclinic_cell(polymer_29k):
         eo1, fo1, _ = oracle.evaluate(top.desc, prm, x1, box6, skip_all_pairs=True)
         check_terms(e1, eo1, TERMS_AMBER, 1e-9)
         assert rel_rms(f1, fo1) <= 1e-8
+
+
+def test_sheared_basis_is_binned_in_its_reduced_cell(polymer_29k):
+    """The cubic 29k workload written in the sheared basis a, a + b, a + b + c
+    (edges 66, 93.3, 114.3 A; angles 35.3, 54.7, 45 degrees).  Its columns in
+    that basis would be too wide for the minimum-image search; the library
+    bins, searches and grids it in the reduced basis of the same lattice (the
+    cube), so the minimum-image search runs (search mode 0).  Bars as above:
+    FP64 vs the oracle in the reduced basis 1e-9 per term and 1e-8 force
+    rel-RMS, virial 1e-9; mixed vs FP64 force rel-RMS 1e-5; the direct-space
+    and bonded terms equal those of the cubic box (same lattice) to 1e-12,
+    FP64."""
+    print("case polymer_29k in a sheared basis, reduced binning cell", flush=True)
+    prm_path, rst_path = polymer_29k
+    top = N.Topology(prm_path, rst_path, N.SEM_AMBER)
+    box6 = [float(v) for v in sheared_box(top.box)]
+    x = np.ascontiguousarray(np.asarray(top.coords, dtype=np.float64).reshape(-1, 3))
+    kw = dict(lj_cutoff=12.0, lj_switch=11.0, ewald_tol=1e-5, pme_grid=(72, 72, 72))
+    prm = N.make_params(N.SEM_AMBER, N.PREC_FP64, **kw)
+    res = {}
+    for name, prec in (("fp64", N.PREC_FP64), ("mixed", N.PREC_MIXED)):
+        ev = _ev(top, N.make_params(N.SEM_AMBER, prec, **kw), box6, virial=name == "fp64")
+        try:
+            res[name] = ev.evaluate(x, box6[:3])
+            if name == "fp64":
+                w = ev.virial()
+            mode = ev.search_mode()
+        finally:
+            ev.close()
+        print(f"  {name}: search mode {mode}")
+        assert mode == 0
+    e, f, _ = res["fp64"]
+    eo, fo, wo = oracle.evaluate(top.desc, prm, x, reduced_box(box6), skip_all_pairs=True, full_virial=True)
+    check_terms(e, eo, TERMS_AMBER, 1e-9)
+    r = rel_rms(f, fo)
+    dw = np.abs(w - wo).max() / np.abs(wo).max()
+    print(f"  FP64 vs oracle: force rel-RMS {r:.2e}, virial {dw:.2e}")
+    assert r <= 1e-8
+    assert dw <= 1e-9
+    r32 = rel_rms(res["mixed"][1], f)
+    print(f"  mixed force rel-RMS {r32:.2e}")
+    assert r32 <= 1e-5
+    # the same lattice as an orthorhombic box: every term but the reciprocal one
+    ev = _ev(top, prm, [float(v) for v in top.box] + [90.0, 90.0, 90.0])
+    try:
+        ec, _, _ = ev.evaluate(x, [float(v) for v in top.box])
+    finally:
+        ev.close()
+    for k in ("bond", "angle", "dihedral", "urey_bradley", "improper", "cmap", "vdw", "ewald_direct",
+              "ewald_self", "ewald_excl", "vdw14", "ee14"):
+        rel = abs(e[k] - ec[k]) / max(abs(ec[k]), 1e-300)
+        print(f"  {k:14s} sheared vs cubic rel {rel:.2e}")
+        assert rel <= 1e-12, k

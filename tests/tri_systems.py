@@ -3,6 +3,9 @@
   * sheared_box(L): the lattice of an L1 x L2 x L3 orthorhombic box written in
     the sheared basis a, a + b, a + b + c, as edge lengths and angles (the
     same periodic system, every cell-matrix entry non-zero);
+  * reduced_box(box6): the same lattice in its reduced basis (off-diagonal
+    edge components at most half the diagonal ones), as edge lengths and
+    angles: the cell the GPU library evaluates a given cell in;
   * mapped(coords, L, box6): coordinates of a cubic box of edge L carried
     into a triclinic cell of the same edge lengths by x' = H x / L (a
     genuinely triclinic system; molecules are sheared with the cell);
@@ -37,6 +40,24 @@ def cell_vectors(box6):
     cx = lc * cb
     cy = lc * (ca - cb * cg) / sg
     return np.array([[la, lb * cg, cx], [0.0, lb * sg, cy], [0.0, 0.0, np.sqrt(lc * lc - cx * cx - cy * cy)]])
+
+
+def reduced_box(box6):
+    """[|a|, |b'|, |c'|, alpha', beta', gamma'] of the lattice of box6 in the
+    basis a, b' = b - m a, c' = c - k b - l a with |c'_y| <= b_y / 2 and
+    |c'_x|, |b'_x| <= a_x / 2 (whole m, k, l; rounding to nearest)."""
+    h = cell_vectors(box6)
+    a, b, c = h[:, 0].copy(), h[:, 1].copy(), h[:, 2].copy()
+    k1 = np.rint(c[1] / b[1])
+    c -= k1 * b
+    k2 = np.rint(c[0] / a[0])
+    c -= k2 * a
+    k3 = np.rint(b[0] / a[0])
+    b -= k3 * a
+    if k1 == 0 and k2 == 0 and k3 == 0:
+        return [float(v) for v in box6]  # reduced already
+    return [float(np.linalg.norm(a)), float(np.linalg.norm(b)), float(np.linalg.norm(c)),
+            _angle(b, c), _angle(a, c), _angle(a, b)]
 
 
 def mapped(coords, edge, box6):
