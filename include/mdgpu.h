@@ -258,8 +258,8 @@ int mdgpu_adjoint(mdgpu_ctx* ctx, const double* coords, const double box[3], con
  * captures the graph, once per batch.  A later call whose buffer addresses
  * differ from the captured ones, or in which a system's pair list outgrows its
  * capacity (that system is redone on its own), leaves the graph stale, and the
- * batch then evaluates its systems one after another (see
- * mdgpu_batch_graph_state). */
+ * batch then issues the same launches on the same streams directly, without a
+ * graph (see mdgpu_batch_graph_state). */
 typedef struct mdgpu_batch mdgpu_batch;
 
 int mdgpu_batch_create(int device, mdgpu_batch** out);
@@ -303,7 +303,7 @@ int mdgpu_batch_mode(const mdgpu_batch* b, int* mode);
  * reference). */
 int mdgpu_batch_system_config(const mdgpu_batch* b, int index, int* split, int* fft_lines);
 /* 0: no graph yet; 1: the captured graph is in use; 2: the graph went stale
- * and the batch evaluates serially. */
+ * and the batch issues its launches directly (mode 2's passes). */
 int mdgpu_batch_graph_state(const mdgpu_batch* b, int* state);
 /* Number of systems and total atoms. */
 int mdgpu_batch_size(const mdgpu_batch* b, int* n_systems, int64_t* n_atoms);

@@ -9,7 +9,8 @@
 // system's energy row and flags into one buffer, read back with one copy.
 // Replays need no host work per system.  A system whose flags ask for a redo
 // (pair list over capacity, cluster too wide for the minimum-image search) is
-// evaluated again on its own, and the batch evaluates serially from then on.
+// evaluated again on its own, and from then on the batch issues the graph's
+// launches directly on the same streams (mode 2), without a graph.
 //
 // Each system keeps its own pair-tile split and FFT block shape.  The
 // autotuner sets them per size class (systems ordered by atom count, cut into
@@ -56,8 +57,9 @@ struct mdgpu_batch {
     // The graph is captured once per batch.  Capturing a second time on the
     // same batch (after destroying the first graph) ended the process with a
     // host segmentation fault on the MI355X and the cause is not found, so a
-    // batch whose captured graph went stale evaluates serially instead.
-    int graph_state = 0;              // 0 none yet, 1 captured and used, 2 stale: serial from now on
+    // batch whose captured graph went stale issues the same launches on the
+    // same streams directly instead (mode 2).
+    int graph_state = 0;              // 0 none yet, 1 captured and used, 2 stale: direct launches from now on
     const double** d_src = nullptr;   // per-system energy rows (device pointers)
     double* d_all = nullptr;          // [n_sys x kRow]
     double* h_all = nullptr;          // pinned copy
@@ -247,7 +249,7 @@ int run_serial(mdgpu_batch* b, const double* x, mdgpu_energies* out_e, double* f
 
 int run_graph(mdgpu_batch* b, const double* x, mdgpu_energies* out_e, double* f) {
     if (b->graph_state == 1 && (b->dirty || !b->exec || x != b->cap_x || f != b->cap_f)) b->graph_state = 2;
-    if (b->graph_state == 2) return run_serial(b, x, out_e, f);
+    if (b->graph_state == 2) return run_streams(b, x, out_e, f);
     if (b->graph_state == 0) {
         // one serial pass sizes every pair list before the single capture
         int st = run_serial(b, x, nullptr, f);

@@ -122,7 +122,8 @@ def test_class_tuner_then_single_capture_host_path():
     buffers and keeps the faster of graph and direct launches.  With the tuned
     per-system launch shapes the serial mode, the directly issued passes
     (mode 2) and the graph agree bit for bit, on the host path too; a call
-    with other buffers makes the graph stale and the batch evaluates serially.
+    with other buffers makes the graph stale and the batch issues the same
+    launches directly (mode 2's passes), bit for bit as before.
     Against separate contexts (default shape) the FP32 partial sums of the
     i-forces are grouped differently, so the bar there is 2e-6."""
     dev = torch.device("cuda:0")
