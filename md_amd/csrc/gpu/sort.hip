@@ -46,7 +46,7 @@ constexpr int kColMax = kColBucket;  // atoms per column the in-LDS sort handles
 constexpr int kBinThreads = 1024;
 constexpr int kMaxCols = 8192;
 constexpr int kMaxSortBlocks = 256;
-constexpr int kAtomsPerSortBlock = 1024;
+constexpr int kAtomsPerSortBlock = 512;
 
 // Excluded and 1-4 pairs whose minimum-image distances bound the tiles that
 // can hold an excluded pair (n1 = n2 = 0: not needed).
