@@ -384,6 +384,7 @@ def bench_scale(torch, N, systems, reps=100, warmup=20):
         ev.evaluate_device(x.data_ptr(), top.box, f.data_ptr())
         counts = ev.counts()
         ev.set_counting(False)
+        tuned = ev.tune(x.data_ptr(), top.box, f.data_ptr(), reps=6)
         ev.set_timing(3)
         t = timed_steps(torch, ev, x.data_ptr(), f.data_ptr(), top.box, reps, warmup)
         nb_total, nb_launches = ev.timing()["nonbonded"]
@@ -396,6 +397,7 @@ def bench_scale(torch, N, systems, reps=100, warmup=20):
     return {"workload": "scale_100k: %d atoms, 100 A box, AMBER, rc 10 A (switch 9 A), tol 1e-5, PME 108^3, "
                         "mixed form" % top.n_atoms,
             "evals_per_s": reps / t, "atom_evals_per_s": top.n_atoms * reps / t, "work": counts,
+            "tuned_shapes": {"split": tuned[0], "fft_lines": tuned[1] or 16, "tune_median_ms_direct": tuned[2]},
             "kernel_ms": kernels_ms,
             "roofline": {"bound": "valu", "kernel": "pair kernel (mixed, AMBER, switched LJ + erfc Ewald)",
                          "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -405,9 +407,11 @@ def bench_scale(torch, N, systems, reps=100, warmup=20):
             "pme_roofline": pme_roofline(kernels_ms, top.n_atoms, kg)}
 
 
-def profile_workload(name, steps, warmup):
+def profile_workload(name, steps, warmup, shapes=None):
     """Only `steps` mixed-form evaluations of one workload after `warmup`
-    (graph replay), for profiler runs: polymer_29k or scale_100k."""
+    (graph replay), for profiler runs: polymer_29k or scale_100k, with the
+    given (split, fft_lines) launch shapes (no tuning pass, whose candidate
+    launches would mix into the profile)."""
     import torch
     from md_amd import native as N
     from md_amd import systems
@@ -422,9 +426,11 @@ def profile_workload(name, steps, warmup):
     ev = N.Evaluator(0)
     ev.upload(top.desc)
     ev.set_params(N.make_params(N.SEM_AMBER, N.PREC_MIXED, **kw))
+    if shapes:
+        ev.set_shapes(*shapes)
     t = timed_steps(torch, ev, x.data_ptr(), f.data_ptr(), top.box, steps, warmup)
     print(json.dumps({"profile_workload": name, "atoms": top.n_atoms, "steps": steps, "warmup": warmup,
-                      "evals_per_s": steps / t}), flush=True)
+                      "shapes": ev.shapes(), "evals_per_s": steps / t}), flush=True)
     ev.close()
 
 
@@ -521,8 +527,12 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 64-frame parameter-gradient run")
     ap.add_argument("--no-md", action="store_true", help="skip the minimisation + dynamics run")
     ap.add_argument("--no-scale", action="store_true", help="skip the ~100k-atom system")
+    ap.add_argument("--tune-reps", type=int, default=10,
+                    help="evaluations per candidate in the launch-shape autotuner")
     ap.add_argument("--profile-workload", choices=["polymer_29k", "scale_100k"], default=None,
                     help="only time --steps mixed evaluations of this workload (for profiler runs)")
+    ap.add_argument("--shapes", type=int, nargs=2, metavar=("SPLIT", "FFT_LINES"), default=None,
+                    help="with --profile-workload: these launch shapes instead of the defaults (4 8)")
     ap.add_argument("--triclinic", action="store_true",
                     help="also time the same system written as a triclinic cell (sheared basis)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -533,7 +543,7 @@ def main():
         copy_calibration()
         return
     if args.profile_workload:
-        profile_workload(args.profile_workload, args.steps, args.warmup)
+        profile_workload(args.profile_workload, args.steps, args.warmup, args.shapes)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X; "
@@ -578,6 +588,11 @@ def main():
     counts = ev32.counts()
     ev32.set_counting(False)
 
+    # launch shapes picked by the context's autotuner (every pair-kernel
+    # split and FFT block shape timed on these buffers, the fastest kept)
+    tuned32 = ev32.tune(x.data_ptr(), box, f32.data_ptr(), reps=args.tune_reps)
+    tuned64 = ev64.tune(x.data_ptr(), box, f64.data_ptr(), reps=max(3, args.tune_reps // 2))
+
     # the timed region: in one step of every 16 the pair kernel is bracketed by
     # two HIP events on the evaluator's stream, so its launch duration is
     # measured live; the steps replay a captured graph of the evaluation (the
@@ -590,8 +605,8 @@ def main():
         raise SystemExit(f"pair-kernel events timed {nb_launches} of {args.warmup + args.steps} steps")
     ev32.set_timing(0)
     nb_ms = nb_total / max(nb_launches, 1)
-    steps64 = args.fp64_steps or max(10, args.steps // 5)
-    t64 = timed_steps(torch, ev64, x.data_ptr(), f64.data_ptr(), box, steps64, max(2, args.warmup // 5))
+    steps64 = args.fp64_steps or max(20, args.steps // 2)
+    t64 = timed_steps(torch, ev64, x.data_ptr(), f64.data_ptr(), box, steps64, max(5, args.warmup // 4))
     rate32 = args.steps / t32
     rate64 = steps64 / t64
 
@@ -638,6 +653,10 @@ def main():
         "speedup_mixed_vs_fp64": rate32 / rate64,
         "host_buffer_evals_per_s": rate_host,
         "eval_path": "captured-graph replay" if graph_replay else "direct launches",
+        "tuned_shapes": {"mixed": {"split": tuned32[0], "fft_lines": tuned32[1] or 16,
+                                   "tune_median_ms_direct": tuned32[2]},
+                         "fp64": {"split": tuned64[0], "fft_lines": tuned64[1] or 16,
+                                  "tune_median_ms_direct": tuned64[2]}},
         "force_rel_rms_mixed_vs_fp64": rms,
         "energy_rel_err_mixed_vs_fp64": term_err,
         "work": counts,

@@ -333,6 +333,27 @@ int mdgpu_set_graph(mdgpu_ctx* ctx, int enable);
 /* 0: no graph captured yet, 1: evaluations replay a captured graph,
  * 2: direct launches for good (-1 for a null context). */
 int mdgpu_graph_state(const mdgpu_ctx* ctx);
+/* Autotuner of one context (no counterpart in the reference, whose loops
+ * have no launch shapes to choose).  Before the first graph evaluation
+ * (graph state 0, else MDGPU_ERR_STATE), every pair-kernel split (1, 2, 4, 8
+ * waves per i-cluster list) and FFT block shape (16, 8, 4 lines per block)
+ * is timed over `reps` evaluations of d_coords in `box` with direct launches
+ * (host clock around each synchronous evaluation, median kept), and the
+ * fastest pair is kept for the context's later evaluations.  Mixed-form
+ * forces are the same bits for every choice (fixed-point sums); energies and
+ * FP64 forces are summed in a fixed order for a given choice and differ
+ * between choices by rounding only.  The graph setting and timing mode and totals are as before
+ * afterwards; the list-build counters include the tuning evaluations.
+ * Returns the chosen split and FFT lines (0 = 16) and the median time (ms). */
+int mdgpu_tune(mdgpu_ctx* ctx, const double* d_coords, const double box[3], double* d_forces, int reps, int* split,
+               int* fft_lines, double* ms);
+/* Launch shapes of one context, set by hand or read back (no counterpart in
+ * the reference): split = waves per i-cluster list in the pair kernel
+ * (1..8, default 4), fft_lines = cap on FFT lines per block (0 = 16, else a
+ * power of two <= 16; default 8).  A change is refused (MDGPU_ERR_STATE)
+ * while the context replays a captured graph (graph state 1). */
+int mdgpu_set_shapes(mdgpu_ctx* ctx, int split, int fft_lines);
+int mdgpu_get_shapes(const mdgpu_ctx* ctx, int* split, int* fft_lines);
 /* Enable (1) or disable (0) counting of pairs within the cutoff. */
 int mdgpu_set_counting(mdgpu_ctx* ctx, int enable);
 

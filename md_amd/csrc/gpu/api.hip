@@ -7,6 +7,7 @@
 #include "ctx.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1427,6 +1428,81 @@ int mdgpu_set_graph(mdgpu_ctx* c, int enable) {
 }
 
 int mdgpu_graph_state(const mdgpu_ctx* c) { return c ? c->graph_state : -1; }
+
+int mdgpu_set_shapes(mdgpu_ctx* c, int split, int fft_lines) {
+    if (!c) return MDGPU_ERR_ARG;
+    if (split < 1 || split > 8 || fft_lines < 0 || fft_lines > 16 || (fft_lines & (fft_lines - 1)) != 0)
+        return set_error(c, MDGPU_ERR_ARG, "split 1..8, fft_lines 0 or a power of two <= 16");
+    if (c->graph_state == 1 && (split != c->split || fft_lines != c->fft_lines))
+        return set_error(c, MDGPU_ERR_STATE, "launch shapes are fixed while the context replays its graph");
+    c->split = split;
+    c->fft_lines = fft_lines;
+    return MDGPU_OK;
+}
+
+int mdgpu_get_shapes(const mdgpu_ctx* c, int* split, int* fft_lines) {
+    if (!c) return MDGPU_ERR_ARG;
+    if (split) *split = c->split;
+    if (fft_lines) *fft_lines = c->fft_lines;
+    return MDGPU_OK;
+}
+
+int mdgpu_tune(mdgpu_ctx* c, const double* d_coords, const double box[3], double* d_forces, int reps, int* split,
+               int* fft_lines, double* ms) {
+    if (!c || !d_coords || !box || reps < 1) return set_error(c, MDGPU_ERR_ARG, "bad argument");
+    if (!c->have_system || !c->have_params) return set_error(c, MDGPU_ERR_STATE, "upload a system and set params first");
+    if (c->graph_state != 0) return set_error(c, MDGPU_ERR_STATE, "tune before the first graph evaluation");
+    // Every candidate is timed with direct launches (no capture while tuning)
+    // and without timing events; both settings come back afterwards.
+    const bool use_graph = c->use_graph;
+    const mdg::Timing timing = c->timing;
+    c->use_graph = false;
+    c->timing.mode = 0;
+    const int split0 = c->split, fft0 = c->fft_lines;
+    const int cand_split[] = {1, 2, 4, 8};
+    const int cand_fft[] = {0, 8, 4};
+    std::vector<double> t(static_cast<size_t>(reps));
+    int best_p = split0, best_f = fft0;
+    double best = 0;
+    bool have_best = false;
+    int st = MDGPU_OK;
+    for (int p : cand_split) {
+        for (int f : cand_fft) {
+            c->split = p;
+            c->fft_lines = f;
+            st = eval_impl(c, d_coords, box, nullptr, d_forces, nullptr, false);  // settles the lists
+            for (int r = 0; r < reps && st == MDGPU_OK; ++r) {
+                const auto t0 = std::chrono::steady_clock::now();
+                st = eval_impl(c, d_coords, box, nullptr, d_forces, nullptr, false);
+                t[r] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            }
+            if (st != MDGPU_OK) break;
+            // the median evaluation: robust to a stray slow one
+            std::nth_element(t.begin(), t.begin() + reps / 2, t.end());
+            const double med = t[reps / 2];
+            if (!have_best || med < best) {
+                best = med;
+                best_p = p;
+                best_f = f;
+                have_best = true;
+            }
+        }
+        if (st != MDGPU_OK) break;
+    }
+    c->use_graph = use_graph;
+    c->timing = timing;
+    if (st != MDGPU_OK) {
+        c->split = split0;
+        c->fft_lines = fft0;
+        return st;
+    }
+    c->split = best_p;
+    c->fft_lines = best_f;
+    if (split) *split = best_p;
+    if (fft_lines) *fft_lines = best_f;
+    if (ms) *ms = best;
+    return MDGPU_OK;
+}
 
 int mdgpu_set_virial(mdgpu_ctx* c, int enable) {
     if (!c) return MDGPU_ERR_ARG;

@@ -144,6 +144,10 @@ def gpu_lib() -> C.CDLL:
         lib.mdgpu_set_counting.argtypes = [C.c_void_p, C.c_int]
         lib.mdgpu_set_graph.argtypes = [C.c_void_p, C.c_int]
         lib.mdgpu_graph_state.argtypes = [C.c_void_p]
+        lib.mdgpu_set_shapes.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        lib.mdgpu_get_shapes.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        lib.mdgpu_tune.argtypes = [C.c_void_p, C.c_void_p, P_F64, C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int), P_F64]
         lib.mdgpu_set_verlet.argtypes = [C.c_void_p, C.c_double]
         lib.mdgpu_set_box_angles.argtypes = [C.c_void_p, P_F64]
         lib.mdgpu_get_box_angles.argtypes = [C.c_void_p, P_F64]
@@ -560,6 +564,26 @@ class Evaluator:
     def graph_state(self) -> int:
         """0 no graph yet, 1 replaying a captured evaluation, 2 direct launches for good."""
         return int(self.lib.mdgpu_graph_state(self.ctx))
+
+    def set_shapes(self, split: int, fft_lines: int):
+        """Pair-kernel split (1..8) and FFT lines per block (0 = 16, 8, 4, ...)."""
+        self._check(self.lib.mdgpu_set_shapes(self.ctx, int(split), int(fft_lines)))
+
+    def shapes(self):
+        sp, fl = C.c_int(), C.c_int()
+        self._check(self.lib.mdgpu_get_shapes(self.ctx, C.byref(sp), C.byref(fl)))
+        return sp.value, fl.value
+
+    def tune(self, d_coords: int, box, d_forces: int = 0, reps: int = 10):
+        """Time every pair-kernel split and FFT block shape on these device
+        buffers (before the first graph evaluation) and keep the fastest;
+        returns (split, fft_lines, median ms per evaluation)."""
+        b = np.ascontiguousarray(box, dtype=np.float64)
+        sp, fl, ms = C.c_int(), C.c_int(), C.c_double()
+        self._check(self.lib.mdgpu_tune(self.ctx, C.c_void_p(d_coords), b.ctypes.data_as(P_F64),
+                                        C.c_void_p(d_forces or None), int(reps), C.byref(sp), C.byref(fl),
+                                        C.byref(ms)))
+        return sp.value, fl.value, ms.value
 
     def set_verlet(self, skin: float):
         """Keep the pair list across evaluations with this skin (A) until an atom

@@ -109,3 +109,56 @@ def test_sampled_pair_timing_with_graph_replay(polymer_2k):
         assert 0.0 < ms / n < 10.0
     finally:
         ev.close()
+
+
+@pytest.mark.parametrize("prec", [N.PREC_MIXED, N.PREC_FP64], ids=["mixed", "fp64"])
+def test_tune_keeps_results_and_graph(prec, polymer_2k):
+    """mdgpu_tune times every split / FFT block shape, keeps the fastest and
+    leaves the context as it was: no graph yet, timing totals unchanged, and
+    evaluations that agree with an untuned context (mixed forces bit for bit,
+    energies and FP64 forces to rounding)."""
+    prm_path, rst_path = polymer_2k
+    top = N.Topology(prm_path, rst_path, N.SEM_AMBER)
+    print(f"case polymer_2k tune prec={prec}", flush=True)
+    prm = N.make_params(N.SEM_AMBER, prec, lj_cutoff=10.0, lj_switch=9.0, ewald_tol=1e-5, pme_grid=(32, 32, 32))
+    box = list(np.asarray(top.box, dtype=np.float64).reshape(-1)[:3])
+    dev = torch.device("cuda", 0)
+    xd = torch.tensor(_frames(top, 1)[0], device=dev)
+    ref, out = N.Evaluator(0), N.Evaluator(0)
+    try:
+        ref.upload(top.desc)
+        ref.set_params(prm)
+        fr = torch.zeros_like(xd)
+        er, _ = ref.evaluate_device(xd.data_ptr(), box, fr.data_ptr())
+        out.upload(top.desc)
+        out.set_params(prm)
+        out.set_timing(3)
+        fd = torch.zeros_like(xd)
+        split, lines, ms = out.tune(xd.data_ptr(), box, fd.data_ptr(), reps=3)
+        print(f"  tuned split {split} fft_lines {lines} median {ms:.4f} ms")
+        assert split in (1, 2, 4, 8) and lines in (0, 8, 4) and 0.0 < ms < 100.0
+        assert out.graph_state() == 0
+        assert out.timing()["nonbonded"] == (0.0, 0)
+        for _ in range(3):  # the second captures, the third replays
+            e, _ = out.evaluate_device(xd.data_ptr(), box, fd.data_ptr())
+        assert out.graph_state() == 1
+        torch.cuda.synchronize()
+        f_ref, f_out = fr.cpu().numpy(), fd.cpu().numpy()
+        for key in KEYS:
+            assert abs(e[key] - er[key]) <= 1e-12 * max(abs(er[key]), 1.0), (key, e[key], er[key])
+        if prec == N.PREC_MIXED:
+            assert np.array_equal(f_out, f_ref)
+        else:
+            assert np.abs(f_out - f_ref).max() <= 1e-11 * np.abs(f_ref).max()
+        # once a graph exists the shapes are fixed
+        assert out.shapes() == (split, lines)
+        with pytest.raises(N.GpuError):
+            out.tune(xd.data_ptr(), box, fd.data_ptr(), reps=1)
+        out.set_shapes(split, lines)  # unchanged: accepted
+        with pytest.raises(N.GpuError):
+            out.set_shapes(2 if split != 2 else 4, lines)
+        with pytest.raises(N.GpuError):
+            ref.set_shapes(3, 5)  # FFT lines must be a power of two
+    finally:
+        ref.close()
+        out.close()
