@@ -567,28 +567,32 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search(const SearchArg
 // entry with its j atom in all eight tiles.
 constexpr int kExclCap = 32;
 
-// The i-cluster's exclusion lists as registers: issued at the start of the
-// search so that their dependent loads overlap the candidate scan.
+// The i-cluster's exclusion lists as registers, loaded in three dependent
+// steps (the atoms of the cluster, their list bounds, the entries) that the
+// search issues beside its own loads, so that each rides on a round trip the
+// search makes anyway.
 struct ExclRegs {
     int a_l, e0, en;
     int v[kCluster];  // entry `lane` of atom t's list
 };
 
-__device__ __forceinline__ ExclRegs excl_lists_load(const SearchArgs& a, int ci, int lane) {
-    ExclRegs r;
-    r.e0 = 0;
-    r.en = 0;
-    r.a_l = lane < kCluster ? a.slot_atom[ci * kCluster + lane] : -1;
-    if (r.a_l >= 0) {
-        r.e0 = a.excl_start[r.a_l];
-        r.en = a.excl_start[r.a_l + 1] - r.e0;
-    }
+// second step: list bounds of lane's atom (lanes 0..7; a_l < 0 elsewhere),
+// kept as loaded (en holds the end until the third step)
+__device__ __forceinline__ void excl_bounds(const SearchArgs& a, ExclRegs& r) {
+    const int k = r.a_l >= 0 ? r.a_l : 0;
+    r.e0 = a.excl_start[k];
+    r.en = a.excl_start[k + 1];
+}
+
+// third step: entry `lane` of each atom's list (the first kExclCap)
+__device__ __forceinline__ void excl_entries(const SearchArgs& a, ExclRegs& r, int lane) {
+    r.en = r.a_l >= 0 ? r.en - r.e0 : 0;
+    r.e0 = r.a_l >= 0 ? r.e0 : 0;
 #pragma unroll
     for (int t = 0; t < kCluster; ++t) {
         const int n_t = min(__shfl(r.en, t, kWave), kExclCap), s_t = __shfl(r.e0, t, kWave);
         r.v[t] = lane < n_t ? a.excl_list[s_t + lane] : -1;
     }
-    return r;
 }
 
 // The j atoms of the first kNearPre groups of eight near tiles are loaded at
@@ -686,15 +690,29 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     __shared__ int2 s_near[kSearchWaves][kNearCap];
     __shared__ int s_ex[kSearchWaves][kCluster][kExclCap];
     __shared__ int s_cand[kSearchWaves][kCandCap];
-    if (a.vs && !a.vs->build) return;
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
-    const int ci = __builtin_amdgcn_readfirstlane(blockIdx.x * kSearchWaves + wv);
-    const int ncl = min(*a.ncl_dev, a.max_clusters);
-    if (ci >= ncl) return;  // whole wave leaves; the kernel has no block barrier
+    const int ci_w = __builtin_amdgcn_readfirstlane(blockIdx.x * kSearchWaves + wv);
+    // First round trip: the Verlet build flag and the cluster count with
+    // everything indexed by the cluster alone.  The arrays hold max_clusters
+    // entries and the index is clamped to them, so these loads stay in
+    // bounds for a wave past the clusters, and none of their values is used
+    // as an index before the checks below.
+    const int ci = min(ci_w, a.max_clusters - 1);
+    const R4* pos = static_cast<const R4*>(a.pos);
+    const int build = a.vs ? a.vs->build : 1;
+    const int ncl_dev = *a.ncl_dev;
     const int col = a.cluster_col[ci];
-    const int cx = col % a.ncx, cy = col / a.ncx;
     const double4 oi = a.origin[ci];
     const float4 hi = a.half[ci];
+    const R4 pi_l = pos[ci * kCluster + (lane & 7)];  // the i-cluster's atoms relative to its origin
+    ExclRegs xr{};
+    xr.a_l = a.amber && lane < kCluster ? a.slot_atom[ci * kCluster + lane] : -1;
+    const float excl_rmax = a.amber ? *a.excl_rmax : 0.f;
+    const int ncl = min(ncl_dev, a.max_clusters);
+    // the whole wave leaves (the kernel has no block barrier): a kept
+    // Verlet list, or a wave past the clusters
+    if (!build || ci_w >= ncl) return;
+    const int cx = col % a.ncx, cy = col / a.ncx;
     const int ny = min(2 * a.dcy + 1, a.ncy);
     if (hi.x > hlx || hi.y > hly || hi.z > hlz || 2 * ny > kWave) {
         if (lane == 0) {
@@ -707,13 +725,10 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     const size_t base_i = static_cast<size_t>(ci) * a.cap;
     R4* delta = reinterpret_cast<R4*>(a.list_delta);
     double rex = 0.0;
-    if (a.amber) rex = static_cast<double>(*a.excl_rmax) + 0.01;
+    if (a.amber) rex = static_cast<double>(excl_rmax) + 0.01;
     const double rex2 = rex * rex;
-    ExclRegs xr{};
-    if (a.amber) xr = excl_lists_load(a, ci, lane);
-    // the i-cluster's atoms relative to its origin (used after the ranges)
-    const R4* pos = static_cast<const R4*>(a.pos);
-    const R4 pi_l = pos[ci * kCluster + (lane & 7)];
+    // second round trip: the exclusion-list bounds beside the column ranges
+    if (a.amber) excl_bounds(a, xr);
 
     // ranges: lane r = 2 * row + segment, each clipped to the half window
     // [ci, ci + hw) (circular)
@@ -786,6 +801,9 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     const int r_pre = r_inc - r_len;
     const int total = __shfl(r_inc, kWave - 1, kWave);
     const int nr = 2 * ny;
+    // third round trip: the exclusion-list entries beside the first
+    // candidates' boxes (needed only after the candidate walk)
+    if (a.amber) excl_entries(a, xr, lane);
     // the stream in LDS: range k occupies [r_pre_k, r_pre_k + r_len_k)
     int* cand_s = s_cand[wv];
     const bool staged = total <= kCandCap;
