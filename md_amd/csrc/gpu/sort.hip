@@ -570,7 +570,8 @@ constexpr int kExclCap = 32;
 // The i-cluster's exclusion lists as registers, loaded in three dependent
 // steps (the atoms of the cluster, their list bounds, the entries) that the
 // search issues beside its own loads, so that each rides on a round trip the
-// search makes anyway.
+// search makes anyway: the cluster's own records, the column ranges, and the
+// near tiles' j atoms.
 struct ExclRegs {
     int a_l, e0, en;
     int v[kCluster];  // entry `lane` of atom t's list
@@ -727,6 +728,16 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     double rex = 0.0;
     if (a.amber) rex = static_cast<double>(excl_rmax) + 0.01;
     const double rex2 = rex * rex;
+    // the i-cluster's atoms relative to its origin, in scalar registers (read
+    // before the next loads are issued: the counter that a wait for a load
+    // checks also covers every load issued after it)
+    real pix[kCluster], piy[kCluster], piz[kCluster];
+#pragma unroll
+    for (int k = 0; k < kCluster; ++k) {
+        pix[k] = bcast_lane(pi_l.x, k);
+        piy[k] = bcast_lane(pi_l.y, k);
+        piz[k] = bcast_lane(pi_l.z, k);
+    }
     // second round trip: the exclusion-list bounds beside the column ranges
     if (a.amber) excl_bounds(a, xr);
 
@@ -801,9 +812,6 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
     const int r_pre = r_inc - r_len;
     const int total = __shfl(r_inc, kWave - 1, kWave);
     const int nr = 2 * ny;
-    // third round trip: the exclusion-list entries beside the first
-    // candidates' boxes (needed only after the candidate walk)
-    if (a.amber) excl_entries(a, xr, lane);
     // the stream in LDS: range k occupies [r_pre_k, r_pre_k + r_len_k)
     int* cand_s = s_cand[wv];
     const bool staged = total <= kCandCap;
@@ -831,14 +839,6 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
         return q < total ? cj : -1;
     };
 
-    // the i-cluster's atoms relative to its origin, in scalar registers
-    real pix[kCluster], piy[kCluster], piz[kCluster];
-#pragma unroll
-    for (int k = 0; k < kCluster; ++k) {
-        pix[k] = bcast_lane(pi_l.x, k);
-        piy[k] = bcast_lane(pi_l.y, k);
-        piz[k] = bcast_lane(pi_l.z, k);
-    }
     const real rp2 = static_cast<real>(a.rp2);
 
     int count = 0, nnear = 0;
@@ -936,6 +936,9 @@ __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const Search
         if (lane == 0) atomicOr(a.flags + 3, 1u);
         nnear = kNearCap;
     }
+    // the exclusion-list entries, issued beside the near tiles' j atoms
+    // (one round trip for both)
+    excl_entries(a, xr, lane);
     __builtin_amdgcn_wave_barrier();
     near_tile_masks(a, ci, lane, xr, s_ex[wv], s_near[wv], nnear, base_i);
 }
