@@ -234,9 +234,8 @@ __global__ __launch_bounds__(kSortThreads) void k_col_sort(int ncol, int* __rest
                                                            int nbin, const ClusterOut<real> o,
                                                            const VerletState* __restrict__ vs) {
     constexpr int kPer = kColMax / kSortThreads;  // atoms per thread at most
-    __shared__ unsigned long long s_key[kColMax];  // keys; after ranking, x by rank
-    __shared__ double s_y[kColMax], s_z[kColMax];
-    __shared__ int s_sorted[kColMax];
+    __shared__ unsigned long long s_key[kColMax];  // the column's sort keys
+    __shared__ int s_sorted[kColMax];              // atom by rank
     __shared__ int s_red[kSortThreads / kWave];
     __shared__ int s_n, s_last;
     if (vs && !vs->build) return;
@@ -293,18 +292,10 @@ __global__ __launch_bounds__(kSortThreads) void k_col_sort(int ncol, int* __rest
         }
     }
     __syncthreads();
-    double* s_x = reinterpret_cast<double*>(s_key);
 #pragma unroll
     for (int p = 0; p < kPer; ++p) {
         const int k = static_cast<int>(threadIdx.x) + p * kSortThreads;
-        if (k < n) {
-            const int id = static_cast<int>(key[p] & 0xffffffffull);
-            const int r = rank[p];
-            s_sorted[r] = id;
-            s_x[r] = wrapped[3 * id];
-            s_y[r] = wrapped[3 * id + 1];
-            s_z[r] = wrapped[3 * id + 2];
-        }
+        if (k < n) s_sorted[rank[p]] = static_cast<int>(key[p] & 0xffffffffull);
     }
     __syncthreads();
     const int nslots = min(ncl * kCluster, kColMax);
@@ -318,11 +309,15 @@ __global__ __launch_bounds__(kSortThreads) void k_col_sort(int ncol, int* __rest
         // the ranks are a permutation of the column's keys (each atom's key
         // is placed once); an index outside the system never reaches memory
         if (atom >= o.n_atoms) atom = -1;
-        double x = 0, y = 0, z = 0;
+        // the atom's coordinates, charge and type in one round trip
+        double x = 0, y = 0, z = 0, q = 0;
+        int type = o.pad_type;
         if (atom >= 0) {
-            x = s_x[k];
-            y = s_y[k];
-            z = s_z[k];
+            x = wrapped[3 * atom];
+            y = wrapped[3 * atom + 1];
+            z = wrapped[3 * atom + 2];
+            q = o.charge[atom];
+            type = o.type[atom];
         }
         double lo[3] = {atom >= 0 ? x : 1e300, atom >= 0 ? y : 1e300, atom >= 0 ? z : 1e300};
         double hi[3] = {atom >= 0 ? x : -1e300, atom >= 0 ? y : -1e300, atom >= 0 ? z : -1e300};
@@ -343,8 +338,8 @@ __global__ __launch_bounds__(kSortThreads) void k_col_sort(int ncol, int* __rest
             p.x = static_cast<real>(x - ox);
             p.y = static_cast<real>(y - oy);
             p.z = static_cast<real>(z - oz);
-            p.w = static_cast<real>(o.charge[atom]);
-            o.slot_meta[slot] = make_int2(o.type[atom], atom);
+            p.w = static_cast<real>(q);
+            o.slot_meta[slot] = make_int2(type, atom);
         } else {
             p.x = p.y = p.z = static_cast<real>(kFarAway);
             p.w = 0;
@@ -361,7 +356,7 @@ __global__ __launch_bounds__(kSortThreads) void k_col_sort(int ncol, int* __rest
                 wq.x = wv[0];
                 wq.y = wv[1];
                 wq.z = wv[2];
-                wq.w = static_cast<real>(o.charge[atom]);
+                wq.w = static_cast<real>(q);
             }
             o.pme.fl[slot] = packed;
             reinterpret_cast<typename Vec4<real>::T*>(o.pme.w)[slot] = wq;
