@@ -673,12 +673,15 @@ __device__ __forceinline__ double bcast_lane(double v, int l) {
 //
 // The candidate clusters of the rows in reach form at most two contiguous
 // index ranges per row (the row's x window, split where it wraps).  Lane r
-// loads the bounds of range r, clipped to j >= i for the half list; the
+// loads the bounds of range r, clipped to the circular half window; the
 // ranges are then walked as one stream of candidates, 64 per step, with the
 // next step's cluster boxes loaded while the current step is tested.  The
 // stream's cluster indices are written once into LDS (streams of up to
 // kCandCap candidates; a longer one maps each index through the ranges).
-constexpr int kCandCap = 1024;
+// 768 holds the longest streams of a water-density system at rc 12 A (467)
+// or 10 A (675) and keeps a block's LDS at 24.6 KB, so six blocks fit a CU
+// (as many as the registers allow) where 1,024 allowed five.
+constexpr int kCandCap = 768;
 template <typename real>
 __global__ __launch_bounds__(kSearchWaves * kWave) void k_search_mi(const SearchArgs a, double hlx, double hly,
                                                                      double hlz) {

@@ -126,3 +126,34 @@ def test_cfg2_mixed_bit_identical_on_repeat(cfg2):
         assert ev.graph_state() == 1
     finally:
         ev.close()
+
+
+def test_long_candidate_streams_match_oracle(cfg2):
+    """rc 18 A (switch 17 A) on the same system: a quarter of the
+    i-clusters then have more candidate j-clusters (up to 874) than the
+    search stages in LDS (768), so for them the search maps stream
+    positions through its column ranges instead; the
+    fast minimum-image search must still run (search mode 0) and give the
+    oracle's result at the FP64 bars, and the mixed form its usual bar."""
+    kw = dict(lj_cutoff=18.0, lj_switch=17.0, ewald_tol=1e-5, pme_grid=(72, 72, 72))
+    print(f"case polymer_29k rc 18 A, {cfg2.n_atoms} atoms", flush=True)
+    out = {}
+    for prec in (N.PREC_FP64, N.PREC_MIXED):
+        ev = N.Evaluator(0)
+        try:
+            ev.upload(cfg2.desc)
+            ev.set_params(N.make_params(N.SEM_AMBER, prec, **kw))
+            out[prec] = ev.evaluate(cfg2.coords, cfg2.box)
+            assert ev.search_mode() == 0
+        finally:
+            ev.close()
+    e, f, _ = out[N.PREC_FP64]
+    p = N.make_params(N.SEM_AMBER, N.PREC_FP64, **kw)
+    eo, fo, _ = oracle.evaluate(cfg2.desc, p, cfg2.coords, cfg2.box, skip_all_pairs=True)
+    bad = [k for k in TERMS if not abs(e[k] - eo[k]) <= 1e-9 * abs(eo[k]) + 1e-12]
+    r = _rms(f, fo)
+    r32 = _rms(out[N.PREC_MIXED][1], f)
+    print(f"  fp64 vs oracle force rel-RMS {r:.2e}, terms off {bad}; mixed vs fp64 {r32:.2e}")
+    assert not bad, bad
+    assert r <= 1e-8
+    assert r32 <= 1e-5
