@@ -342,7 +342,9 @@ int mdgpu_graph_state(const mdgpu_ctx* ctx);
  * fastest pair is kept for the context's later evaluations.  Mixed-form
  * forces are the same bits for every choice (fixed-point sums); energies and
  * FP64 forces are summed in a fixed order for a given choice and differ
- * between choices by rounding only.  The graph setting and timing mode and totals are as before
+ * between choices by the rounding of their partial sums (the split decides
+ * which wave sums which pairs; FP32 partial energies in the mixed form, about
+ * 1e-9 of the total).  The graph setting and timing mode and totals are as before
  * afterwards; the list-build counters include the tuning evaluations.
  * Returns the chosen split and FFT lines (0 = 16) and the median time (ms). */
 int mdgpu_tune(mdgpu_ctx* ctx, const double* d_coords, const double box[3], double* d_forces, int reps, int* split,

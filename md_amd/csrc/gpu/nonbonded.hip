@@ -484,8 +484,14 @@ __device__ __forceinline__ float sep2(float dx, float dy, float dz) {
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dx, dx, dy * dy));
 }
 
+// Register budget of eight waves per SIMD (64 VGPRs): the pair loop fits
+// without spills (the FP64 bonded and correction blocks of the same launch
+// spill a few values; they are a small share of it).  LDS still holds a CU
+// to three blocks of eight waves for 30 LJ types; the tighter allocation
+// measured as fast as six waves' (80 VGPRs) at 29k atoms and faster at 100k.
+// Seven waves' (72 VGPRs) was slower.
 template <bool kSwitch, bool kCount>
-__global__ __launch_bounds__(kWavesOf<float> * kWave) __attribute__((amdgpu_waves_per_eu(kCount ? 1 : 6))) void
+__global__ __launch_bounds__(kWavesOf<float> * kWave) __attribute__((amdgpu_waves_per_eu(kCount ? 1 : 8))) void
 k_nonbonded_c(const NbArgs<float> a, const BArgs<float> ba, const CArgs ca, int npb, int nbb) {
     constexpr int kWaves = kWavesOf<float>;
     if (static_cast<int>(blockIdx.x) >= npb) {

@@ -116,7 +116,7 @@ def test_tune_keeps_results_and_graph(prec, polymer_2k):
     """mdgpu_tune times every split / FFT block shape, keeps the fastest and
     leaves the context as it was: no graph yet, timing totals unchanged, and
     evaluations that agree with an untuned context (mixed forces bit for bit,
-    energies and FP64 forces to rounding)."""
+    energies and FP64 forces to the rounding of their partial sums)."""
     prm_path, rst_path = polymer_2k
     top = N.Topology(prm_path, rst_path, N.SEM_AMBER)
     print(f"case polymer_2k tune prec={prec}", flush=True)
@@ -144,8 +144,12 @@ def test_tune_keeps_results_and_graph(prec, polymer_2k):
         assert out.graph_state() == 1
         torch.cuda.synchronize()
         f_ref, f_out = fr.cpu().numpy(), fd.cpu().numpy()
+        # energies: summed in a fixed order for a given split, but the split
+        # decides which wave sums which pairs (FP32 partial sums in the mixed
+        # form, FP64 in the FP64 form)
+        ebar = 1e-7 if prec == N.PREC_MIXED else 1e-12
         for key in KEYS:
-            assert abs(e[key] - er[key]) <= 1e-12 * max(abs(er[key]), 1.0), (key, e[key], er[key])
+            assert abs(e[key] - er[key]) <= ebar * max(abs(er[key]), 1.0), (key, e[key], er[key])
         if prec == N.PREC_MIXED:
             assert np.array_equal(f_out, f_ref)
         else:
