@@ -338,7 +338,8 @@ int mdgpu_graph_state(const mdgpu_ctx* ctx);
  * (graph state 0, else MDGPU_ERR_STATE), every pair-kernel split (1, 2, 4, 8
  * waves per i-cluster list) and FFT block shape (16, 8, 4 lines per block)
  * is timed over `reps` evaluations of d_coords in `box` with direct launches
- * (host clock around each synchronous evaluation, median kept), and the
+ * (host clock around each synchronous evaluation; two rounds over the
+ * candidates, each candidate's time the lower of its two medians), and the
  * fastest pair is kept for the context's later evaluations.  Mixed-form
  * forces are the same bits for every choice (fixed-point sums); energies and
  * FP64 forces are summed in a fixed order for a given choice and differ
