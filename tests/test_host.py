@@ -93,3 +93,17 @@ def test_noalloc_query_rejects_bad_arguments_before_any_launch():
     bad = np.array([30.0, 0.0, 30.0])
     assert f(0, fake, fake, 10, bad.ctypes.data_as(N.P_F64), 1.0, None, fake, None) == 1
     assert f(10 ** 6, fake, fake, 10, bp, 1.0, None, fake, None) == 1
+
+
+def test_shape_entry_points_reject_a_null_context():
+    """mdgpu_tune / mdgpu_set_shapes / mdgpu_get_shapes check their context
+    before any device work: a null one gives MDGPU_ERR_ARG."""
+    import ctypes
+    lib = N.gpu_lib()
+    box = np.array([30.0, 30.0, 30.0])
+    bp = box.ctypes.data_as(N.P_F64)
+    fake = ctypes.c_void_p(0x1000)   # never dereferenced: the context check comes first
+    sp, fl, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    assert lib.mdgpu_tune(None, fake, bp, None, 3, ctypes.byref(sp), ctypes.byref(fl), ctypes.byref(ms)) == 1
+    assert lib.mdgpu_set_shapes(None, 2, 8) == 1
+    assert lib.mdgpu_get_shapes(None, ctypes.byref(sp), ctypes.byref(fl)) == 1
