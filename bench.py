@@ -243,8 +243,7 @@ def bench_md(torch, N, systems, top, p32, box, steps=200, warmup=20, dt=0.0005):
     The water hydrogens get the CHARMM TIP3P LJ radius
     (sigma 0.4 A, eps 0.046 kcal/mol): without it a hydrogen and an unlike
     charge have no repulsion and a minimiser can follow a hydrogen into
-    another atom.  The launch shapes are tuned once on the starting
-    geometry.  Each MD leg starts from the minimised coordinates;
+    another atom.  Each MD leg starts from the minimised coordinates;
     steps/s over `steps` timed steps after `warmup` untimed ones."""
     dev = torch.device("cuda:0")
     arr = systems.with_hydrogen_lj(top.arrays, int(top.arrays["type"][-1]))
@@ -257,8 +256,6 @@ def bench_md(torch, N, systems, top, p32, box, steps=200, warmup=20, dt=0.0005):
     x = torch.tensor(top.coords.reshape(-1), dtype=torch.float64, device=dev)
     f = torch.zeros_like(x)
     e0, _ = ev.evaluate_device(x.data_ptr(), box, f.data_ptr())
-    # launch shapes for the whole leg, tuned on the starting geometry (kept list)
-    tuned = ev.tune(x.data_ptr(), box, f.data_ptr(), reps=6)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     res = ev.minimize(x.data_ptr(), box, max_iter=300, rms_force_tol=1.0, max_step=0.1)
@@ -299,7 +296,6 @@ def bench_md(torch, N, systems, top, p32, box, steps=200, warmup=20, dt=0.0005):
                         "L-BFGS (max 300 iterations, "
                         "RMS force 1 kcal/mol/A, 0.1 A step cap, skin 1 A), then velocity Verlet NVE, "
                         f"dt {dt * 1000:.1f} fs, 300 K start",
-            "tuned_shapes": {"split": tuned[0], "fft_lines": tuned[1] or 16},
             "minimize": {"energy_before": e0["total"], "energy_after": res["energy"],
                          "iterations": res["iterations"], "evaluations": res["evaluations"],
                          "rms_force": res["rms_force"], "max_force": res["max_force"], "seconds": t_min,
